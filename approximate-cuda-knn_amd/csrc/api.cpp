@@ -1,0 +1,245 @@
+// extern "C" boundary of libaknn.so (declarations and reference citations in
+// include/aknn.h).  Every entry point catches everything: nothing throws across
+// the ABI.
+#include "aknn.h"
+
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+#include "io.hpp"
+
+struct aknn_handle {
+  bool debug = false;
+  std::string data_file, results_file, error;
+  aknn::Params params;
+  bool have_options = false;
+  int device = -1;
+  std::unique_ptr<aknn::Engine> engine;
+  bool calculated = false;
+};
+
+namespace {
+
+int fail(aknn_handle* h, int code, const std::string& msg) {
+  if (h) h->error = msg;
+  return code;
+}
+
+template <typename F>
+int guarded(aknn_handle* h, F&& f) {
+  if (!h) return AKNN_ERR_ARGUMENT;
+  try {
+    h->error.clear();
+    return f();
+  } catch (const aknn::HipError& e) {
+    return fail(h, AKNN_ERR_DEVICE, e.what);
+  } catch (const std::invalid_argument& e) {
+    return fail(h, AKNN_ERR_ARGUMENT, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(h, AKNN_ERR_DEVICE, "out of host memory");
+  } catch (const std::exception& e) {
+    return fail(h, AKNN_ERR_STATE, e.what());
+  } catch (...) {
+    return fail(h, AKNN_ERR_STATE, "unknown error");
+  }
+}
+
+int setup(aknn_handle* h, const float* x, int64_t n, int d, int k, int device, int distance, int trees, int props,
+          int min_size) {
+  aknn::Params p = h->params;
+  p.k = k;
+  p.trees = trees;
+  p.leaf = min_size;
+  p.props = props;
+  p.metric = distance;
+  if (!h->engine || h->device != device) {
+    h->engine.reset();
+    h->engine = std::make_unique<aknn::Engine>(device);
+    h->device = device;
+  }
+  h->engine->set_data(x, n, d, p);
+  h->params = p;
+  h->calculated = false;
+  return AKNN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+aknn_handle* aknn_create(int debug_mode) {
+  try {
+    auto* h = new aknn_handle();
+    h->debug = debug_mode != 0;
+    return h;
+  } catch (...) {
+    return nullptr;
+  }
+}
+
+void aknn_destroy(aknn_handle* h) {
+  try {
+    delete h;
+  } catch (...) {
+  }
+}
+
+int aknn_set_data_file(aknn_handle* h, const char* path) {
+  return guarded(h, [&] {
+    if (!path) return fail(h, AKNN_ERR_ARGUMENT, "null path");
+    h->data_file = path;
+    return AKNN_OK;
+  });
+}
+
+int aknn_set_results_file(aknn_handle* h, const char* path) {
+  return guarded(h, [&] {
+    if (!path) return fail(h, AKNN_ERR_ARGUMENT, "null path");
+    h->results_file = path;
+    return AKNN_OK;
+  });
+}
+
+int aknn_initialize(aknn_handle* h, int n, int d, int k, int device, int distance, int klaster, int trees,
+                    int propagations, int min_size) {
+  return guarded(h, [&] {
+    if (n < 1 || d < 1) return fail(h, AKNN_ERR_ARGUMENT, "N and D must be positive");
+    std::vector<float> x;
+    std::vector<int32_t> labels;
+    const std::string err = aknn::load_csv(h->data_file, n, d, klaster != 0, x, labels);
+    if (!err.empty()) return fail(h, AKNN_ERR_IO, err);
+    return setup(h, x.data(), n, d, k, device, distance, trees, propagations, min_size);
+  });
+}
+
+int aknn_initialize_host(aknn_handle* h, const float* x, int64_t n, int d, int k, int device, int distance,
+                         int trees, int propagations, int min_size) {
+  return guarded(h, [&] {
+    if (!x) return fail(h, AKNN_ERR_ARGUMENT, "null data");
+    return setup(h, x, n, d, k, device, distance, trees, propagations, min_size);
+  });
+}
+
+int aknn_calculate(aknn_handle* h) {
+  return guarded(h, [&] {
+    if (!h->engine) return fail(h, AKNN_ERR_STATE, "not initialized");
+    h->engine->calculate();
+    h->calculated = true;
+    return AKNN_OK;
+  });
+}
+
+int aknn_save_result(aknn_handle* h) {
+  return guarded(h, [&] {
+    if (!h->engine || !h->calculated) return fail(h, AKNN_ERR_STATE, "nothing calculated");
+    const int64_t nk = h->engine->n() * h->params.k;
+    std::vector<int32_t> ids(nk);
+    std::vector<float> vals(nk);
+    h->engine->get_graph(ids.data(), vals.data());
+    const std::string err = aknn::write_result(h->results_file, h->debug, h->params.metric, h->engine->n(),
+                                               h->params.k, ids.data(), vals.data());
+    if (!err.empty()) return fail(h, AKNN_ERR_IO, err);
+    return AKNN_OK;
+  });
+}
+
+int aknn_deinitialize(aknn_handle* h) {
+  return guarded(h, [&] {
+    h->engine.reset();
+    h->device = -1;
+    h->calculated = false;
+    return AKNN_OK;
+  });
+}
+
+int aknn_set_option(aknn_handle* h, const char* name, int64_t value) {
+  return guarded(h, [&] {
+    if (!name) return fail(h, AKNN_ERR_ARGUMENT, "null option name");
+    const std::string n(name);
+    if (n == "seed") {
+      if (value < 0 || value > 0xffffffffLL) return fail(h, AKNN_ERR_ARGUMENT, "seed must fit in 32 bits");
+      h->params.seed = static_cast<uint32_t>(value);
+    } else if (n == "rng_order") {
+      if (value != 0 && value != 1) return fail(h, AKNN_ERR_ARGUMENT, "rng_order is 0 or 1");
+      h->params.rng_order = static_cast<int>(value);
+    } else if (n == "cosine_fixed") {
+      h->params.cosine_fixed = value != 0;
+    } else if (n == "level_batch") {
+      if (value < 1 || value > 4096) return fail(h, AKNN_ERR_ARGUMENT, "level_batch must be in [1, 4096]");
+      h->params.level_batch = static_cast<int>(value);
+    } else {
+      return fail(h, AKNN_ERR_ARGUMENT, "unknown option " + n);
+    }
+    // options apply to the next calculate(); the data stays uploaded
+    if (h->engine) h->engine->set_options(h->params);
+    return AKNN_OK;
+  });
+}
+
+int aknn_get_graph(aknn_handle* h, int32_t* ids, float* vals) {
+  return guarded(h, [&] {
+    if (!h->engine || !h->calculated) return fail(h, AKNN_ERR_STATE, "nothing calculated");
+    if (!ids || !vals) return fail(h, AKNN_ERR_ARGUMENT, "null output");
+    h->engine->get_graph(ids, vals);
+    return AKNN_OK;
+  });
+}
+
+int aknn_get_normalized(aknn_handle* h, float* x) {
+  return guarded(h, [&] {
+    if (!h->engine || !h->calculated) return fail(h, AKNN_ERR_STATE, "nothing calculated");
+    if (!x) return fail(h, AKNN_ERR_ARGUMENT, "null output");
+    h->engine->get_normalized(x);
+    return AKNN_OK;
+  });
+}
+
+int aknn_get_forest(aknn_handle* h, int tree, int32_t* perm, int32_t* range_start, int32_t* range_count,
+                    int32_t* iterations) {
+  return guarded(h, [&] {
+    if (!h->engine || !h->calculated) return fail(h, AKNN_ERR_STATE, "nothing calculated");
+    if (!perm || !range_start || !range_count) return fail(h, AKNN_ERR_ARGUMENT, "null output");
+    h->engine->get_forest(tree, perm, range_start, range_count);
+    if (iterations) *iterations = h->engine->tree_iterations(tree);
+    return AKNN_OK;
+  });
+}
+
+int aknn_get_stage_times(aknn_handle* h, double* ms, int n) {
+  return guarded(h, [&] {
+    if (!h->engine || !h->calculated) return fail(h, AKNN_ERR_STATE, "nothing calculated");
+    const aknn::StageTimes& t = h->engine->times();
+    const double v[5] = {t.normalize, t.forest, t.initial, t.propagation, t.total};
+    for (int i = 0; i < n && i < 5; ++i) ms[i] = v[i];
+    return AKNN_OK;
+  });
+}
+
+int aknn_get_kernel_stats(aknn_handle* h, const char* name, int64_t* launches, double* total_ms, double* units) {
+  return guarded(h, [&] {
+    if (!h->engine || !h->calculated) return fail(h, AKNN_ERR_STATE, "nothing calculated");
+    if (!name) return fail(h, AKNN_ERR_ARGUMENT, "null name");
+    const std::string n(name);
+    const aknn::KernelStats* s = nullptr;
+    if (n == "leaf_knn")
+      s = &h->engine->stats_leaf_knn;
+    else if (n == "propagate")
+      s = &h->engine->stats_propagate;
+    else if (n == "forest")
+      s = &h->engine->stats_forest;
+    else
+      return fail(h, AKNN_ERR_ARGUMENT, "unknown kernel " + n);
+    if (launches) *launches = s->launches;
+    if (total_ms) *total_ms = s->total_ms;
+    if (units) *units = s->units;
+    return AKNN_OK;
+  });
+}
+
+const char* aknn_last_error(aknn_handle* h) { return h ? h->error.c_str() : "null handle"; }
+
+}  // extern "C"

@@ -1,0 +1,132 @@
+// Device helpers shared by the kernels: the exact FP32 distance chains and
+// wave64 utilities.
+//
+// Bit-exactness rules (the reference is nvcc-compiled with default FMA
+// contraction and IEEE division/sqrt, euclideandistancematrixgpu.cu:243-273,
+// :540-570, :840-881):
+//   * every multiply-add of a chain is an explicit fmaf, every other operation
+//     an explicit single operation, accumulated in dimension order;
+//   * dimensions are processed in chunks of 256, each chunk's partial starting
+//     at 0.0f and being added into the total;
+//   * sqrtf and '/' are the correctly rounded forms (built with
+//     -fhip-fp32-correctly-rounded-divide-sqrt), denormals are preserved.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "aknn_common.hpp"
+
+namespace aknn {
+
+constexpr int kWave = 64;
+
+// (key, id) lexicographic order used by every selection: ties go to the lower id.
+__device__ __forceinline__ bool key_less(float ka, int32_t ia, float kb, int32_t ib) {
+  return ka < kb || (ka == kb && ia < ib);
+}
+
+// Final key of a pair from its accumulated totals.
+template <int METRIC>
+__device__ __forceinline__ float finish_key(float tot, float aa, float bb) {
+  if constexpr (METRIC == kCosine) {
+    const float c = tot / (sqrtf(aa) * sqrtf(bb));
+    return static_cast<float>(1.0 - static_cast<double>(c));
+  } else {
+    return sqrtf(tot);
+  }
+}
+
+// One step of a chain: EUCLIDEAN fma(t, t, p), TAXICAB p + |t|, COSINE fma(a, b, p).
+template <int METRIC>
+__device__ __forceinline__ float chain_step(float a, float b, float p) {
+  if constexpr (METRIC == kEuclidean) {
+    const float t = a - b;
+    return __builtin_fmaf(t, t, p);
+  } else if constexpr (METRIC == kTaxicab) {
+    return p + __builtin_fabsf(a - b);
+  } else {
+    return __builtin_fmaf(a, b, p);
+  }
+}
+
+// Chain over dimensions [0, d) of row a (any address space) and row b
+// (global, 16-byte aligned, padded to a multiple of 4).  Returns the total.
+template <int METRIC>
+__device__ __forceinline__ float row_chain(const float* a, const float* __restrict__ b, int d) {
+  float tot = 0.0f;
+  for (int c0 = 0; c0 < d; c0 += 256) {
+    const int c1 = c0 + 256 < d ? c0 + 256 : d;
+    float p = 0.0f;
+    int dd = c0;
+    for (; dd + 4 <= c1; dd += 4) {
+      const float4 bv = *reinterpret_cast<const float4*>(b + dd);
+      p = chain_step<METRIC>(a[dd + 0], bv.x, p);
+      p = chain_step<METRIC>(a[dd + 1], bv.y, p);
+      p = chain_step<METRIC>(a[dd + 2], bv.z, p);
+      p = chain_step<METRIC>(a[dd + 3], bv.w, p);
+    }
+    for (; dd < c1; ++dd) p = chain_step<METRIC>(a[dd], b[dd], p);
+    tot = tot + p;
+  }
+  return tot;
+}
+
+// Per-point squared norm with the reference's COSINE chain (fma(a, a, p)
+// chunked by 256); the reference recomputes it for every pair, the value is
+// the same every time.
+__device__ __forceinline__ float norm_chain(const float* __restrict__ a, int d) {
+  float tot = 0.0f;
+  for (int c0 = 0; c0 < d; c0 += 256) {
+    const int c1 = c0 + 256 < d ? c0 + 256 : d;
+    float p = 0.0f;
+    for (int dd = c0; dd < c1; ++dd) p = __builtin_fmaf(a[dd], a[dd], p);
+    tot = tot + p;
+  }
+  return tot;
+}
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+
+// Inserts (key, id) into the sorted list held by lanes [0, k) of the wave
+// (lane l holds entry l) and drops the last entry.  The caller guarantees the
+// new entry is strictly below the current last entry and not yet present.
+__device__ __forceinline__ void wave_list_insert(float& key, int32_t& id, int k, float nk, int32_t nid) {
+  const int lane = lane_id();
+  const bool smaller = lane < k && key_less(key, id, nk, nid);
+  const int pos = __popcll(__ballot(smaller));
+  const float pk = __shfl_up(key, 1);
+  const int32_t pid = __shfl_up(id, 1);
+  if (lane < k && lane > pos) {
+    key = pk;
+    id = pid;
+  }
+  if (lane == pos) {
+    key = nk;
+    id = nid;
+  }
+}
+
+// Bitonic sort of the (key, id) pairs held one per lane over the whole wave
+// (ascending; lanes beyond the list must hold +inf / INT32_MAX).
+__device__ __forceinline__ void wave_sort(float& key, int32_t& id) {
+  const int lane = lane_id();
+  for (int size = 2; size <= kWave; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      const float ok = __shfl_xor(key, stride);
+      const int32_t oid = __shfl_xor(id, stride);
+      const bool up = (lane & size) == 0;
+      const bool lower = (lane & stride) == 0;
+      const bool other_less = key_less(ok, oid, key, id);
+      // lower lane keeps the min when sorting up, the max when sorting down
+      const bool take = lower == up ? other_less : !other_less;
+      if (take) {
+        key = ok;
+        id = oid;
+      }
+    }
+  }
+}
+
+}  // namespace aknn

@@ -1,0 +1,317 @@
+// Host orchestration of the k-NN graph build on one GPU.
+//
+// calculate() is the reference's timed bracket (euclideandistancematrixgpu.cu:2861-2896):
+// normalize, forest, stub + initial k-NN, propagation rounds.  Differences in
+// structure (not in results):
+//   * data stays row-major in HBM (one point = one contiguous row padded to a
+//     multiple of 4 floats), so neighbour gathers are single 16-byte streams;
+//   * the forest never leaves the device: no per-tree D2H/H2D round trip of
+//     nodes and leaves (.cu:2653-2655, :1997-2065);
+//   * the per-level host check of the reference (.cu:2602-2604) becomes one
+//     check per batch of levels; levels past the end of a tree are no-ops and
+//     the rand() stream is rewound to the first draw the tree did not use;
+//   * one stream, no per-launch synchronisation.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <stdexcept>
+
+namespace aknn {
+
+namespace {
+constexpr int kNumEvents = 8;
+}
+
+Engine::Engine(int device) : device_(device) {
+  int count = 0;
+  AKNN_HIP_TRY(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count) throw HipError("device " + std::to_string(device) + " is not present");
+  AKNN_HIP_TRY(hipSetDevice(device_));
+  AKNN_HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  for (int i = 0; i < kNumEvents; ++i) {
+    hipEvent_t e;
+    AKNN_HIP_TRY(hipEventCreate(&e));
+    events_.push_back(e);
+  }
+  AKNN_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&ctl_host_), sizeof(ForestCtl), hipHostMallocDefault));
+}
+
+Engine::~Engine() {
+  (void)hipSetDevice(device_);
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  release();
+  for (auto e : events_) (void)hipEventDestroy(e);
+  for (auto e : launch_events_) (void)hipEventDestroy(e);
+  if (ctl_host_) (void)hipHostFree(ctl_host_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+hipEvent_t Engine::ev(int i) { return events_.at(i); }
+
+hipEvent_t Engine::launch_ev(int64_t i) {
+  while (static_cast<int64_t>(launch_events_.size()) <= i) {
+    hipEvent_t e;
+    AKNN_HIP_TRY(hipEventCreate(&e));
+    launch_events_.push_back(e);
+  }
+  return launch_events_[i];
+}
+
+template <typename T>
+T* Engine::alloc(size_t count) {
+  void* p = nullptr;
+  AKNN_HIP_TRY(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+  allocs_.push_back(p);
+  return static_cast<T*>(p);
+}
+
+void Engine::release() {
+  for (void* p : allocs_) (void)hipFree(p);
+  allocs_.clear();
+  x_ = x_raw_ = norms_ = mean_ = nullptr;
+  gmax_ = nullptr;
+  gids_[0] = gids_[1] = nullptr;
+  gvals_[0] = gvals_[1] = nullptr;
+  perm_ = nullptr;
+  leaves_ = nullptr;
+  rs_ = rc_ = nullptr;
+  ncand_ = nullptr;
+  fb_ = ForestBuffers{};
+  cps_ = CosineParityScratch{};
+}
+
+void Engine::set_data(const float* x_rowmajor, int64_t n, int d, const Params& p) {
+  if (n < 1 || d < 1) throw std::invalid_argument("N and D must be positive");
+  if (p.k < 1 || p.k > 64) throw std::invalid_argument("k must be in [1, 64]");
+  if (n < static_cast<int64_t>(p.k) + 1)
+    throw std::invalid_argument("N must be at least k+1 (the reference walks past the root otherwise)");
+  if (n > 0x7fffffffLL) throw std::invalid_argument("N must fit in int32");
+  if (p.trees < 1) throw std::invalid_argument("the number of trees must be at least 1");
+  if (p.leaf < 0 || p.props < 0) throw std::invalid_argument("leaf size and propagations must be >= 0");
+  if (p.metric < kEuclidean || p.metric > kCosine) throw std::invalid_argument("unknown distance");
+  AKNN_HIP_TRY(hipSetDevice(device_));
+  AKNN_HIP_TRY(hipStreamSynchronize(stream_));
+  release();
+  p_ = p;
+  n_ = n;
+  d_ = d;
+  ld_ = (d + 3) / 4 * 4;
+  leaf_eff_ = std::max(p.leaf, 1);
+
+  x_ = alloc<float>(static_cast<size_t>(n) * ld_);
+  x_raw_ = alloc<float>(static_cast<size_t>(n) * ld_);
+  AKNN_HIP_TRY(hipMemcpy2DAsync(x_raw_, ld_ * sizeof(float), x_rowmajor, d * sizeof(float), d * sizeof(float), n,
+                                hipMemcpyHostToDevice, stream_));
+  if (ld_ != d)
+    AKNN_HIP_TRY(hipMemset2DAsync(reinterpret_cast<char*>(x_raw_) + d * sizeof(float), ld_ * sizeof(float), 0,
+                                  (ld_ - d) * sizeof(float), n, stream_));
+  if (p.metric == kCosine) norms_ = alloc<float>(n);
+  mean_ = alloc<float>(d);
+  gmax_ = alloc<uint32_t>(1);
+  const size_t nk = static_cast<size_t>(n) * p.k;
+  for (int b = 0; b < 2; ++b) {
+    gids_[b] = alloc<int32_t>(nk);
+    gvals_[b] = alloc<float>(nk);
+  }
+  perm_ = alloc<int32_t>(static_cast<size_t>(n) * p.trees);
+  leaves_ = alloc<Leaf>(static_cast<size_t>(n) * p.trees);
+  const int64_t blocks = (n + 1023) / 1024;
+  fb_.n = n;
+  fb_.max_buckets = n / (leaf_eff_ + 1) + 2;
+  for (int b = 0; b < 2; ++b) {
+    fb_.act_ids[b] = alloc<int32_t>(n);
+    fb_.act_bkt[b] = alloc<int32_t>(n);
+    fb_.bkt[b] = alloc<Bucket>(fb_.max_buckets);
+  }
+  fb_.rmask = alloc<uint64_t>(blocks * 16);
+  fb_.wpre = alloc<int32_t>(blocks * 16);
+  fb_.blk_r = alloc<int32_t>(blocks);
+  fb_.blk_roff = alloc<int32_t>(blocks);
+  fb_.ctl = alloc<ForestCtl>(1);
+  ncand_ = alloc<unsigned long long>(std::max(p.props, 1));
+  AKNN_HIP_TRY(hipStreamSynchronize(stream_));
+}
+
+void Engine::build_forest() {
+  nleaves_.assign(p_.trees, 0);
+  iters_.assign(p_.trees, 0);
+  active_sum_ = 0;
+  stats_forest = KernelStats{};
+  const int n32 = static_cast<int>(n_);
+  for (int t = 0; t < p_.trees; ++t) {
+    const size_t base = draws_.pos();
+    // root pivots (.cu:2481-2487)
+    int32_t q1 = pick_index(draws_.at(base), n32);
+    int32_t q2 = pick_index(draws_.at(base + 1), n32);
+    if (q1 == q2) q2 = (q1 + 1) % n32;
+    int32_t* perm = perm_ + static_cast<int64_t>(t) * n_;
+    Leaf* leaves = leaves_ + static_cast<int64_t>(t) * n_;
+    launch_forest_init(fb_, perm, leaves, n_, q1, q2, leaf_eff_, p_.k + 1, stream_);
+    int level = 0;
+    for (;;) {
+      for (int b = 0; b < p_.level_batch; ++b, ++level) {
+        const int32_t first = draws_.at(base + 2 + 2 * static_cast<size_t>(level));
+        const int32_t second = draws_.at(base + 3 + 2 * static_cast<size_t>(level));
+        // Leaf7(rand(), rand()) (.cu:2589): argument evaluation order is the host compiler's
+        const int32_t r1 = p_.rng_order == 0 ? second : first;
+        const int32_t r2 = p_.rng_order == 0 ? first : second;
+        const int32_t v1 = static_cast<int32_t>((static_cast<double>(r1) / 2147483647.0) * 2147483647.0);
+        const int32_t v2 = static_cast<int32_t>((static_cast<double>(r2) / 2147483647.0) * 2147483647.0);
+        launch_forest_level(fb_, x_, d_, ld_, perm, leaves, level, v1, v2, leaf_eff_, p_.k + 1, stream_);
+      }
+      AKNN_HIP_TRY(hipMemcpyAsync(ctl_host_, fb_.ctl, sizeof(ForestCtl), hipMemcpyDeviceToHost, stream_));
+      AKNN_HIP_TRY(hipStreamSynchronize(stream_));
+      if (ctl_host_->done) break;
+      if (level > n_ + 2) throw HipError("forest level loop did not terminate");
+    }
+    iters_[t] = ctl_host_->iterations;
+    nleaves_[t] = ctl_host_->nleaves;
+    active_sum_ += ctl_host_->active_sum;
+    draws_.advance(2 + 2 * static_cast<size_t>(ctl_host_->iterations));
+  }
+  // algorithmic bytes: every active point's row and its bookkeeping per level,
+  // plus the leaf array write (T * N * 8 with the leaf list)
+  stats_forest.units = static_cast<double>(active_sum_) * (4.0 * d_ + 16.0) + 8.0 * n_ * p_.trees;
+}
+
+void Engine::initial_knn() {
+  stats_leaf_knn = KernelStats{};
+  int32_t* ids = gids_[cur_];
+  float* vals = gvals_[cur_];
+  launch_stub_init(ids, vals, n_ * p_.k, stream_);
+  const bool parity_cosine = p_.metric == kCosine && !p_.cosine_fixed;
+  if (parity_cosine) {
+    if (!rs_) {
+      rs_ = alloc<int32_t>(static_cast<size_t>(n_) * p_.trees);
+      rc_ = alloc<int32_t>(static_cast<size_t>(n_) * p_.trees);
+    }
+    for (int t = 0; t < p_.trees; ++t)
+      launch_leaf_ranges(perm_ + t * n_, leaves_ + t * n_, nleaves_[t], rs_ + t * n_, rc_ + t * n_, stream_);
+    if (!cps_.bits) {
+      cps_.groups = static_cast<int>(std::min<int64_t>(n_, 256));
+      cps_.words = (n_ + 31) / 32;
+      cps_.summary_words = (cps_.words + 31) / 32;
+      cps_.max_cand = n_;
+      cps_.bits = alloc<uint32_t>(static_cast<size_t>(cps_.groups) * cps_.words);
+      cps_.summary = alloc<uint32_t>(static_cast<size_t>(cps_.groups) * cps_.summary_words);
+      cps_.cand = alloc<int32_t>(static_cast<size_t>(cps_.groups) * cps_.max_cand);
+      cps_.key = alloc<float>(static_cast<size_t>(cps_.groups) * cps_.max_cand);
+      AKNN_HIP_TRY(hipMemsetAsync(cps_.bits, 0, sizeof(uint32_t) * cps_.groups * cps_.words, stream_));
+      AKNN_HIP_TRY(hipMemsetAsync(cps_.summary, 0, sizeof(uint32_t) * cps_.groups * cps_.summary_words, stream_));
+    }
+    const int64_t dim_idx = std::min<int64_t>(static_cast<int64_t>(p_.k) * p_.k + p_.k, n_);
+    AKNN_HIP_TRY(hipEventRecord(launch_ev(0), stream_));
+    launch_cosine_parity_initial(x_, d_, ld_, norms_, n_, p_.k, p_.trees, rs_, rc_, perm_, dim_idx - p_.k, cps_, ids,
+                                 vals, stream_);
+    AKNN_HIP_TRY(hipEventRecord(launch_ev(1), stream_));
+    stats_leaf_knn.launches = 1;
+    return;
+  }
+  for (int t = 0; t < p_.trees; ++t) {
+    AKNN_HIP_TRY(hipEventRecord(launch_ev(2 * t), stream_));
+    launch_leaf_knn(p_.metric, x_, d_, ld_, norms_, perm_ + t * n_, leaves_ + t * n_, nleaves_[t], leaf_eff_, p_.k,
+                    ids, vals, stream_);
+    AKNN_HIP_TRY(hipEventRecord(launch_ev(2 * t + 1), stream_));
+  }
+  stats_leaf_knn.launches = p_.trees;
+}
+
+void Engine::propagate() {
+  stats_propagate = KernelStats{};
+  const bool flip = p_.metric == kCosine && !p_.cosine_fixed;
+  const int base = 2 * p_.trees;
+  AKNN_HIP_TRY(hipMemsetAsync(ncand_, 0, sizeof(unsigned long long) * std::max(p_.props, 1), stream_));
+  for (int r = 0; r < p_.props; ++r) {
+    AKNN_HIP_TRY(hipEventRecord(launch_ev(base + 2 * r), stream_));
+    launch_propagate(p_.metric, flip, x_, d_, ld_, norms_, n_, p_.k, gids_[cur_], gvals_[cur_], gids_[cur_ ^ 1],
+                     gvals_[cur_ ^ 1], ncand_ + r, stream_);
+    AKNN_HIP_TRY(hipEventRecord(launch_ev(base + 2 * r + 1), stream_));
+    cur_ ^= 1;  // buffer swap (.cu:1969-1975)
+  }
+  stats_propagate.launches = p_.props;
+}
+
+void Engine::calculate() {
+  AKNN_HIP_TRY(hipSetDevice(device_));
+  cur_ = 0;
+  draws_.reset(p_.seed);
+  AKNN_HIP_TRY(hipMemcpyAsync(x_, x_raw_, sizeof(float) * n_ * ld_, hipMemcpyDeviceToDevice, stream_));
+  AKNN_HIP_TRY(hipEventRecord(ev(0), stream_));
+  launch_normalize(x_, n_, d_, ld_, mean_, gmax_, stream_);
+  if (norms_) launch_norms(x_, n_, d_, ld_, norms_, stream_);
+  AKNN_HIP_TRY(hipEventRecord(ev(1), stream_));
+  build_forest();
+  AKNN_HIP_TRY(hipEventRecord(ev(2), stream_));
+  initial_knn();
+  AKNN_HIP_TRY(hipEventRecord(ev(3), stream_));
+  propagate();
+  AKNN_HIP_TRY(hipEventRecord(ev(6), stream_));
+  AKNN_HIP_TRY(hipEventSynchronize(ev(6)));
+  float a = 0, b = 0, c = 0, dd = 0, e = 0;
+  AKNN_HIP_TRY(hipEventElapsedTime(&a, ev(0), ev(1)));
+  AKNN_HIP_TRY(hipEventElapsedTime(&b, ev(1), ev(2)));
+  AKNN_HIP_TRY(hipEventElapsedTime(&c, ev(2), ev(3)));
+  AKNN_HIP_TRY(hipEventElapsedTime(&dd, ev(3), ev(6)));
+  AKNN_HIP_TRY(hipEventElapsedTime(&e, ev(0), ev(6)));
+  times_ = StageTimes{a, b, c, dd, e};
+  for (int64_t l = 0; l < stats_leaf_knn.launches; ++l) {
+    float ms = 0;
+    AKNN_HIP_TRY(hipEventElapsedTime(&ms, launch_ev(2 * l), launch_ev(2 * l + 1)));
+    stats_leaf_knn.total_ms += ms;
+  }
+  std::vector<unsigned long long> nc(std::max(p_.props, 1));
+  AKNN_HIP_TRY(hipMemcpy(nc.data(), ncand_, sizeof(unsigned long long) * nc.size(), hipMemcpyDeviceToHost));
+  for (int r = 0; r < p_.props; ++r) {
+    float ms = 0;
+    AKNN_HIP_TRY(hipEventElapsedTime(&ms, launch_ev(2 * p_.trees + 2 * r), launch_ev(2 * p_.trees + 2 * r + 1)));
+    stats_propagate.total_ms += ms;
+    const double k = p_.k;
+    // per point: own row (8k), k neighbour id rows (4k^2), x_i (4D), output
+    // row (8k); per unique new candidate its row (4D)
+    stats_propagate.units += static_cast<double>(n_) * (16.0 * k + 4.0 * k * k + 4.0 * d_) + 4.0 * d_ * nc[r];
+  }
+  stats_forest.launches = 0;
+  for (int t = 0; t < p_.trees; ++t) stats_forest.launches += iters_[t];
+  stats_forest.total_ms = b;
+  // leaf k-NN flops: 3D per pair (subtract + multiply-add), pairs = every
+  // owner against its start node minus itself
+  if (!(p_.metric == kCosine && !p_.cosine_fixed)) {
+    std::vector<Leaf> host;
+    double pairs = 0;
+    for (int t = 0; t < p_.trees; ++t) {
+      host.resize(nleaves_[t]);
+      AKNN_HIP_TRY(hipMemcpy(host.data(), leaves_ + t * n_, sizeof(Leaf) * nleaves_[t], hipMemcpyDeviceToHost));
+      for (const Leaf& l : host) pairs += static_cast<double>(l.size) * (l.rcount - 1);
+    }
+    stats_leaf_knn.units = pairs * 3.0 * d_;
+  }
+}
+
+void Engine::get_graph(int32_t* ids, float* vals) const {
+  const size_t nk = static_cast<size_t>(n_) * p_.k;
+  AKNN_HIP_TRY(hipMemcpy(ids, gids_[cur_], nk * sizeof(int32_t), hipMemcpyDeviceToHost));
+  AKNN_HIP_TRY(hipMemcpy(vals, gvals_[cur_], nk * sizeof(float), hipMemcpyDeviceToHost));
+}
+
+void Engine::get_normalized(float* x) const {
+  AKNN_HIP_TRY(hipMemcpy2D(x, d_ * sizeof(float), x_, ld_ * sizeof(float), d_ * sizeof(float), n_,
+                           hipMemcpyDeviceToHost));
+}
+
+void Engine::get_forest(int tree, int32_t* perm, int32_t* rs, int32_t* rc) {
+  if (tree < 0 || tree >= p_.trees || tree >= static_cast<int>(nleaves_.size()))
+    throw std::invalid_argument("no such tree (run calculate first)");
+  if (!rs_) {
+    rs_ = alloc<int32_t>(static_cast<size_t>(n_) * p_.trees);
+    rc_ = alloc<int32_t>(static_cast<size_t>(n_) * p_.trees);
+  }
+  launch_leaf_ranges(perm_ + tree * n_, leaves_ + tree * n_, nleaves_[tree], rs_ + tree * n_, rc_ + tree * n_,
+                     stream_);
+  AKNN_HIP_TRY(hipStreamSynchronize(stream_));
+  AKNN_HIP_TRY(hipMemcpy(perm, perm_ + tree * n_, n_ * sizeof(int32_t), hipMemcpyDeviceToHost));
+  AKNN_HIP_TRY(hipMemcpy(rs, rs_ + tree * n_, n_ * sizeof(int32_t), hipMemcpyDeviceToHost));
+  AKNN_HIP_TRY(hipMemcpy(rc, rc_ + tree * n_, n_ * sizeof(int32_t), hipMemcpyDeviceToHost));
+}
+
+}  // namespace aknn

@@ -1,0 +1,113 @@
+// The k-NN graph engine: device buffers and the calculate() bracket of
+// EuclideanDistanceMatrixGPU (euclideandistancematrixgpu.cu:2861-2896).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "aknn_common.hpp"
+#include "glibc_rand.hpp"
+#include "kernels.hpp"
+
+namespace aknn {
+
+struct Params {
+  int k = 0;
+  int trees = 0;
+  int leaf = 0;   // minSize: a bucket with more points than this is split
+  int props = 0;
+  int metric = kEuclidean;
+  uint32_t seed = 1;      // rand() stream seed (the reference never seeds: 1)
+  int rng_order = 0;      // 0: Leaf7 gets (second, first) draw of a level; 1: (first, second)
+  int cosine_fixed = 0;   // 1: carry COSINE distances instead of the 1 - key flip
+  int level_batch = 16;   // forest levels enqueued per host check
+};
+
+// Per-stage times of the last calculate(), milliseconds.
+struct StageTimes {
+  double normalize = 0, forest = 0, initial = 0, propagation = 0, total = 0;
+};
+
+// Launch-level statistics of one kernel family in the last calculate().
+struct KernelStats {
+  int64_t launches = 0;
+  double total_ms = 0;
+  double units = 0;  // algorithmic bytes or flops summed over launches
+};
+
+class Engine {
+ public:
+  explicit Engine(int device);
+  ~Engine();
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+
+  void set_data(const float* x_rowmajor, int64_t n, int d, const Params& p);
+  void calculate();
+  // seed, rng_order, cosine_fixed, level_batch of p (shape parameters are fixed by set_data)
+  void set_options(const Params& p) {
+    p_.seed = p.seed;
+    p_.rng_order = p.rng_order;
+    p_.cosine_fixed = p.cosine_fixed;
+    p_.level_batch = p.level_batch;
+  }
+
+  void get_graph(int32_t* ids, float* vals) const;
+  void get_normalized(float* x) const;
+  void get_forest(int tree, int32_t* perm, int32_t* rs, int32_t* rc);
+  int tree_iterations(int tree) const { return tree < static_cast<int>(iters_.size()) ? iters_[tree] : -1; }
+
+  const StageTimes& times() const { return times_; }
+  KernelStats stats_leaf_knn, stats_propagate, stats_forest;
+
+  int64_t n() const { return n_; }
+  int d() const { return d_; }
+  const Params& params() const { return p_; }
+  int device() const { return device_; }
+
+ private:
+  void release();
+  template <typename T>
+  T* alloc(size_t count);
+  void build_forest();
+  void initial_knn();
+  void propagate();
+
+  int device_;
+  hipStream_t stream_ = nullptr;
+  std::vector<void*> allocs_;
+  std::vector<hipEvent_t> events_;
+  hipEvent_t ev(int i);
+  hipEvent_t launch_ev(int64_t i);
+  std::vector<hipEvent_t> launch_events_;
+
+  Params p_;
+  int64_t n_ = 0;
+  int d_ = 0, ld_ = 0;
+  int leaf_eff_ = 1;
+  float* x_ = nullptr;
+  float* x_raw_ = nullptr;  // pristine copy: calculate() normalizes x_ in place, re-runs start from here
+  float* norms_ = nullptr;
+  float* mean_ = nullptr;
+  uint32_t* gmax_ = nullptr;
+  int32_t* gids_[2] = {nullptr, nullptr};
+  float* gvals_[2] = {nullptr, nullptr};
+  int cur_ = 0;
+  int32_t* perm_ = nullptr;  // [trees][n]
+  Leaf* leaves_ = nullptr;   // [trees][n]
+  std::vector<int> nleaves_, iters_;
+  ForestBuffers fb_{};
+  ForestCtl* ctl_host_ = nullptr;
+  unsigned long long* ncand_ = nullptr;
+  int32_t* rs_ = nullptr;  // [trees][n], allocated on demand
+  int32_t* rc_ = nullptr;
+  CosineParityScratch cps_{};
+  int64_t active_sum_ = 0;
+  DrawStream draws_;
+  StageTimes times_;
+};
+
+}  // namespace aknn

@@ -1,0 +1,338 @@
+// Random-projection forest, one level of one tree per three launches.
+//
+// Semantics follow buildUpTheTrees / makePartitionOfLeaf0..9
+// (euclideandistancematrixgpu.cu:2130-2686): a bucket with more than `leaf`
+// points is split by the sign of x . (x_p2 - x_p1) (an FMA chain in dimension
+// order, threshold 0), the pivot p1 is forced LEFT and p2 RIGHT, every bucket
+// at a level shares one (rand(), rand()) pair for its children's pivots, and
+// the loop stops at the first level with nothing to split.
+//
+// Layout (MI355X-first, not the reference's per-position bookkeeping): only
+// buckets that still split are kept, as a compact list of records plus a
+// compact array of their points (ids in bucket order).  Finished buckets are
+// written once, straight into the leaf array and a leaf list.  A level is
+//   side:    one lane per active point: projection chain -> one ballot bit,
+//            per-1024-point block counts and per-wave prefixes;
+//   level:   one workgroup: scan of the block counts, the RIGHT count of
+//            every bucket from popcounts, child records, pivots, compaction;
+//   scatter: one lane per active point: stable rank from popcounts, write to
+//            the next level's compact arrays or to the final leaf array.
+// The partition is stable (the reference's atomic order is one legal order;
+// this fixes the one where each side keeps the bucket's order).
+#include "device_util.hpp"
+#include "kernels.hpp"
+
+namespace aknn {
+
+namespace {
+
+constexpr int kSideBlock = 1024;             // points per scan block
+constexpr int kWordsPerBlock = kSideBlock / 64;
+
+__device__ __forceinline__ bool side_is_right(const float* __restrict__ x, int d, int ld, int32_t id, int32_t p1,
+                                              int32_t p2) {
+  const float* xc = x + static_cast<int64_t>(id) * ld;
+  const float* xa = x + static_cast<int64_t>(p1) * ld;
+  const float* xb = x + static_cast<int64_t>(p2) * ld;
+  float s = 0.0f;
+  int dd = 0;
+  for (; dd + 4 <= d; dd += 4) {
+    const float4 c = *reinterpret_cast<const float4*>(xc + dd);
+    const float4 a = *reinterpret_cast<const float4*>(xa + dd);
+    const float4 b = *reinterpret_cast<const float4*>(xb + dd);
+    s = __builtin_fmaf(c.x, b.x - a.x, s);
+    s = __builtin_fmaf(c.y, b.y - a.y, s);
+    s = __builtin_fmaf(c.z, b.z - a.z, s);
+    s = __builtin_fmaf(c.w, b.w - a.w, s);
+  }
+  for (; dd < d; ++dd) s = __builtin_fmaf(xc[dd], xb[dd] - xa[dd], s);
+  return !(s < 0.0f);  // side = s < 0 ? LEFT : RIGHT (.cu:2157-2161)
+}
+
+__global__ __launch_bounds__(256) void forest_init_kernel(ForestBuffers fb, int32_t* __restrict__ perm,
+                                                          Leaf* __restrict__ leaves, int64_t n, int32_t q1, int32_t q2,
+                                                          int leaf_eff) {
+  const bool root_splits = n > leaf_eff;
+  for (int64_t m = blockIdx.x * 256ll + threadIdx.x; m < n; m += gridDim.x * 256ll) {
+    fb.act_ids[0][m] = static_cast<int32_t>(m);
+    fb.act_bkt[0][m] = 0;
+    if (!root_splits) perm[m] = static_cast<int32_t>(m);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ForestCtl c{};
+    c.nact[0] = root_splits ? static_cast<int32_t>(n) : 0;
+    c.nbkt[0] = root_splits ? 1 : 0;
+    if (root_splits) {
+      Bucket b{};
+      b.astart = 0;
+      b.size = static_cast<int32_t>(n);
+      b.fstart = 0;
+      b.q1 = q1;
+      b.q2 = q2;
+      b.rstart = 0;
+      b.rcount = static_cast<int32_t>(n);
+      b.child[0] = b.child[1] = -1;
+      fb.bkt[0][0] = b;
+    } else {
+      leaves[0] = Leaf{0, static_cast<int32_t>(n), 0, static_cast<int32_t>(n)};
+      c.nleaves = 1;
+    }
+    *fb.ctl = c;
+  }
+}
+
+__global__ __launch_bounds__(256) void forest_side_kernel(const float* __restrict__ x, int d, int ld,
+                                                          const int32_t* __restrict__ act_ids,
+                                                          const int32_t* __restrict__ act_bkt,
+                                                          const Bucket* __restrict__ bkt, const ForestCtl* ctl,
+                                                          int parity, uint64_t* __restrict__ rmask,
+                                                          int32_t* __restrict__ wpre, int32_t* __restrict__ blk_r) {
+  __shared__ int wcnt[kWordsPerBlock];
+  const int nact = ctl->nact[parity];
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  for (int64_t blk = blockIdx.x; blk * kSideBlock < nact; blk += gridDim.x) {
+    for (int sub = 0; sub < kSideBlock / 256; ++sub) {
+      const int64_t m = blk * kSideBlock + sub * 256 + threadIdx.x;
+      bool right = false;
+      if (m < nact) {
+        const Bucket& b = bkt[act_bkt[m]];
+        const int32_t rel = static_cast<int32_t>(m) - b.astart;
+        if (rel == b.q2) {
+          right = true;  // Leaf2 writes RIGHT for p2 last (.cu:2178-2179)
+        } else if (rel != b.q1) {
+          right = side_is_right(x, d, ld, act_ids[m], act_ids[b.astart + b.q1], act_ids[b.astart + b.q2]);
+        }
+      }
+      const uint64_t bal = __ballot(right);
+      const int w = sub * 4 + wave;
+      if (lane == 0) {
+        rmask[blk * kWordsPerBlock + w] = bal;
+        wcnt[w] = __popcll(bal);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int acc = 0;
+      for (int w = 0; w < kWordsPerBlock; ++w) {
+        wpre[blk * kWordsPerBlock + w] = acc;
+        acc += wcnt[w];
+      }
+      blk_r[blk] = acc;
+    }
+    __syncthreads();
+  }
+}
+
+// Inclusive scan over a 1024-thread block; *total receives the block sum.
+__device__ __forceinline__ int block_scan(int v, int* sh, int* total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(v, o);
+    if (lane >= o) v += t;
+  }
+  if (lane == 63) sh[wave] = v;
+  __syncthreads();
+  if (wave == 0) {
+    int w = lane < nw ? sh[lane] : 0;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int t = __shfl_up(w, o);
+      if (lane >= o) w += t;
+    }
+    if (lane < nw) sh[lane] = w;
+  }
+  __syncthreads();
+  const int res = v + (wave > 0 ? sh[wave - 1] : 0);
+  *total = sh[nw - 1];
+  __syncthreads();
+  return res;
+}
+
+__device__ __forceinline__ int32_t rights_before(int64_t pos, int nact, int32_t total, const int32_t* blk_roff,
+                                                 const int32_t* wpre, const uint64_t* rmask) {
+  if (pos >= nact) return total;
+  const uint64_t w = rmask[pos >> 6];
+  const uint64_t below = (1ull << (pos & 63)) - 1ull;
+  return blk_roff[pos >> 10] + wpre[pos >> 6] + __popcll(w & below);
+}
+
+__global__ __launch_bounds__(1024) void forest_level_kernel(Bucket* __restrict__ bcur, Bucket* __restrict__ bnext,
+                                                            const int32_t* __restrict__ blk_r,
+                                                            int32_t* __restrict__ blk_roff,
+                                                            const uint64_t* __restrict__ rmask,
+                                                            const int32_t* __restrict__ wpre, ForestCtl* ctl,
+                                                            int parity, int level, int32_t v1, int32_t v2,
+                                                            int leaf_eff, int kp1, Leaf* __restrict__ leaves) {
+  __shared__ int sh[16];
+  const int nb = ctl->nbkt[parity];
+  if (nb == 0) {
+    // The reference's last loop iteration: nothing splits, the pair is drawn anyway.
+    if (threadIdx.x == 0) {
+      if (ctl->done == 0) {
+        ctl->done = 1;
+        ctl->iterations = level + 1;
+      }
+      ctl->nbkt[parity ^ 1] = 0;
+      ctl->nact[parity ^ 1] = 0;
+    }
+    return;
+  }
+  const int nact = ctl->nact[parity];
+  const int nblk = (nact + kSideBlock - 1) / kSideBlock;
+  int run = 0;
+  for (int base = 0; base < nblk; base += 1024) {
+    const int i = base + threadIdx.x;
+    const int v = i < nblk ? blk_r[i] : 0;
+    int tot;
+    const int incl = block_scan(v, sh, &tot);
+    if (i < nblk) blk_roff[i] = run + incl - v;
+    run += tot;
+  }
+  const int32_t total_r = run;
+  __syncthreads();
+
+  int run_b = 0, run_a = 0;
+  for (int base = 0; base < nb; base += 1024) {
+    const int bi = base + threadIdx.x;
+    Bucket b{};
+    int32_t sz[2] = {0, 0}, rbef = 0;
+    bool act[2] = {false, false};
+    if (bi < nb) {
+      b = bcur[bi];
+      rbef = rights_before(b.astart, nact, total_r, blk_roff, wpre, rmask);
+      const int32_t rend = rights_before(static_cast<int64_t>(b.astart) + b.size, nact, total_r, blk_roff, wpre, rmask);
+      sz[1] = rend - rbef;
+      sz[0] = b.size - sz[1];
+      act[0] = sz[0] > leaf_eff;
+      act[1] = sz[1] > leaf_eff;
+    }
+    const int nchild = static_cast<int>(act[0]) + static_cast<int>(act[1]);
+    const int asz = (act[0] ? sz[0] : 0) + (act[1] ? sz[1] : 0);
+    int tot_c, tot_a;
+    const int ic = block_scan(nchild, sh, &tot_c);
+    const int ia = block_scan(asz, sh, &tot_a);
+    if (bi < nb) {
+      int cidx = run_b + ic - nchild;
+      int cast = run_a + ia - asz;
+      b.rbefore = rbef;
+      for (int c = 0; c < 2; ++c) {
+        const int32_t size = sz[c];
+        const int32_t fst = b.fstart + (c ? sz[0] : 0);
+        const bool own_range = size >= kp1;
+        const int32_t rstart = own_range ? fst : b.rstart;
+        const int32_t rcount = own_range ? size : b.rcount;
+        b.cfs[c] = fst;
+        if (act[c]) {
+          // Leaf7 pivots (.cu:2317-2347); size >= 2 here
+          int32_t q1 = v1 % size, q2 = v2 % size;
+          if (q1 == q2) q2 = (q1 + 1) % size;
+          Bucket nbk{};
+          nbk.astart = cast;
+          nbk.size = size;
+          nbk.fstart = fst;
+          nbk.q1 = q1;
+          nbk.q2 = q2;
+          nbk.rstart = rstart;
+          nbk.rcount = rcount;
+          nbk.child[0] = nbk.child[1] = -1;
+          bnext[cidx] = nbk;
+          b.child[c] = cidx;
+          b.cast[c] = cast;
+          ++cidx;
+          cast += size;
+        } else {
+          const int slot = atomicAdd(&ctl->nleaves, 1);
+          leaves[slot] = Leaf{fst, size, rstart, rcount};
+          b.child[c] = -1;
+          b.cast[c] = 0;
+        }
+      }
+      bcur[bi] = b;
+    }
+    run_b += tot_c;
+    run_a += tot_a;
+  }
+  if (threadIdx.x == 0) {
+    ctl->nbkt[parity ^ 1] = run_b;
+    ctl->nact[parity ^ 1] = run_a;
+    ctl->active_sum += nact;
+  }
+}
+
+__global__ __launch_bounds__(256) void forest_scatter_kernel(const int32_t* __restrict__ act_ids,
+                                                             const int32_t* __restrict__ act_bkt,
+                                                             const Bucket* __restrict__ bkt,
+                                                             const uint64_t* __restrict__ rmask,
+                                                             const int32_t* __restrict__ wpre,
+                                                             const int32_t* __restrict__ blk_roff,
+                                                             const ForestCtl* ctl, int parity,
+                                                             int32_t* __restrict__ act_ids_n,
+                                                             int32_t* __restrict__ act_bkt_n, int32_t* __restrict__ perm) {
+  const int nact = ctl->nact[parity];
+  for (int64_t m = blockIdx.x * 256ll + threadIdx.x; m < nact; m += gridDim.x * 256ll) {
+    const int32_t id = act_ids[m];
+    const Bucket& b = bkt[act_bkt[m]];
+    const uint64_t w = rmask[m >> 6];
+    const int bit = static_cast<int>((w >> (m & 63)) & 1ull);
+    const int32_t rb = blk_roff[m >> 10] + wpre[m >> 6] + __popcll(w & ((1ull << (m & 63)) - 1ull));
+    const int32_t rank = bit ? rb - b.rbefore : (static_cast<int32_t>(m) - b.astart) - (rb - b.rbefore);
+    const int32_t child = b.child[bit];
+    if (child >= 0) {
+      act_ids_n[b.cast[bit] + rank] = id;
+      act_bkt_n[b.cast[bit] + rank] = child;
+    } else {
+      perm[b.cfs[bit] + rank] = id;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void leaf_ranges_kernel(const int32_t* __restrict__ perm,
+                                                          const Leaf* __restrict__ leaves, int nleaves,
+                                                          int32_t* __restrict__ rs, int32_t* __restrict__ rc) {
+  const int li = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (li >= nleaves) return;
+  const Leaf l = leaves[li];
+  for (int p = threadIdx.x & 63; p < l.size; p += 64) {
+    const int32_t pt = perm[l.start + p];
+    rs[pt] = l.rstart;
+    rc[pt] = l.rcount;
+  }
+}
+
+int clamp_grid(int64_t g, int cap) { return static_cast<int>(g < 1 ? 1 : (g > cap ? cap : g)); }
+
+}  // namespace
+
+void launch_forest_init(const ForestBuffers& fb, int32_t* perm, Leaf* leaves, int64_t n, int32_t q1, int32_t q2,
+                        int leaf_eff, int kp1, hipStream_t s) {
+  (void)kp1;
+  hipLaunchKernelGGL(forest_init_kernel, dim3(clamp_grid((n + 255) / 256, 4096)), dim3(256), 0, s, fb, perm, leaves,
+                     n, q1, q2, leaf_eff);
+  AKNN_HIP_TRY(hipGetLastError());
+}
+
+void launch_forest_level(const ForestBuffers& fb, const float* x, int d, int ld, int32_t* perm, Leaf* leaves,
+                         int level, int32_t v1, int32_t v2, int leaf_eff, int kp1, hipStream_t s) {
+  const int par = level & 1;
+  hipLaunchKernelGGL(forest_side_kernel, dim3(clamp_grid((fb.n + kSideBlock - 1) / kSideBlock, 2048)), dim3(256), 0,
+                     s, x, d, ld, fb.act_ids[par], fb.act_bkt[par], fb.bkt[par], fb.ctl, par, fb.rmask, fb.wpre,
+                     fb.blk_r);
+  AKNN_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(forest_level_kernel, dim3(1), dim3(1024), 0, s, fb.bkt[par], fb.bkt[par ^ 1], fb.blk_r,
+                     fb.blk_roff, fb.rmask, fb.wpre, fb.ctl, par, level, v1, v2, leaf_eff, kp1, leaves);
+  AKNN_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(forest_scatter_kernel, dim3(clamp_grid((fb.n + 255) / 256, 8192)), dim3(256), 0, s,
+                     fb.act_ids[par], fb.act_bkt[par], fb.bkt[par], fb.rmask, fb.wpre, fb.blk_roff, fb.ctl, par,
+                     fb.act_ids[par ^ 1], fb.act_bkt[par ^ 1], perm);
+  AKNN_HIP_TRY(hipGetLastError());
+}
+
+void launch_leaf_ranges(const int32_t* perm, const Leaf* leaves, int nleaves, int32_t* rs, int32_t* rc,
+                        hipStream_t s) {
+  if (nleaves <= 0) return;
+  hipLaunchKernelGGL(leaf_ranges_kernel, dim3((nleaves + 3) / 4), dim3(256), 0, s, perm, leaves, nleaves, rs, rc);
+  AKNN_HIP_TRY(hipGetLastError());
+}
+
+}  // namespace aknn

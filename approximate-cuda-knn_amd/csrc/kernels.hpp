@@ -1,0 +1,69 @@
+// Host-side launchers of the HIP kernels (one translation unit per stage).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "aknn_common.hpp"
+
+namespace aknn {
+
+// ---- normalization (euclideandistancematrixgpu.cu:19-85) -------------------
+// x: [n][ld] row-major, d <= ld.  mean: [d] scratch, gmax: [1] scratch.
+void launch_normalize(float* x, int64_t n, int d, int ld, float* mean, uint32_t* gmax, hipStream_t s);
+// Per-point COSINE squared norms (the chain of .cu:849-865 for one operand).
+void launch_norms(const float* x, int64_t n, int d, int ld, float* norms, hipStream_t s);
+
+// ---- forest (euclideandistancematrixgpu.cu:2130-2686) -----------------------
+struct ForestBuffers {
+  int32_t* act_ids[2];
+  int32_t* act_bkt[2];
+  Bucket* bkt[2];
+  uint64_t* rmask;
+  int32_t* wpre;
+  int32_t* blk_r;
+  int32_t* blk_roff;
+  ForestCtl* ctl;
+  int64_t n;
+  int64_t max_buckets;
+};
+void launch_forest_init(const ForestBuffers& fb, int32_t* perm, Leaf* leaves, int64_t n, int32_t q1, int32_t q2,
+                        int leaf_eff, int kp1, hipStream_t s);
+void launch_forest_level(const ForestBuffers& fb, const float* x, int d, int ld, int32_t* perm, Leaf* leaves,
+                         int level, int32_t v1, int32_t v2, int leaf_eff, int kp1, hipStream_t s);
+// Per-point start ranges from a leaf list (for export and the COSINE parity path).
+void launch_leaf_ranges(const int32_t* perm, const Leaf* leaves, int nleaves, int32_t* rs, int32_t* rc,
+                        hipStream_t s);
+
+// ---- graph -------------------------------------------------------------------
+void launch_stub_init(int32_t* ids, float* vals, int64_t count, hipStream_t s);
+
+// Initial leaf k-NN for one tree, merged into the graph rows in place
+// (EUCLIDEAN / TAXICAB / COSINE-fixed; exact top-k by (key, id)).
+void launch_leaf_knn(int metric, const float* x, int d, int ld, const float* norms, const int32_t* perm,
+                     const Leaf* leaves, int nleaves, int max_leaf, int k, int32_t* ids, float* vals,
+                     hipStream_t s);
+
+// One Jacobi propagation round, ids_in/vals_in -> ids_out/vals_out.
+// flip: the COSINE parity store rule (stored = 1 - key).  ncand: optional
+// device counter of unique new candidates (for byte accounting).
+void launch_propagate(int metric, bool flip, const float* x, int d, int ld, const float* norms, int64_t n, int k,
+                      const int32_t* ids_in, const float* vals_in, int32_t* ids_out, float* vals_out,
+                      unsigned long long* ncand, hipStream_t s);
+
+// COSINE parity initial pass: ascending-id chunks of `cap` candidates over the
+// union of all trees' start ranges, with the store-time flip after each chunk.
+struct CosineParityScratch {
+  uint32_t* bits;     // [groups][words]
+  uint32_t* summary;  // [groups][summary_words]
+  int32_t* cand;      // [groups][max_cand]
+  float* key;         // [groups][max_cand]
+  int64_t words, summary_words, max_cand;
+  int groups;
+};
+void launch_cosine_parity_initial(const float* x, int d, int ld, const float* norms, int64_t n, int k, int trees,
+                                  const int32_t* rs, const int32_t* rc, const int32_t* perm, int64_t cap,
+                                  const CosineParityScratch& sc, int32_t* ids, float* vals, hipStream_t s);
+
+}  // namespace aknn

@@ -1,0 +1,104 @@
+// Normalization kernels: restate normalizeDataStep1/2
+// (euclideandistancematrixgpu.cu:19-42, :44-85) on a row-major [n][ld] matrix.
+//
+// Step 1 keeps the reference's per-dimension sequential FP32 sum: one lane per
+// dimension walks all points in order (the rounding of a sequential sum depends
+// on the order, so this chain cannot be split).  A wave covers 64 adjacent
+// dimensions, so every row access is one coalesced segment, and the loads run
+// ahead of the dependent adds.  Centring, the max-abs reduction over the
+// dimensions below 256 and the scaling are plain parallel HBM passes.
+#include "device_util.hpp"
+#include "kernels.hpp"
+
+namespace aknn {
+
+namespace {
+
+constexpr int kMeanUnroll = 16;
+
+__global__ __launch_bounds__(64) void mean_kernel(const float* __restrict__ x, int64_t n, int d, int ld,
+                                                  float* __restrict__ mean) {
+  const int dd = blockIdx.x * 64 + threadIdx.x;
+  if (dd >= d) return;
+  const float* p = x + dd;
+  float m = 0.0f;
+  int64_t i = 0;
+  for (; i + kMeanUnroll <= n; i += kMeanUnroll) {
+    float v[kMeanUnroll];
+#pragma unroll
+    for (int u = 0; u < kMeanUnroll; ++u) v[u] = p[(i + u) * ld];
+#pragma unroll
+    for (int u = 0; u < kMeanUnroll; ++u) m = m + v[u];
+  }
+  for (; i < n; ++i) m = m + p[i * ld];
+  mean[dd] = m / static_cast<float>(n);
+}
+
+__global__ __launch_bounds__(256) void center_kernel(float* __restrict__ x, int64_t n, int d, int ld,
+                                                     const float* __restrict__ mean, uint32_t* __restrict__ gmax) {
+  __shared__ float red[256];
+  const int dm = d < 256 ? d : 256;
+  float local = 0.0f;
+  const int64_t total = n * ld;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < total; e += gridDim.x * 256ll) {
+    const int dd = static_cast<int>(e % ld);
+    if (dd < d) {
+      const float v = x[e] - mean[dd];
+      x[e] = v;
+      if (dd < dm) {
+        const float a = __builtin_fabsf(v);
+        local = a > local ? a : local;
+      }
+    }
+  }
+  red[threadIdx.x] = local;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s && red[threadIdx.x + s] > red[threadIdx.x]) red[threadIdx.x] = red[threadIdx.x + s];
+    __syncthreads();
+  }
+  // non-negative floats order like their bit patterns
+  if (threadIdx.x == 0) atomicMax(gmax, __float_as_uint(red[0]));
+}
+
+__global__ __launch_bounds__(256) void scale_kernel(float* __restrict__ x, int64_t n, int d, int ld,
+                                                    const uint32_t* __restrict__ gmax) {
+  const int dm = d < 256 ? d : 256;
+  const float g = __uint_as_float(*gmax);
+  const int64_t total = n * ld;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < total; e += gridDim.x * 256ll) {
+    const int dd = static_cast<int>(e % ld);
+    if (dd < dm) x[e] = x[e] / g;
+  }
+}
+
+__global__ __launch_bounds__(256) void norms_kernel(const float* __restrict__ x, int64_t n, int d, int ld,
+                                                    float* __restrict__ norms) {
+  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+  if (i < n) norms[i] = norm_chain(x + i * ld, d);
+}
+
+int grid_for(int64_t work, int64_t per_block, int cap) {
+  int64_t g = (work + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  return static_cast<int>(g > cap ? cap : g);
+}
+
+}  // namespace
+
+void launch_normalize(float* x, int64_t n, int d, int ld, float* mean, uint32_t* gmax, hipStream_t s) {
+  AKNN_HIP_TRY(hipMemsetAsync(gmax, 0, sizeof(uint32_t), s));
+  hipLaunchKernelGGL(mean_kernel, dim3((d + 63) / 64), dim3(64), 0, s, x, n, d, ld, mean);
+  AKNN_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(center_kernel, dim3(grid_for(n * ld, 256, 8192)), dim3(256), 0, s, x, n, d, ld, mean, gmax);
+  AKNN_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(scale_kernel, dim3(grid_for(n * ld, 256, 8192)), dim3(256), 0, s, x, n, d, ld, gmax);
+  AKNN_HIP_TRY(hipGetLastError());
+}
+
+void launch_norms(const float* x, int64_t n, int d, int ld, float* norms, hipStream_t s) {
+  hipLaunchKernelGGL(norms_kernel, dim3(grid_for(n, 256, 1 << 30)), dim3(256), 0, s, x, n, d, ld, norms);
+  AKNN_HIP_TRY(hipGetLastError());
+}
+
+}  // namespace aknn

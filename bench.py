@@ -1,0 +1,207 @@
+"""Benchmark: k-NN graph build throughput and recall on one MI355X.
+
+A step is one calculate(): normalize, RP forest, initial leaf k-NN and the
+propagation rounds over the configuration's synthetic points, which are
+uploaded to HBM before the timed region.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--config C2]
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(ROOT, "approximate-cuda-knn_amd")
+
+METRIC = "kNN-graph build s + recall@k vs FP64 exact; plain vs chosen form, 1 GPU"
+PEAK_HBM_GBS = 8000.0
+PEAK_FP32_TFLOPS = 157.3
+
+
+def _load(name, path):
+    spec = importlib.util.spec_from_file_location(name, path)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def exact_recall(xn, ids, k, metric, queries, device="cuda"):
+    """recall@k against FP64 exact k-NN (self excluded, ties at the k-th
+    distance count as hits).  Candidates come from an FP64 GEMM-form pass
+    (k + 32 best), their distances are then recomputed directly in FP64."""
+    import torch
+
+    X = torch.from_numpy(xn).to(device=device, dtype=torch.float64)
+    n = X.shape[0]
+    sq = (X * X).sum(1)
+    nrm = sq.sqrt()
+    hits = 0
+    extra = min(n - 1, k + 32)
+
+    def direct(q, cand):  # q: [b, d], cand: [b, m, d]
+        diff = cand - q[:, None, :]
+        if metric == 0:
+            return (diff * diff).sum(-1)
+        if metric == 1:
+            return diff.abs().sum(-1)
+        dots = (cand * q[:, None, :]).sum(-1)
+        return 1.0 - dots / (cand.norm(dim=-1) * q.norm(dim=-1)[:, None])
+
+    qs = torch.from_numpy(np.asarray(queries)).to(device)
+    for s in range(0, len(qs), 256):
+        qi = qs[s:s + 256]
+        Q = X[qi]
+        if metric == 0:
+            approx = sq[qi][:, None] + sq[None, :] - 2.0 * (Q @ X.T)
+        elif metric == 1:
+            approx = torch.cdist(Q, X, p=1)
+        else:
+            approx = 1.0 - (Q @ X.T) / (nrm[qi][:, None] * nrm[None, :])
+        approx[torch.arange(len(qi), device=device), qi] = float("inf")
+        cand = approx.topk(extra, dim=1, largest=False).indices
+        exact = direct(Q, X[cand])
+        exact[cand == qi[:, None]] = float("inf")
+        kth = exact.sort(dim=1).values[:, k - 1]
+        ret = torch.from_numpy(ids[queries[s:s + 256]].astype(np.int64)).to(device)
+        valid = (ret >= 0) & (ret != qi[:, None])
+        dret = direct(Q, X[ret.clamp(min=0)])
+        hits += int(((dret <= kth[:, None]) & valid).sum().item())
+    return hits / (len(queries) * k)
+
+
+def cpu_baseline(datasets, config, cfg, sample_n):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+
+    x = datasets.generate(config, n=sample_n)
+    m = oracle_lib.METRICS[cfg["metric"]]
+    t0 = time.perf_counter()
+    oracle_lib.run(x, cfg["k"], cfg["trees"], cfg["leaf"], cfg["props"], m)
+    dt = time.perf_counter() - t0
+    return {
+        "value": sample_n / dt,
+        "unit": "points/s",
+        "cores": int(oracle_lib.lib().orc_num_threads()),
+        "kind": "port",
+        "sample": (f"{config} distribution at N={sample_n} (same D, k, trees, leaf, propagations, metric): whole "
+                   f"pipeline in {dt:.1f} s on the C oracle (OpenMP over points)"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="C2")
+    ap.add_argument("--n", type=int, default=None, help="override the number of points")
+    ap.add_argument("--queries", type=int, default=10000)
+    ap.add_argument("--cpu-sample", type=int, default=50000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-recall", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              "(the multi-GPU sharding is a design only, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    if int(os.environ.get("LOCAL_RANK", "0")) != 0:
+        return
+
+    aknn = _load("aknn_binding", os.path.join(PKG_DIR, "aknn.py"))
+    datasets = _load("aknn_datasets", os.path.join(PKG_DIR, "datasets.py"))
+    cfg = dict(datasets.CONFIGS[args.config])
+    n = args.n or cfg["n"]
+    metric = aknn.METRICS[cfg["metric"]]
+    print(f"[bench] generating {args.config} n={n} d={cfg['d']}", file=sys.stderr, flush=True)
+    x = datasets.generate(args.config, n=n)
+
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except ImportError:
+        torch = None
+        sync = lambda: None  # noqa: E731
+
+    g = aknn.Graph()
+    g.initialize(x, cfg["k"], cfg["trees"], cfg["leaf"], cfg["props"], metric)
+    for w in range(args.warmup):
+        g.calculate()
+        print(f"[bench] warmup {w}: {g.stage_times()['total']:.1f} ms", file=sys.stderr, flush=True)
+    sync()
+    t0 = time.perf_counter()
+    stage = []
+    for s in range(args.steps):
+        g.calculate()
+        stage.append(g.stage_times())
+    sync()
+    wall = (time.perf_counter() - t0) / args.steps
+    for s, st in enumerate(stage):
+        print(f"[bench] step {s}: " + " ".join(f"{k}={v:.1f}" for k, v in st.items()), file=sys.stderr, flush=True)
+    ev_total = float(np.mean([st["total"] for st in stage]))
+    stats = {name: g.kernel_stats(name) for name in ("leaf_knn", "propagate", "forest")}
+
+    # dominant kernel of the step: the largest summed launch time
+    dom = max(("leaf_knn", "propagate"), key=lambda nm: stats[nm]["total_ms"])
+    s = stats[dom]
+    avg_s = s["total_ms"] / max(s["launches"], 1) / 1e3
+    per_launch = s["units"] / max(s["launches"], 1)
+    if dom == "leaf_knn":
+        achieved = per_launch / avg_s / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel": "leaf_knn_kernel",
+                "engine": "FP32 VALU (exact chains, no MFMA)"}
+    else:
+        achieved = per_launch / avg_s / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": achieved / PEAK_HBM_GBS, "traffic": None, "kernel": "propagate_kernel"}
+    prof = os.path.join(ROOT, "profiles", f"traffic_{roof['kernel']}_{args.config}.json")
+    if os.path.exists(prof) and args.n is None:
+        with open(prof) as f:
+            roof["traffic"] = json.load(f).get("hbm_bytes_per_launch")
+
+    line = {
+        "metric": METRIC,
+        "value": n / wall,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.config, "n": n, "d": cfg["d"], "k": cfg["k"], "trees": cfg["trees"],
+                   "leaf": cfg["leaf"], "propagations": cfg["props"], "distance": cfg["metric"],
+                   "form": "plain"},
+        "build_s": ev_total / 1e3,
+        "stages_ms": {k: float(np.mean([st[k] for st in stage])) for k in stage[0]},
+        "kernels": stats,
+        "roofline": roof,
+    }
+    if not args.no_recall and torch is not None and torch.cuda.is_available():
+        rng = np.random.default_rng(1234)
+        q = np.sort(rng.choice(n, size=min(args.queries, n), replace=False))
+        ids, _ = g.graph()
+        xn = g.normalized()
+        t1 = time.perf_counter()
+        line["recall_at_k"] = exact_recall(xn, ids, cfg["k"], metric, q)
+        line["recall_queries"] = int(len(q))
+        print(f"[bench] recall {line['recall_at_k']:.4f} ({time.perf_counter() - t1:.1f} s)", file=sys.stderr,
+              flush=True)
+    g.close()
+    if not args.no_cpu:
+        print(f"[bench] CPU baseline on {args.cpu_sample} points", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(datasets, args.config, cfg, min(args.cpu_sample, n))
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

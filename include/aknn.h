@@ -1,0 +1,124 @@
+/*
+ * aknn: approximate k-NN graph builder for AMD Instinct MI355X (gfx950).
+ *
+ * C ABI of libaknn.so.  It replaces the public interface of the reference's
+ * class EuclideanDistanceMatrixGPU (gosteq/Approximate-CUDA-kNN,
+ * euclideandistancematrixgpu.h:50-63) one call per method, so a binding
+ * (ctypes, cgo, JNI, a C++ wrapper) can drive the same lifecycle:
+ *
+ *   create -> set_data_file -> set_results_file -> initialize -> calculate
+ *          -> save_result -> deinitialize -> destroy
+ *
+ * Conventions: every call returns AKNN_OK (0) or a nonzero status and never
+ * throws; aknn_last_error() gives the message.  The handle owns all device
+ * memory; arrays passed in or out are owned by the caller.  One handle per
+ * host thread.  Arrays are plain host pointers, row-major.
+ */
+#ifndef AKNN_H
+#define AKNN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* euclideandistancematrixgpu.h:16-18 */
+#define AKNN_DISTANCE_EUCLIDEAN 0
+#define AKNN_DISTANCE_TAXICAB 1
+#define AKNN_DISTANCE_COSINE 2
+
+#define AKNN_OK 0
+#define AKNN_ERR_ARGUMENT 1
+#define AKNN_ERR_IO 2
+#define AKNN_ERR_DEVICE 3
+#define AKNN_ERR_STATE 4
+
+typedef struct aknn_handle aknn_handle;
+
+/* EuclideanDistanceMatrixGPU(bool debugMode), euclideandistancematrixgpu.cu:2696-2702.
+ * debug_mode != 0 makes aknn_save_result also write <results>DEBUG. */
+aknn_handle* aknn_create(int debug_mode);
+/* ~EuclideanDistanceMatrixGPU(), euclideandistancematrixgpu.cu:2704-2706 */
+void aknn_destroy(aknn_handle* h);
+
+/* setDataFile, euclideandistancematrixgpu.cu:2708-2710 */
+int aknn_set_data_file(aknn_handle* h, const char* path);
+/* setResultsFile, euclideandistancematrixgpu.cu:2898-2900 */
+int aknn_set_results_file(aknn_handle* h, const char* path);
+
+/* initialize(N, D, k, device, typeOfDistance, klaster, trees, propagations,
+ * minSize), euclideandistancematrixgpu.cu:2759-2819: selects the GPU, reads
+ * the CSV data file (two header lines, then N lines of D comma-separated
+ * values, plus a trailing integer label when klaster != 0, .cu:2712-2757) and
+ * uploads it.  Rejects inputs the reference cannot run correctly: N < k+1,
+ * trees < 1, missing or short rows. */
+int aknn_initialize(aknn_handle* h, int n, int d, int k, int device, int distance, int klaster, int trees,
+                    int propagations, int min_size);
+
+/* Same as aknn_initialize with the points given in memory instead of a file
+ * (x: n*d floats, row-major, point-major).  No reference counterpart: it is
+ * initialize() with loadData() (.cu:2712-2757) replaced by a copy. */
+int aknn_initialize_host(aknn_handle* h, const float* x, int64_t n, int d, int k, int device, int distance,
+                         int trees, int propagations, int min_size);
+
+/* calculate(), euclideandistancematrixgpu.cu:2861-2896: normalize, forest,
+ * initial k-NN, propagation rounds, all on the device.  Re-runnable: every call
+ * starts again from the uploaded points. */
+int aknn_calculate(aknn_handle* h);
+
+/* saveResultToResultFile(), euclideandistancematrixgpu.cu:2908-2978: writes
+ * "N;k;8\n", the 8-byte marker 0x01020304, then per point k {int32 id,
+ * float32 value} records sorted by value (COSINE: descending), ties by lower
+ * id; <results>VALIDATION; <results>DEBUG when created with debug_mode. */
+int aknn_save_result(aknn_handle* h);
+
+/* deinitialize(), euclideandistancematrixgpu.cu:2821-2859 */
+int aknn_deinitialize(aknn_handle* h);
+
+/* Options, set before aknn_calculate (no reference counterpart; defaults
+ * reproduce the reference):
+ *   "seed"          rand() stream seed (default 1: the reference never seeds)
+ *   "rng_order"     0: each level's Leaf7(rand(), rand()) receives (second,
+ *                   first) draw, as g++ evaluates arguments right to left; 1:
+ *                   (first, second)
+ *   "cosine_fixed"  0: COSINE stores 1 - key on every selection and re-keys
+ *                   carried entries by it (.cu:1007, :806); 1: carry distances
+ *   "level_batch"   forest levels enqueued per host check (default 16) */
+int aknn_set_option(aknn_handle* h, const char* name, int64_t value);
+
+/* The graph as the result file holds it before sorting: ids[n*k], vals[n*k]
+ * (the device-to-host copy of .cu:2911-2915; rows sorted by (value, id)). */
+int aknn_get_graph(aknn_handle* h, int32_t* ids, float* vals);
+
+/* Points after normalization (.cu:19-85), n*d floats, row-major. */
+int aknn_get_normalized(aknn_handle* h, float* x);
+
+/* Tree `tree` of the last calculate(): perm[n] is the leaf array's point order
+ * (TreeNodeLeaf::entityNumber, .cu:2391-2405); range_start[p], range_count[p]
+ * give, for point p, the positions in perm of its start node
+ * (findGraphTraversalStartPoint, .cu:87-105, and the leftmost-leaf descent,
+ * .cu:166-176).  iterations (may be NULL) receives the number of level-loop
+ * iterations (.cu:2509-2637), i.e. rand() pairs the tree drew after its root. */
+int aknn_get_forest(aknn_handle* h, int tree, int32_t* perm, int32_t* range_start, int32_t* range_count,
+                    int32_t* iterations);
+
+/* Times of the last calculate() in ms: [0] normalize, [1] forest, [2] initial
+ * k-NN, [3] propagation, [4] total (the reference's event bracket,
+ * .cu:2867-2893).  n: capacity of ms. */
+int aknn_get_stage_times(aknn_handle* h, double* ms, int n);
+
+/* Kernel statistics of the last calculate(): name is "leaf_knn",
+ * "propagate" or "forest".  launches, summed launch time (HIP events on the
+ * launch stream), and algorithmic units summed over the launches (flops for
+ * leaf_knn, bytes for propagate and forest). */
+int aknn_get_kernel_stats(aknn_handle* h, const char* name, int64_t* launches, double* total_ms, double* units);
+
+/* Message of the last failed call on h (empty if none). */
+const char* aknn_last_error(aknn_handle* h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AKNN_H */

@@ -1,0 +1,158 @@
+"""GPU path vs the CPU oracle, bit for bit, through the C ABI.
+
+Every case compares the HIP build with the oracle on identical seeded inputs
+at sizes the oracle finishes in seconds: normalized points, every tree's leaf
+order and start ranges (and the rand() iterations it consumed), and the graph
+ids and FP32 values.  Integer outputs and float values are compared exactly.
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import PKG_DIR, TESTS_DIR
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+def gpu_run(aknn, x, k, trees, leaf, props, metric, **opts):
+    print(f"[gpu] n={len(x)} d={x.shape[1]} k={k} T={trees} L={leaf} P={props} metric={metric} {opts}", flush=True)
+    g = aknn.Graph()
+    g.initialize(x, k, trees, leaf, props, metric, **opts)
+    g.calculate()
+    return g
+
+
+def check_forest(oracle, g, x_norm, trees, leaf, k, seed=1, order=0):
+    perm, rs, rc, info = oracle.build_forest(x_norm, trees, leaf, k, seed=seed, rng_order=order)
+    for t in range(trees):
+        gp, grs, grc, it = g.forest(t)
+        assert (gp == perm[t]).all(), f"tree {t} leaf order"
+        assert (grs == rs[t]).all() and (grc == rc[t]).all(), f"tree {t} start ranges"
+        assert it == info[t][0], f"tree {t} iterations"
+
+
+def check_graph(oracle, g, x_raw, k, trees, leaf, props, metric, seed=1, order=0, cosine_fixed=0):
+    xn, ids, vals, _ = oracle.run(x_raw, k, trees, leaf, props, metric, seed=seed, rng_order=order,
+                                  cosine_fixed=cosine_fixed)
+    gx = g.normalized()
+    assert (bits(gx) == bits(xn)).all(), "normalized points"
+    gi, gv = g.graph()
+    bad = np.nonzero((gi != ids).any(1) | (bits(gv) != bits(vals)).any(1))[0]
+    assert len(bad) == 0, f"{len(bad)} rows differ, first {bad[:5]}: gpu {gi[bad[0]]} {gv[bad[0]]} oracle {ids[bad[0]]} {vals[bad[0]]}"
+    return xn
+
+
+def test_readme_example_cli(tmp_path):
+    """The reference's README example through the drop-in CLI: output bytes match the exact-arithmetic KAT."""
+    kat = json.load(open(os.path.join(TESTS_DIR, "golden", "readme_kat.json")))
+    inp = tmp_path / "input.csv"
+    inp.write_text(kat["csv"])
+    for metric in ("EUCLIDEAN", "TAXICAB", "COSINE"):
+        out = tmp_path / f"out_{metric}"
+        props = "0" if metric == "COSINE" else "10"
+        print(f"[gpu] cli {metric}", flush=True)
+        r = subprocess.run([os.path.join(PKG_DIR, "aknn"), str(inp), str(out), "2", "10", "10", props, metric, "0",
+                            "F", "--debug"], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        assert "has done task in:" in r.stdout
+        assert out.read_bytes().hex() == kat["graphs"][metric]["result_file_hex"]
+        assert (tmp_path / f"out_{metric}VALIDATION").read_text() == "Everything is OK."
+        assert (tmp_path / f"out_{metric}DEBUG").read_text().startswith("NEAR: <0, ")
+
+
+def test_cli_exit_codes_with_device(tmp_path):
+    exe = os.path.join(PKG_DIR, "aknn")
+    r = subprocess.run([exe, str(tmp_path / "missing.csv"), "o", "2", "1", "1", "1", "EUCLIDEAN", "0", "F"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 7 and "Error opening the file" in r.stdout
+    bad = tmp_path / "short.csv"
+    bad.write_text("4\n2\n1, 2\n3, 4\n")
+    r = subprocess.run([exe, str(bad), "o", "2", "1", "1", "1", "EUCLIDEAN", "0", "F"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 8 and "Initialization failed." in r.stdout
+
+
+def test_c1_end_to_end_bit_exact(aknn, oracle, datasets):
+    """Config 1: N=20,000, D=32, 10 clusters, k=10, T=10, L=32, P=3, EUCLIDEAN."""
+    x = datasets.gmm_c1()[0]
+    g = gpu_run(aknn, x, 10, 10, 32, 3, 0)
+    xn = check_graph(oracle, g, x, 10, 10, 32, 3, 0)
+    check_forest(oracle, g, xn, 10, 32, 10)
+    st = g.stage_times()
+    assert st["total"] > 0
+
+
+@pytest.mark.parametrize("metric", [0, 1])
+@pytest.mark.parametrize("props", [0, 2])
+def test_small_bit_exact(aknn, oracle, datasets, metric, props):
+    x = datasets.gmm_c1(3000, 24, clusters=6, seed=3)[0]
+    g = gpu_run(aknn, x, 8, 4, 20, props, metric)
+    check_graph(oracle, g, x, 8, 4, 20, props, metric)
+
+
+@pytest.mark.parametrize("cosine_fixed", [0, 1])
+@pytest.mark.parametrize("props", [0, 2])
+def test_cosine_bit_exact(aknn, oracle, datasets, cosine_fixed, props):
+    x = datasets.gmm_c1(2500, 20, clusters=5, seed=4)[0]
+    g = gpu_run(aknn, x, 6, 3, 24, props, 2, cosine_fixed=cosine_fixed)
+    check_graph(oracle, g, x, 6, 3, 24, props, 2, cosine_fixed=cosine_fixed)
+
+
+@pytest.mark.parametrize("metric", [0, 1, 2])
+def test_wide_rows_bit_exact(aknn, oracle, datasets, metric):
+    """D=300 > 256: the normalization quirk and 256-dimension chunked sums."""
+    x = datasets.sparse_c3(1200, 300, clusters=8, seed=6)
+    g = gpu_run(aknn, x, 5, 2, 40, 1, metric)
+    check_graph(oracle, g, x, 5, 2, 40, 1, metric)
+
+
+@pytest.mark.parametrize("leaf,k", [(0, 3), (1, 4), (7, 12), (5000, 9)])
+def test_forest_edge_leaf_sizes(aknn, oracle, datasets, leaf, k):
+    x = datasets.uniform_c2(1500, 12, seed=leaf + 1)
+    g = gpu_run(aknn, x, k, 3, leaf, 1, 0)
+    xn = check_graph(oracle, g, x, k, 3, leaf, 1, 0)
+    check_forest(oracle, g, xn, 3, leaf, k)
+
+
+@pytest.mark.parametrize("seed,order", [(2, 0), (1, 1), (987654321, 0)])
+def test_rng_options(aknn, oracle, datasets, seed, order):
+    x = datasets.gmm_c1(2000, 16, clusters=4, seed=9)[0]
+    g = gpu_run(aknn, x, 6, 3, 16, 1, 0, seed=seed, rng_order=order)
+    xn = check_graph(oracle, g, x, 6, 3, 16, 1, 0, seed=seed, order=order)
+    check_forest(oracle, g, xn, 3, 16, 6, seed=seed, order=order)
+
+
+def test_level_batch_does_not_change_results(aknn, datasets):
+    x = datasets.gmm_c1(3000, 16, clusters=5, seed=2)[0]
+    a = gpu_run(aknn, x, 8, 3, 8, 1, 0, level_batch=1).graph()
+    b = gpu_run(aknn, x, 8, 3, 8, 1, 0, level_batch=64).graph()
+    assert (a[0] == b[0]).all() and (bits(a[1]) == bits(b[1])).all()
+
+
+def test_recalculate_is_identical(aknn, datasets):
+    x = datasets.gmm_c1(4000, 16, clusters=5, seed=8)[0]
+    g = gpu_run(aknn, x, 10, 4, 16, 2, 0)
+    a = g.graph()
+    g.calculate()
+    b = g.graph()
+    assert (a[0] == b[0]).all() and (bits(a[1]) == bits(b[1])).all()
+
+
+def test_k_equals_n_minus_one(aknn, oracle, datasets):
+    x = datasets.uniform_c2(40, 5, seed=3)
+    g = gpu_run(aknn, x, 39, 2, 4, 1, 0)
+    check_graph(oracle, g, x, 39, 2, 4, 1, 0)
+
+
+def test_rejects_k_at_least_n(aknn, datasets):
+    x = datasets.uniform_c2(10, 3)
+    g = aknn.Graph()
+    with pytest.raises(aknn.AknnError, match="k\\+1"):
+        g.initialize(x, 10, 2, 4, 1, 0)
