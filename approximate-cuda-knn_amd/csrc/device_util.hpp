@@ -89,6 +89,15 @@ __device__ __forceinline__ float norm_chain(const float* __restrict__ a, int d) 
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2:
+// block b runs on XCD b % 8.  This maps XCD x to the contiguous logical range
+// [x * grid/8, (x+1) * grid/8), so neighbouring work (the owner chunks of one
+// leaf, spatially close points) shares one L2.  grid must be a multiple of 8.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t grid) {
+  return (b & 7) * (grid >> 3) + (b >> 3);
+}
+inline int64_t xcd_grid(int64_t blocks) { return (blocks + 7) / 8 * 8; }
+
 // Inserts (key, id) into the sorted list held by lanes [0, k) of the wave
 // (lane l holds entry l) and drops the last entry.  The caller guarantees the
 // new entry is strictly below the current last entry and not yet present.

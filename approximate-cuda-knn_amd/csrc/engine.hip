@@ -224,7 +224,7 @@ void Engine::propagate() {
   AKNN_HIP_TRY(hipMemsetAsync(ncand_, 0, sizeof(unsigned long long) * std::max(p_.props, 1), stream_));
   for (int r = 0; r < p_.props; ++r) {
     AKNN_HIP_TRY(hipEventRecord(launch_ev(base + 2 * r), stream_));
-    launch_propagate(p_.metric, flip, x_, d_, ld_, norms_, n_, p_.k, gids_[cur_], gvals_[cur_], gids_[cur_ ^ 1],
+    launch_propagate(p_.metric, flip, x_, d_, ld_, norms_, perm_, n_, p_.k, gids_[cur_], gvals_[cur_], gids_[cur_ ^ 1],
                      gvals_[cur_ ^ 1], ncand_ + r, stream_);
     AKNN_HIP_TRY(hipEventRecord(launch_ev(base + 2 * r + 1), stream_));
     cur_ ^= 1;  // buffer swap (.cu:1969-1975)

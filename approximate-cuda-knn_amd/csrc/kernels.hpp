@@ -46,9 +46,11 @@ void launch_leaf_knn(int metric, const float* x, int d, int ld, const float* nor
                      hipStream_t s);
 
 // One Jacobi propagation round, ids_in/vals_in -> ids_out/vals_out.
-// flip: the COSINE parity store rule (stored = 1 - key).  ncand: optional
-// device counter of unique new candidates (for byte accounting).
-void launch_propagate(int metric, bool flip, const float* x, int d, int ld, const float* norms, int64_t n, int k,
+// flip: the COSINE parity store rule (stored = 1 - key).  order: processing
+// order of the points (nullable: 0..n-1); results do not depend on it.
+// ncand: optional device counter of unique new candidates (byte accounting).
+void launch_propagate(int metric, bool flip, const float* x, int d, int ld, const float* norms,
+                      const int32_t* order, int64_t n, int k,
                       const int32_t* ids_in, const float* vals_in, int32_t* ids_out, float* vals_out,
                       unsigned long long* ncand, hipStream_t s);
 

@@ -53,29 +53,37 @@ __device__ __forceinline__ void list_insert(float (&lk)[KMAX], int32_t (&li)[KMA
   }
 }
 
+constexpr int kStageFloats = 2 * kDc * (kTile + kPad);
+static_assert(kStageFloats >= kTile * (kTile + 1), "the distance block reuses the staging tiles");
+
 template <int METRIC, int KMAX, bool BIGD>
 __global__ __launch_bounds__(64) void leaf_knn_kernel(const float* __restrict__ x, int d, int ld,
                                                       const float* __restrict__ norms, const int32_t* __restrict__ perm,
-                                                      const Leaf* __restrict__ leaves, int k,
-                                                      int32_t* __restrict__ gids, float* __restrict__ gvals) {
-  __shared__ __attribute__((aligned(16))) float As[kDc][kTile + kPad];
-  __shared__ __attribute__((aligned(16))) float Bs[kDc][kTile + kPad];
-  __shared__ float Dt[kTile][kTile + 1];
-  __shared__ int32_t oid[kTile];
+                                                      const Leaf* __restrict__ leaves, int nleaves, int chunks_per_leaf,
+                                                      int k, int32_t* __restrict__ gids, float* __restrict__ gvals) {
+  // The transposed staging tiles (dimension-major) and the 64x64 distance
+  // block are never live at the same time and share one LDS buffer.
+  __shared__ __attribute__((aligned(16))) float smem[kStageFloats];
+  float(*As)[kTile + kPad] = reinterpret_cast<float(*)[kTile + kPad]>(smem);
+  float(*Bs)[kTile + kPad] = reinterpret_cast<float(*)[kTile + kPad]>(smem + kDc * (kTile + kPad));
+  float(*Dt)[kTile + 1] = reinterpret_cast<float(*)[kTile + 1]>(smem);
   __shared__ int32_t cid[kTile];
   __shared__ float onorm[kTile];
   __shared__ float cnorm[kTile];
 
-  const Leaf L = leaves[blockIdx.x];
-  const int o0 = blockIdx.y * kTile;
+  const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t leaf_idx = lb / chunks_per_leaf;
+  if (leaf_idx >= nleaves) return;
+  const Leaf L = leaves[leaf_idx];
+  const int o0 = static_cast<int>(lb % chunks_per_leaf) * kTile;
   if (o0 >= L.size) return;
   const int lane = threadIdx.x;
   const int nown = L.size - o0 < kTile ? L.size - o0 : kTile;
   const bool has_owner = lane < nown;
   const int32_t my_id = has_owner ? perm[L.start + o0 + lane] : -1;
-  oid[lane] = my_id;
   if constexpr (METRIC == kCosine) onorm[lane] = has_owner ? norms[my_id] : 1.0f;
 
+  // this owner's running top-k, seeded with its graph row (earlier trees)
   float lk[KMAX];
   int32_t li[KMAX];
 #pragma unroll
@@ -94,6 +102,7 @@ __global__ __launch_bounds__(64) void leaf_knn_kernel(const float* __restrict__ 
     }
 
   const int r = lane >> 3, c = lane & 7;
+  const float* orow = x + static_cast<int64_t>(my_id < 0 ? 0 : my_id) * ld;
   for (int cb = 0; cb < L.rcount; cb += kTile) {
     const int ncand = L.rcount - cb < kTile ? L.rcount - cb : kTile;
     __syncthreads();
@@ -114,7 +123,6 @@ __global__ __launch_bounds__(64) void leaf_knn_kernel(const float* __restrict__ 
         for (int j = 0; j < 8; ++j) tot[i][j] = 0.0f;
     }
 
-    const float* orow = x + static_cast<int64_t>(my_id < 0 ? 0 : my_id) * ld;
     const float* crow = x + static_cast<int64_t>(my_c < 0 ? 0 : my_c) * ld;
     for (int d0 = 0; d0 < d; d0 += kDc) {
       if constexpr (BIGD) {
@@ -160,6 +168,7 @@ __global__ __launch_bounds__(64) void leaf_knn_kernel(const float* __restrict__ 
           for (int j = 0; j < 8; ++j) acc[i][j] = chain_step<METRIC>(a[i], b[j], acc[i][j]);
       }
     }
+    __syncthreads();
 #pragma unroll
     for (int i = 0; i < 8; ++i)
 #pragma unroll
@@ -174,9 +183,17 @@ __global__ __launch_bounds__(64) void leaf_knn_kernel(const float* __restrict__ 
       }
     __syncthreads();
     if (has_owner) {
+      // each lane first marks the candidates below its threshold, then
+      // inserts only those: lanes do not wait on each other's insertions
+      uint64_t surv = 0;
       for (int j = 0; j < ncand; ++j) {
         const int32_t cj = cid[j];
-        if (cj == my_id) continue;
+        surv |= static_cast<uint64_t>(cj != my_id && key_less(Dt[lane][j], cj, thk, thi)) << j;
+      }
+      while (surv) {
+        const int j = __ffsll(static_cast<long long>(surv)) - 1;
+        surv &= surv - 1;
+        const int32_t cj = cid[j];
         const float key = Dt[lane][j];
         if (!key_less(key, cj, thk, thi)) continue;
         bool dup = false;
@@ -206,9 +223,10 @@ __global__ __launch_bounds__(64) void leaf_knn_kernel(const float* __restrict__ 
 template <int METRIC, int KMAX, bool BIGD>
 void launch_t(const float* x, int d, int ld, const float* norms, const int32_t* perm, const Leaf* leaves, int nleaves,
               int max_leaf, int k, int32_t* ids, float* vals, hipStream_t s) {
-  const dim3 grid(nleaves, (max_leaf + kTile - 1) / kTile);
-  hipLaunchKernelGGL((leaf_knn_kernel<METRIC, KMAX, BIGD>), grid, dim3(kTile), 0, s, x, d, ld, norms, perm, leaves, k,
-                     ids, vals);
+  const int chunks = (max_leaf + kTile - 1) / kTile;
+  const int64_t grid = xcd_grid(static_cast<int64_t>(nleaves) * chunks);
+  hipLaunchKernelGGL((leaf_knn_kernel<METRIC, KMAX, BIGD>), dim3(static_cast<unsigned>(grid)), dim3(kTile), 0, s, x,
+                     d, ld, norms, perm, leaves, nleaves, chunks, k, ids, vals);
   AKNN_HIP_TRY(hipGetLastError());
 }
 

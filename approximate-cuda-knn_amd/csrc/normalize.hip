@@ -3,10 +3,10 @@
 //
 // Step 1 keeps the reference's per-dimension sequential FP32 sum: one lane per
 // dimension walks all points in order (the rounding of a sequential sum depends
-// on the order, so this chain cannot be split).  A wave covers 64 adjacent
-// dimensions, so every row access is one coalesced segment, and the loads run
-// ahead of the dependent adds.  Centring, the max-abs reduction over the
-// dimensions below 256 and the scaling are plain parallel HBM passes.
+// on the order, so this chain cannot be split).  The chain's speed is set by
+// how many rows are in flight, so fifteen waves stage rows through LDS for the
+// one wave that adds.  Centring, the max-abs reduction over the dimensions
+// below 256 and the scaling are plain parallel HBM passes.
 #include "device_util.hpp"
 #include "kernels.hpp"
 
@@ -14,24 +14,50 @@ namespace aknn {
 
 namespace {
 
-constexpr int kMeanUnroll = 16;
+constexpr int kMeanRows = 256;  // rows per LDS tile (64 dimensions wide)
 
-__global__ __launch_bounds__(64) void mean_kernel(const float* __restrict__ x, int64_t n, int d, int ld,
-                                                  float* __restrict__ mean) {
-  const int dd = blockIdx.x * 64 + threadIdx.x;
-  if (dd >= d) return;
-  const float* p = x + dd;
+// One workgroup per 64 dimensions.  Wave 0 holds the 64 sequential sums (one
+// lane per dimension) and walks the rows of a tile in order; waves 1..15
+// stream the next tile into the other half of an LDS ring meanwhile, so a
+// whole tile of rows is in flight per memory latency instead of the handful
+// one lane could keep outstanding.
+__global__ __launch_bounds__(1024) void mean_kernel(const float* __restrict__ x, int64_t n, int d, int ld,
+                                                    float* __restrict__ mean) {
+  __shared__ float ring[2][kMeanRows][64];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int dd = blockIdx.x * 64 + lane;
+  const int64_t ntiles = (n + kMeanRows - 1) / kMeanRows;
+  constexpr int kLoaders = 15;
+  constexpr int kPer = (kMeanRows + kLoaders - 1) / kLoaders;
+  auto load_tile = [&](int buf, int64_t t) {
+    float v[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int r = wave - 1 + kLoaders * j;
+      const int64_t row = t * kMeanRows + r;
+      v[j] = (r < kMeanRows && row < n && dd < d) ? x[row * ld + dd] : 0.0f;
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int r = wave - 1 + kLoaders * j;
+      if (r < kMeanRows) ring[buf][r][lane] = v[j];
+    }
+  };
+  if (wave > 0) load_tile(0, 0);
+  __syncthreads();
   float m = 0.0f;
-  int64_t i = 0;
-  for (; i + kMeanUnroll <= n; i += kMeanUnroll) {
-    float v[kMeanUnroll];
-#pragma unroll
-    for (int u = 0; u < kMeanUnroll; ++u) v[u] = p[(i + u) * ld];
-#pragma unroll
-    for (int u = 0; u < kMeanUnroll; ++u) m = m + v[u];
+  for (int64_t t = 0; t < ntiles; ++t) {
+    if (wave > 0) {
+      if (t + 1 < ntiles) load_tile(static_cast<int>((t + 1) & 1), t + 1);
+    } else {
+      const int64_t left = n - t * kMeanRows;
+      const int rows = left < kMeanRows ? static_cast<int>(left) : kMeanRows;
+      const float* col = &ring[t & 1][0][lane];
+      for (int r = 0; r < rows; ++r) m = m + col[r * 64];
+    }
+    __syncthreads();
   }
-  for (; i < n; ++i) m = m + p[i * ld];
-  mean[dd] = m / static_cast<float>(n);
+  if (wave == 0 && dd < d) mean[dd] = m / static_cast<float>(n);
 }
 
 __global__ __launch_bounds__(256) void center_kernel(float* __restrict__ x, int64_t n, int d, int ld,
@@ -88,7 +114,7 @@ int grid_for(int64_t work, int64_t per_block, int cap) {
 
 void launch_normalize(float* x, int64_t n, int d, int ld, float* mean, uint32_t* gmax, hipStream_t s) {
   AKNN_HIP_TRY(hipMemsetAsync(gmax, 0, sizeof(uint32_t), s));
-  hipLaunchKernelGGL(mean_kernel, dim3((d + 63) / 64), dim3(64), 0, s, x, n, d, ld, mean);
+  hipLaunchKernelGGL(mean_kernel, dim3((d + 63) / 64), dim3(1024), 0, s, x, n, d, ld, mean);
   AKNN_HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(center_kernel, dim3(grid_for(n * ld, 256, 8192)), dim3(256), 0, s, x, n, d, ld, mean, gmax);
   AKNN_HIP_TRY(hipGetLastError());

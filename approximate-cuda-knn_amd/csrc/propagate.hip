@@ -39,8 +39,9 @@ __device__ __forceinline__ bool hash_insert(int32_t* hs, uint32_t mask, int shif
 
 template <int METRIC, bool FLIP>
 __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict__ x, int d, int ld,
-                                                        const float* __restrict__ norms, int64_t n, int k, int hs_log2,
-                                                        int cand_cap, int xi_stride,
+                                                        const float* __restrict__ norms,
+                                                        const int32_t* __restrict__ order, int64_t n, int k,
+                                                        int hs_log2, int cand_cap, int xi_stride,
                                                         const int32_t* __restrict__ ids_in,
                                                         const float* __restrict__ vals_in, int32_t* __restrict__ ids_out,
                                                         float* __restrict__ vals_out,
@@ -54,8 +55,12 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
   int32_t* cand = hs + hs_size;
   int32_t* cnt = cand + cand_cap;
 
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + wave;
-  if (i >= n) return;
+  // Points are taken in `order` (the first tree's leaf order): waves running at
+  // the same time then own nearby points whose candidate rows overlap, and
+  // each XCD walks a contiguous part of that order, so its L2 sees the reuse.
+  const int64_t pidx = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+  if (pidx >= n) return;
+  const int64_t i = order != nullptr ? order[pidx] : pidx;
 
   float key = __builtin_inff();
   int32_t id = INT_MAX;
@@ -129,7 +134,8 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
 }
 
 template <int METRIC, bool FLIP>
-void launch_t(const float* x, int d, int ld, const float* norms, int64_t n, int k, const int32_t* ids_in,
+void launch_t(const float* x, int d, int ld, const float* norms, const int32_t* order, int64_t n, int k,
+              const int32_t* ids_in,
               const float* vals_in, int32_t* ids_out, float* vals_out, unsigned long long* ncand, hipStream_t s) {
   int hs_log2 = 7;
   while ((1 << hs_log2) < 2 * (k * k + k + 1)) ++hs_log2;
@@ -138,30 +144,30 @@ void launch_t(const float* x, int d, int ld, const float* norms, int64_t n, int 
   const size_t per_wave = (static_cast<size_t>(1 << hs_log2) + cand_cap + xi_stride + 4) * 4;
   const size_t shmem = per_wave * kWavesPerBlock;
   if (shmem > 160 * 1024) throw HipError("propagation: k or d too large for LDS");
-  const int64_t blocks = (n + kWavesPerBlock - 1) / kWavesPerBlock;
+  const int64_t blocks = xcd_grid((n + kWavesPerBlock - 1) / kWavesPerBlock);
   hipLaunchKernelGGL((propagate_kernel<METRIC, FLIP>), dim3(static_cast<unsigned>(blocks)), dim3(256), shmem, s, x, d,
-                     ld, norms, n, k, hs_log2, cand_cap, xi_stride, ids_in, vals_in, ids_out, vals_out, ncand);
+                     ld, norms, order, n, k, hs_log2, cand_cap, xi_stride, ids_in, vals_in, ids_out, vals_out, ncand);
   AKNN_HIP_TRY(hipGetLastError());
 }
 
 }  // namespace
 
-void launch_propagate(int metric, bool flip, const float* x, int d, int ld, const float* norms, int64_t n, int k,
-                      const int32_t* ids_in, const float* vals_in, int32_t* ids_out, float* vals_out,
+void launch_propagate(int metric, bool flip, const float* x, int d, int ld, const float* norms,
+                      const int32_t* order, int64_t n, int k, const int32_t* ids_in, const float* vals_in, int32_t* ids_out, float* vals_out,
                       unsigned long long* ncand, hipStream_t s) {
   if (k > 64) throw HipError("propagation supports k <= 64");
   switch (metric) {
     case kEuclidean:
-      launch_t<kEuclidean, false>(x, d, ld, norms, n, k, ids_in, vals_in, ids_out, vals_out, ncand, s);
+      launch_t<kEuclidean, false>(x, d, ld, norms, order, n, k, ids_in, vals_in, ids_out, vals_out, ncand, s);
       break;
     case kTaxicab:
-      launch_t<kTaxicab, false>(x, d, ld, norms, n, k, ids_in, vals_in, ids_out, vals_out, ncand, s);
+      launch_t<kTaxicab, false>(x, d, ld, norms, order, n, k, ids_in, vals_in, ids_out, vals_out, ncand, s);
       break;
     case kCosine:
       if (flip)
-        launch_t<kCosine, true>(x, d, ld, norms, n, k, ids_in, vals_in, ids_out, vals_out, ncand, s);
+        launch_t<kCosine, true>(x, d, ld, norms, order, n, k, ids_in, vals_in, ids_out, vals_out, ncand, s);
       else
-        launch_t<kCosine, false>(x, d, ld, norms, n, k, ids_in, vals_in, ids_out, vals_out, ncand, s);
+        launch_t<kCosine, false>(x, d, ld, norms, order, n, k, ids_in, vals_in, ids_out, vals_out, ncand, s);
       break;
     default:
       throw HipError("unknown distance");
