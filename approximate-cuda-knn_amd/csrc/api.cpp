@@ -171,6 +171,8 @@ int aknn_set_option(aknn_handle* h, const char* name, int64_t value) {
     } else if (n == "level_batch") {
       if (value < 1 || value > 4096) return fail(h, AKNN_ERR_ARGUMENT, "level_batch must be in [1, 4096]");
       h->params.level_batch = static_cast<int>(value);
+    } else if (n == "skip_settled") {
+      h->params.skip_settled = value != 0;
     } else {
       return fail(h, AKNN_ERR_ARGUMENT, "unknown option " + n);
     }

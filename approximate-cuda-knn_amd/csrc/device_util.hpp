@@ -60,6 +60,8 @@ __device__ __forceinline__ float row_chain(const float* a, const float* __restri
     const int c1 = c0 + 256 < d ? c0 + 256 : d;
     float p = 0.0f;
     int dd = c0;
+    // eight 16-byte loads in flight per lane before their FMAs
+#pragma unroll 8
     for (; dd + 4 <= c1; dd += 4) {
       const float4 bv = *reinterpret_cast<const float4*>(b + dd);
       p = chain_step<METRIC>(a[dd + 0], bv.x, p);
@@ -71,6 +73,39 @@ __device__ __forceinline__ float row_chain(const float* a, const float* __restri
     tot = tot + p;
   }
   return tot;
+}
+
+// Same chain for exactly d = 4*NV <= 256 dimensions (one chunk), with the
+// whole row of b loaded into registers before the first step: one memory
+// latency per row instead of one per batch of loads.  Kernels are
+// instantiated per row width (no in-kernel switch over widths), so each
+// instantiation has a single straight-line load path.
+template <int METRIC, int NV>
+__device__ __forceinline__ float row_chain_regs(const float* a, const float* __restrict__ b) {
+  float4 v[NV];
+#pragma unroll
+  for (int j = 0; j < NV; ++j) v[j] = *reinterpret_cast<const float4*>(b + 4 * j);
+  float p = 0.0f;
+#pragma unroll
+  for (int j = 0; j < NV; ++j) {
+    p = chain_step<METRIC>(a[4 * j + 0], v[j].x, p);
+    p = chain_step<METRIC>(a[4 * j + 1], v[j].y, p);
+    p = chain_step<METRIC>(a[4 * j + 2], v[j].z, p);
+    p = chain_step<METRIC>(a[4 * j + 3], v[j].w, p);
+  }
+  return 0.0f + p;
+}
+
+// Row-width class of a kernel instantiation: rows of exactly 4*NV floats in
+// registers (NV = 8, 16, 24, 32), or 0 for any width, streamed.
+inline int row_width_class(int d) {
+  switch (d) {
+    case 32: return 8;
+    case 64: return 16;
+    case 96: return 24;
+    case 128: return 32;
+    default: return 0;
+  }
 }
 
 // Per-point squared norm with the reference's COSINE chain (fma(a, a, p)

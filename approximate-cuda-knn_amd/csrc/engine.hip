@@ -77,6 +77,7 @@ void Engine::release() {
   leaves_ = nullptr;
   rs_ = rc_ = nullptr;
   ncand_ = nullptr;
+  changed_[0] = changed_[1] = nullptr;
   fb_ = ForestBuffers{};
   cps_ = CosineParityScratch{};
 }
@@ -130,6 +131,8 @@ void Engine::set_data(const float* x_rowmajor, int64_t n, int d, const Params& p
   fb_.blk_roff = alloc<int32_t>(blocks);
   fb_.ctl = alloc<ForestCtl>(1);
   ncand_ = alloc<unsigned long long>(std::max(p.props, 1));
+  changed_[0] = alloc<uint8_t>(n);
+  changed_[1] = alloc<uint8_t>(n);
   AKNN_HIP_TRY(hipStreamSynchronize(stream_));
 }
 
@@ -223,9 +226,15 @@ void Engine::propagate() {
   const int base = 2 * p_.trees;
   AKNN_HIP_TRY(hipMemsetAsync(ncand_, 0, sizeof(unsigned long long) * std::max(p_.props, 1), stream_));
   for (int r = 0; r < p_.props; ++r) {
+    PropagateHistory hist;
+    if (p_.skip_settled) {
+      hist.ids_prev = r > 0 ? gids_[cur_ ^ 1] : nullptr;
+      hist.changed_prev = changed_[(r + 1) & 1];
+      hist.changed_out = changed_[r & 1];
+    }
     AKNN_HIP_TRY(hipEventRecord(launch_ev(base + 2 * r), stream_));
     launch_propagate(p_.metric, flip, x_, d_, ld_, norms_, perm_, n_, p_.k, gids_[cur_], gvals_[cur_], gids_[cur_ ^ 1],
-                     gvals_[cur_ ^ 1], ncand_ + r, stream_);
+                     gvals_[cur_ ^ 1], ncand_ + r, hist, stream_);
     AKNN_HIP_TRY(hipEventRecord(launch_ev(base + 2 * r + 1), stream_));
     cur_ ^= 1;  // buffer swap (.cu:1969-1975)
   }

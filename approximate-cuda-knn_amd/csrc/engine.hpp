@@ -24,6 +24,7 @@ struct Params {
   int rng_order = 0;      // 0: Leaf7 gets (second, first) draw of a level; 1: (first, second)
   int cosine_fixed = 0;   // 1: carry COSINE distances instead of the 1 - key flip
   int level_batch = 16;   // forest levels enqueued per host check
+  int skip_settled = 1;   // propagation skips neighbours whose rows were examined last round (same results)
 };
 
 // Per-stage times of the last calculate(), milliseconds.
@@ -53,6 +54,7 @@ class Engine {
     p_.rng_order = p.rng_order;
     p_.cosine_fixed = p.cosine_fixed;
     p_.level_batch = p.level_batch;
+    p_.skip_settled = p.skip_settled;
   }
 
   void get_graph(int32_t* ids, float* vals) const;
@@ -102,6 +104,7 @@ class Engine {
   ForestBuffers fb_{};
   ForestCtl* ctl_host_ = nullptr;
   unsigned long long* ncand_ = nullptr;
+  uint8_t* changed_[2] = {nullptr, nullptr};
   int32_t* rs_ = nullptr;  // [trees][n], allocated on demand
   int32_t* rc_ = nullptr;
   CosineParityScratch cps_{};

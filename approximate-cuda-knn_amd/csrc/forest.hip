@@ -29,23 +29,44 @@ namespace {
 constexpr int kSideBlock = 1024;             // points per scan block
 constexpr int kWordsPerBlock = kSideBlock / 64;
 
+// Projection chain x . (x_p2 - x_p1) in dimension order.  NV > 0: the row has
+// exactly 4*NV floats and the point's row is loaded into registers in one
+// batch (one memory latency; the pivot rows are shared by the bucket and come
+// from cache).  NV = 0: any width, streamed.  Kernels are instantiated per
+// width, so each has a single load path.
+template <int NV>
 __device__ __forceinline__ bool side_is_right(const float* __restrict__ x, int d, int ld, int32_t id, int32_t p1,
                                               int32_t p2) {
   const float* xc = x + static_cast<int64_t>(id) * ld;
   const float* xa = x + static_cast<int64_t>(p1) * ld;
   const float* xb = x + static_cast<int64_t>(p2) * ld;
   float s = 0.0f;
-  int dd = 0;
-  for (; dd + 4 <= d; dd += 4) {
-    const float4 c = *reinterpret_cast<const float4*>(xc + dd);
-    const float4 a = *reinterpret_cast<const float4*>(xa + dd);
-    const float4 b = *reinterpret_cast<const float4*>(xb + dd);
-    s = __builtin_fmaf(c.x, b.x - a.x, s);
-    s = __builtin_fmaf(c.y, b.y - a.y, s);
-    s = __builtin_fmaf(c.z, b.z - a.z, s);
-    s = __builtin_fmaf(c.w, b.w - a.w, s);
+  if constexpr (NV > 0) {
+    float4 c[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) c[j] = *reinterpret_cast<const float4*>(xc + 4 * j);
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      const float4 a = *reinterpret_cast<const float4*>(xa + 4 * j);
+      const float4 b = *reinterpret_cast<const float4*>(xb + 4 * j);
+      s = __builtin_fmaf(c[j].x, b.x - a.x, s);
+      s = __builtin_fmaf(c[j].y, b.y - a.y, s);
+      s = __builtin_fmaf(c[j].z, b.z - a.z, s);
+      s = __builtin_fmaf(c[j].w, b.w - a.w, s);
+    }
+  } else {
+    int dd = 0;
+    for (; dd + 4 <= d; dd += 4) {
+      const float4 c = *reinterpret_cast<const float4*>(xc + dd);
+      const float4 a = *reinterpret_cast<const float4*>(xa + dd);
+      const float4 b = *reinterpret_cast<const float4*>(xb + dd);
+      s = __builtin_fmaf(c.x, b.x - a.x, s);
+      s = __builtin_fmaf(c.y, b.y - a.y, s);
+      s = __builtin_fmaf(c.z, b.z - a.z, s);
+      s = __builtin_fmaf(c.w, b.w - a.w, s);
+    }
+    for (; dd < d; ++dd) s = __builtin_fmaf(xc[dd], xb[dd] - xa[dd], s);
   }
-  for (; dd < d; ++dd) s = __builtin_fmaf(xc[dd], xb[dd] - xa[dd], s);
   return !(s < 0.0f);  // side = s < 0 ? LEFT : RIGHT (.cu:2157-2161)
 }
 
@@ -81,6 +102,7 @@ __global__ __launch_bounds__(256) void forest_init_kernel(ForestBuffers fb, int3
   }
 }
 
+template <int NV>
 __global__ __launch_bounds__(256) void forest_side_kernel(const float* __restrict__ x, int d, int ld,
                                                           const int32_t* __restrict__ act_ids,
                                                           const int32_t* __restrict__ act_bkt,
@@ -101,7 +123,7 @@ __global__ __launch_bounds__(256) void forest_side_kernel(const float* __restric
         if (rel == b.q2) {
           right = true;  // Leaf2 writes RIGHT for p2 last (.cu:2178-2179)
         } else if (rel != b.q1) {
-          right = side_is_right(x, d, ld, act_ids[m], act_ids[b.astart + b.q1], act_ids[b.astart + b.q2]);
+          right = side_is_right<NV>(x, d, ld, act_ids[m], act_ids[b.astart + b.q1], act_ids[b.astart + b.q2]);
         }
       }
       const uint64_t bal = __ballot(right);
@@ -315,9 +337,28 @@ void launch_forest_init(const ForestBuffers& fb, int32_t* perm, Leaf* leaves, in
 void launch_forest_level(const ForestBuffers& fb, const float* x, int d, int ld, int32_t* perm, Leaf* leaves,
                          int level, int32_t v1, int32_t v2, int leaf_eff, int kp1, hipStream_t s) {
   const int par = level & 1;
-  hipLaunchKernelGGL(forest_side_kernel, dim3(clamp_grid((fb.n + kSideBlock - 1) / kSideBlock, 2048)), dim3(256), 0,
-                     s, x, d, ld, fb.act_ids[par], fb.act_bkt[par], fb.bkt[par], fb.ctl, par, fb.rmask, fb.wpre,
-                     fb.blk_r);
+  const dim3 sg(clamp_grid((fb.n + kSideBlock - 1) / kSideBlock, 2048));
+  switch (row_width_class(d)) {
+    case 8:
+      hipLaunchKernelGGL(forest_side_kernel<8>, sg, dim3(256), 0, s, x, d, ld, fb.act_ids[par], fb.act_bkt[par],
+                         fb.bkt[par], fb.ctl, par, fb.rmask, fb.wpre, fb.blk_r);
+      break;
+    case 16:
+      hipLaunchKernelGGL(forest_side_kernel<16>, sg, dim3(256), 0, s, x, d, ld, fb.act_ids[par], fb.act_bkt[par],
+                         fb.bkt[par], fb.ctl, par, fb.rmask, fb.wpre, fb.blk_r);
+      break;
+    case 24:
+      hipLaunchKernelGGL(forest_side_kernel<24>, sg, dim3(256), 0, s, x, d, ld, fb.act_ids[par], fb.act_bkt[par],
+                         fb.bkt[par], fb.ctl, par, fb.rmask, fb.wpre, fb.blk_r);
+      break;
+    case 32:
+      hipLaunchKernelGGL(forest_side_kernel<32>, sg, dim3(256), 0, s, x, d, ld, fb.act_ids[par], fb.act_bkt[par],
+                         fb.bkt[par], fb.ctl, par, fb.rmask, fb.wpre, fb.blk_r);
+      break;
+    default:
+      hipLaunchKernelGGL(forest_side_kernel<0>, sg, dim3(256), 0, s, x, d, ld, fb.act_ids[par], fb.act_bkt[par],
+                         fb.bkt[par], fb.ctl, par, fb.rmask, fb.wpre, fb.blk_r);
+  }
   AKNN_HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(forest_level_kernel, dim3(1), dim3(1024), 0, s, fb.bkt[par], fb.bkt[par ^ 1], fb.blk_r,
                      fb.blk_roff, fb.rmask, fb.wpre, fb.ctl, par, level, v1, v2, leaf_eff, kp1, leaves);

@@ -45,14 +45,26 @@ void launch_leaf_knn(int metric, const float* x, int d, int ld, const float* nor
                      const Leaf* leaves, int nleaves, int max_leaf, int k, int32_t* ids, float* vals,
                      hipStream_t s);
 
+// What the previous round left behind, so a round can skip neighbours whose
+// rows were already fully examined (results are unchanged by the skip).
+// ids_prev: the previous round's input graph (it is the buffer this round
+// writes; nullable: no skipping); changed_prev[j]: row j changed in the
+// previous round; changed_out[i]: written by this round (nullable).
+struct PropagateHistory {
+  const int32_t* ids_prev = nullptr;
+  const uint8_t* changed_prev = nullptr;
+  uint8_t* changed_out = nullptr;
+};
+
 // One Jacobi propagation round, ids_in/vals_in -> ids_out/vals_out.
-// flip: the COSINE parity store rule (stored = 1 - key).  order: processing
-// order of the points (nullable: 0..n-1); results do not depend on it.
-// ncand: optional device counter of unique new candidates (byte accounting).
+// flip: the COSINE parity store rule (stored = 1 - key; no skipping then,
+// since any candidate triggers the flip).  order: processing order of the
+// points (nullable: 0..n-1); results do not depend on it.  ncand: optional
+// device counter of examined new candidates (byte accounting).
 void launch_propagate(int metric, bool flip, const float* x, int d, int ld, const float* norms,
                       const int32_t* order, int64_t n, int k,
                       const int32_t* ids_in, const float* vals_in, int32_t* ids_out, float* vals_out,
-                      unsigned long long* ncand, hipStream_t s);
+                      unsigned long long* ncand, const PropagateHistory& hist, hipStream_t s);
 
 // COSINE parity initial pass: ascending-id chunks of `cap` candidates over the
 // union of all trees' start ranges, with the store-time flip after each chunk.

@@ -37,15 +37,18 @@ __device__ __forceinline__ bool hash_insert(int32_t* hs, uint32_t mask, int shif
   }
 }
 
-template <int METRIC, bool FLIP>
+template <int METRIC, bool FLIP, int NV>
 __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict__ x, int d, int ld,
                                                         const float* __restrict__ norms,
                                                         const int32_t* __restrict__ order, int64_t n, int k,
                                                         int hs_log2, int cand_cap, int xi_stride,
                                                         const int32_t* __restrict__ ids_in,
-                                                        const float* __restrict__ vals_in, int32_t* __restrict__ ids_out,
+                                                        const float* __restrict__ vals_in, int32_t* ids_out,
                                                         float* __restrict__ vals_out,
-                                                        unsigned long long* __restrict__ ncand) {
+                                                        unsigned long long* __restrict__ ncand,
+                                                        const int32_t* ids_prev,
+                                                        const uint8_t* __restrict__ changed_prev,
+                                                        uint8_t* __restrict__ changed_out) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int hs_size = 1 << hs_log2;
@@ -68,6 +71,19 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
     key = vals_in[i * k + lane];
     id = ids_in[i * k + lane];
   }
+  // Neighbours whose whole row was already a candidate set of i last round
+  // need no expansion: j was in i's previous row and j's row did not change
+  // in the previous round.  Every such candidate was then in last round's
+  // pool of i, and a candidate that lost there has k better entries in i's
+  // current row, so it cannot enter now (ids_prev is the buffer this round
+  // writes; row i of it is read here before this wave overwrites it).
+  bool expand = true;
+  if (ids_prev != nullptr) {
+    const int32_t pid = lane < k ? ids_prev[i * k + lane] : INT_MAX;
+    bool was_prev = false;
+    for (int l = 0; l < k; ++l) was_prev |= __shfl(pid, l) == id;
+    if (lane < k && id >= 0 && was_prev && changed_prev[id] == 0) expand = false;
+  }
   for (int s = lane; s < hs_size; s += 64) hs[s] = -1;
   for (int dd = lane; dd < d; dd += 64) xi[dd] = x[i * ld + dd];
   if (lane == 0) *cnt = 0;
@@ -84,7 +100,8 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
   for (int f = lane; f < kk + (64 - kk % 64) % 64; f += 64) {
     const int hi = f / k, lo = f - (f / k) * k;
     const int32_t nb = __shfl(id, hi < 64 ? hi : 0);
-    if (f < kk && nb >= 0) {
+    const bool ex = __shfl(static_cast<int>(expand), hi < 64 ? hi : 0) != 0;
+    if (f < kk && nb >= 0 && ex) {
       const int32_t c = ids_in[static_cast<int64_t>(nb) * k + lo];
       if (c >= 0 && hash_insert(hs, mask, shift, c)) {
         const int pos = atomicAdd(cnt, 1);
@@ -97,6 +114,7 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
   const int m = *cnt;
   if (ncand != nullptr && lane == 0) atomicAdd(ncand, static_cast<unsigned long long>(m));
 
+  bool changed = false;
   if (m > 0) {
     float thk = __shfl(key, k - 1);
     int32_t thi = __shfl(id, k - 1);
@@ -106,7 +124,12 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
       const int32_t c = ci < m ? cand[ci] : -1;
       float dk = __builtin_inff();
       if (c >= 0) {
-        const float t = row_chain<METRIC>(xi, x + static_cast<int64_t>(c) * ld, d);
+        const float* xc = x + static_cast<int64_t>(c) * ld;
+        float t;
+        if constexpr (NV > 0)
+          t = row_chain_regs<METRIC, NV>(xi, xc);
+        else
+          t = row_chain<METRIC>(xi, xc, d);
         dk = finish_key<METRIC>(t, ni, METRIC == kCosine ? norms[c] : 1.0f);
       }
       uint64_t surv = __ballot(c >= 0 && key_less(dk, c, thk, thi));
@@ -116,6 +139,7 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
         const float nk = __shfl(dk, src);
         const int32_t nid = __shfl(c, src);
         if (key_less(nk, nid, thk, thi)) {
+          changed = true;
           wave_list_insert(key, id, k, nk, nid);
           thk = __shfl(key, k - 1);
           thi = __shfl(id, k - 1);
@@ -125,18 +149,20 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
     if constexpr (FLIP) {
       if (lane < k) key = static_cast<float>(1.0 - static_cast<double>(key));
       wave_sort(key, id);
+      changed = true;
     }
   }
+  if (changed_out != nullptr && lane == 0) changed_out[i] = changed ? 1 : 0;
   if (lane < k) {
     ids_out[i * k + lane] = id;
     vals_out[i * k + lane] = key;
   }
 }
 
-template <int METRIC, bool FLIP>
-void launch_t(const float* x, int d, int ld, const float* norms, const int32_t* order, int64_t n, int k,
-              const int32_t* ids_in,
-              const float* vals_in, int32_t* ids_out, float* vals_out, unsigned long long* ncand, hipStream_t s) {
+template <int METRIC, bool FLIP, int NV>
+void launch_w(const float* x, int d, int ld, const float* norms, const int32_t* order, int64_t n, int k,
+              const int32_t* ids_in, const float* vals_in, int32_t* ids_out, float* vals_out,
+              unsigned long long* ncand, const PropagateHistory& hist, hipStream_t s) {
   int hs_log2 = 7;
   while ((1 << hs_log2) < 2 * (k * k + k + 1)) ++hs_log2;
   const int cand_cap = k * k;
@@ -145,29 +171,41 @@ void launch_t(const float* x, int d, int ld, const float* norms, const int32_t* 
   const size_t shmem = per_wave * kWavesPerBlock;
   if (shmem > 160 * 1024) throw HipError("propagation: k or d too large for LDS");
   const int64_t blocks = xcd_grid((n + kWavesPerBlock - 1) / kWavesPerBlock);
-  hipLaunchKernelGGL((propagate_kernel<METRIC, FLIP>), dim3(static_cast<unsigned>(blocks)), dim3(256), shmem, s, x, d,
-                     ld, norms, order, n, k, hs_log2, cand_cap, xi_stride, ids_in, vals_in, ids_out, vals_out, ncand);
+  hipLaunchKernelGGL((propagate_kernel<METRIC, FLIP, NV>), dim3(static_cast<unsigned>(blocks)), dim3(256), shmem, s, x, d,
+                     ld, norms, order, n, k, hs_log2, cand_cap, xi_stride, ids_in, vals_in, ids_out, vals_out, ncand,
+                     FLIP ? nullptr : hist.ids_prev, hist.changed_prev, hist.changed_out);
   AKNN_HIP_TRY(hipGetLastError());
+}
+
+// Candidate rows are streamed (width class 0) for every width: the
+// register-row instantiations measured slower here (fewer waves in flight at
+// 196 VGPRs outweigh the deeper per-lane prefetch).
+template <int METRIC, bool FLIP>
+void launch_t(const float* x, int d, int ld, const float* norms, const int32_t* order, int64_t n, int k,
+              const int32_t* ids_in, const float* vals_in, int32_t* ids_out, float* vals_out,
+              unsigned long long* ncand, const PropagateHistory& hist, hipStream_t s) {
+  launch_w<METRIC, FLIP, 0>(x, d, ld, norms, order, n, k, ids_in, vals_in, ids_out, vals_out, ncand, hist, s);
 }
 
 }  // namespace
 
 void launch_propagate(int metric, bool flip, const float* x, int d, int ld, const float* norms,
-                      const int32_t* order, int64_t n, int k, const int32_t* ids_in, const float* vals_in, int32_t* ids_out, float* vals_out,
-                      unsigned long long* ncand, hipStream_t s) {
+                      const int32_t* order, int64_t n, int k, const int32_t* ids_in, const float* vals_in,
+                      int32_t* ids_out, float* vals_out, unsigned long long* ncand, const PropagateHistory& hist,
+                      hipStream_t s) {
   if (k > 64) throw HipError("propagation supports k <= 64");
   switch (metric) {
     case kEuclidean:
-      launch_t<kEuclidean, false>(x, d, ld, norms, order, n, k, ids_in, vals_in, ids_out, vals_out, ncand, s);
+      launch_t<kEuclidean, false>(x, d, ld, norms, order, n, k, ids_in, vals_in, ids_out, vals_out, ncand, hist, s);
       break;
     case kTaxicab:
-      launch_t<kTaxicab, false>(x, d, ld, norms, order, n, k, ids_in, vals_in, ids_out, vals_out, ncand, s);
+      launch_t<kTaxicab, false>(x, d, ld, norms, order, n, k, ids_in, vals_in, ids_out, vals_out, ncand, hist, s);
       break;
     case kCosine:
       if (flip)
-        launch_t<kCosine, true>(x, d, ld, norms, order, n, k, ids_in, vals_in, ids_out, vals_out, ncand, s);
+        launch_t<kCosine, true>(x, d, ld, norms, order, n, k, ids_in, vals_in, ids_out, vals_out, ncand, hist, s);
       else
-        launch_t<kCosine, false>(x, d, ld, norms, order, n, k, ids_in, vals_in, ids_out, vals_out, ncand, s);
+        launch_t<kCosine, false>(x, d, ld, norms, order, n, k, ids_in, vals_in, ids_out, vals_out, ncand, hist, s);
       break;
     default:
       throw HipError("unknown distance");

@@ -84,7 +84,12 @@ int aknn_deinitialize(aknn_handle* h);
  *                   (first, second)
  *   "cosine_fixed"  0: COSINE stores 1 - key on every selection and re-keys
  *                   carried entries by it (.cu:1007, :806); 1: carry distances
- *   "level_batch"   forest levels enqueued per host check (default 16) */
+ *   "level_batch"   forest levels enqueued per host check (default 16)
+ *   "skip_settled"  1 (default): a propagation round does not re-examine the
+ *                   neighbours of a neighbour whose row it already examined
+ *                   in the previous round unchanged (EUCLIDEAN, TAXICAB and
+ *                   COSINE with carried distances; the results are the same
+ *                   as without the skip); 0: examine all */
 int aknn_set_option(aknn_handle* h, const char* name, int64_t value);
 
 /* The graph as the result file holds it before sorting: ids[n*k], vals[n*k]

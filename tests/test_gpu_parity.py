@@ -156,3 +156,16 @@ def test_rejects_k_at_least_n(aknn, datasets):
     g = aknn.Graph()
     with pytest.raises(aknn.AknnError, match="k\\+1"):
         g.initialize(x, 10, 2, 4, 1, 0)
+
+
+@pytest.mark.parametrize("metric,fixed", [(0, 0), (1, 0), (2, 1)])
+def test_skip_settled_changes_nothing(aknn, oracle, datasets, metric, fixed):
+    """Propagation's skip of already-examined neighbours gives the same graph (and the oracle's)."""
+    x = datasets.gmm_c1(4000, 16, clusters=6, seed=21)[0]
+    a = gpu_run(aknn, x, 8, 2, 24, 5, metric, cosine_fixed=fixed, skip_settled=1)
+    b = gpu_run(aknn, x, 8, 2, 24, 5, metric, cosine_fixed=fixed, skip_settled=0)
+    ga, gb = a.graph(), b.graph()
+    assert (ga[0] == gb[0]).all() and (bits(ga[1]) == bits(gb[1])).all()
+    sa, sb = a.kernel_stats("propagate"), b.kernel_stats("propagate")
+    assert sa["units"] < sb["units"]  # fewer candidate rows examined
+    check_graph(oracle, a, x, 8, 2, 24, 5, metric, cosine_fixed=fixed)
