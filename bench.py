@@ -9,6 +9,7 @@ uploaded to HBM before the timed region.  Prints one JSON line.
 import argparse
 import importlib.util
 import json
+import re
 import os
 import sys
 import time
@@ -102,13 +103,13 @@ def main():
     ap.add_argument("--config", default="C2")
     ap.add_argument("--n", type=int, default=None, help="override the number of points")
     ap.add_argument("--queries", type=int, default=10000)
-    ap.add_argument("--cpu-sample", type=int, default=50000)
+    ap.add_argument("--cpu-sample", type=int, default=200000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-recall", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
-              "(the multi-GPU sharding is a design only, see DESIGN.md)", file=sys.stderr)
+              "(the multi-GPU sharding is not built: one GPU per box)", file=sys.stderr)
         sys.exit(2)
     if int(os.environ.get("LOCAL_RANK", "0")) != 0:
         return
@@ -160,10 +161,14 @@ def main():
         achieved = per_launch / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": achieved / PEAK_HBM_GBS, "traffic": None, "kernel": "propagate_kernel"}
-    prof = os.path.join(ROOT, "profiles", f"traffic_{roof['kernel']}_{args.config}.json")
+    # HBM bytes per launch of this kernel on this workload, from the committed
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against copy kernels)
+    prof = os.path.join(ROOT, "profiles", f"traffic_{roof['kernel']}_{args.config}.md")
     if os.path.exists(prof) and args.n is None:
         with open(prof) as f:
-            roof["traffic"] = json.load(f).get("hbm_bytes_per_launch")
+            m = re.search(r"^hbm_bytes_per_launch:\s*([0-9.eE+]+)", f.read(), re.M)
+        if m:
+            roof["traffic"] = float(m.group(1))
 
     line = {
         "metric": METRIC,
