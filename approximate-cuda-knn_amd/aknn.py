@@ -21,7 +21,7 @@ EXPORTS = (
     "aknn_create", "aknn_destroy", "aknn_set_data_file", "aknn_set_results_file", "aknn_initialize",
     "aknn_initialize_host", "aknn_calculate", "aknn_save_result", "aknn_deinitialize", "aknn_set_option",
     "aknn_get_graph", "aknn_get_normalized", "aknn_get_forest", "aknn_get_stage_times",
-    "aknn_get_kernel_stats", "aknn_last_error",
+    "aknn_get_kernel_stats", "aknn_get_candidate_counts", "aknn_last_error",
 )
 
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -62,6 +62,7 @@ def load():
     L.aknn_get_stage_times.argtypes = [H, _f64p, ctypes.c_int]
     L.aknn_get_kernel_stats.argtypes = [H, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64),
                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    L.aknn_get_candidate_counts.argtypes = [H, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     L.aknn_last_error.argtypes = [H]
     L.aknn_last_error.restype = ctypes.c_char_p
     for name in EXPORTS:
@@ -157,6 +158,13 @@ class Graph:
         self._check(self._L.aknn_get_kernel_stats(self._h, name.encode(), ctypes.byref(launches),
                                                   ctypes.byref(total), ctypes.byref(units)))
         return {"launches": launches.value, "total_ms": total.value, "units": units.value}
+
+    def candidate_counts(self):
+        """(examined, rescored) propagation candidates of the last calculate()."""
+        ex = ctypes.c_double()
+        rs = ctypes.c_double()
+        self._check(self._L.aknn_get_candidate_counts(self._h, ctypes.byref(ex), ctypes.byref(rs)))
+        return ex.value, rs.value
 
     def deinitialize(self):
         self._check(self._L.aknn_deinitialize(self._h))

@@ -173,6 +173,8 @@ int aknn_set_option(aknn_handle* h, const char* name, int64_t value) {
       h->params.level_batch = static_cast<int>(value);
     } else if (n == "skip_settled") {
       h->params.skip_settled = value != 0;
+    } else if (n == "sketch") {
+      h->params.sketch = value != 0;
     } else {
       return fail(h, AKNN_ERR_ARGUMENT, "unknown option " + n);
     }
@@ -238,6 +240,15 @@ int aknn_get_kernel_stats(aknn_handle* h, const char* name, int64_t* launches, d
     if (launches) *launches = s->launches;
     if (total_ms) *total_ms = s->total_ms;
     if (units) *units = s->units;
+    return AKNN_OK;
+  });
+}
+
+int aknn_get_candidate_counts(aknn_handle* h, double* examined, double* rescored) {
+  return guarded(h, [&] {
+    if (!h->engine || !h->calculated) return fail(h, AKNN_ERR_STATE, "nothing calculated");
+    if (examined) *examined = h->engine->stats_propagate.examined;
+    if (rescored) *rescored = h->engine->stats_propagate.rescored;
     return AKNN_OK;
   });
 }

@@ -96,6 +96,102 @@ __device__ __forceinline__ float row_chain_regs(const float* a, const float* __r
   return 0.0f + p;
 }
 
+// Chain of a FP16 sketch row b (global, ldh halves, 16-byte aligned, zero
+// padded) against a query sketch a held as floats (zero padded to ldh): one
+// unchunked chain in FP32, each half converted exactly.  Feeds
+// sketch_lower_bound only, never a stored value.
+__device__ __forceinline__ float half_lo(uint32_t w) {
+  return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(w & 0xffffu)));
+}
+__device__ __forceinline__ float half_hi(uint32_t w) {
+  return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(w >> 16)));
+}
+
+// G 16-byte loads (8 halves each) issued together, then their 8G steps.
+template <int METRIC, int G>
+__device__ __forceinline__ float sketch_group(const float* a, const uint4* __restrict__ b, float p) {
+  uint4 v[G];
+#pragma unroll
+  for (int j = 0; j < G; ++j) v[j] = b[j];
+#pragma unroll
+  for (int j = 0; j < G; ++j) {
+    const float* aj = a + 8 * j;
+    p = chain_step<METRIC>(aj[0], half_lo(v[j].x), p);
+    p = chain_step<METRIC>(aj[1], half_hi(v[j].x), p);
+    p = chain_step<METRIC>(aj[2], half_lo(v[j].y), p);
+    p = chain_step<METRIC>(aj[3], half_hi(v[j].y), p);
+    p = chain_step<METRIC>(aj[4], half_lo(v[j].z), p);
+    p = chain_step<METRIC>(aj[5], half_hi(v[j].z), p);
+    p = chain_step<METRIC>(aj[6], half_lo(v[j].w), p);
+    p = chain_step<METRIC>(aj[7], half_hi(v[j].w), p);
+  }
+  return p;
+}
+
+template <int METRIC>
+__device__ __forceinline__ float sketch_chain(const float* a, const _Float16* __restrict__ b, int ldh) {
+  const uint4* bv = reinterpret_cast<const uint4*>(b);
+  float p = 0.0f;
+  int dd = 0;
+  // 128 bytes in flight per lane (a whole row in flight measured slower: the
+  // extra registers cost more waves than the deeper prefetch gains)
+  for (; dd + 64 <= ldh; dd += 64) p = sketch_group<METRIC, 8>(a + dd, bv + dd / 8, p);
+  for (; dd < ldh; dd += 8) p = sketch_group<METRIC, 1>(a + dd, bv + dd / 8, p);
+  return p;
+}
+
+// Constants of the sketch bounds for rows of d dimensions.  kap bounds the
+// relative error of any FP32 chain of the pipeline over d terms (the exact
+// chunked chains take at most d + d/256 + 3 roundings per term, the sketch
+// chain d + 1) with a factor 2 to spare, which also covers the FP64 rounding
+// of the bound itself; tiny bounds the absolute error of subnormal results.
+struct SketchBound {
+  double kap;
+  double tiny;
+};
+inline SketchBound sketch_bound(int d) {
+  const double u = 1.0 / 16777216.0;  // 2^-24
+  return {2.0 * (d + d / 256 + 8) * u, (d + 16) * 1.401298464324817e-45};
+}
+
+// A lower bound of the exact FP32 key (finish_key of the exact chain) of the
+// pair (x, y), from the sketch chain p of (xh, yh), the sketch radii ex, ey
+// (L2; L1 for TAXICAB) and, for COSINE, the exact squared-norm chains aa, bb.
+//   EUCLIDEAN: ||x-y|| >= sqrt(p (1-kap)) - ex - ey (triangle inequality), and
+//     the exact key >= ||x-y|| (1-kap).
+//   TAXICAB: |x-y|_1 >= p (1-kap) - ex - ey, and key >= sqrt(|x-y|_1 (1-kap)).
+//   COSINE: the exact key is 1 - fl(tot / den) with den the same FP32 product
+//     of roots as finish_key, and tot <= p + kap (A+ex)(B+ey) + ex B
+//     + (A+ex) ey + kap A B, with A >= ||x||, B >= ||y|| from aa, bb.
+// Non-finite or degenerate inputs give -inf: the exact path then decides.
+template <int METRIC>
+__device__ __forceinline__ double sketch_lower_bound(float p, float ex, float ey, float aa, float bb,
+                                                     const SketchBound& sb) {
+  const double k1 = 1.0 - sb.kap;
+  double lb;
+  if constexpr (METRIC == kEuclidean) {
+    const double s = fmax(0.0, static_cast<double>(p) * k1 - sb.tiny);
+    const double r = fmax(0.0, sqrt(s) - static_cast<double>(ex) - static_cast<double>(ey));
+    lb = sqrt(fmax(0.0, r * r * k1 - sb.tiny)) * k1;
+  } else if constexpr (METRIC == kTaxicab) {
+    const double s =
+        fmax(0.0, static_cast<double>(p) * k1 - sb.tiny - static_cast<double>(ex) - static_cast<double>(ey));
+    lb = sqrt(fmax(0.0, s * k1 - sb.tiny)) * k1;
+  } else {
+    const float den = sqrtf(aa) * sqrtf(bb);
+    if (!(den > 0.0f) || !(den <= 3.4e38f)) return -__builtin_inf();
+    const double a = sqrt((static_cast<double>(aa) + sb.tiny) * (1.0 + 2.0 * sb.kap));
+    const double b = sqrt((static_cast<double>(bb) + sb.tiny) * (1.0 + 2.0 * sb.kap));
+    const double xe = ex, ye = ey;
+    const double up = static_cast<double>(p) + sb.kap * (a + xe) * (b + ye) + xe * b + (a + xe) * ye +
+                      sb.kap * a * b + 2.0 * sb.tiny;
+    double q = up / static_cast<double>(den);
+    q += fabs(q) * 2.384185791015625e-07 + 1e-40;  // the FP32 quotient's rounding, 2^-22 relative
+    lb = (1.0 - q) - 9.5367431640625e-07;          // the FP32 store of 1 - quotient, 2^-20 absolute
+  }
+  return lb == lb ? lb : -__builtin_inf();
+}
+
 // Row-width class of a kernel instantiation: rows of exactly 4*NV floats in
 // registers (NV = 8, 16, 24, 32), or 0 for any width, streamed.
 inline int row_width_class(int d) {

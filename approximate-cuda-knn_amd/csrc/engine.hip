@@ -69,7 +69,8 @@ T* Engine::alloc(size_t count) {
 void Engine::release() {
   for (void* p : allocs_) (void)hipFree(p);
   allocs_.clear();
-  x_ = x_raw_ = norms_ = mean_ = nullptr;
+  x_ = x_raw_ = norms_ = mean_ = eps_ = nullptr;
+  xh_ = nullptr;
   gmax_ = nullptr;
   gids_[0] = gids_[1] = nullptr;
   gvals_[0] = gvals_[1] = nullptr;
@@ -98,6 +99,7 @@ void Engine::set_data(const float* x_rowmajor, int64_t n, int d, const Params& p
   n_ = n;
   d_ = d;
   ld_ = (d + 3) / 4 * 4;
+  ldh_ = (d + 7) / 8 * 8;
   leaf_eff_ = std::max(p.leaf, 1);
 
   x_ = alloc<float>(static_cast<size_t>(n) * ld_);
@@ -130,7 +132,7 @@ void Engine::set_data(const float* x_rowmajor, int64_t n, int d, const Params& p
   fb_.blk_r = alloc<int32_t>(blocks);
   fb_.blk_roff = alloc<int32_t>(blocks);
   fb_.ctl = alloc<ForestCtl>(1);
-  ncand_ = alloc<unsigned long long>(std::max(p.props, 1));
+  ncand_ = alloc<unsigned long long>(2 * static_cast<size_t>(std::max(p.props, 1)));
   changed_[0] = alloc<uint8_t>(n);
   changed_[1] = alloc<uint8_t>(n);
   AKNN_HIP_TRY(hipStreamSynchronize(stream_));
@@ -224,17 +226,35 @@ void Engine::propagate() {
   stats_propagate = KernelStats{};
   const bool flip = p_.metric == kCosine && !p_.cosine_fixed;
   const int base = 2 * p_.trees;
-  AKNN_HIP_TRY(hipMemsetAsync(ncand_, 0, sizeof(unsigned long long) * std::max(p_.props, 1), stream_));
+  AKNN_HIP_TRY(hipMemsetAsync(ncand_, 0, sizeof(unsigned long long) * 2 * std::max(p_.props, 1), stream_));
   for (int r = 0; r < p_.props; ++r) {
-    PropagateHistory hist;
+    PropagateArgs a;
+    a.metric = p_.metric;
+    a.flip = flip;
+    a.x = x_;
+    a.d = d_;
+    a.ld = ld_;
+    if (p_.sketch) {
+      a.xh = xh_;
+      a.ldh = ldh_;
+      a.eps = eps_;
+    }
+    a.norms = norms_;
+    a.order = perm_;
+    a.n = n_;
+    a.k = p_.k;
+    a.ids_in = gids_[cur_];
+    a.vals_in = gvals_[cur_];
+    a.ids_out = gids_[cur_ ^ 1];
+    a.vals_out = gvals_[cur_ ^ 1];
+    a.ncand = ncand_ + 2 * r;
     if (p_.skip_settled) {
-      hist.ids_prev = r > 0 ? gids_[cur_ ^ 1] : nullptr;
-      hist.changed_prev = changed_[(r + 1) & 1];
-      hist.changed_out = changed_[r & 1];
+      a.hist.ids_prev = r > 0 ? gids_[cur_ ^ 1] : nullptr;
+      a.hist.changed_prev = changed_[(r + 1) & 1];
+      a.hist.changed_out = changed_[r & 1];
     }
     AKNN_HIP_TRY(hipEventRecord(launch_ev(base + 2 * r), stream_));
-    launch_propagate(p_.metric, flip, x_, d_, ld_, norms_, perm_, n_, p_.k, gids_[cur_], gvals_[cur_], gids_[cur_ ^ 1],
-                     gvals_[cur_ ^ 1], ncand_ + r, hist, stream_);
+    launch_propagate(a, stream_);
     AKNN_HIP_TRY(hipEventRecord(launch_ev(base + 2 * r + 1), stream_));
     cur_ ^= 1;  // buffer swap (.cu:1969-1975)
   }
@@ -245,10 +265,16 @@ void Engine::calculate() {
   AKNN_HIP_TRY(hipSetDevice(device_));
   cur_ = 0;
   draws_.reset(p_.seed);
+  const bool sketch = p_.sketch && p_.props > 0;
+  if (sketch && !xh_) {
+    xh_ = alloc<_Float16>(static_cast<size_t>(n_) * ldh_);
+    eps_ = alloc<float>(n_);
+  }
   AKNN_HIP_TRY(hipMemcpyAsync(x_, x_raw_, sizeof(float) * n_ * ld_, hipMemcpyDeviceToDevice, stream_));
   AKNN_HIP_TRY(hipEventRecord(ev(0), stream_));
   launch_normalize(x_, n_, d_, ld_, mean_, gmax_, stream_);
   if (norms_) launch_norms(x_, n_, d_, ld_, norms_, stream_);
+  if (sketch) launch_sketch(x_, n_, d_, ld_, p_.metric, xh_, ldh_, eps_, stream_);
   AKNN_HIP_TRY(hipEventRecord(ev(1), stream_));
   build_forest();
   AKNN_HIP_TRY(hipEventRecord(ev(2), stream_));
@@ -269,16 +295,26 @@ void Engine::calculate() {
     AKNN_HIP_TRY(hipEventElapsedTime(&ms, launch_ev(2 * l), launch_ev(2 * l + 1)));
     stats_leaf_knn.total_ms += ms;
   }
-  std::vector<unsigned long long> nc(std::max(p_.props, 1));
+  std::vector<unsigned long long> nc(2 * static_cast<size_t>(std::max(p_.props, 1)));
   AKNN_HIP_TRY(hipMemcpy(nc.data(), ncand_, sizeof(unsigned long long) * nc.size(), hipMemcpyDeviceToHost));
+  stats_propagate.examined = stats_propagate.rescored = 0;
   for (int r = 0; r < p_.props; ++r) {
     float ms = 0;
     AKNN_HIP_TRY(hipEventElapsedTime(&ms, launch_ev(2 * p_.trees + 2 * r), launch_ev(2 * p_.trees + 2 * r + 1)));
     stats_propagate.total_ms += ms;
     const double k = p_.k;
     // per point: own row (8k), k neighbour id rows (4k^2), x_i (4D), output
-    // row (8k); per unique new candidate its row (4D)
-    stats_propagate.units += static_cast<double>(n_) * (16.0 * k + 4.0 * k * k + 4.0 * d_) + 4.0 * d_ * nc[r];
+    // row (8k); per unique new candidate its row (4D), or with the sketch its
+    // sketch row and radius (2 ldh + 4) plus 4D for each exactly re-scored one
+    double per_point = 16.0 * k + 4.0 * k * k + 4.0 * d_;
+    double cand = 4.0 * d_ * nc[2 * r];
+    if (sketch) {
+      per_point += 2.0 * ldh_ + 4.0;
+      cand = (2.0 * ldh_ + 4.0) * nc[2 * r] + 4.0 * d_ * nc[2 * r + 1];
+    }
+    stats_propagate.units += static_cast<double>(n_) * per_point + cand;
+    stats_propagate.examined += static_cast<double>(nc[2 * r]);
+    stats_propagate.rescored += static_cast<double>(sketch ? nc[2 * r + 1] : nc[2 * r]);
   }
   stats_forest.launches = 0;
   for (int t = 0; t < p_.trees; ++t) stats_forest.launches += iters_[t];

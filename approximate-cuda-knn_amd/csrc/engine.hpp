@@ -25,6 +25,7 @@ struct Params {
   int cosine_fixed = 0;   // 1: carry COSINE distances instead of the 1 - key flip
   int level_batch = 16;   // forest levels enqueued per host check
   int skip_settled = 1;   // propagation skips neighbours whose rows were examined last round (same results)
+  int sketch = 0;         // propagation screens candidates with the FP16 sketch bound (same results)
 };
 
 // Per-stage times of the last calculate(), milliseconds.
@@ -37,6 +38,8 @@ struct KernelStats {
   int64_t launches = 0;
   double total_ms = 0;
   double units = 0;  // algorithmic bytes or flops summed over launches
+  double examined = 0;  // propagation: new candidates examined
+  double rescored = 0;  // propagation: candidates whose exact distance was computed
 };
 
 class Engine {
@@ -55,6 +58,7 @@ class Engine {
     p_.cosine_fixed = p.cosine_fixed;
     p_.level_batch = p.level_batch;
     p_.skip_settled = p.skip_settled;
+    p_.sketch = p.sketch;
   }
 
   void get_graph(int32_t* ids, float* vals) const;
@@ -93,6 +97,9 @@ class Engine {
   float* x_ = nullptr;
   float* x_raw_ = nullptr;  // pristine copy: calculate() normalizes x_ in place, re-runs start from here
   float* norms_ = nullptr;
+  _Float16* xh_ = nullptr;  // FP16 sketch [n][ldh_], allocated on first use
+  float* eps_ = nullptr;    // sketch radii [n]
+  int ldh_ = 0;
   float* mean_ = nullptr;
   uint32_t* gmax_ = nullptr;
   int32_t* gids_[2] = {nullptr, nullptr};

@@ -59,12 +59,36 @@ struct PropagateHistory {
 // One Jacobi propagation round, ids_in/vals_in -> ids_out/vals_out.
 // flip: the COSINE parity store rule (stored = 1 - key; no skipping then,
 // since any candidate triggers the flip).  order: processing order of the
-// points (nullable: 0..n-1); results do not depend on it.  ncand: optional
-// device counter of examined new candidates (byte accounting).
-void launch_propagate(int metric, bool flip, const float* x, int d, int ld, const float* norms,
-                      const int32_t* order, int64_t n, int k,
-                      const int32_t* ids_in, const float* vals_in, int32_t* ids_out, float* vals_out,
-                      unsigned long long* ncand, const PropagateHistory& hist, hipStream_t s);
+// points (nullable: 0..n-1); results do not depend on it.  xh/eps: FP16
+// sketch and radii (nullable: exact distances for every candidate; results do
+// not depend on it).  ncand: optional device counters [examined new
+// candidates, exactly re-scored candidates] (byte accounting).
+struct PropagateArgs {
+  int metric = kEuclidean;
+  bool flip = false;
+  const float* x = nullptr;
+  int d = 0, ld = 0;
+  const _Float16* xh = nullptr;
+  int ldh = 0;
+  const float* eps = nullptr;
+  const float* norms = nullptr;
+  const int32_t* order = nullptr;
+  int64_t n = 0;
+  int k = 0;
+  const int32_t* ids_in = nullptr;
+  const float* vals_in = nullptr;
+  int32_t* ids_out = nullptr;
+  float* vals_out = nullptr;
+  unsigned long long* ncand = nullptr;
+  PropagateHistory hist;
+};
+void launch_propagate(const PropagateArgs& a, hipStream_t s);
+
+// ---- FP16 sketch of the normalized points -----------------------------------
+// xh: [n][ldh] halves (ldh % 8 == 0, ldh >= d); eps[n]: upper bounds of the
+// rounding error's norm (L1 for TAXICAB, L2 otherwise).
+void launch_sketch(const float* x, int64_t n, int d, int ld, int metric, _Float16* xh, int ldh, float* eps,
+                   hipStream_t s);
 
 // COSINE parity initial pass: ascending-id chunks of `cap` candidates over the
 // union of all trees' start ranges, with the store-time flip after each chunk.

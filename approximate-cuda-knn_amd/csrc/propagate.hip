@@ -16,6 +16,13 @@
 // from LDS by broadcast, giving the exact FP32 chain.  The running top-k lives
 // one entry per lane; candidates below the current k-th entry (few after the
 // first round) are inserted with ballot/shuffle steps.
+//
+// With the FP16 sketch (SK), a first pass reads each candidate's half-size
+// sketch row instead and keeps only candidates whose proven lower bound of the
+// exact key is not above the current k-th key (sketch.hip); the survivors are
+// compacted in place and take the exact pass above.  A dropped candidate's
+// exact key is above the k-th key, so it could not have entered: the stored
+// graph is the same bit for bit, with about half the row bytes read.
 #include <climits>
 
 #include "device_util.hpp"
@@ -27,21 +34,26 @@ namespace {
 
 constexpr int kWavesPerBlock = 4;
 
-__device__ __forceinline__ bool hash_insert(int32_t* hs, uint32_t mask, int shift, int32_t key) {
-  uint32_t h = (static_cast<uint32_t>(key) * 2654435761u) >> shift;
+// Open-addressing set of ids in LDS (-1 = empty), linear probing.  The table
+// has at least twice as many slots as ids ever inserted, so probing ends.
+__device__ __forceinline__ bool hash_insert(int32_t* hs, uint32_t size, int32_t key) {
+  uint32_t h = static_cast<uint32_t>(
+      (static_cast<uint64_t>(static_cast<uint32_t>(key) * 2654435761u) * size) >> 32);
   for (;;) {
     const int32_t old = atomicCAS(&hs[h], -1, key);
     if (old == -1) return true;
     if (old == key) return false;
-    h = (h + 1) & mask;
+    h = h + 1 == size ? 0 : h + 1;
   }
 }
 
-template <int METRIC, bool FLIP, int NV>
+template <int METRIC, bool FLIP, int NV, bool SK>
 __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict__ x, int d, int ld,
+                                                        const _Float16* __restrict__ xh, int ldh,
+                                                        const float* __restrict__ eps, SketchBound sb,
                                                         const float* __restrict__ norms,
                                                         const int32_t* __restrict__ order, int64_t n, int k,
-                                                        int hs_log2, int cand_cap, int xi_stride,
+                                                        int hs_size, int cand_cap, int xi_stride,
                                                         const int32_t* __restrict__ ids_in,
                                                         const float* __restrict__ vals_in, int32_t* ids_out,
                                                         float* __restrict__ vals_out,
@@ -51,10 +63,11 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
                                                         uint8_t* __restrict__ changed_out) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int hs_size = 1 << hs_log2;
-  const size_t per_wave_words = static_cast<size_t>(hs_size) + cand_cap + xi_stride + 4;
-  float* xi = reinterpret_cast<float*>(smem) + per_wave_words * wave;  // 16-byte aligned (xi_stride % 4 == 0)
-  int32_t* hs = reinterpret_cast<int32_t*>(xi + xi_stride);
+  // per wave: x_i (FP32, plain form) or x_i's sketch as floats (SK), the
+  // hash set, the candidate list and its counter; xi_stride is a multiple of 4
+  const size_t per_wave_words = static_cast<size_t>(xi_stride) + hs_size + cand_cap + 4;
+  float* xrow = reinterpret_cast<float*>(smem) + per_wave_words * wave;  // 16-byte aligned
+  int32_t* hs = reinterpret_cast<int32_t*>(xrow + xi_stride);
   int32_t* cand = hs + hs_size;
   int32_t* cnt = cand + cand_cap;
 
@@ -85,15 +98,22 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
     if (lane < k && id >= 0 && was_prev && changed_prev[id] == 0) expand = false;
   }
   for (int s = lane; s < hs_size; s += 64) hs[s] = -1;
-  for (int dd = lane; dd < d; dd += 64) xi[dd] = x[i * ld + dd];
+  // the exact pass reads x_i from LDS in the plain form; with the sketch only
+  // few candidates take it and x_i is read from global memory (cached)
+  const float* xi = SK ? x + i * ld : xrow;
+  const float* xs = xrow;
+  if constexpr (SK) {
+    for (int dd = lane; dd < ldh; dd += 64) xrow[dd] = static_cast<float>(xh[i * ldh + dd]);
+  } else {
+    for (int dd = lane; dd < d; dd += 64) xrow[dd] = x[i * ld + dd];
+  }
   if (lane == 0) *cnt = 0;
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  const uint32_t mask = static_cast<uint32_t>(hs_size - 1);
-  const int shift = 32 - hs_log2;
-  if (lane == 0) hash_insert(hs, mask, shift, static_cast<int32_t>(i));
+  const uint32_t hsz = static_cast<uint32_t>(hs_size);
+  if (lane == 0) hash_insert(hs, hsz, static_cast<int32_t>(i));
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  if (lane < k && id >= 0) hash_insert(hs, mask, shift, id);
+  if (lane < k && id >= 0) hash_insert(hs, hsz, id);
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   // two-hop ids (.cu:1078-1085)
   const int kk = k * k;
@@ -103,7 +123,7 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
     const bool ex = __shfl(static_cast<int>(expand), hi < 64 ? hi : 0) != 0;
     if (f < kk && nb >= 0 && ex) {
       const int32_t c = ids_in[static_cast<int64_t>(nb) * k + lo];
-      if (c >= 0 && hash_insert(hs, mask, shift, c)) {
+      if (c >= 0 && hash_insert(hs, hsz, c)) {
         const int pos = atomicAdd(cnt, 1);
         cand[pos] = c;
       }
@@ -111,7 +131,7 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
   }
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
-  const int m = *cnt;
+  int m = *cnt;
   if (ncand != nullptr && lane == 0) atomicAdd(ncand, static_cast<unsigned long long>(m));
 
   bool changed = false;
@@ -119,6 +139,30 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
     float thk = __shfl(key, k - 1);
     int32_t thi = __shfl(id, k - 1);
     const float ni = METRIC == kCosine ? norms[i] : 1.0f;
+    if constexpr (SK) {
+      // sketch pass: survivors are written back to cand[] in order; slot
+      // w + rank <= base + lane was already read by this batch
+      const float ei = eps[i];
+      int w = 0;
+      for (int base = 0; base < m; base += 64) {
+        const int ci = base + lane;
+        const int32_t c = ci < m ? cand[ci] : -1;
+        bool pass = false;
+        if (c >= 0) {
+          const float p = sketch_chain<METRIC>(xs, xh + static_cast<int64_t>(c) * ldh, ldh);
+          const double lb =
+              sketch_lower_bound<METRIC>(p, ei, eps[c], ni, METRIC == kCosine ? norms[c] : 1.0f, sb);
+          pass = !(lb > static_cast<double>(thk));
+        }
+        const uint64_t bal = __ballot(pass);
+        if (pass) cand[w + __popcll(bal & ((1ull << lane) - 1))] = c;
+        w += __popcll(bal);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      m = w;
+      if (ncand != nullptr && lane == 0) atomicAdd(ncand + 1, static_cast<unsigned long long>(m));
+    }
     for (int base = 0; base < m; base += 64) {
       const int ci = base + lane;
       const int32_t c = ci < m ? cand[ci] : -1;
@@ -159,21 +203,22 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
   }
 }
 
-template <int METRIC, bool FLIP, int NV>
-void launch_w(const float* x, int d, int ld, const float* norms, const int32_t* order, int64_t n, int k,
-              const int32_t* ids_in, const float* vals_in, int32_t* ids_out, float* vals_out,
-              unsigned long long* ncand, const PropagateHistory& hist, hipStream_t s) {
-  int hs_log2 = 7;
-  while ((1 << hs_log2) < 2 * (k * k + k + 1)) ++hs_log2;
-  const int cand_cap = k * k;
-  const int xi_stride = (d + 3) / 4 * 4;
-  const size_t per_wave = (static_cast<size_t>(1 << hs_log2) + cand_cap + xi_stride + 4) * 4;
+template <int METRIC, bool FLIP, int NV, bool SK>
+void launch_w(const PropagateArgs& a, hipStream_t s) {
+  // at most 1 + k + k*k ids enter the set: at least twice that many slots
+  int hs_size = 128;
+  while (hs_size < 2 * (a.k * a.k + a.k + 1)) hs_size *= 2;
+  const int cand_cap = a.k * a.k;
+  const int xi_stride = SK ? a.ldh : (a.d + 3) / 4 * 4;
+  const size_t per_wave = (static_cast<size_t>(xi_stride) + hs_size + cand_cap + 4) * 4;
   const size_t shmem = per_wave * kWavesPerBlock;
   if (shmem > 160 * 1024) throw HipError("propagation: k or d too large for LDS");
-  const int64_t blocks = xcd_grid((n + kWavesPerBlock - 1) / kWavesPerBlock);
-  hipLaunchKernelGGL((propagate_kernel<METRIC, FLIP, NV>), dim3(static_cast<unsigned>(blocks)), dim3(256), shmem, s, x, d,
-                     ld, norms, order, n, k, hs_log2, cand_cap, xi_stride, ids_in, vals_in, ids_out, vals_out, ncand,
-                     FLIP ? nullptr : hist.ids_prev, hist.changed_prev, hist.changed_out);
+  const int64_t blocks = xcd_grid((a.n + kWavesPerBlock - 1) / kWavesPerBlock);
+  const SketchBound sb = sketch_bound(a.d);
+  hipLaunchKernelGGL((propagate_kernel<METRIC, FLIP, NV, SK>), dim3(static_cast<unsigned>(blocks)), dim3(256), shmem,
+                     s, a.x, a.d, a.ld, a.xh, a.ldh, a.eps, sb, a.norms, a.order, a.n, a.k, hs_size, cand_cap,
+                     xi_stride, a.ids_in, a.vals_in, a.ids_out, a.vals_out, a.ncand,
+                     FLIP ? nullptr : a.hist.ids_prev, a.hist.changed_prev, a.hist.changed_out);
   AKNN_HIP_TRY(hipGetLastError());
 }
 
@@ -181,31 +226,31 @@ void launch_w(const float* x, int d, int ld, const float* norms, const int32_t* 
 // register-row instantiations measured slower here (fewer waves in flight at
 // 196 VGPRs outweigh the deeper per-lane prefetch).
 template <int METRIC, bool FLIP>
-void launch_t(const float* x, int d, int ld, const float* norms, const int32_t* order, int64_t n, int k,
-              const int32_t* ids_in, const float* vals_in, int32_t* ids_out, float* vals_out,
-              unsigned long long* ncand, const PropagateHistory& hist, hipStream_t s) {
-  launch_w<METRIC, FLIP, 0>(x, d, ld, norms, order, n, k, ids_in, vals_in, ids_out, vals_out, ncand, hist, s);
+void launch_t(const PropagateArgs& a, hipStream_t s) {
+  if (a.xh != nullptr)
+    launch_w<METRIC, FLIP, 0, true>(a, s);
+  else
+    launch_w<METRIC, FLIP, 0, false>(a, s);
 }
 
 }  // namespace
 
-void launch_propagate(int metric, bool flip, const float* x, int d, int ld, const float* norms,
-                      const int32_t* order, int64_t n, int k, const int32_t* ids_in, const float* vals_in,
-                      int32_t* ids_out, float* vals_out, unsigned long long* ncand, const PropagateHistory& hist,
-                      hipStream_t s) {
-  if (k > 64) throw HipError("propagation supports k <= 64");
-  switch (metric) {
+void launch_propagate(const PropagateArgs& a, hipStream_t s) {
+  if (a.k > 64) throw HipError("propagation supports k <= 64");
+  if (a.xh != nullptr && (a.eps == nullptr || a.ldh % 8 != 0 || a.ldh < a.d))
+    throw HipError("propagation: malformed sketch");
+  switch (a.metric) {
     case kEuclidean:
-      launch_t<kEuclidean, false>(x, d, ld, norms, order, n, k, ids_in, vals_in, ids_out, vals_out, ncand, hist, s);
+      launch_t<kEuclidean, false>(a, s);
       break;
     case kTaxicab:
-      launch_t<kTaxicab, false>(x, d, ld, norms, order, n, k, ids_in, vals_in, ids_out, vals_out, ncand, hist, s);
+      launch_t<kTaxicab, false>(a, s);
       break;
     case kCosine:
-      if (flip)
-        launch_t<kCosine, true>(x, d, ld, norms, order, n, k, ids_in, vals_in, ids_out, vals_out, ncand, hist, s);
+      if (a.flip)
+        launch_t<kCosine, true>(a, s);
       else
-        launch_t<kCosine, false>(x, d, ld, norms, order, n, k, ids_in, vals_in, ids_out, vals_out, ncand, hist, s);
+        launch_t<kCosine, false>(a, s);
       break;
     default:
       throw HipError("unknown distance");

@@ -95,12 +95,79 @@ def cpu_baseline(datasets, config, cfg, sample_n):
     }
 
 
+# The two forms the metric compares.  Both give the same graph bit for bit
+# (checked on every run below); the chosen form only computes less.
+FORMS = {
+    "plain": {"sketch": 0},
+    "chosen": {"sketch": 1},
+}
+
+
+def roofline(stats, config, form):
+    """Roofline of the step's dominant kernel from its live launch statistics."""
+    dom = max(("leaf_knn", "propagate"), key=lambda nm: stats[nm]["total_ms"])
+    s = stats[dom]
+    avg_s = s["total_ms"] / max(s["launches"], 1) / 1e3
+    per_launch = s["units"] / max(s["launches"], 1)
+    if dom == "leaf_knn":
+        achieved = per_launch / avg_s / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel": "leaf_knn_kernel",
+                "engine": "FP32 VALU (exact chains, no MFMA)"}
+    else:
+        achieved = per_launch / avg_s / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": achieved / PEAK_HBM_GBS, "traffic": None, "kernel": "propagate_kernel"}
+    # HBM bytes per launch of this kernel on this workload and form, from the
+    # committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against
+    # copy kernels)
+    prof = os.path.join(ROOT, "profiles", f"traffic_{roof['kernel']}_{config}_{form}.md")
+    if os.path.exists(prof):
+        with open(prof) as f:
+            m = re.search(r"^hbm_bytes_per_launch:\s*([0-9.eE+]+)", f.read(), re.M)
+        if m:
+            roof["traffic"] = float(m.group(1))
+    return roof
+
+
+def run_form(g, name, opts, args, n, sync):
+    for k, v in opts.items():
+        g.set_option(k, v)
+    for w in range(args.warmup):
+        g.calculate()
+        print(f"[bench] {name} warmup {w}: {g.stage_times()['total']:.1f} ms", file=sys.stderr, flush=True)
+    sync()
+    t0 = time.perf_counter()
+    stage = []
+    for s in range(args.steps):
+        g.calculate()
+        stage.append(g.stage_times())
+    sync()
+    wall = (time.perf_counter() - t0) / args.steps
+    for s, st in enumerate(stage):
+        print(f"[bench] {name} step {s}: " + " ".join(f"{k}={v:.1f}" for k, v in st.items()), file=sys.stderr,
+              flush=True)
+    stats = {nm: g.kernel_stats(nm) for nm in ("leaf_knn", "propagate", "forest")}
+    ex, rs = g.candidate_counts()
+    stats["propagate"]["candidates_examined"] = ex
+    stats["propagate"]["candidates_rescored"] = rs
+    return {
+        "value": n / wall,
+        "ms_per_step": wall * 1e3,
+        "build_s": float(np.mean([st["total"] for st in stage])) / 1e3,
+        "stages_ms": {k: float(np.mean([st[k] for st in stage])) for k in stage[0]},
+        "kernels": stats,
+        "options": opts,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="C2")
+    ap.add_argument("--form", choices=("both", "plain", "chosen"), default="both")
     ap.add_argument("--n", type=int, default=None, help="override the number of points")
     ap.add_argument("--queries", type=int, default=10000)
     ap.add_argument("--cpu-sample", type=int, default=200000)
@@ -131,70 +198,48 @@ def main():
 
     g = aknn.Graph()
     g.initialize(x, cfg["k"], cfg["trees"], cfg["leaf"], cfg["props"], metric)
-    for w in range(args.warmup):
-        g.calculate()
-        print(f"[bench] warmup {w}: {g.stage_times()['total']:.1f} ms", file=sys.stderr, flush=True)
-    sync()
-    t0 = time.perf_counter()
-    stage = []
-    for s in range(args.steps):
-        g.calculate()
-        stage.append(g.stage_times())
-    sync()
-    wall = (time.perf_counter() - t0) / args.steps
-    for s, st in enumerate(stage):
-        print(f"[bench] step {s}: " + " ".join(f"{k}={v:.1f}" for k, v in st.items()), file=sys.stderr, flush=True)
-    ev_total = float(np.mean([st["total"] for st in stage]))
-    stats = {name: g.kernel_stats(name) for name in ("leaf_knn", "propagate", "forest")}
-
-    # dominant kernel of the step: the largest summed launch time
-    dom = max(("leaf_knn", "propagate"), key=lambda nm: stats[nm]["total_ms"])
-    s = stats[dom]
-    avg_s = s["total_ms"] / max(s["launches"], 1) / 1e3
-    per_launch = s["units"] / max(s["launches"], 1)
-    if dom == "leaf_knn":
-        achieved = per_launch / avg_s / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel": "leaf_knn_kernel",
-                "engine": "FP32 VALU (exact chains, no MFMA)"}
-    else:
-        achieved = per_launch / avg_s / 1e9
-        roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": achieved / PEAK_HBM_GBS, "traffic": None, "kernel": "propagate_kernel"}
-    # HBM bytes per launch of this kernel on this workload, from the committed
-    # rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against copy kernels)
-    prof = os.path.join(ROOT, "profiles", f"traffic_{roof['kernel']}_{args.config}.md")
-    if os.path.exists(prof) and args.n is None:
-        with open(prof) as f:
-            m = re.search(r"^hbm_bytes_per_launch:\s*([0-9.eE+]+)", f.read(), re.M)
-        if m:
-            roof["traffic"] = float(m.group(1))
-
+    names = ["plain", "chosen"] if args.form == "both" else [args.form]
+    results, graphs = {}, {}
+    for name in names:
+        results[name] = run_form(g, name, FORMS[name], args, n, sync)
+        graphs[name] = g.graph()
+    head = names[-1]
+    r = results[head]
+    cfg_name = args.config if args.n is None else f"{args.config}_n{n}"
     line = {
         "metric": METRIC,
-        "value": n / wall,
+        "value": r["value"],
         "unit": "points/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3,
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": args.config, "n": n, "d": cfg["d"], "k": cfg["k"], "trees": cfg["trees"],
-                   "leaf": cfg["leaf"], "propagations": cfg["props"], "distance": cfg["metric"],
-                   "form": "plain"},
-        "build_s": ev_total / 1e3,
-        "stages_ms": {k: float(np.mean([st[k] for st in stage])) for k in stage[0]},
-        "kernels": stats,
-        "roofline": roof,
+                   "leaf": cfg["leaf"], "propagations": cfg["props"], "distance": cfg["metric"], "form": head},
+        "build_s": r["build_s"],
+        "stages_ms": r["stages_ms"],
+        "kernels": r["kernels"],
+        "roofline": roofline(r["kernels"], cfg_name, head),
     }
+    if len(names) > 1:
+        other = results["plain"]
+        other["roofline"] = roofline(other["kernels"], cfg_name, "plain")
+        line["plain"] = other
+        line["chosen_vs_plain"] = r["value"] / other["value"]
+        ga, gb = graphs["plain"], graphs["chosen"]
+        line["forms_identical"] = bool((ga[0] == gb[0]).all() and
+                                       (ga[1].view(np.uint32) == gb[1].view(np.uint32)).all())
+        print(f"[bench] chosen/plain {line['chosen_vs_plain']:.3f}, identical graphs: {line['forms_identical']}",
+              file=sys.stderr, flush=True)
     if not args.no_recall and torch is not None and torch.cuda.is_available():
         rng = np.random.default_rng(1234)
         q = np.sort(rng.choice(n, size=min(args.queries, n), replace=False))
-        ids, _ = g.graph()
+        ids, _ = graphs[head]
         xn = g.normalized()
         t1 = time.perf_counter()
         line["recall_at_k"] = exact_recall(xn, ids, cfg["k"], metric, q)

@@ -89,7 +89,12 @@ int aknn_deinitialize(aknn_handle* h);
  *                   neighbours of a neighbour whose row it already examined
  *                   in the previous round unchanged (EUCLIDEAN, TAXICAB and
  *                   COSINE with carried distances; the results are the same
- *                   as without the skip); 0: examine all */
+ *                   as without the skip); 0: examine all
+ *   "sketch"        1: propagation first screens each candidate with an FP16
+ *                   copy of the points and a proven lower bound of its exact
+ *                   FP32 key, and computes the exact key only where that bound
+ *                   does not exclude it (the results are the same); 0 (default):
+ *                   exact keys for every candidate */
 int aknn_set_option(aknn_handle* h, const char* name, int64_t value);
 
 /* The graph as the result file holds it before sorting: ids[n*k], vals[n*k]
@@ -118,6 +123,12 @@ int aknn_get_stage_times(aknn_handle* h, double* ms, int n);
  * launch stream), and algorithmic units summed over the launches (flops for
  * leaf_knn, bytes for propagate and forest). */
 int aknn_get_kernel_stats(aknn_handle* h, const char* name, int64_t* launches, double* total_ms, double* units);
+
+/* Propagation candidate counts of the last calculate(), summed over rounds:
+ * new candidates examined, and those whose exact FP32 distance was computed
+ * (all of them without "sketch").  No reference counterpart: the byte
+ * accounting of the kernel statistics above. */
+int aknn_get_candidate_counts(aknn_handle* h, double* examined, double* rescored);
 
 /* Message of the last failed call on h (empty if none). */
 const char* aknn_last_error(aknn_handle* h);

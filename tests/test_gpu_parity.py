@@ -169,3 +169,51 @@ def test_skip_settled_changes_nothing(aknn, oracle, datasets, metric, fixed):
     sa, sb = a.kernel_stats("propagate"), b.kernel_stats("propagate")
     assert sa["units"] < sb["units"]  # fewer candidate rows examined
     check_graph(oracle, a, x, 8, 2, 24, 5, metric, cosine_fixed=fixed)
+
+
+def _same_graph(a, b):
+    ga, gb = a.graph(), b.graph()
+    assert (ga[0] == gb[0]).all() and (bits(ga[1]) == bits(gb[1])).all()
+
+
+@pytest.mark.parametrize("metric,fixed", [(0, 0), (1, 0), (2, 1), (2, 0)])
+def test_sketch_changes_nothing(aknn, oracle, datasets, metric, fixed):
+    """The FP16 sketch screen of propagation gives the same graph (and the oracle's), re-scoring fewer candidates."""
+    x = datasets.gmm_c1(4000, 40, clusters=6, seed=31)[0]
+    a = gpu_run(aknn, x, 8, 2, 24, 4, metric, cosine_fixed=fixed, sketch=1)
+    b = gpu_run(aknn, x, 8, 2, 24, 4, metric, cosine_fixed=fixed, sketch=0)
+    _same_graph(a, b)
+    ex, rs = a.candidate_counts()
+    ex0, rs0 = b.candidate_counts()
+    print(f"[gpu] examined {ex:.0f} rescored {rs:.0f} (plain {rs0:.0f})", flush=True)
+    assert ex == ex0 == rs0 and 0 < rs < ex
+    check_graph(oracle, a, x, 8, 2, 24, 4, metric, cosine_fixed=fixed)
+
+
+def _adversarial(n=3000, d=300, seed=5):
+    """Exact duplicates (zero distances, id ties), values far beyond FP16 in the
+    dimensions the normalization leaves unscaled (d >= 256), and rows in the
+    FP16 subnormal range."""
+    rng = np.random.default_rng(seed)
+    x = rng.normal(size=(n, d)).astype(np.float32)
+    x[100:130] = x[100]                       # 30 identical points (fewer than the leaf bound: a
+    #                                           bucket of identical points can never split)
+    x[400:700:3] = x[400] + np.float32(1e-6) * rng.normal(size=(100, d)).astype(np.float32)
+    x[700:760, 280:] *= np.float32(1e6)       # beyond FP16 after normalization (unscaled tail)
+    x[800:900] = x[0] + np.float32(3e-5) * rng.normal(size=(100, d)).astype(np.float32)
+    return x
+
+
+@pytest.mark.parametrize("metric,fixed", [(0, 0), (1, 0), (2, 1), (2, 0)])
+def test_sketch_adversarial_inputs(aknn, oracle, metric, fixed):
+    x = _adversarial()
+    a = gpu_run(aknn, x, 10, 2, 40, 3, metric, cosine_fixed=fixed, sketch=1)
+    b = gpu_run(aknn, x, 10, 2, 40, 3, metric, cosine_fixed=fixed, sketch=0)
+    _same_graph(a, b)
+    check_graph(oracle, a, x, 10, 2, 40, 3, metric, cosine_fixed=fixed)
+
+
+def test_sketch_c1_bit_exact(aknn, oracle, datasets):
+    x = datasets.gmm_c1()[0]
+    g = gpu_run(aknn, x, 10, 10, 32, 3, 0, sketch=1)
+    check_graph(oracle, g, x, 10, 10, 32, 3, 0)
