@@ -62,7 +62,8 @@ def load():
     L.aknn_get_stage_times.argtypes = [H, _f64p, ctypes.c_int]
     L.aknn_get_kernel_stats.argtypes = [H, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64),
                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
-    L.aknn_get_candidate_counts.argtypes = [H, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    L.aknn_get_candidate_counts.argtypes = [H, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
+                                            ctypes.POINTER(ctypes.c_double)]
     L.aknn_last_error.argtypes = [H]
     L.aknn_last_error.restype = ctypes.c_char_p
     for name in EXPORTS:
@@ -159,11 +160,13 @@ class Graph:
                                                   ctypes.byref(total), ctypes.byref(units)))
         return {"launches": launches.value, "total_ms": total.value, "units": units.value}
 
-    def candidate_counts(self):
-        """(examined, rescored) propagation candidates of the last calculate()."""
+    def candidate_counts(self, stage="propagate"):
+        """(examined, rescored) candidates of the last calculate(): propagation
+        candidates, or for stage "leaf_knn" the pairs screened on the matrix
+        cores and those re-scored exactly."""
         ex = ctypes.c_double()
         rs = ctypes.c_double()
-        self._check(self._L.aknn_get_candidate_counts(self._h, ctypes.byref(ex), ctypes.byref(rs)))
+        self._check(self._L.aknn_get_candidate_counts(self._h, stage.encode(), ctypes.byref(ex), ctypes.byref(rs)))
         return ex.value, rs.value
 
     def deinitialize(self):

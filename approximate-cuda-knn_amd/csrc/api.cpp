@@ -175,6 +175,8 @@ int aknn_set_option(aknn_handle* h, const char* name, int64_t value) {
       h->params.skip_settled = value != 0;
     } else if (n == "sketch") {
       h->params.sketch = value != 0;
+    } else if (n == "leaf_mfma") {
+      h->params.leaf_mfma = value != 0;
     } else {
       return fail(h, AKNN_ERR_ARGUMENT, "unknown option " + n);
     }
@@ -235,7 +237,15 @@ int aknn_get_kernel_stats(aknn_handle* h, const char* name, int64_t* launches, d
       s = &h->engine->stats_propagate;
     else if (n == "forest")
       s = &h->engine->stats_forest;
-    else
+    else if (n == "leaf_mfma") {
+      // the screened (matrix-core) launches of the leaf k-NN: GEMM-form flops
+      const aknn::KernelStats& l = h->engine->stats_leaf_knn;
+      const int ldh = (h->engine->d() + 7) / 8 * 8;
+      if (launches) *launches = l.screened_launches;
+      if (total_ms) *total_ms = l.screened_ms;
+      if (units) *units = 2.0 * ldh * l.screened;
+      return AKNN_OK;
+    } else
       return fail(h, AKNN_ERR_ARGUMENT, "unknown kernel " + n);
     if (launches) *launches = s->launches;
     if (total_ms) *total_ms = s->total_ms;
@@ -244,11 +254,20 @@ int aknn_get_kernel_stats(aknn_handle* h, const char* name, int64_t* launches, d
   });
 }
 
-int aknn_get_candidate_counts(aknn_handle* h, double* examined, double* rescored) {
+int aknn_get_candidate_counts(aknn_handle* h, const char* stage, double* examined, double* rescored) {
   return guarded(h, [&] {
     if (!h->engine || !h->calculated) return fail(h, AKNN_ERR_STATE, "nothing calculated");
-    if (examined) *examined = h->engine->stats_propagate.examined;
-    if (rescored) *rescored = h->engine->stats_propagate.rescored;
+    if (!stage) return fail(h, AKNN_ERR_ARGUMENT, "null stage");
+    const std::string n(stage);
+    const aknn::KernelStats* s = nullptr;
+    if (n == "propagate")
+      s = &h->engine->stats_propagate;
+    else if (n == "leaf_knn")
+      s = &h->engine->stats_leaf_knn;
+    else
+      return fail(h, AKNN_ERR_ARGUMENT, "unknown stage " + n);
+    if (examined) *examined = n == "propagate" ? s->examined : s->screened;
+    if (rescored) *rescored = s->rescored;
     return AKNN_OK;
   });
 }

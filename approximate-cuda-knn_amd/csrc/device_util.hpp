@@ -75,6 +75,30 @@ __device__ __forceinline__ float row_chain(const float* a, const float* __restri
   return tot;
 }
 
+// Same chain with both rows in global memory (16-byte aligned, padded to a
+// multiple of 4): 16-byte loads from both, eight in flight per lane.
+template <int METRIC>
+__device__ __forceinline__ float row_chain_global(const float* __restrict__ a, const float* __restrict__ b, int d) {
+  float tot = 0.0f;
+  for (int c0 = 0; c0 < d; c0 += 256) {
+    const int c1 = c0 + 256 < d ? c0 + 256 : d;
+    float p = 0.0f;
+    int dd = c0;
+#pragma unroll 4
+    for (; dd + 4 <= c1; dd += 4) {
+      const float4 av = *reinterpret_cast<const float4*>(a + dd);
+      const float4 bv = *reinterpret_cast<const float4*>(b + dd);
+      p = chain_step<METRIC>(av.x, bv.x, p);
+      p = chain_step<METRIC>(av.y, bv.y, p);
+      p = chain_step<METRIC>(av.z, bv.z, p);
+      p = chain_step<METRIC>(av.w, bv.w, p);
+    }
+    for (; dd < c1; ++dd) p = chain_step<METRIC>(a[dd], b[dd], p);
+    tot = tot + p;
+  }
+  return tot;
+}
+
 // Same chain for exactly d = 4*NV <= 256 dimensions (one chunk), with the
 // whole row of b loaded into registers before the first step: one memory
 // latency per row instead of one per batch of loads.  Kernels are

@@ -69,7 +69,8 @@ T* Engine::alloc(size_t count) {
 void Engine::release() {
   for (void* p : allocs_) (void)hipFree(p);
   allocs_.clear();
-  x_ = x_raw_ = norms_ = mean_ = eps_ = nullptr;
+  x_ = x_raw_ = norms_ = mean_ = eps_ = nh_ = nullptr;
+  leaf_rescored_ = nullptr;
   xh_ = nullptr;
   gmax_ = nullptr;
   gids_[0] = gids_[1] = nullptr;
@@ -213,10 +214,17 @@ void Engine::initial_knn() {
     stats_leaf_knn.launches = 1;
     return;
   }
+  const bool mfma = leaf_mfma_active();
+  if (mfma) AKNN_HIP_TRY(hipMemsetAsync(leaf_rescored_, 0, sizeof(unsigned long long), stream_));
   for (int t = 0; t < p_.trees; ++t) {
     AKNN_HIP_TRY(hipEventRecord(launch_ev(2 * t), stream_));
-    launch_leaf_knn(p_.metric, x_, d_, ld_, norms_, perm_ + t * n_, leaves_ + t * n_, nleaves_[t], leaf_eff_, p_.k,
-                    ids, vals, stream_);
+    // the first tree meets stub rows (nothing to screen against): plain kernel
+    if (mfma && t > 0)
+      launch_leaf_mfma(p_.metric, x_, d_, ld_, xh_, ldh_, eps_, nh_, perm_ + t * n_, leaves_ + t * n_, nleaves_[t],
+                       leaf_eff_, p_.k, ids, vals, leaf_rescored_, stream_);
+    else
+      launch_leaf_knn(p_.metric, x_, d_, ld_, norms_, perm_ + t * n_, leaves_ + t * n_, nleaves_[t], leaf_eff_,
+                      p_.k, ids, vals, stream_);
     AKNN_HIP_TRY(hipEventRecord(launch_ev(2 * t + 1), stream_));
   }
   stats_leaf_knn.launches = p_.trees;
@@ -265,16 +273,18 @@ void Engine::calculate() {
   AKNN_HIP_TRY(hipSetDevice(device_));
   cur_ = 0;
   draws_.reset(p_.seed);
-  const bool sketch = p_.sketch && p_.props > 0;
+  const bool sketch = (p_.sketch && p_.props > 0) || leaf_mfma_active();
   if (sketch && !xh_) {
     xh_ = alloc<_Float16>(static_cast<size_t>(n_) * ldh_);
     eps_ = alloc<float>(n_);
+    nh_ = alloc<float>(n_);
+    leaf_rescored_ = alloc<unsigned long long>(1);
   }
   AKNN_HIP_TRY(hipMemcpyAsync(x_, x_raw_, sizeof(float) * n_ * ld_, hipMemcpyDeviceToDevice, stream_));
   AKNN_HIP_TRY(hipEventRecord(ev(0), stream_));
   launch_normalize(x_, n_, d_, ld_, mean_, gmax_, stream_);
   if (norms_) launch_norms(x_, n_, d_, ld_, norms_, stream_);
-  if (sketch) launch_sketch(x_, n_, d_, ld_, p_.metric, xh_, ldh_, eps_, stream_);
+  if (sketch) launch_sketch(x_, n_, d_, ld_, p_.metric, xh_, ldh_, eps_, nh_, stream_);
   AKNN_HIP_TRY(hipEventRecord(ev(1), stream_));
   build_forest();
   AKNN_HIP_TRY(hipEventRecord(ev(2), stream_));
@@ -290,10 +300,15 @@ void Engine::calculate() {
   AKNN_HIP_TRY(hipEventElapsedTime(&dd, ev(3), ev(6)));
   AKNN_HIP_TRY(hipEventElapsedTime(&e, ev(0), ev(6)));
   times_ = StageTimes{a, b, c, dd, e};
+  const bool mfma = leaf_mfma_active();
   for (int64_t l = 0; l < stats_leaf_knn.launches; ++l) {
     float ms = 0;
     AKNN_HIP_TRY(hipEventElapsedTime(&ms, launch_ev(2 * l), launch_ev(2 * l + 1)));
     stats_leaf_knn.total_ms += ms;
+    if (mfma && l > 0) {
+      stats_leaf_knn.screened_ms += ms;
+      stats_leaf_knn.screened_launches += 1;
+    }
   }
   std::vector<unsigned long long> nc(2 * static_cast<size_t>(std::max(p_.props, 1)));
   AKNN_HIP_TRY(hipMemcpy(nc.data(), ncand_, sizeof(unsigned long long) * nc.size(), hipMemcpyDeviceToHost));
@@ -323,13 +338,22 @@ void Engine::calculate() {
   // owner against its start node minus itself
   if (!(p_.metric == kCosine && !p_.cosine_fixed)) {
     std::vector<Leaf> host;
-    double pairs = 0;
+    double pairs = 0, screened = 0;
     for (int t = 0; t < p_.trees; ++t) {
       host.resize(nleaves_[t]);
       AKNN_HIP_TRY(hipMemcpy(host.data(), leaves_ + t * n_, sizeof(Leaf) * nleaves_[t], hipMemcpyDeviceToHost));
-      for (const Leaf& l : host) pairs += static_cast<double>(l.size) * (l.rcount - 1);
+      for (const Leaf& l : host) {
+        pairs += static_cast<double>(l.size) * (l.rcount - 1);
+        if (mfma && t > 0) screened += static_cast<double>(l.size) * (l.rcount - 1);
+      }
     }
     stats_leaf_knn.units = pairs * 3.0 * d_;
+    stats_leaf_knn.screened = screened;
+    if (mfma) {
+      unsigned long long r = 0;
+      AKNN_HIP_TRY(hipMemcpy(&r, leaf_rescored_, sizeof(r), hipMemcpyDeviceToHost));
+      stats_leaf_knn.rescored = static_cast<double>(r);
+    }
   }
 }
 

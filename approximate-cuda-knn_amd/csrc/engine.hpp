@@ -26,6 +26,7 @@ struct Params {
   int level_batch = 16;   // forest levels enqueued per host check
   int skip_settled = 1;   // propagation skips neighbours whose rows were examined last round (same results)
   int sketch = 0;         // propagation screens candidates with the FP16 sketch bound (same results)
+  int leaf_mfma = 0;      // leaf k-NN of trees after the first screens pairs on the matrix cores (same results)
 };
 
 // Per-stage times of the last calculate(), milliseconds.
@@ -39,7 +40,11 @@ struct KernelStats {
   double total_ms = 0;
   double units = 0;  // algorithmic bytes or flops summed over launches
   double examined = 0;  // propagation: new candidates examined
-  double rescored = 0;  // propagation: candidates whose exact distance was computed
+  double rescored = 0;  // propagation: candidates whose exact distance was computed;
+                        // leaf k-NN: pairs of screened trees whose exact distance was computed
+  double screened = 0;  // leaf k-NN: pairs screened on the matrix cores
+  int64_t screened_launches = 0;
+  double screened_ms = 0;
 };
 
 class Engine {
@@ -59,6 +64,7 @@ class Engine {
     p_.level_batch = p.level_batch;
     p_.skip_settled = p.skip_settled;
     p_.sketch = p.sketch;
+    p_.leaf_mfma = p.leaf_mfma;
   }
 
   void get_graph(int32_t* ids, float* vals) const;
@@ -78,6 +84,9 @@ class Engine {
   void release();
   template <typename T>
   T* alloc(size_t count);
+  bool leaf_mfma_active() const {
+    return p_.leaf_mfma && p_.metric == kEuclidean && p_.trees > 1;
+  }
   void build_forest();
   void initial_knn();
   void propagate();
@@ -99,6 +108,8 @@ class Engine {
   float* norms_ = nullptr;
   _Float16* xh_ = nullptr;  // FP16 sketch [n][ldh_], allocated on first use
   float* eps_ = nullptr;    // sketch radii [n]
+  float* nh_ = nullptr;     // sketch squared norms [n]
+  unsigned long long* leaf_rescored_ = nullptr;
   int ldh_ = 0;
   float* mean_ = nullptr;
   uint32_t* gmax_ = nullptr;

@@ -86,9 +86,17 @@ void launch_propagate(const PropagateArgs& a, hipStream_t s);
 
 // ---- FP16 sketch of the normalized points -----------------------------------
 // xh: [n][ldh] halves (ldh % 8 == 0, ldh >= d); eps[n]: upper bounds of the
-// rounding error's norm (L1 for TAXICAB, L2 otherwise).
+// rounding error's norm (L1 for TAXICAB, L2 otherwise); nh[n]: ||xh||^2.
 void launch_sketch(const float* x, int64_t n, int d, int ld, int metric, _Float16* xh, int ldh, float* eps,
-                   hipStream_t s);
+                   float* nh, hipStream_t s);
+
+// Initial leaf k-NN for one tree, EUCLIDEAN, with the candidate pairs screened
+// by half-precision MFMA dot products of the sketch and proven lower bounds;
+// the same graph as launch_leaf_knn.  rescored: optional device counter of
+// pairs whose exact distance was computed.
+void launch_leaf_mfma(int metric, const float* x, int d, int ld, const _Float16* xh, int ldh, const float* eps,
+                      const float* nh, const int32_t* perm, const Leaf* leaves, int nleaves, int max_leaf, int k,
+                      int32_t* ids, float* vals, unsigned long long* rescored, hipStream_t s);
 
 // COSINE parity initial pass: ascending-id chunks of `cap` candidates over the
 // union of all trees' start ranges, with the store-time flip after each chunk.

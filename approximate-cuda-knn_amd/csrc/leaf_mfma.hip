@@ -1,0 +1,351 @@
+// Initial leaf k-NN for one tree with half-precision MFMA screening (EUCLIDEAN).
+//
+// Same contract and results as leaf_knn.hip (the exact FP32 top-k by
+// (key, id) of the graph row merged with the tree's start-node candidates,
+// euclideandistancematrixgpu.cu:107-415), computed differently:
+//
+//   1. the 64 owners x 64 candidates block of dot products of the FP16 sketch
+//      rows (sketch.hip) is formed on the matrix cores,
+//      v_mfma_f32_32x32x16_f16 with FP32 accumulation;
+//   2. every pair gets a proven lower bound of its exact FP32 key from
+//      ||xh||^2 + ||yh||^2 - 2 xh.yh, the accumulation error, the sketch radii
+//      and the FP32 chain error; a pair whose bound is above the owner's
+//      current k-th key cannot enter and is dropped;
+//   3. the surviving pairs (near the boundary: a few percent once the graph
+//      rows hold an earlier tree's neighbours) are compacted into a list and
+//      re-scored with the exact FP32 chain, spread over all 64 lanes, then
+//      inserted by their owners exactly as in leaf_knn.hip.
+//
+// The stored keys are the exact chain's, so the graph is the same bit for bit
+// as the plain kernel's.  Lane layout of the 32x32x16 MFMA (checked on the
+// device): A[i][k] is in lane i + 32*(k/8), element k%8; B[k][j] in lane
+// j + 32*(k/8), element k%8; D[i][j] in lane j + 32*((i/4)%2), register
+// 4*(i/8) + i%4.
+#include <climits>
+
+#include "device_util.hpp"
+#include "kernels.hpp"
+
+namespace aknn {
+
+namespace {
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kTile = 64;
+
+template <int KMAX>
+__device__ __forceinline__ void list_insert(float (&lk)[KMAX], int32_t (&li)[KMAX], int k, float key, int32_t id) {
+#pragma unroll
+  for (int p = KMAX - 1; p > 0; --p) {
+    if (p < k) {
+      if (key_less(key, id, lk[p - 1], li[p - 1])) {
+        lk[p] = lk[p - 1];
+        li[p] = li[p - 1];
+      } else if (key_less(key, id, lk[p], li[p])) {
+        lk[p] = key;
+        li[p] = id;
+      }
+    }
+  }
+  if (key_less(key, id, lk[0], li[0])) {
+    lk[0] = key;
+    li[0] = id;
+  }
+}
+
+// 8 halves of a sketch row starting at half kk, or zeros past the row.
+__device__ __forceinline__ half8 piece(const _Float16* __restrict__ row, int kk, int ldh) {
+  uint4 v = make_uint4(0u, 0u, 0u, 0u);
+  if (row != nullptr && kk < ldh) v = *reinterpret_cast<const uint4*>(row + kk);
+  return __builtin_bit_cast(half8, v);
+}
+
+// Owner threshold in sketch space: the smallest T such that a pair whose
+// sketch distance exceeds T + eps_candidate has an exact key above thk
+// (T >= thk / (1 - kap) + eps_owner, rounded up).
+__device__ __forceinline__ float owner_threshold(float thk, float eo, double kap) {
+  const double t = (static_cast<double>(thk) + 1e-30) / (1.0 - kap) + static_cast<double>(eo);
+  float f = static_cast<float>(t);
+  if (static_cast<double>(f) < t) f = __uint_as_float(__float_as_uint(f) + 1u);
+  return f;
+}
+
+struct MfmaLeafArgs {
+  const float* x;
+  int d, ld;
+  const _Float16* xh;
+  int ldh;
+  const float* eps;
+  const float* nh;
+  const int32_t* perm;
+  const Leaf* leaves;
+  int nleaves, chunks_per_leaf, k;
+  int32_t* gids;
+  float* gvals;
+  double kap;    // FP32 chain error of the exact keys (sketch_bound)
+  float c1m;     // 1 - (MFMA accumulation error + evaluation margin), relative to ||xh||^2 + ||yh||^2
+  unsigned long long* rescored;  // optional: exactly re-scored pairs
+};
+
+template <int KMAX>
+__global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
+  __shared__ __attribute__((aligned(16))) int32_t oid[kTile];
+  __shared__ int32_t cid[kTile];
+  __shared__ __attribute__((aligned(16))) float o_n[kTile];
+  __shared__ __attribute__((aligned(16))) float o_t[kTile];
+  __shared__ float c_n[kTile], c_e[kTile], kb[kTile];
+  __shared__ uint64_t survm[kTile];
+  __shared__ uint16_t plist[kTile * kTile];
+
+  const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t leaf_idx = lb / a.chunks_per_leaf;
+  if (leaf_idx >= a.nleaves) return;
+  const Leaf L = a.leaves[leaf_idx];
+  const int o0 = static_cast<int>(lb % a.chunks_per_leaf) * kTile;
+  if (o0 >= L.size) return;
+  const int lane = threadIdx.x;
+  const int half = lane >> 5, col = lane & 31;
+  const int nown = L.size - o0 < kTile ? L.size - o0 : kTile;
+  const bool has_owner = lane < nown;
+  const int32_t my_id = has_owner ? a.perm[L.start + o0 + lane] : -1;
+  const int k = a.k;
+
+  float lk[KMAX];
+  int32_t li[KMAX];
+#pragma unroll
+  for (int p = 0; p < KMAX; ++p) {
+    const bool v = has_owner && p < k;
+    lk[p] = v ? a.gvals[static_cast<int64_t>(my_id) * k + p] : __builtin_inff();
+    li[p] = v ? a.gids[static_cast<int64_t>(my_id) * k + p] : INT_MAX;
+  }
+  float thk = __builtin_inff();
+  int32_t thi = INT_MAX;
+#pragma unroll
+  for (int p = 0; p < KMAX; ++p)
+    if (p == k - 1) {
+      thk = lk[p];
+      thi = li[p];
+    }
+  const float my_eps = has_owner ? a.eps[my_id] : 0.0f;
+  oid[lane] = my_id;
+  o_n[lane] = has_owner ? a.nh[my_id] : 0.0f;
+  o_t[lane] = owner_threshold(thk, my_eps, a.kap);
+  __syncthreads();
+  const int32_t oa = oid[col], ob = oid[32 + col];
+  const _Float16* arow0 = oa >= 0 ? a.xh + static_cast<int64_t>(oa) * a.ldh : nullptr;
+  const _Float16* arow1 = ob >= 0 ? a.xh + static_cast<int64_t>(ob) * a.ldh : nullptr;
+  const int steps = (a.ldh + 15) / 16;
+  unsigned long long nres = 0;
+
+  for (int cb = 0; cb < L.rcount; cb += kTile) {
+    const int ncand = L.rcount - cb < kTile ? L.rcount - cb : kTile;
+    __syncthreads();
+    const int32_t my_c = lane < ncand ? a.perm[L.rstart + cb + lane] : -1;
+    cid[lane] = my_c;
+    c_n[lane] = my_c >= 0 ? a.nh[my_c] : 0.0f;
+    c_e[lane] = my_c >= 0 ? a.eps[my_c] : 0.0f;
+    survm[lane] = 0;
+    __syncthreads();
+
+    // 1. dot products of the sketches: acc[ot][ct] is the 32x32 block of
+    //    owners 32*ot.. against candidates 32*ct..
+    const int32_t ca = cid[col], cbb = cid[32 + col];
+    const _Float16* brow0 = ca >= 0 ? a.xh + static_cast<int64_t>(ca) * a.ldh : nullptr;
+    const _Float16* brow1 = cbb >= 0 ? a.xh + static_cast<int64_t>(cbb) * a.ldh : nullptr;
+    f32x16 acc00, acc01, acc10, acc11;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc00[r] = acc01[r] = acc10[r] = acc11[r] = 0.0f;
+#pragma unroll 2
+    for (int s = 0; s < steps; ++s) {
+      const int kk = 16 * s + 8 * half;
+      const half8 A0 = piece(arow0, kk, a.ldh), A1 = piece(arow1, kk, a.ldh);
+      const half8 B0 = piece(brow0, kk, a.ldh), B1 = piece(brow1, kk, a.ldh);
+      acc00 = __builtin_amdgcn_mfma_f32_32x32x16_f16(A0, B0, acc00, 0, 0, 0);
+      acc01 = __builtin_amdgcn_mfma_f32_32x32x16_f16(A0, B1, acc01, 0, 0, 0);
+      acc10 = __builtin_amdgcn_mfma_f32_32x32x16_f16(A1, B0, acc10, 0, 0, 0);
+      acc11 = __builtin_amdgcn_mfma_f32_32x32x16_f16(A1, B1, acc11, 0, 0, 0);
+    }
+
+    // 2. screen: this lane holds candidate columns j = 32*ct + col against
+    //    owner rows i = 32*ot + 8*g + 4*half + q (register 4*g + q)
+    int32_t cjv[2];
+    float cnv[2], cev[2];
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct) {
+      cjv[ct] = cid[32 * ct + col];
+      cnv[ct] = c_n[32 * ct + col];
+      cev[ct] = c_e[32 * ct + col];
+    }
+#pragma unroll
+    for (int ot = 0; ot < 2; ++ot) {
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int i0 = 32 * ot + 8 * g + 4 * half;
+        const float4 on4 = *reinterpret_cast<const float4*>(&o_n[i0]);
+        const float4 to4 = *reinterpret_cast<const float4*>(&o_t[i0]);
+        const int4 oi4 = *reinterpret_cast<const int4*>(&oid[i0]);
+        const float on[4] = {on4.x, on4.y, on4.z, on4.w};
+        const float to[4] = {to4.x, to4.y, to4.z, to4.w};
+        const int32_t oi[4] = {oi4.x, oi4.y, oi4.z, oi4.w};
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+          const f32x16& acc = ot == 0 ? (ct == 0 ? acc00 : acc01) : (ct == 0 ? acc10 : acc11);
+          uint32_t hit = 0;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const float s_lo = __builtin_fmaf(-2.0f, acc[4 * g + q], (on[q] + cnv[ct]) * a.c1m);
+            const float t = (to[q] + cev[ct]) * 1.0000002384185791f;                // (1 + 4u)
+            const float t2 = __builtin_fmaf(t, t, 1e-30f) * 1.0000004768371582f;   // (1 + 8u)
+            hit |= static_cast<uint32_t>(oi[q] >= 0 && cjv[ct] != oi[q] && !(s_lo > t2)) << q;
+          }
+          if (cjv[ct] >= 0 && hit != 0) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+              if (hit & (1u << q))
+                atomicOr(reinterpret_cast<unsigned long long*>(&survm[i0 + q]), 1ull << (32 * ct + col));
+          }
+        }
+      }
+    }
+    __syncthreads();
+
+    // 3. survivors: owner-major pair list, exact keys on all lanes, owners
+    //    insert.  A survivor already in the owner's row (found by an earlier
+    //    tree: close points share start nodes) is dropped here; its exact key
+    //    is already stored.
+    uint64_t m = survm[lane];
+    if (m != 0) {
+      uint64_t mm = m;
+      while (mm) {
+        const int c = __ffsll(static_cast<long long>(mm)) - 1;
+        mm &= mm - 1;
+        const int32_t cj = cid[c];
+        bool dup = false;
+#pragma unroll
+        for (int pp = 0; pp < KMAX; ++pp) dup |= (pp < k) && li[pp] == cj;
+        if (dup) m &= ~(1ull << c);
+      }
+    }
+    const int cnt = __popcll(m);
+    int off = cnt;
+#pragma unroll
+    for (int dl = 1; dl < 64; dl <<= 1) {
+      const int t = __shfl_up(off, dl);
+      if (lane >= dl) off += t;
+    }
+    const int total = __shfl(off, 63);
+    off -= cnt;
+    if (total == 0) continue;
+    {
+      uint64_t mm = m;
+      int w = off;
+      while (mm) {
+        const int c = __ffsll(static_cast<long long>(mm)) - 1;
+        mm &= mm - 1;
+        plist[w++] = static_cast<uint16_t>((lane << 6) | c);
+      }
+    }
+    nres += total;
+    __syncthreads();
+    for (int base = 0; base < total; base += kTile) {
+      const int p = base + lane;
+      float key = 0.0f;
+      if (p < total) {
+        const int e = plist[p];
+        const float* xo = a.x + static_cast<int64_t>(oid[e >> 6]) * a.ld;
+        const float* xc = a.x + static_cast<int64_t>(cid[e & 63]) * a.ld;
+        key = finish_key<kEuclidean>(row_chain_global<kEuclidean>(xo, xc, a.d), 1.0f, 1.0f);
+      }
+      kb[lane] = key;
+      __syncthreads();
+      if (has_owner) {
+        const int lo = off > base ? off : base;
+        const int hi = off + cnt < base + kTile ? off + cnt : base + kTile;
+        for (int q = lo; q < hi; ++q) {
+          const int32_t cj = cid[plist[q] & 63];
+          const float kq = kb[q - base];
+          if (!key_less(kq, cj, thk, thi)) continue;
+          bool dup = false;
+#pragma unroll
+          for (int pp = 0; pp < KMAX; ++pp) dup |= (pp < k) && li[pp] == cj;
+          if (dup) continue;
+          list_insert<KMAX>(lk, li, k, kq, cj);
+#pragma unroll
+          for (int pp = 0; pp < KMAX; ++pp)
+            if (pp == k - 1) {
+              thk = lk[pp];
+              thi = li[pp];
+            }
+        }
+      }
+      __syncthreads();
+    }
+    if (has_owner && cnt > 0) o_t[lane] = owner_threshold(thk, my_eps, a.kap);
+  }
+  if (a.rescored != nullptr && lane == 0 && nres > 0) atomicAdd(a.rescored, nres);
+  if (has_owner) {
+#pragma unroll
+    for (int p = 0; p < KMAX; ++p)
+      if (p < k) {
+        a.gids[static_cast<int64_t>(my_id) * k + p] = li[p];
+        a.gvals[static_cast<int64_t>(my_id) * k + p] = lk[p];
+      }
+  }
+}
+
+template <int KMAX>
+void launch_k(const MfmaLeafArgs& a, hipStream_t s) {
+  const int64_t grid = xcd_grid(static_cast<int64_t>(a.nleaves) * a.chunks_per_leaf);
+  hipLaunchKernelGGL((leaf_mfma_kernel<KMAX>), dim3(static_cast<unsigned>(grid)), dim3(kTile), 0, s, a);
+  AKNN_HIP_TRY(hipGetLastError());
+}
+
+}  // namespace
+
+void launch_leaf_mfma(int metric, const float* x, int d, int ld, const _Float16* xh, int ldh, const float* eps,
+                      const float* nh, const int32_t* perm, const Leaf* leaves, int nleaves, int max_leaf, int k,
+                      int32_t* ids, float* vals, unsigned long long* rescored, hipStream_t s) {
+  if (nleaves <= 0) return;
+  if (metric != kEuclidean) throw HipError("MFMA leaf screening supports EUCLIDEAN");
+  if (ldh % 8 != 0 || ldh < d) throw HipError("MFMA leaf screening: malformed sketch");
+  MfmaLeafArgs a{};
+  a.x = x;
+  a.d = d;
+  a.ld = ld;
+  a.xh = xh;
+  a.ldh = ldh;
+  a.eps = eps;
+  a.nh = nh;
+  a.perm = perm;
+  a.leaves = leaves;
+  a.nleaves = nleaves;
+  a.chunks_per_leaf = (max_leaf + kTile - 1) / kTile;
+  a.k = k;
+  a.gids = ids;
+  a.gvals = vals;
+  a.kap = sketch_bound(d).kap;
+  // accumulation error of the MFMA sums over ldh terms (|G - xh.yh| <=
+  // (ldh + 16) u sqrt(nx ny) <= (ldh + 16) u (nx + ny) / 2, taken 8 times
+  // over for unspecified internal rounding), the FP32 rounding of the stored
+  // sketch norms and of the test itself
+  const double u = 1.0 / 16777216.0;
+  const double km = 8.0 * (ldh + 16) * u;
+  a.c1m = static_cast<float>(1.0 - (2.0 * km + 16.0 * u));
+  a.rescored = rescored;
+  if (k <= 8)
+    launch_k<8>(a, s);
+  else if (k <= 16)
+    launch_k<16>(a, s);
+  else if (k <= 24)
+    launch_k<24>(a, s);
+  else if (k <= 32)
+    launch_k<32>(a, s);
+  else if (k <= 64)
+    launch_k<64>(a, s);
+  else
+    throw HipError("leaf k-NN supports k <= 64");
+}
+
+}  // namespace aknn

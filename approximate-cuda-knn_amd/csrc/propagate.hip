@@ -63,10 +63,13 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
                                                         uint8_t* __restrict__ changed_out) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  // per wave: x_i (FP32, plain form) or x_i's sketch as floats (SK), the
-  // hash set, the candidate list and its counter; xi_stride is a multiple of 4
+  // per wave: x_i (FP32, plain form) or, with the sketch, x_i's sketch as
+  // floats, x_i (FP32) and two staged candidate rows; then the hash set, the
+  // candidate list and its counter.  xi_stride is a multiple of 4.
   const size_t per_wave_words = static_cast<size_t>(xi_stride) + hs_size + cand_cap + 4;
   float* xrow = reinterpret_cast<float*>(smem) + per_wave_words * wave;  // 16-byte aligned
+  float* xf = xrow + (SK ? ldh : 0);
+  float* stg = xf + ld;
   int32_t* hs = reinterpret_cast<int32_t*>(xrow + xi_stride);
   int32_t* cand = hs + hs_size;
   int32_t* cnt = cand + cand_cap;
@@ -100,10 +103,11 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
   for (int s = lane; s < hs_size; s += 64) hs[s] = -1;
   // the exact pass reads x_i from LDS in the plain form; with the sketch only
   // few candidates take it and x_i is read from global memory (cached)
-  const float* xi = SK ? x + i * ld : xrow;
+  const float* xi = SK ? xf : xrow;
   const float* xs = xrow;
   if constexpr (SK) {
     for (int dd = lane; dd < ldh; dd += 64) xrow[dd] = static_cast<float>(xh[i * ldh + dd]);
+    for (int dd = lane; dd < ld; dd += 64) xf[dd] = x[i * ld + dd];
   } else {
     for (int dd = lane; dd < d; dd += 64) xrow[dd] = x[i * ld + dd];
   }
@@ -163,18 +167,42 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
       m = w;
       if (ncand != nullptr && lane == 0) atomicAdd(ncand + 1, static_cast<unsigned long long>(m));
     }
-    for (int base = 0; base < m; base += 64) {
+    // exact pass: in the plain form one candidate per lane streams its row;
+    // with the sketch the few survivors' rows are first staged in LDS two at
+    // a time by the whole wave (one memory round trip), then chained from LDS
+    const int step = SK ? 2 : 64;
+    for (int base = 0; base < m; base += step) {
       const int ci = base + lane;
-      const int32_t c = ci < m ? cand[ci] : -1;
+      int32_t c = -1;
       float dk = __builtin_inff();
-      if (c >= 0) {
-        const float* xc = x + static_cast<int64_t>(c) * ld;
-        float t;
-        if constexpr (NV > 0)
-          t = row_chain_regs<METRIC, NV>(xi, xc);
-        else
-          t = row_chain<METRIC>(xi, xc, d);
-        dk = finish_key<METRIC>(t, ni, METRIC == kCosine ? norms[c] : 1.0f);
+      if constexpr (SK) {
+        const int nrow = m - base < 2 ? m - base : 2;
+        const int q4 = ld / 4;
+        __builtin_amdgcn_wave_barrier();
+        for (int f = lane; f < nrow * q4; f += 64) {
+          const int r = f / q4, q = f - r * q4;
+          const int32_t cr = cand[base + r];
+          *reinterpret_cast<float4*>(stg + r * ld + 4 * q) =
+              *reinterpret_cast<const float4*>(x + static_cast<int64_t>(cr) * ld + 4 * q);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (lane < nrow) {
+          c = cand[ci];
+          const float t = row_chain<METRIC>(xi, stg + lane * ld, d);
+          dk = finish_key<METRIC>(t, ni, METRIC == kCosine ? norms[c] : 1.0f);
+        }
+      } else {
+        c = ci < m ? cand[ci] : -1;
+        if (c >= 0) {
+          const float* xc = x + static_cast<int64_t>(c) * ld;
+          float t;
+          if constexpr (NV > 0)
+            t = row_chain_regs<METRIC, NV>(xi, xc);
+          else
+            t = row_chain<METRIC>(xi, xc, d);
+          dk = finish_key<METRIC>(t, ni, METRIC == kCosine ? norms[c] : 1.0f);
+        }
       }
       uint64_t surv = __ballot(c >= 0 && key_less(dk, c, thk, thi));
       while (surv) {
@@ -209,7 +237,7 @@ void launch_w(const PropagateArgs& a, hipStream_t s) {
   int hs_size = 128;
   while (hs_size < 2 * (a.k * a.k + a.k + 1)) hs_size *= 2;
   const int cand_cap = a.k * a.k;
-  const int xi_stride = SK ? a.ldh : (a.d + 3) / 4 * 4;
+  const int xi_stride = SK ? a.ldh + 3 * a.ld : (a.d + 3) / 4 * 4;
   const size_t per_wave = (static_cast<size_t>(xi_stride) + hs_size + cand_cap + 4) * 4;
   const size_t shmem = per_wave * kWavesPerBlock;
   if (shmem > 160 * 1024) throw HipError("propagation: k or d too large for LDS");
@@ -237,6 +265,7 @@ void launch_t(const PropagateArgs& a, hipStream_t s) {
 
 void launch_propagate(const PropagateArgs& a, hipStream_t s) {
   if (a.k > 64) throw HipError("propagation supports k <= 64");
+  if (a.ld % 4 != 0 || a.ld < a.d) throw HipError("propagation: row stride must be a multiple of 4 covering d");
   if (a.xh != nullptr && (a.eps == nullptr || a.ldh % 8 != 0 || a.ldh < a.d))
     throw HipError("propagation: malformed sketch");
   switch (a.metric) {

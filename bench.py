@@ -99,7 +99,7 @@ def cpu_baseline(datasets, config, cfg, sample_n):
 # (checked on every run below); the chosen form only computes less.
 FORMS = {
     "plain": {"sketch": 0},
-    "chosen": {"sketch": 1},
+    "chosen": {"sketch": 1, "leaf_mfma": 1},
 }
 
 
@@ -151,6 +151,11 @@ def run_form(g, name, opts, args, n, sync):
     ex, rs = g.candidate_counts()
     stats["propagate"]["candidates_examined"] = ex
     stats["propagate"]["candidates_rescored"] = rs
+    if opts.get("leaf_mfma"):
+        stats["leaf_mfma"] = g.kernel_stats("leaf_mfma")
+        scr, res = g.candidate_counts("leaf_knn")
+        stats["leaf_mfma"]["pairs_screened"] = scr
+        stats["leaf_mfma"]["pairs_rescored"] = res
     return {
         "value": n / wall,
         "ms_per_step": wall * 1e3,

@@ -94,7 +94,12 @@ int aknn_deinitialize(aknn_handle* h);
  *                   copy of the points and a proven lower bound of its exact
  *                   FP32 key, and computes the exact key only where that bound
  *                   does not exclude it (the results are the same); 0 (default):
- *                   exact keys for every candidate */
+ *                   exact keys for every candidate
+ *   "leaf_mfma"     1: EUCLIDEAN leaf k-NN of every tree after the first
+ *                   screens its pairs with FP16 dot products on the matrix
+ *                   cores and proven lower bounds, computing exact FP32 keys
+ *                   only for pairs the bound does not exclude (the results are
+ *                   the same); 0 (default): exact keys for every pair */
 int aknn_set_option(aknn_handle* h, const char* name, int64_t value);
 
 /* The graph as the result file holds it before sorting: ids[n*k], vals[n*k]
@@ -119,16 +124,20 @@ int aknn_get_forest(aknn_handle* h, int tree, int32_t* perm, int32_t* range_star
 int aknn_get_stage_times(aknn_handle* h, double* ms, int n);
 
 /* Kernel statistics of the last calculate(): name is "leaf_knn",
- * "propagate" or "forest".  launches, summed launch time (HIP events on the
- * launch stream), and algorithmic units summed over the launches (flops for
- * leaf_knn, bytes for propagate and forest). */
+ * "propagate", "forest" or "leaf_mfma" (the screened leaf k-NN launches
+ * alone).  launches, summed launch time (HIP events on the launch stream), and
+ * algorithmic units summed over the launches (flops for leaf_knn: 3D per
+ * pair; GEMM-form flops for leaf_mfma: 2 * padded D per pair; bytes for
+ * propagate and forest). */
 int aknn_get_kernel_stats(aknn_handle* h, const char* name, int64_t* launches, double* total_ms, double* units);
 
-/* Propagation candidate counts of the last calculate(), summed over rounds:
- * new candidates examined, and those whose exact FP32 distance was computed
- * (all of them without "sketch").  No reference counterpart: the byte
+/* Candidate counts of the last calculate().  stage "propagate": new
+ * candidates examined over all rounds, and those whose exact FP32 distance was
+ * computed (all of them without "sketch").  stage "leaf_knn": pairs screened
+ * on the matrix cores (trees after the first, with "leaf_mfma"), and those
+ * whose exact distance was computed.  No reference counterpart: the work
  * accounting of the kernel statistics above. */
-int aknn_get_candidate_counts(aknn_handle* h, double* examined, double* rescored);
+int aknn_get_candidate_counts(aknn_handle* h, const char* stage, double* examined, double* rescored);
 
 /* Message of the last failed call on h (empty if none). */
 const char* aknn_last_error(aknn_handle* h);

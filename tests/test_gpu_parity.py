@@ -217,3 +217,32 @@ def test_sketch_c1_bit_exact(aknn, oracle, datasets):
     x = datasets.gmm_c1()[0]
     g = gpu_run(aknn, x, 10, 10, 32, 3, 0, sketch=1)
     check_graph(oracle, g, x, 10, 10, 32, 3, 0)
+
+
+@pytest.mark.parametrize("n,d,k,trees,leaf,seed", [(4000, 40, 8, 4, 24, 41), (3000, 128, 20, 3, 100, 42),
+                                                   (1500, 300, 10, 3, 60, 43)])
+def test_leaf_mfma_changes_nothing(aknn, oracle, datasets, n, d, k, trees, leaf, seed):
+    """Matrix-core screening of the leaf pairs (EUCLIDEAN) gives the plain graph and the oracle's."""
+    x = datasets.gmm_c1(n, d, clusters=6, seed=seed)[0]
+    a = gpu_run(aknn, x, k, trees, leaf, 0, 0, leaf_mfma=1)
+    b = gpu_run(aknn, x, k, trees, leaf, 0, 0, leaf_mfma=0)
+    _same_graph(a, b)
+    scr, res = a.candidate_counts("leaf_knn")
+    print(f"[gpu] leaf pairs screened {scr:.0f} rescored {res:.0f}", flush=True)
+    assert scr > 0 and res < scr
+    check_graph(oracle, a, x, k, trees, leaf, 0, 0)
+
+
+def test_leaf_mfma_uniform_and_adversarial(aknn, oracle, datasets):
+    x = datasets.uniform_c2(5000, 128, seed=44)
+    a = gpu_run(aknn, x, 20, 4, 256, 1, 0, leaf_mfma=1, sketch=1)
+    check_graph(oracle, a, x, 20, 4, 256, 1, 0)
+    x = _adversarial()
+    a = gpu_run(aknn, x, 10, 3, 40, 0, 0, leaf_mfma=1)
+    check_graph(oracle, a, x, 10, 3, 40, 0, 0)
+
+
+def test_chosen_form_c1_bit_exact(aknn, oracle, datasets):
+    x = datasets.gmm_c1()[0]
+    g = gpu_run(aknn, x, 10, 10, 32, 3, 0, sketch=1, leaf_mfma=1)
+    check_graph(oracle, g, x, 10, 10, 32, 3, 0)
