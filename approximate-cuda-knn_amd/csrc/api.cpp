@@ -264,9 +264,11 @@ int aknn_get_candidate_counts(aknn_handle* h, const char* stage, double* examine
       s = &h->engine->stats_propagate;
     else if (n == "leaf_knn")
       s = &h->engine->stats_leaf_knn;
+    else if (n == "forest")
+      s = &h->engine->stats_forest;
     else
       return fail(h, AKNN_ERR_ARGUMENT, "unknown stage " + n);
-    if (examined) *examined = n == "propagate" ? s->examined : s->screened;
+    if (examined) *examined = n == "leaf_knn" ? s->screened : s->examined;
     if (rescored) *rescored = s->rescored;
     return AKNN_OK;
   });

@@ -144,6 +144,7 @@ void Engine::build_forest() {
   iters_.assign(p_.trees, 0);
   active_sum_ = 0;
   stats_forest = KernelStats{};
+  if (fb_.amb_count) AKNN_HIP_TRY(hipMemsetAsync(fb_.amb_count, 0, sizeof(unsigned long long), stream_));
   const int n32 = static_cast<int>(n_);
   for (int t = 0; t < p_.trees; ++t) {
     const size_t base = draws_.pos();
@@ -176,9 +177,18 @@ void Engine::build_forest() {
     active_sum_ += ctl_host_->active_sum;
     draws_.advance(2 + 2 * static_cast<size_t>(ctl_host_->iterations));
   }
-  // algorithmic bytes: every active point's row and its bookkeeping per level,
-  // plus the leaf array write (T * N * 8 with the leaf list)
+  // algorithmic bytes: every active point's row (or its sketch row, radius
+  // and norm, plus the exact row of each undecided point) and its bookkeeping
+  // per level, plus the leaf array write (T * N * 8 with the leaf list)
   stats_forest.units = static_cast<double>(active_sum_) * (4.0 * d_ + 16.0) + 8.0 * n_ * p_.trees;
+  if (fb_.amb_count) {
+    unsigned long long amb = 0;
+    AKNN_HIP_TRY(hipMemcpy(&amb, fb_.amb_count, sizeof(amb), hipMemcpyDeviceToHost));
+    stats_forest.examined = static_cast<double>(active_sum_);
+    stats_forest.rescored = static_cast<double>(amb);
+    stats_forest.units = static_cast<double>(active_sum_) * (2.0 * ldh_ + 8.0 + 16.0) + 4.0 * d_ * amb +
+                         8.0 * n_ * p_.trees;
+  }
 }
 
 void Engine::initial_knn() {
@@ -273,13 +283,18 @@ void Engine::calculate() {
   AKNN_HIP_TRY(hipSetDevice(device_));
   cur_ = 0;
   draws_.reset(p_.seed);
-  const bool sketch = (p_.sketch && p_.props > 0) || leaf_mfma_active();
+  const bool sketch = p_.sketch || leaf_mfma_active();
   if (sketch && !xh_) {
     xh_ = alloc<_Float16>(static_cast<size_t>(n_) * ldh_);
     eps_ = alloc<float>(n_);
     nh_ = alloc<float>(n_);
-    leaf_rescored_ = alloc<unsigned long long>(1);
+    leaf_rescored_ = alloc<unsigned long long>(2);
   }
+  fb_.xh = p_.sketch ? xh_ : nullptr;
+  fb_.ldh = ldh_;
+  fb_.eps = p_.sketch ? eps_ : nullptr;
+  fb_.nh = p_.sketch ? nh_ : nullptr;
+  fb_.amb_count = p_.sketch ? leaf_rescored_ + 1 : nullptr;
   AKNN_HIP_TRY(hipMemcpyAsync(x_, x_raw_, sizeof(float) * n_ * ld_, hipMemcpyDeviceToDevice, stream_));
   AKNN_HIP_TRY(hipEventRecord(ev(0), stream_));
   launch_normalize(x_, n_, d_, ld_, mean_, gmax_, stream_);

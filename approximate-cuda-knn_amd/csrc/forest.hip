@@ -146,6 +146,129 @@ __global__ __launch_bounds__(256) void forest_side_kernel(const float* __restric
   }
 }
 
+// Side of a point decided from its FP16 sketch, or 0 when the sketch cannot
+// decide.  The projection of the sketch, s' = sum xh_d * w_d with the exact
+// w_d = x_p2,d - x_p1,d of the chain, differs from the exact FP32 chain s by
+// at most ||w|| (eps + kap (||xh|| + eps)) (the sketch radius and the two
+// chains' rounding), so |s'| above that bound fixes the sign of s: +1 RIGHT
+// (s > 0), -1 LEFT (s < 0).  ||w|| is bounded from the same loop.
+__device__ __forceinline__ int sketch_side(const float* __restrict__ x, int ld, const _Float16* __restrict__ xh,
+                                           int ldh, float eo, float no, int32_t id, int32_t p1, int32_t p2,
+                                           const SketchBound& sb) {
+  const _Float16* hc = xh + static_cast<int64_t>(id) * ldh;
+  const float* xa = x + static_cast<int64_t>(p1) * ld;
+  const float* xb = x + static_cast<int64_t>(p2) * ld;
+  float sp = 0.0f, ww = 0.0f;
+  int dd = 0;
+#pragma unroll 4
+  for (; dd + 8 <= ld; dd += 8) {
+    const uint4 h = *reinterpret_cast<const uint4*>(hc + dd);
+    const float4 a0 = *reinterpret_cast<const float4*>(xa + dd);
+    const float4 a1 = *reinterpret_cast<const float4*>(xa + dd + 4);
+    const float4 b0 = *reinterpret_cast<const float4*>(xb + dd);
+    const float4 b1 = *reinterpret_cast<const float4*>(xb + dd + 4);
+    const float w[8] = {b0.x - a0.x, b0.y - a0.y, b0.z - a0.z, b0.w - a0.w,
+                        b1.x - a1.x, b1.y - a1.y, b1.z - a1.z, b1.w - a1.w};
+    const float hv[8] = {half_lo(h.x), half_hi(h.x), half_lo(h.y), half_hi(h.y),
+                         half_lo(h.z), half_hi(h.z), half_lo(h.w), half_hi(h.w)};
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      sp = __builtin_fmaf(hv[j], w[j], sp);
+      ww = __builtin_fmaf(w[j], w[j], ww);
+    }
+  }
+  if (dd < ld) {  // ld % 8 == 4: one group of four
+    const uint2 h = *reinterpret_cast<const uint2*>(hc + dd);
+    const float4 a0 = *reinterpret_cast<const float4*>(xa + dd);
+    const float4 b0 = *reinterpret_cast<const float4*>(xb + dd);
+    const float w[4] = {b0.x - a0.x, b0.y - a0.y, b0.z - a0.z, b0.w - a0.w};
+    const float hv[4] = {half_lo(h.x), half_hi(h.x), half_lo(h.y), half_hi(h.y)};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      sp = __builtin_fmaf(hv[j], w[j], sp);
+      ww = __builtin_fmaf(w[j], w[j], ww);
+    }
+  }
+  const double wn = sqrt(static_cast<double>(ww)) * (1.0 + sb.kap);
+  const double xn = sqrt(static_cast<double>(no)) * (1.0 + sb.kap);
+  const double e = eo;
+  const double bound = wn * (e + sb.kap * (xn + e)) + 2.0 * sb.tiny;
+  if (static_cast<double>(sp) > bound) return 1;
+  if (static_cast<double>(sp) < -bound) return -1;
+  return 0;
+}
+
+// Side kernel with the sketch: one pass decides most points from their half
+// rows; the undecided ones of a 1024-point block are queued in LDS and take
+// the exact chain together, so a straggler does not hold a whole wave.
+__global__ __launch_bounds__(256) void forest_side_sketch_kernel(
+    const float* __restrict__ x, int d, int ld, const _Float16* __restrict__ xh, int ldh,
+    const float* __restrict__ eps, const float* __restrict__ nh, SketchBound sb, const int32_t* __restrict__ act_ids,
+    const int32_t* __restrict__ act_bkt, const Bucket* __restrict__ bkt, const ForestCtl* ctl, int parity,
+    uint64_t* __restrict__ rmask, int32_t* __restrict__ wpre, int32_t* __restrict__ blk_r,
+    unsigned long long* __restrict__ amb_count) {
+  __shared__ uint64_t bits[kWordsPerBlock];
+  __shared__ int wcnt[kWordsPerBlock];
+  __shared__ int16_t amb[kSideBlock];
+  __shared__ int namb;
+  const int nact = ctl->nact[parity];
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  for (int64_t blk = blockIdx.x; blk * kSideBlock < nact; blk += gridDim.x) {
+    if (threadIdx.x == 0) namb = 0;
+    __syncthreads();
+    for (int sub = 0; sub < kSideBlock / 256; ++sub) {
+      const int ml = sub * 256 + threadIdx.x;
+      const int64_t m = blk * kSideBlock + ml;
+      bool right = false;
+      if (m < nact) {
+        const Bucket& b = bkt[act_bkt[m]];
+        const int32_t rel = static_cast<int32_t>(m) - b.astart;
+        if (rel == b.q2) {
+          right = true;  // Leaf2 writes RIGHT for p2 last (.cu:2178-2179)
+        } else if (rel != b.q1) {
+          const int32_t id = act_ids[m];
+          const int r = sketch_side(x, ld, xh, ldh, eps[id], nh[id], id, act_ids[b.astart + b.q1],
+                                    act_ids[b.astart + b.q2], sb);
+          if (r > 0) {
+            right = true;
+          } else if (r == 0) {
+            const int pos = atomicAdd(&namb, 1);
+            amb[pos] = static_cast<int16_t>(ml);
+          }
+        }
+      }
+      const uint64_t bal = __ballot(right);
+      if (lane == 0) bits[sub * 4 + wave] = bal;
+    }
+    __syncthreads();
+    const int na = namb;
+    for (int t = threadIdx.x; t < na; t += 256) {
+      const int ml = amb[t];
+      const int64_t m = blk * kSideBlock + ml;
+      const Bucket& b = bkt[act_bkt[m]];
+      if (side_is_right<0>(x, d, ld, act_ids[m], act_ids[b.astart + b.q1], act_ids[b.astart + b.q2]))
+        atomicOr(reinterpret_cast<unsigned long long*>(&bits[ml >> 6]), 1ull << (ml & 63));
+    }
+    __syncthreads();
+    if (threadIdx.x < kWordsPerBlock) {
+      rmask[blk * kWordsPerBlock + threadIdx.x] = bits[threadIdx.x];
+      wcnt[threadIdx.x] = __popcll(bits[threadIdx.x]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int acc = 0;
+      for (int w = 0; w < kWordsPerBlock; ++w) {
+        wpre[blk * kWordsPerBlock + w] = acc;
+        acc += wcnt[w];
+      }
+      blk_r[blk] = acc;
+      if (amb_count != nullptr && na > 0) atomicAdd(amb_count, static_cast<unsigned long long>(na));
+    }
+    __syncthreads();
+  }
+}
+
 // Inclusive scan over a 1024-thread block; *total receives the block sum.
 __device__ __forceinline__ int block_scan(int v, int* sh, int* total) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -338,7 +461,13 @@ void launch_forest_level(const ForestBuffers& fb, const float* x, int d, int ld,
                          int level, int32_t v1, int32_t v2, int leaf_eff, int kp1, hipStream_t s) {
   const int par = level & 1;
   const dim3 sg(clamp_grid((fb.n + kSideBlock - 1) / kSideBlock, 2048));
-  switch (row_width_class(d)) {
+  if (fb.xh != nullptr) {
+    if (fb.eps == nullptr || fb.nh == nullptr || fb.ldh % 8 != 0 || fb.ldh < d || ld % 4 != 0)
+      throw HipError("forest: malformed sketch");
+    hipLaunchKernelGGL(forest_side_sketch_kernel, sg, dim3(256), 0, s, x, d, ld, fb.xh, fb.ldh, fb.eps, fb.nh,
+                       sketch_bound(d), fb.act_ids[par], fb.act_bkt[par], fb.bkt[par], fb.ctl, par, fb.rmask, fb.wpre,
+                       fb.blk_r, fb.amb_count);
+  } else switch (row_width_class(d)) {
     case 8:
       hipLaunchKernelGGL(forest_side_kernel<8>, sg, dim3(256), 0, s, x, d, ld, fb.act_ids[par], fb.act_bkt[par],
                          fb.bkt[par], fb.ctl, par, fb.rmask, fb.wpre, fb.blk_r);

@@ -27,6 +27,13 @@ struct ForestBuffers {
   ForestCtl* ctl;
   int64_t n;
   int64_t max_buckets;
+  // FP16 sketch for the side test (nullable: exact chains for every point;
+  // the partition is the same), and a counter of points it left undecided
+  const _Float16* xh = nullptr;
+  int ldh = 0;
+  const float* eps = nullptr;
+  const float* nh = nullptr;
+  unsigned long long* amb_count = nullptr;
 };
 void launch_forest_init(const ForestBuffers& fb, int32_t* perm, Leaf* leaves, int64_t n, int32_t q1, int32_t q2,
                         int leaf_eff, int kp1, hipStream_t s);

@@ -151,6 +151,10 @@ def run_form(g, name, opts, args, n, sync):
     ex, rs = g.candidate_counts()
     stats["propagate"]["candidates_examined"] = ex
     stats["propagate"]["candidates_rescored"] = rs
+    if opts.get("sketch"):
+        ex, rs = g.candidate_counts("forest")
+        stats["forest"]["side_tests"] = ex
+        stats["forest"]["side_tests_exact"] = rs
     if opts.get("leaf_mfma"):
         stats["leaf_mfma"] = g.kernel_stats("leaf_mfma")
         scr, res = g.candidate_counts("leaf_knn")

@@ -90,11 +90,13 @@ int aknn_deinitialize(aknn_handle* h);
  *                   in the previous round unchanged (EUCLIDEAN, TAXICAB and
  *                   COSINE with carried distances; the results are the same
  *                   as without the skip); 0: examine all
- *   "sketch"        1: propagation first screens each candidate with an FP16
- *                   copy of the points and a proven lower bound of its exact
- *                   FP32 key, and computes the exact key only where that bound
- *                   does not exclude it (the results are the same); 0 (default):
- *                   exact keys for every candidate
+ *   "sketch"        1: an FP16 copy of the points with proven error bounds
+ *                   decides most forest side tests (the exact chain runs where
+ *                   the bound leaves the sign open), and propagation screens
+ *                   each candidate with a lower bound of its exact FP32 key,
+ *                   computing the exact key only where the bound does not
+ *                   exclude it (the results are the same); 0 (default): exact
+ *                   chains everywhere
  *   "leaf_mfma"     1: EUCLIDEAN leaf k-NN of every tree after the first
  *                   screens its pairs with FP16 dot products on the matrix
  *                   cores and proven lower bounds, computing exact FP32 keys
@@ -135,8 +137,10 @@ int aknn_get_kernel_stats(aknn_handle* h, const char* name, int64_t* launches, d
  * candidates examined over all rounds, and those whose exact FP32 distance was
  * computed (all of them without "sketch").  stage "leaf_knn": pairs screened
  * on the matrix cores (trees after the first, with "leaf_mfma"), and those
- * whose exact distance was computed.  No reference counterpart: the work
- * accounting of the kernel statistics above. */
+ * whose exact distance was computed.  stage "forest": side tests of active
+ * points over all levels, and those the sketch left to the exact chain (0
+ * without "sketch").  No reference counterpart: the work accounting of the
+ * kernel statistics above. */
 int aknn_get_candidate_counts(aknn_handle* h, const char* stage, double* examined, double* rescored);
 
 /* Message of the last failed call on h (empty if none). */

@@ -210,13 +210,15 @@ def test_sketch_adversarial_inputs(aknn, oracle, metric, fixed):
     a = gpu_run(aknn, x, 10, 2, 40, 3, metric, cosine_fixed=fixed, sketch=1)
     b = gpu_run(aknn, x, 10, 2, 40, 3, metric, cosine_fixed=fixed, sketch=0)
     _same_graph(a, b)
-    check_graph(oracle, a, x, 10, 2, 40, 3, metric, cosine_fixed=fixed)
+    xn = check_graph(oracle, a, x, 10, 2, 40, 3, metric, cosine_fixed=fixed)
+    check_forest(oracle, a, xn, 2, 40, 10)
 
 
 def test_sketch_c1_bit_exact(aknn, oracle, datasets):
     x = datasets.gmm_c1()[0]
     g = gpu_run(aknn, x, 10, 10, 32, 3, 0, sketch=1)
-    check_graph(oracle, g, x, 10, 10, 32, 3, 0)
+    xn = check_graph(oracle, g, x, 10, 10, 32, 3, 0)
+    check_forest(oracle, g, xn, 10, 32, 10)
 
 
 @pytest.mark.parametrize("n,d,k,trees,leaf,seed", [(4000, 40, 8, 4, 24, 41), (3000, 128, 20, 3, 100, 42),
@@ -246,3 +248,15 @@ def test_chosen_form_c1_bit_exact(aknn, oracle, datasets):
     x = datasets.gmm_c1()[0]
     g = gpu_run(aknn, x, 10, 10, 32, 3, 0, sketch=1, leaf_mfma=1)
     check_graph(oracle, g, x, 10, 10, 32, 3, 0)
+
+
+@pytest.mark.parametrize("d,metric", [(20, 0), (128, 0), (300, 1), (33, 2)])
+def test_forest_sketch_same_trees(aknn, oracle, datasets, d, metric):
+    """The sketch-decided side tests give the oracle's trees (leaf order, start ranges, rand() use)."""
+    x = datasets.gmm_c1(3000, d, clusters=5, seed=50 + d)[0]
+    g = gpu_run(aknn, x, 8, 3, 20, 1, metric, sketch=1, cosine_fixed=1)
+    ex, rs = g.candidate_counts("forest")
+    print(f"[gpu] side tests {ex:.0f} exact {rs:.0f}", flush=True)
+    assert ex > 0 and rs < ex
+    xn = check_graph(oracle, g, x, 8, 3, 20, 1, metric, cosine_fixed=1)
+    check_forest(oracle, g, xn, 3, 20, 8)
