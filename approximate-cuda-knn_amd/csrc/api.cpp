@@ -177,6 +177,8 @@ int aknn_set_option(aknn_handle* h, const char* name, int64_t value) {
       h->params.sketch = value != 0;
     } else if (n == "leaf_mfma") {
       h->params.leaf_mfma = value != 0;
+    } else if (n == "sketch_int8") {
+      h->params.sketch_int8 = value != 0;
     } else {
       return fail(h, AKNN_ERR_ARGUMENT, "unknown option " + n);
     }

@@ -69,7 +69,9 @@ T* Engine::alloc(size_t count) {
 void Engine::release() {
   for (void* p : allocs_) (void)hipFree(p);
   allocs_.clear();
-  x_ = x_raw_ = norms_ = mean_ = eps_ = nh_ = nullptr;
+  x_ = x_raw_ = norms_ = mean_ = eps_ = nh_ = s8_ = e8_ = nullptr;
+  q8_ = nullptr;
+  a8_ = nullptr;
   leaf_rescored_ = nullptr;
   xh_ = nullptr;
   gmax_ = nullptr;
@@ -101,6 +103,7 @@ void Engine::set_data(const float* x_rowmajor, int64_t n, int d, const Params& p
   d_ = d;
   ld_ = (d + 3) / 4 * 4;
   ldh_ = (d + 7) / 8 * 8;
+  ld8_ = (d + 15) / 16 * 16;
   leaf_eff_ = std::max(p.leaf, 1);
 
   x_ = alloc<float>(static_cast<size_t>(n) * ld_);
@@ -252,10 +255,17 @@ void Engine::propagate() {
     a.x = x_;
     a.d = d_;
     a.ld = ld_;
-    if (p_.sketch) {
+    if (propagation_screened()) {
       a.xh = xh_;
       a.ldh = ldh_;
       a.eps = eps_;
+      if (p_.sketch_int8 && p_.metric == kEuclidean) {
+        a.q8 = q8_;
+        a.ld8 = ld8_;
+        a.s8 = s8_;
+        a.a8 = a8_;
+        a.e8 = e8_;
+      }
     }
     a.norms = norms_;
     a.order = perm_;
@@ -290,6 +300,13 @@ void Engine::calculate() {
     nh_ = alloc<float>(n_);
     leaf_rescored_ = alloc<unsigned long long>(2);
   }
+  const bool sketch8 = propagation_screened() && p_.sketch_int8 && p_.metric == kEuclidean && p_.props > 0;
+  if (sketch8 && !q8_) {
+    q8_ = alloc<int8_t>(static_cast<size_t>(n_) * ld8_);
+    s8_ = alloc<float>(n_);
+    a8_ = alloc<int32_t>(n_);
+    e8_ = alloc<float>(n_);
+  }
   fb_.xh = p_.sketch ? xh_ : nullptr;
   fb_.ldh = ldh_;
   fb_.eps = p_.sketch ? eps_ : nullptr;
@@ -300,6 +317,7 @@ void Engine::calculate() {
   launch_normalize(x_, n_, d_, ld_, mean_, gmax_, stream_);
   if (norms_) launch_norms(x_, n_, d_, ld_, norms_, stream_);
   if (sketch) launch_sketch(x_, n_, d_, ld_, p_.metric, xh_, ldh_, eps_, nh_, stream_);
+  if (sketch8) launch_sketch8(x_, n_, d_, ld_, q8_, ld8_, s8_, a8_, e8_, stream_);
   AKNN_HIP_TRY(hipEventRecord(ev(1), stream_));
   build_forest();
   AKNN_HIP_TRY(hipEventRecord(ev(2), stream_));
@@ -338,13 +356,15 @@ void Engine::calculate() {
     // sketch row and radius (2 ldh + 4) plus 4D for each exactly re-scored one
     double per_point = 16.0 * k + 4.0 * k * k + 4.0 * d_;
     double cand = 4.0 * d_ * nc[2 * r];
-    if (sketch) {
-      per_point += 2.0 * ldh_ + 4.0;
-      cand = (2.0 * ldh_ + 4.0) * nc[2 * r] + 4.0 * d_ * nc[2 * r + 1];
+    if (propagation_screened()) {
+      const bool i8 = p_.sketch_int8 && p_.metric == kEuclidean;
+      const double row = i8 ? ld8_ + 12.0 : 2.0 * ldh_ + 4.0;  // sketch row + per-row scalars
+      per_point += row;
+      cand = row * nc[2 * r] + 4.0 * d_ * nc[2 * r + 1];
     }
     stats_propagate.units += static_cast<double>(n_) * per_point + cand;
     stats_propagate.examined += static_cast<double>(nc[2 * r]);
-    stats_propagate.rescored += static_cast<double>(sketch ? nc[2 * r + 1] : nc[2 * r]);
+    stats_propagate.rescored += static_cast<double>(propagation_screened() ? nc[2 * r + 1] : nc[2 * r]);
   }
   stats_forest.launches = 0;
   for (int t = 0; t < p_.trees; ++t) stats_forest.launches += iters_[t];

@@ -27,6 +27,7 @@ struct Params {
   int skip_settled = 1;   // propagation skips neighbours whose rows were examined last round (same results)
   int sketch = 0;         // propagation screens candidates with the FP16 sketch bound (same results)
   int leaf_mfma = 0;      // leaf k-NN of trees after the first screens pairs on the matrix cores (same results)
+  int sketch_int8 = 0;    // with sketch, EUCLIDEAN propagation screens with the int8 sketch (same results)
 };
 
 // Per-stage times of the last calculate(), milliseconds.
@@ -65,6 +66,7 @@ class Engine {
     p_.skip_settled = p.skip_settled;
     p_.sketch = p.sketch;
     p_.leaf_mfma = p.leaf_mfma;
+    p_.sketch_int8 = p.sketch_int8;
   }
 
   void get_graph(int32_t* ids, float* vals) const;
@@ -84,6 +86,9 @@ class Engine {
   void release();
   template <typename T>
   T* alloc(size_t count);
+  // propagation screens with the sketch (not in COSINE parity mode, whose
+  // flipped stored values give no distance threshold)
+  bool propagation_screened() const { return p_.sketch && !(p_.metric == kCosine && !p_.cosine_fixed); }
   bool leaf_mfma_active() const {
     return p_.leaf_mfma && p_.metric == kEuclidean && p_.trees > 1;
   }
@@ -109,6 +114,11 @@ class Engine {
   _Float16* xh_ = nullptr;  // FP16 sketch [n][ldh_], allocated on first use
   float* eps_ = nullptr;    // sketch radii [n]
   float* nh_ = nullptr;     // sketch squared norms [n]
+  int8_t* q8_ = nullptr;    // int8 sketch [n][ld8_], scales, sums of squares, radii
+  float* s8_ = nullptr;
+  int32_t* a8_ = nullptr;
+  float* e8_ = nullptr;
+  int ld8_ = 0;
   unsigned long long* leaf_rescored_ = nullptr;
   int ldh_ = 0;
   float* mean_ = nullptr;

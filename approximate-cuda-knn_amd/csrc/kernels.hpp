@@ -78,6 +78,12 @@ struct PropagateArgs {
   const _Float16* xh = nullptr;
   int ldh = 0;
   const float* eps = nullptr;
+  // int8 sketch (EUCLIDEAN; used instead of xh when set)
+  const int8_t* q8 = nullptr;
+  int ld8 = 0;
+  const float* s8 = nullptr;
+  const int32_t* a8 = nullptr;
+  const float* e8 = nullptr;
   const float* norms = nullptr;
   const int32_t* order = nullptr;
   int64_t n = 0;
@@ -96,6 +102,11 @@ void launch_propagate(const PropagateArgs& a, hipStream_t s);
 // rounding error's norm (L1 for TAXICAB, L2 otherwise); nh[n]: ||xh||^2.
 void launch_sketch(const float* x, int64_t n, int d, int ld, int metric, _Float16* xh, int ldh, float* eps,
                    float* nh, hipStream_t s);
+
+// Int8 sketch: q8 [n][ld8] (ld8 % 16 == 0, ld8 >= d), per-row scale s8,
+// a8 = sum q^2, e8 = upper bound of ||x - s8 q8||_2.
+void launch_sketch8(const float* x, int64_t n, int d, int ld, int8_t* q8, int ld8, float* s8, int32_t* a8, float* e8,
+                    hipStream_t s);
 
 // Initial leaf k-NN for one tree, EUCLIDEAN, with the candidate pairs screened
 // by half-precision MFMA dot products of the sketch and proven lower bounds;

@@ -34,6 +34,7 @@ namespace {
 
 constexpr int kWavesPerBlock = 4;
 
+
 // Open-addressing set of ids in LDS (-1 = empty), linear probing.  The table
 // has at least twice as many slots as ids ever inserted, so probing ends.
 __device__ __forceinline__ bool hash_insert(int32_t* hs, uint32_t size, int32_t key) {
@@ -47,10 +48,14 @@ __device__ __forceinline__ bool hash_insert(int32_t* hs, uint32_t size, int32_t 
   }
 }
 
-template <int METRIC, bool FLIP, int NV, bool SK>
+// SKM: 0 exact pass only, 1 FP16 sketch screen, 2 int8 sketch screen.
+template <int METRIC, bool FLIP, int SKM>
 __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict__ x, int d, int ld,
                                                         const _Float16* __restrict__ xh, int ldh,
                                                         const float* __restrict__ eps, SketchBound sb,
+                                                        const int8_t* __restrict__ q8, int ld8,
+                                                        const float* __restrict__ s8, const int32_t* __restrict__ a8,
+                                                        const float* __restrict__ e8,
                                                         const float* __restrict__ norms,
                                                         const int32_t* __restrict__ order, int64_t n, int k,
                                                         int hs_size, int cand_cap, int xi_stride,
@@ -68,7 +73,8 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
   // candidate list and its counter.  xi_stride is a multiple of 4.
   const size_t per_wave_words = static_cast<size_t>(xi_stride) + hs_size + cand_cap + 4;
   float* xrow = reinterpret_cast<float*>(smem) + per_wave_words * wave;  // 16-byte aligned
-  float* xf = xrow + (SK ? ldh : 0);
+  constexpr bool SK = SKM > 0;
+  float* xf = xrow + (SKM == 1 ? ldh : (SKM == 2 ? ld8 / 4 : 0));
   float* stg = xf + ld;
   int32_t* hs = reinterpret_cast<int32_t*>(xrow + xi_stride);
   int32_t* cand = hs + hs_size;
@@ -100,73 +106,165 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
     for (int l = 0; l < k; ++l) was_prev |= __shfl(pid, l) == id;
     if (lane < k && id >= 0 && was_prev && changed_prev[id] == 0) expand = false;
   }
-  for (int s = lane; s < hs_size; s += 64) hs[s] = -1;
-  // the exact pass reads x_i from LDS in the plain form; with the sketch only
-  // few candidates take it and x_i is read from global memory (cached)
+  if constexpr (!SK) {
+    for (int s = lane; s < hs_size; s += 64) hs[s] = -1;
+  } else {
+    if (lane < k) hs[lane] = id;  // i's row, for the screen's membership test
+  }
   const float* xi = SK ? xf : xrow;
   const float* xs = xrow;
-  if constexpr (SK) {
+  if constexpr (SKM == 1) {
     for (int dd = lane; dd < ldh; dd += 64) xrow[dd] = static_cast<float>(xh[i * ldh + dd]);
+    for (int dd = lane; dd < ld; dd += 64) xf[dd] = x[i * ld + dd];
+  } else if constexpr (SKM == 2) {
+    for (int dd = lane; dd < ld8 / 4; dd += 64)
+      reinterpret_cast<int32_t*>(xrow)[dd] = reinterpret_cast<const int32_t*>(q8 + i * ld8)[dd];
     for (int dd = lane; dd < ld; dd += 64) xf[dd] = x[i * ld + dd];
   } else {
     for (int dd = lane; dd < d; dd += 64) xrow[dd] = x[i * ld + dd];
   }
-  if (lane == 0) *cnt = 0;
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  const uint32_t hsz = static_cast<uint32_t>(hs_size);
-  if (lane == 0) hash_insert(hs, hsz, static_cast<int32_t>(i));
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  if (lane < k && id >= 0) hash_insert(hs, hsz, id);
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  // two-hop ids (.cu:1078-1085)
   const int kk = k * k;
-  for (int f = lane; f < kk + (64 - kk % 64) % 64; f += 64) {
-    const int hi = f / k, lo = f - (f / k) * k;
-    const int32_t nb = __shfl(id, hi < 64 ? hi : 0);
-    const bool ex = __shfl(static_cast<int>(expand), hi < 64 ? hi : 0) != 0;
-    if (f < kk && nb >= 0 && ex) {
-      const int32_t c = ids_in[static_cast<int64_t>(nb) * k + lo];
-      if (c >= 0 && hash_insert(hs, hsz, c)) {
-        const int pos = atomicAdd(cnt, 1);
-        cand[pos] = c;
+  int m = 0;     // candidates for the exact pass
+  int nnew = 0;  // new candidates (.cu:1117: none -> the row is copied, no flip)
+  float thk = __shfl(key, k - 1);
+  int32_t thi = __shfl(id, k - 1);
+  const float ni = METRIC == kCosine ? norms[i] : 1.0f;
+  if constexpr (!SK) {
+    // two-hop ids (.cu:1078-1085), deduplicated in the LDS hash set that
+    // holds i and i's row
+    if (lane == 0) *cnt = 0;
+    const uint32_t hsz = static_cast<uint32_t>(hs_size);
+    if (lane == 0) hash_insert(hs, hsz, static_cast<int32_t>(i));
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (lane < k && id >= 0) hash_insert(hs, hsz, id);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    for (int f = lane; f < kk + (64 - kk % 64) % 64; f += 64) {
+      const int hi = f / k, lo = f - (f / k) * k;
+      const int32_t nb = __shfl(id, hi < 64 ? hi : 0);
+      const bool ex = __shfl(static_cast<int>(expand), hi < 64 ? hi : 0) != 0;
+      if (f < kk && nb >= 0 && ex) {
+        const int32_t c = ids_in[static_cast<int64_t>(nb) * k + lo];
+        if (c >= 0 && hash_insert(hs, hsz, c)) {
+          const int pos = atomicAdd(cnt, 1);
+          cand[pos] = c;
+        }
       }
     }
-  }
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  int m = *cnt;
-  if (ncand != nullptr && lane == 0) atomicAdd(ncand, static_cast<unsigned long long>(m));
-
-  bool changed = false;
-  if (m > 0) {
-    float thk = __shfl(key, k - 1);
-    int32_t thi = __shfl(id, k - 1);
-    const float ni = METRIC == kCosine ? norms[i] : 1.0f;
-    if constexpr (SK) {
-      // sketch pass: survivors are written back to cand[] in order; slot
-      // w + rank <= base + lane was already read by this batch
-      const float ei = eps[i];
-      int w = 0;
-      for (int base = 0; base < m; base += 64) {
-        const int ci = base + lane;
-        const int32_t c = ci < m ? cand[ci] : -1;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    m = *cnt;
+    nnew = m;
+    if (ncand != nullptr && lane == 0) atomicAdd(ncand, static_cast<unsigned long long>(m));
+  } else {
+    // Two-hop ids gathered and screened in one pass, without a set: an id
+    // equal to i or in i's row is dropped by a compare against the row; a
+    // repeated id is screened again and, if it survives twice, refused at
+    // insertion (it is then in the list, or not below the k-th entry).  The
+    // pool's top-k is the same as with deduplication.  Survivors are
+    // appended to cand[] in ballot order.
+    {
+      const float ei = SKM == 2 ? e8[i] : eps[i];
+      const double si = SKM == 2 ? static_cast<double>(s8[i]) : 0.0;
+      const double ti = SKM == 2 ? si * si * static_cast<double>(a8[i]) : 0.0;
+      int w = 0, nex = 0;
+      // the ids of up to eight batches are loaded first (all in flight), so
+      // each batch's sketch loads do not wait behind its id load
+      constexpr int kIdBatches = 8;
+      for (int f1 = 0; f1 < kk; f1 += 64 * kIdBatches) {
+      int32_t cids[kIdBatches];
+#pragma unroll
+      for (int b = 0; b < kIdBatches; ++b) {
+        const int f = f1 + 64 * b + lane;
+        const int hi = f / k, lo = f - hi * k;
+        const int src = hi < 64 ? hi : 0;
+        const int32_t nb = __shfl(id, src);
+        const bool ex = __shfl(static_cast<int>(expand), src) != 0;
+        cids[b] = -1;
+        if (f < kk && nb >= 0 && ex) cids[b] = ids_in[static_cast<int64_t>(nb) * k + lo];
+      }
+#pragma unroll
+      for (int b = 0; b < kIdBatches; ++b) {
+        int32_t c = cids[b];
+        if (c == static_cast<int32_t>(i)) c = -1;
+        if (c >= 0) {
+          bool in_row = false;
+          for (int l = 0; l < k; ++l) in_row |= hs[l] == c;
+          if (in_row) c = -1;
+        }
+        cids[b] = c;
+      }
+      for (int b = 0; b < kIdBatches && f1 + 64 * b < kk; ++b) {
+        int32_t c = -1;
+#pragma unroll
+        for (int bb = 0; bb < kIdBatches; ++bb)
+          if (bb == b) c = cids[bb];
+        nex += __popcll(__ballot(c >= 0));
         bool pass = false;
         if (c >= 0) {
-          const float p = sketch_chain<METRIC>(xs, xh + static_cast<int64_t>(c) * ldh, ldh);
-          const double lb =
-              sketch_lower_bound<METRIC>(p, ei, eps[c], ni, METRIC == kCosine ? norms[c] : 1.0f, sb);
+          double lb;
+          if constexpr (SKM == 2) {
+            // exact integer dot product of the int8 rows, eight 16-byte loads in flight
+            const int4* qc = reinterpret_cast<const int4*>(q8 + static_cast<int64_t>(c) * ld8);
+            const int4* qi = reinterpret_cast<const int4*>(xrow);
+            int dot = 0;
+            const int pieces = ld8 / 16;
+            int g = 0;
+            for (; g + 8 <= pieces; g += 8) {
+              int4 v[8];
+#pragma unroll
+              for (int j = 0; j < 8; ++j) v[j] = qc[g + j];
+#pragma unroll
+              for (int j = 0; j < 8; ++j) {
+                const int4 u = qi[g + j];
+                dot = __builtin_amdgcn_sdot4(v[j].x, u.x, dot, false);
+                dot = __builtin_amdgcn_sdot4(v[j].y, u.y, dot, false);
+                dot = __builtin_amdgcn_sdot4(v[j].z, u.z, dot, false);
+                dot = __builtin_amdgcn_sdot4(v[j].w, u.w, dot, false);
+              }
+            }
+            for (; g < pieces; ++g) {
+              const int4 v = qc[g], u = qi[g];
+              dot = __builtin_amdgcn_sdot4(v.x, u.x, dot, false);
+              dot = __builtin_amdgcn_sdot4(v.y, u.y, dot, false);
+              dot = __builtin_amdgcn_sdot4(v.z, u.z, dot, false);
+              dot = __builtin_amdgcn_sdot4(v.w, u.w, dot, false);
+            }
+            // ||s_i q_i - s_c q_c||^2 exactly from the integers, evaluated in
+            // FP64 with its rounding taken off; then the radii (triangle
+            // inequality) and the exact chain's error, as sketch_lower_bound
+            const double sc = static_cast<double>(s8[c]);
+            const double tc = sc * sc * static_cast<double>(a8[c]);
+            const double sq = ti + tc - 2.0 * si * sc * static_cast<double>(dot);
+            const double slo = sq - 1e-12 * (ti + tc) - sb.tiny;
+            const double r = fmax(0.0, sqrt(fmax(0.0, slo)) - static_cast<double>(ei) - static_cast<double>(e8[c]));
+            const double k1 = 1.0 - sb.kap;
+            lb = sqrt(fmax(0.0, r * r * k1 - sb.tiny)) * k1;
+          } else {
+            const float p = sketch_chain<METRIC>(xs, xh + static_cast<int64_t>(c) * ldh, ldh);
+            lb = sketch_lower_bound<METRIC>(p, ei, eps[c], ni, METRIC == kCosine ? norms[c] : 1.0f, sb);
+          }
           pass = !(lb > static_cast<double>(thk));
         }
         const uint64_t bal = __ballot(pass);
         if (pass) cand[w + __popcll(bal & ((1ull << lane) - 1))] = c;
         w += __popcll(bal);
       }
+      }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
       m = w;
-      if (ncand != nullptr && lane == 0) atomicAdd(ncand + 1, static_cast<unsigned long long>(m));
+      nnew = nex;
+      if (ncand != nullptr && lane == 0) {
+        atomicAdd(ncand, static_cast<unsigned long long>(nex));
+        atomicAdd(ncand + 1, static_cast<unsigned long long>(m));
+      }
     }
+  }
+
+  bool changed = false;
+  if (nnew > 0) {
     // exact pass: in the plain form one candidate per lane streams its row;
     // with the sketch the few survivors' rows are first staged in LDS two at
     // a time by the whole wave (one memory round trip), then chained from LDS
@@ -196,12 +294,7 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
         c = ci < m ? cand[ci] : -1;
         if (c >= 0) {
           const float* xc = x + static_cast<int64_t>(c) * ld;
-          float t;
-          if constexpr (NV > 0)
-            t = row_chain_regs<METRIC, NV>(xi, xc);
-          else
-            t = row_chain<METRIC>(xi, xc, d);
-          dk = finish_key<METRIC>(t, ni, METRIC == kCosine ? norms[c] : 1.0f);
+          dk = finish_key<METRIC>(row_chain<METRIC>(xi, xc, d), ni, METRIC == kCosine ? norms[c] : 1.0f);
         }
       }
       uint64_t surv = __ballot(c >= 0 && key_less(dk, c, thk, thi));
@@ -210,7 +303,7 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
         surv &= surv - 1;
         const float nk = __shfl(dk, src);
         const int32_t nid = __shfl(c, src);
-        if (key_less(nk, nid, thk, thi)) {
+        if (key_less(nk, nid, thk, thi) && !(SK && __ballot(lane < k && id == nid) != 0)) {
           changed = true;
           wave_list_insert(key, id, k, nk, nid);
           thk = __shfl(key, k - 1);
@@ -231,20 +324,21 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
   }
 }
 
-template <int METRIC, bool FLIP, int NV, bool SK>
+template <int METRIC, bool FLIP, int SKM>
 void launch_w(const PropagateArgs& a, hipStream_t s) {
   // at most 1 + k + k*k ids enter the set: at least twice that many slots
   int hs_size = 128;
   while (hs_size < 2 * (a.k * a.k + a.k + 1)) hs_size *= 2;
+  if (SKM > 0) hs_size = 64;  // i's row only: the screened path has no set
   const int cand_cap = a.k * a.k;
-  const int xi_stride = SK ? a.ldh + 3 * a.ld : (a.d + 3) / 4 * 4;
+  const int xi_stride = SKM == 1 ? a.ldh + 3 * a.ld : (SKM == 2 ? a.ld8 / 4 + 3 * a.ld : (a.d + 3) / 4 * 4);
   const size_t per_wave = (static_cast<size_t>(xi_stride) + hs_size + cand_cap + 4) * 4;
   const size_t shmem = per_wave * kWavesPerBlock;
   if (shmem > 160 * 1024) throw HipError("propagation: k or d too large for LDS");
   const int64_t blocks = xcd_grid((a.n + kWavesPerBlock - 1) / kWavesPerBlock);
   const SketchBound sb = sketch_bound(a.d);
-  hipLaunchKernelGGL((propagate_kernel<METRIC, FLIP, NV, SK>), dim3(static_cast<unsigned>(blocks)), dim3(256), shmem,
-                     s, a.x, a.d, a.ld, a.xh, a.ldh, a.eps, sb, a.norms, a.order, a.n, a.k, hs_size, cand_cap,
+  hipLaunchKernelGGL((propagate_kernel<METRIC, FLIP, SKM>), dim3(static_cast<unsigned>(blocks)), dim3(256), shmem,
+                     s, a.x, a.d, a.ld, a.xh, a.ldh, a.eps, sb, a.q8, a.ld8, a.s8, a.a8, a.e8, a.norms, a.order, a.n, a.k, hs_size, cand_cap,
                      xi_stride, a.ids_in, a.vals_in, a.ids_out, a.vals_out, a.ncand,
                      FLIP ? nullptr : a.hist.ids_prev, a.hist.changed_prev, a.hist.changed_out);
   AKNN_HIP_TRY(hipGetLastError());
@@ -255,10 +349,23 @@ void launch_w(const PropagateArgs& a, hipStream_t s) {
 // 196 VGPRs outweigh the deeper per-lane prefetch).
 template <int METRIC, bool FLIP>
 void launch_t(const PropagateArgs& a, hipStream_t s) {
+  // COSINE parity: carried entries are keyed by their flipped stored values,
+  // so the k-th entry is no distance threshold and the screen would keep
+  // nearly every candidate; the exact path with its set runs instead
+  if constexpr (FLIP) {
+    launch_w<METRIC, FLIP, 0>(a, s);
+    return;
+  }
+  if constexpr (METRIC == kEuclidean) {
+    if (a.q8 != nullptr) {
+      launch_w<METRIC, FLIP, 2>(a, s);
+      return;
+    }
+  }
   if (a.xh != nullptr)
-    launch_w<METRIC, FLIP, 0, true>(a, s);
+    launch_w<METRIC, FLIP, 1>(a, s);
   else
-    launch_w<METRIC, FLIP, 0, false>(a, s);
+    launch_w<METRIC, FLIP, 0>(a, s);
 }
 
 }  // namespace
@@ -266,6 +373,8 @@ void launch_t(const PropagateArgs& a, hipStream_t s) {
 void launch_propagate(const PropagateArgs& a, hipStream_t s) {
   if (a.k > 64) throw HipError("propagation supports k <= 64");
   if (a.ld % 4 != 0 || a.ld < a.d) throw HipError("propagation: row stride must be a multiple of 4 covering d");
+  if (a.q8 != nullptr && (a.s8 == nullptr || a.a8 == nullptr || a.e8 == nullptr || a.ld8 % 16 != 0 || a.ld8 < a.d))
+    throw HipError("propagation: malformed int8 sketch");
   if (a.xh != nullptr && (a.eps == nullptr || a.ldh % 8 != 0 || a.ldh < a.d))
     throw HipError("propagation: malformed sketch");
   switch (a.metric) {

@@ -97,6 +97,10 @@ int aknn_deinitialize(aknn_handle* h);
  *                   computing the exact key only where the bound does not
  *                   exclude it (the results are the same); 0 (default): exact
  *                   chains everywhere
+ *   "sketch_int8"   1: with "sketch", EUCLIDEAN propagation screens with an
+ *                   int8 copy (per-point scale, exact integer dot products,
+ *                   half the bytes) instead of the FP16 one; 0 (default): FP16
+ *                   (its tighter bound re-scores fewer candidates)
  *   "leaf_mfma"     1: EUCLIDEAN leaf k-NN of every tree after the first
  *                   screens its pairs with FP16 dot products on the matrix
  *                   cores and proven lower bounds, computing exact FP32 keys
