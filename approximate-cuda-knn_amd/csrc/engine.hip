@@ -233,8 +233,8 @@ void Engine::initial_knn() {
     AKNN_HIP_TRY(hipEventRecord(launch_ev(2 * t), stream_));
     // the first tree meets stub rows (nothing to screen against): plain kernel
     if (mfma && t > 0)
-      launch_leaf_mfma(p_.metric, x_, d_, ld_, xh_, ldh_, eps_, nh_, perm_ + t * n_, leaves_ + t * n_, nleaves_[t],
-                       leaf_eff_, p_.k, ids, vals, leaf_rescored_, stream_);
+      launch_leaf_mfma(p_.metric, x_, d_, ld_, xh_, ldh_, eps_, nh_, norms_, perm_ + t * n_, leaves_ + t * n_,
+                       nleaves_[t], leaf_eff_, p_.k, ids, vals, leaf_rescored_, stream_);
     else
       launch_leaf_knn(p_.metric, x_, d_, ld_, norms_, perm_ + t * n_, leaves_ + t * n_, nleaves_[t], leaf_eff_,
                       p_.k, ids, vals, stream_);

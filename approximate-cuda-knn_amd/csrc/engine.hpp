@@ -90,7 +90,7 @@ class Engine {
   // flipped stored values give no distance threshold)
   bool propagation_screened() const { return p_.sketch && !(p_.metric == kCosine && !p_.cosine_fixed); }
   bool leaf_mfma_active() const {
-    return p_.leaf_mfma && p_.metric == kEuclidean && p_.trees > 1;
+    return p_.leaf_mfma && p_.trees > 1 && (p_.metric == kEuclidean || (p_.metric == kCosine && p_.cosine_fixed));
   }
   void build_forest();
   void initial_knn();

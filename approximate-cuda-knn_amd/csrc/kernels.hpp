@@ -108,13 +108,14 @@ void launch_sketch(const float* x, int64_t n, int d, int ld, int metric, _Float1
 void launch_sketch8(const float* x, int64_t n, int d, int ld, int8_t* q8, int ld8, float* s8, int32_t* a8, float* e8,
                     hipStream_t s);
 
-// Initial leaf k-NN for one tree, EUCLIDEAN, with the candidate pairs screened
-// by half-precision MFMA dot products of the sketch and proven lower bounds;
-// the same graph as launch_leaf_knn.  rescored: optional device counter of
-// pairs whose exact distance was computed.
+// Initial leaf k-NN for one tree, EUCLIDEAN or COSINE with carried distances,
+// with the candidate pairs screened by half-precision MFMA dot products of the
+// sketch and proven lower bounds; the same graph as launch_leaf_knn.
+// norms: the COSINE squared-norm chains.  rescored: optional device counter
+// of pairs whose exact distance was computed.
 void launch_leaf_mfma(int metric, const float* x, int d, int ld, const _Float16* xh, int ldh, const float* eps,
-                      const float* nh, const int32_t* perm, const Leaf* leaves, int nleaves, int max_leaf, int k,
-                      int32_t* ids, float* vals, unsigned long long* rescored, hipStream_t s);
+                      const float* nh, const float* norms, const int32_t* perm, const Leaf* leaves, int nleaves,
+                      int max_leaf, int k, int32_t* ids, float* vals, unsigned long long* rescored, hipStream_t s);
 
 // COSINE parity initial pass: ascending-id chunks of `cap` candidates over the
 // union of all trees' start ranges, with the store-time flip after each chunk.

@@ -62,14 +62,28 @@ __device__ __forceinline__ half8 piece(const _Float16* __restrict__ row, int kk,
   return __builtin_bit_cast(half8, v);
 }
 
-// Owner threshold in sketch space: the smallest T such that a pair whose
-// sketch distance exceeds T + eps_candidate has an exact key above thk
-// (T >= thk / (1 - kap) + eps_owner, rounded up).
+// Owner threshold.  EUCLIDEAN: the smallest T such that a pair whose sketch
+// distance exceeds T + eps_candidate has an exact key above thk
+// (T >= thk / (1 - kap) + eps_owner, rounded up).  COSINE: the largest T such
+// that a pair whose exact quotient tot / den is below T has an exact key
+// 1 - quotient above thk (T <= 1 - thk - 2^-20, rounded down; the 2^-20 covers
+// the FP32 rounding of the quotient and of the stored 1 - quotient).
+template <int METRIC>
 __device__ __forceinline__ float owner_threshold(float thk, float eo, double kap) {
-  const double t = (static_cast<double>(thk) + 1e-30) / (1.0 - kap) + static_cast<double>(eo);
-  float f = static_cast<float>(t);
-  if (static_cast<double>(f) < t) f = __uint_as_float(__float_as_uint(f) + 1u);
-  return f;
+  if constexpr (METRIC == kEuclidean) {
+    const double t = (static_cast<double>(thk) + 1e-30) / (1.0 - kap) + static_cast<double>(eo);
+    float f = static_cast<float>(t);
+    if (static_cast<double>(f) < t) f = __uint_as_float(__float_as_uint(f) + 1u);
+    return f;
+  } else {
+    const double t = 1.0 - static_cast<double>(thk) - 9.5367431640625e-07;
+    float f = static_cast<float>(t);
+    if (static_cast<double>(f) > t) {  // next float down
+      const uint32_t b = __float_as_uint(f);
+      f = f > 0.0f ? __uint_as_float(b - 1u) : (f == 0.0f ? -1.401298464e-45f : __uint_as_float(b + 1u));
+    }
+    return f;
+  }
 }
 
 struct MfmaLeafArgs {
@@ -79,6 +93,7 @@ struct MfmaLeafArgs {
   int ldh;
   const float* eps;
   const float* nh;
+  const float* norms;  // COSINE: the exact squared-norm chains
   const int32_t* perm;
   const Leaf* leaves;
   int nleaves, chunks_per_leaf, k;
@@ -86,16 +101,31 @@ struct MfmaLeafArgs {
   float* gvals;
   double kap;    // FP32 chain error of the exact keys (sketch_bound)
   float c1m;     // 1 - (MFMA accumulation error + evaluation margin), relative to ||xh||^2 + ||yh||^2
+  float kmf;     // COSINE: MFMA accumulation error relative to ||xh|| ||yh||
+  float kabf;    // COSINE: kap + 4u
   unsigned long long* rescored;  // optional: exactly re-scored pairs
 };
 
-template <int KMAX>
+// COSINE pair terms: up(G) = G + kM (A+eo)(B+ec) + eo B + (A+eo) ec
+// + (kap + 4u) A B bounds the exact dot chain from the sketch dot G (MFMA
+// error, sketch radii, exact chain error, quotient rounding), with
+// A >= ||x_o||, B >= ||x_c|| from the exact squared norms.
+__device__ __forceinline__ float norm_upper(float aa, double kap, double tiny) {
+  const double t = sqrt((static_cast<double>(aa) + tiny) * (1.0 + 2.0 * kap));
+  float f = static_cast<float>(t);
+  if (static_cast<double>(f) < t) f = __uint_as_float(__float_as_uint(f) + 1u);
+  return f;
+}
+
+template <int METRIC, int KMAX>
 __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
   __shared__ __attribute__((aligned(16))) int32_t oid[kTile];
   __shared__ int32_t cid[kTile];
-  __shared__ __attribute__((aligned(16))) float o_n[kTile];
-  __shared__ __attribute__((aligned(16))) float o_t[kTile];
-  __shared__ float c_n[kTile], c_e[kTile], kb[kTile];
+  __shared__ __attribute__((aligned(16))) float o_n[kTile];  // E: ||xh||^2; C: A (norm bound)
+  __shared__ __attribute__((aligned(16))) float o_t[kTile];  // owner threshold
+  __shared__ __attribute__((aligned(16))) float o_e[kTile];  // C: sketch radius
+  __shared__ __attribute__((aligned(16))) float o_s[kTile];  // C: sqrtf(aa), as the exact key uses
+  __shared__ float c_n[kTile], c_e[kTile], c_s[kTile], kb[kTile];
   __shared__ uint64_t survm[kTile];
   __shared__ uint16_t plist[kTile * kTile];
 
@@ -129,9 +159,18 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
       thi = li[p];
     }
   const float my_eps = has_owner ? a.eps[my_id] : 0.0f;
+  const double tiny = 1e-40;
   oid[lane] = my_id;
-  o_n[lane] = has_owner ? a.nh[my_id] : 0.0f;
-  o_t[lane] = owner_threshold(thk, my_eps, a.kap);
+  if constexpr (METRIC == kEuclidean) {
+    o_n[lane] = has_owner ? a.nh[my_id] : 0.0f;
+  } else {
+    const float aa = has_owner ? a.norms[my_id] : 0.0f;
+    o_n[lane] = norm_upper(aa, a.kap, tiny);
+    o_s[lane] = sqrtf(aa);
+    o_e[lane] = my_eps;
+  }
+  o_t[lane] = owner_threshold<METRIC>(thk, my_eps, a.kap);
+  const float my_aa = METRIC == kCosine && has_owner ? a.norms[my_id] : 1.0f;
   __syncthreads();
   const int32_t oa = oid[col], ob = oid[32 + col];
   const _Float16* arow0 = oa >= 0 ? a.xh + static_cast<int64_t>(oa) * a.ldh : nullptr;
@@ -144,7 +183,13 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
     __syncthreads();
     const int32_t my_c = lane < ncand ? a.perm[L.rstart + cb + lane] : -1;
     cid[lane] = my_c;
-    c_n[lane] = my_c >= 0 ? a.nh[my_c] : 0.0f;
+    if constexpr (METRIC == kEuclidean) {
+      c_n[lane] = my_c >= 0 ? a.nh[my_c] : 0.0f;
+    } else {
+      const float bb = my_c >= 0 ? a.norms[my_c] : 0.0f;
+      c_n[lane] = norm_upper(bb, a.kap, tiny);
+      c_s[lane] = sqrtf(bb);
+    }
     c_e[lane] = my_c >= 0 ? a.eps[my_c] : 0.0f;
     survm[lane] = 0;
     __syncthreads();
@@ -171,12 +216,13 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
     // 2. screen: this lane holds candidate columns j = 32*ct + col against
     //    owner rows i = 32*ot + 8*g + 4*half + q (register 4*g + q)
     int32_t cjv[2];
-    float cnv[2], cev[2];
+    float cnv[2], cev[2], csv[2];
 #pragma unroll
     for (int ct = 0; ct < 2; ++ct) {
       cjv[ct] = cid[32 * ct + col];
       cnv[ct] = c_n[32 * ct + col];
       cev[ct] = c_e[32 * ct + col];
+      csv[ct] = METRIC == kCosine ? c_s[32 * ct + col] : 0.0f;
     }
 #pragma unroll
     for (int ot = 0; ot < 2; ++ot) {
@@ -189,16 +235,37 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
         const float on[4] = {on4.x, on4.y, on4.z, on4.w};
         const float to[4] = {to4.x, to4.y, to4.z, to4.w};
         const int32_t oi[4] = {oi4.x, oi4.y, oi4.z, oi4.w};
+        float oe[4] = {0.0f, 0.0f, 0.0f, 0.0f}, os[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if constexpr (METRIC == kCosine) {
+          const float4 oe4 = *reinterpret_cast<const float4*>(&o_e[i0]);
+          const float4 os4 = *reinterpret_cast<const float4*>(&o_s[i0]);
+          oe[0] = oe4.x; oe[1] = oe4.y; oe[2] = oe4.z; oe[3] = oe4.w;
+          os[0] = os4.x; os[1] = os4.y; os[2] = os4.z; os[3] = os4.w;
+        }
 #pragma unroll
         for (int ct = 0; ct < 2; ++ct) {
           const f32x16& acc = ot == 0 ? (ct == 0 ? acc00 : acc01) : (ct == 0 ? acc10 : acc11);
           uint32_t hit = 0;
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
-            const float s_lo = __builtin_fmaf(-2.0f, acc[4 * g + q], (on[q] + cnv[ct]) * a.c1m);
-            const float t = (to[q] + cev[ct]) * 1.0000002384185791f;                // (1 + 4u)
-            const float t2 = __builtin_fmaf(t, t, 1e-30f) * 1.0000004768371582f;   // (1 + 8u)
-            hit |= static_cast<uint32_t>(oi[q] >= 0 && cjv[ct] != oi[q] && !(s_lo > t2)) << q;
+            bool keep;
+            if constexpr (METRIC == kEuclidean) {
+              const float s_lo = __builtin_fmaf(-2.0f, acc[4 * g + q], (on[q] + cnv[ct]) * a.c1m);
+              const float t = (to[q] + cev[ct]) * 1.0000002384185791f;                // (1 + 4u)
+              const float t2 = __builtin_fmaf(t, t, 1e-30f) * 1.0000004768371582f;   // (1 + 8u)
+              keep = !(s_lo > t2);
+            } else {
+              // reject when up(G) + rounding margin < T_o * den * (1 - 4u)
+              const float G = acc[4 * g + q];
+              const float A = on[q], B = cnv[ct], eo = oe[q], ec = cev[ct];
+              const float Ao = A + eo;
+              const float P = a.kmf * Ao * (B + ec) + eo * B + Ao * ec + a.kabf * A * B;
+              const float lhs = G + P + 4.76837158203125e-07f * (__builtin_fabsf(G) + P + A * B) + 1e-30f;
+              const float den = os[q] * csv[ct];
+              const float rhs = to[q] * den * 0.99999976158142090f - 1e-30f;  // (1 - 4u)
+              keep = !(den > 0.0f && den <= 3.4e38f && lhs < rhs);
+            }
+            hit |= static_cast<uint32_t>(oi[q] >= 0 && cjv[ct] != oi[q] && keep) << q;
           }
           if (cjv[ct] >= 0 && hit != 0) {
 #pragma unroll
@@ -256,7 +323,11 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
         const int e = plist[p];
         const float* xo = a.x + static_cast<int64_t>(oid[e >> 6]) * a.ld;
         const float* xc = a.x + static_cast<int64_t>(cid[e & 63]) * a.ld;
-        key = finish_key<kEuclidean>(row_chain_global<kEuclidean>(xo, xc, a.d), 1.0f, 1.0f);
+        if constexpr (METRIC == kEuclidean)
+          key = finish_key<kEuclidean>(row_chain_global<kEuclidean>(xo, xc, a.d), 1.0f, 1.0f);
+        else
+          key = finish_key<kCosine>(row_chain_global<kCosine>(xo, xc, a.d), a.norms[oid[e >> 6]],
+                                    a.norms[cid[e & 63]]);
       }
       kb[lane] = key;
       __syncthreads();
@@ -282,7 +353,7 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
       }
       __syncthreads();
     }
-    if (has_owner && cnt > 0) o_t[lane] = owner_threshold(thk, my_eps, a.kap);
+    if (has_owner && cnt > 0) o_t[lane] = owner_threshold<METRIC>(thk, my_eps, a.kap);
   }
   if (a.rescored != nullptr && lane == 0 && nres > 0) atomicAdd(a.rescored, nres);
   if (has_owner) {
@@ -295,20 +366,37 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
   }
 }
 
-template <int KMAX>
+template <int METRIC, int KMAX>
 void launch_k(const MfmaLeafArgs& a, hipStream_t s) {
   const int64_t grid = xcd_grid(static_cast<int64_t>(a.nleaves) * a.chunks_per_leaf);
-  hipLaunchKernelGGL((leaf_mfma_kernel<KMAX>), dim3(static_cast<unsigned>(grid)), dim3(kTile), 0, s, a);
+  hipLaunchKernelGGL((leaf_mfma_kernel<METRIC, KMAX>), dim3(static_cast<unsigned>(grid)), dim3(kTile), 0, s, a);
   AKNN_HIP_TRY(hipGetLastError());
+}
+
+template <int METRIC>
+void launch_m(const MfmaLeafArgs& a, int k, hipStream_t s) {
+  if (k <= 8)
+    launch_k<METRIC, 8>(a, s);
+  else if (k <= 16)
+    launch_k<METRIC, 16>(a, s);
+  else if (k <= 24)
+    launch_k<METRIC, 24>(a, s);
+  else if (k <= 32)
+    launch_k<METRIC, 32>(a, s);
+  else if (k <= 64)
+    launch_k<METRIC, 64>(a, s);
+  else
+    throw HipError("leaf k-NN supports k <= 64");
 }
 
 }  // namespace
 
 void launch_leaf_mfma(int metric, const float* x, int d, int ld, const _Float16* xh, int ldh, const float* eps,
-                      const float* nh, const int32_t* perm, const Leaf* leaves, int nleaves, int max_leaf, int k,
-                      int32_t* ids, float* vals, unsigned long long* rescored, hipStream_t s) {
+                      const float* nh, const float* norms, const int32_t* perm, const Leaf* leaves, int nleaves,
+                      int max_leaf, int k, int32_t* ids, float* vals, unsigned long long* rescored, hipStream_t s) {
   if (nleaves <= 0) return;
-  if (metric != kEuclidean) throw HipError("MFMA leaf screening supports EUCLIDEAN");
+  if (metric != kEuclidean && metric != kCosine) throw HipError("MFMA leaf screening supports EUCLIDEAN and COSINE");
+  if (metric == kCosine && norms == nullptr) throw HipError("MFMA leaf screening: COSINE needs the norms");
   if (ldh % 8 != 0 || ldh < d) throw HipError("MFMA leaf screening: malformed sketch");
   MfmaLeafArgs a{};
   a.x = x;
@@ -318,6 +406,7 @@ void launch_leaf_mfma(int metric, const float* x, int d, int ld, const _Float16*
   a.ldh = ldh;
   a.eps = eps;
   a.nh = nh;
+  a.norms = norms;
   a.perm = perm;
   a.leaves = leaves;
   a.nleaves = nleaves;
@@ -333,19 +422,13 @@ void launch_leaf_mfma(int metric, const float* x, int d, int ld, const _Float16*
   const double u = 1.0 / 16777216.0;
   const double km = 8.0 * (ldh + 16) * u;
   a.c1m = static_cast<float>(1.0 - (2.0 * km + 16.0 * u));
+  a.kmf = static_cast<float>(km * (1.0 + 1e-6));
+  a.kabf = static_cast<float>((a.kap + 4.0 * u) * (1.0 + 1e-6));
   a.rescored = rescored;
-  if (k <= 8)
-    launch_k<8>(a, s);
-  else if (k <= 16)
-    launch_k<16>(a, s);
-  else if (k <= 24)
-    launch_k<24>(a, s);
-  else if (k <= 32)
-    launch_k<32>(a, s);
-  else if (k <= 64)
-    launch_k<64>(a, s);
+  if (metric == kEuclidean)
+    launch_m<kEuclidean>(a, k, s);
   else
-    throw HipError("leaf k-NN supports k <= 64");
+    launch_m<kCosine>(a, k, s);
 }
 
 }  // namespace aknn

@@ -241,6 +241,20 @@ def test_leaf_mfma_changes_nothing(aknn, oracle, datasets, n, d, k, trees, leaf,
     check_graph(oracle, a, x, k, trees, leaf, 0, 0)
 
 
+@pytest.mark.parametrize("n,d,k,trees,leaf,seed", [(4000, 40, 8, 4, 24, 45), (2500, 96, 30, 3, 100, 46),
+                                                   (1500, 300, 10, 3, 60, 47)])
+def test_leaf_mfma_cosine_fixed(aknn, oracle, datasets, n, d, k, trees, leaf, seed):
+    """Matrix-core screening of COSINE leaf pairs (carried distances): the plain graph and the oracle's."""
+    x = datasets.gmm_c1(n, d, clusters=6, seed=seed)[0]
+    a = gpu_run(aknn, x, k, trees, leaf, 0, 2, cosine_fixed=1, leaf_mfma=1)
+    b = gpu_run(aknn, x, k, trees, leaf, 0, 2, cosine_fixed=1, leaf_mfma=0)
+    _same_graph(a, b)
+    scr, res = a.candidate_counts("leaf_knn")
+    print(f"[gpu] cosine leaf pairs screened {scr:.0f} rescored {res:.0f}", flush=True)
+    assert scr > 0 and res < scr
+    check_graph(oracle, a, x, k, trees, leaf, 0, 2, cosine_fixed=1)
+
+
 def test_leaf_mfma_uniform_and_adversarial(aknn, oracle, datasets):
     x = datasets.uniform_c2(5000, 128, seed=44)
     a = gpu_run(aknn, x, 20, 4, 256, 1, 0, leaf_mfma=1, sketch=1)
@@ -248,6 +262,8 @@ def test_leaf_mfma_uniform_and_adversarial(aknn, oracle, datasets):
     x = _adversarial()
     a = gpu_run(aknn, x, 10, 3, 40, 0, 0, leaf_mfma=1)
     check_graph(oracle, a, x, 10, 3, 40, 0, 0)
+    a = gpu_run(aknn, x, 10, 3, 40, 1, 2, leaf_mfma=1, sketch=1, cosine_fixed=1)
+    check_graph(oracle, a, x, 10, 3, 40, 1, 2, cosine_fixed=1)
 
 
 def test_chosen_form_c1_bit_exact(aknn, oracle, datasets):
