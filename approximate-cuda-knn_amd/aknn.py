@@ -21,7 +21,7 @@ EXPORTS = (
     "aknn_create", "aknn_destroy", "aknn_set_data_file", "aknn_set_results_file", "aknn_initialize",
     "aknn_initialize_host", "aknn_calculate", "aknn_save_result", "aknn_deinitialize", "aknn_set_option",
     "aknn_get_graph", "aknn_get_normalized", "aknn_get_forest", "aknn_get_stage_times",
-    "aknn_get_kernel_stats", "aknn_get_candidate_counts", "aknn_last_error",
+    "aknn_get_kernel_stats", "aknn_get_candidate_counts", "aknn_exact_knn", "aknn_recall", "aknn_last_error",
 )
 
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -64,6 +64,9 @@ def load():
                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     L.aknn_get_candidate_counts.argtypes = [H, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                             ctypes.POINTER(ctypes.c_double)]
+    _i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+    L.aknn_exact_knn.argtypes = [H, _i64p, ctypes.c_int64, ctypes.c_int, _i32p, _f64p]
+    L.aknn_recall.argtypes = [H, _i64p, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)]
     L.aknn_last_error.argtypes = [H]
     L.aknn_last_error.restype = ctypes.c_char_p
     for name in EXPORTS:
@@ -168,6 +171,21 @@ class Graph:
         rs = ctypes.c_double()
         self._check(self._L.aknn_get_candidate_counts(self._h, stage.encode(), ctypes.byref(ex), ctypes.byref(rs)))
         return ex.value, rs.value
+
+    def exact_knn(self, queries, kq):
+        """FP64 exact k-NN of the query indices: (ids [nq][kq], dist [nq][kq])."""
+        q = np.ascontiguousarray(queries, dtype=np.int64)
+        ids = np.empty((len(q), kq), np.int32)
+        dist = np.empty((len(q), kq), np.float64)
+        self._check(self._L.aknn_exact_knn(self._h, q, len(q), kq, ids, dist))
+        return ids, dist
+
+    def recall(self, queries):
+        """recall@k of the current graph against FP64 exact k-NN of the queries."""
+        q = np.ascontiguousarray(queries, dtype=np.int64)
+        r = ctypes.c_double()
+        self._check(self._L.aknn_recall(self._h, q, len(q), ctypes.byref(r)))
+        return r.value
 
     def deinitialize(self):
         self._check(self._L.aknn_deinitialize(self._h))

@@ -276,6 +276,24 @@ int aknn_get_candidate_counts(aknn_handle* h, const char* stage, double* examine
   });
 }
 
+int aknn_exact_knn(aknn_handle* h, const int64_t* queries, int64_t nq, int kq, int32_t* ids, double* dist) {
+  return guarded(h, [&] {
+    if (!h->engine || !h->calculated) return fail(h, AKNN_ERR_STATE, "nothing calculated");
+    if (nq < 0 || (nq > 0 && (!queries || !ids || !dist))) return fail(h, AKNN_ERR_ARGUMENT, "null array");
+    h->engine->exact_knn(queries, nq, kq, ids, dist);
+    return AKNN_OK;
+  });
+}
+
+int aknn_recall(aknn_handle* h, const int64_t* queries, int64_t nq, double* recall) {
+  return guarded(h, [&] {
+    if (!h->engine || !h->calculated) return fail(h, AKNN_ERR_STATE, "nothing calculated");
+    if (nq < 0 || !recall || (nq > 0 && !queries)) return fail(h, AKNN_ERR_ARGUMENT, "null array");
+    *recall = h->engine->recall(queries, nq);
+    return AKNN_OK;
+  });
+}
+
 const char* aknn_last_error(aknn_handle* h) { return h ? h->error.c_str() : "null handle"; }
 
 }  // extern "C"

@@ -403,6 +403,75 @@ void Engine::get_normalized(float* x) const {
                            hipMemcpyDeviceToHost));
 }
 
+namespace {
+// Device scratch for one call, freed on every exit.
+struct DeviceScratch {
+  std::vector<void*> p;
+  template <typename T>
+  T* get(size_t count) {
+    void* q = nullptr;
+    AKNN_HIP_TRY(hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)));
+    p.push_back(q);
+    return static_cast<T*>(q);
+  }
+  ~DeviceScratch() {
+    for (void* q : p) (void)hipFree(q);
+  }
+};
+}  // namespace
+
+void Engine::exact_knn(const int64_t* queries, int64_t nq, int kq, int32_t* ids, double* dist) {
+  if (nq < 0 || kq < 1 || kq > 64 || kq >= n_) throw std::invalid_argument("exact k-NN needs 1 <= k <= 64, k < N");
+  for (int64_t q = 0; q < nq; ++q)
+    if (queries[q] < 0 || queries[q] >= n_) throw std::invalid_argument("query index out of range");
+  if (nq == 0) return;
+  AKNN_HIP_TRY(hipSetDevice(device_));
+  DeviceScratch sc;
+  int64_t* dq = sc.get<int64_t>(nq);
+  int32_t* di = sc.get<int32_t>(static_cast<size_t>(nq) * kq);
+  double* dd = sc.get<double>(static_cast<size_t>(nq) * kq);
+  double* sq = nullptr;
+  if (p_.metric == kCosine) {
+    sq = sc.get<double>(n_);
+    launch_sq64(x_, n_, d_, ld_, sq, stream_);
+  }
+  AKNN_HIP_TRY(hipMemcpyAsync(dq, queries, sizeof(int64_t) * nq, hipMemcpyHostToDevice, stream_));
+  launch_exact_knn(p_.metric, x_, n_, d_, ld_, sq, dq, nq, kq, di, dd, stream_);
+  AKNN_HIP_TRY(hipMemcpyAsync(ids, di, sizeof(int32_t) * nq * kq, hipMemcpyDeviceToHost, stream_));
+  AKNN_HIP_TRY(hipMemcpyAsync(dist, dd, sizeof(double) * nq * kq, hipMemcpyDeviceToHost, stream_));
+  AKNN_HIP_TRY(hipStreamSynchronize(stream_));
+}
+
+double Engine::recall(const int64_t* queries, int64_t nq) {
+  const int k = p_.k;
+  for (int64_t q = 0; q < nq; ++q)
+    if (queries[q] < 0 || queries[q] >= n_) throw std::invalid_argument("query index out of range");
+  if (nq <= 0) return 0.0;
+  AKNN_HIP_TRY(hipSetDevice(device_));
+  DeviceScratch sc;
+  int64_t* dq = sc.get<int64_t>(nq);
+  int32_t* di = sc.get<int32_t>(static_cast<size_t>(nq) * k);
+  double* dd = sc.get<double>(static_cast<size_t>(nq) * k);
+  double* kth = sc.get<double>(nq);
+  unsigned long long* hits = sc.get<unsigned long long>(1);
+  double* sq = nullptr;
+  if (p_.metric == kCosine) {
+    sq = sc.get<double>(n_);
+    launch_sq64(x_, n_, d_, ld_, sq, stream_);
+  }
+  AKNN_HIP_TRY(hipMemcpyAsync(dq, queries, sizeof(int64_t) * nq, hipMemcpyHostToDevice, stream_));
+  AKNN_HIP_TRY(hipMemsetAsync(hits, 0, sizeof(unsigned long long), stream_));
+  launch_exact_knn(p_.metric, x_, n_, d_, ld_, sq, dq, nq, k, di, dd, stream_);
+  // the k-th exact distance of query q is dd[q * k + k - 1]
+  AKNN_HIP_TRY(hipMemcpy2DAsync(kth, sizeof(double), dd + (k - 1), sizeof(double) * k, sizeof(double), nq,
+                                hipMemcpyDeviceToDevice, stream_));
+  launch_recall(p_.metric, x_, d_, ld_, sq, dq, nq, kth, gids_[cur_], k, hits, stream_);
+  unsigned long long h = 0;
+  AKNN_HIP_TRY(hipMemcpyAsync(&h, hits, sizeof(h), hipMemcpyDeviceToHost, stream_));
+  AKNN_HIP_TRY(hipStreamSynchronize(stream_));
+  return static_cast<double>(h) / (static_cast<double>(nq) * k);
+}
+
 void Engine::get_forest(int tree, int32_t* perm, int32_t* rs, int32_t* rc) {
   if (tree < 0 || tree >= p_.trees || tree >= static_cast<int>(nleaves_.size()))
     throw std::invalid_argument("no such tree (run calculate first)");

@@ -72,6 +72,10 @@ class Engine {
   void get_graph(int32_t* ids, float* vals) const;
   void get_normalized(float* x) const;
   void get_forest(int tree, int32_t* perm, int32_t* rs, int32_t* rc);
+  // FP64 exact k-NN of query points over the normalized points (host arrays)
+  void exact_knn(const int64_t* queries, int64_t nq, int kq, int32_t* ids, double* dist);
+  // recall@k of the current graph against the FP64 exact k-NN of the queries
+  double recall(const int64_t* queries, int64_t nq);
   int tree_iterations(int tree) const { return tree < static_cast<int>(iters_.size()) ? iters_[tree] : -1; }
 
   const StageTimes& times() const { return times_; }

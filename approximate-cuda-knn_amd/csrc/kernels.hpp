@@ -117,6 +117,16 @@ void launch_leaf_mfma(int metric, const float* x, int d, int ld, const _Float16*
                       const float* nh, const float* norms, const int32_t* perm, const Leaf* leaves, int nleaves,
                       int max_leaf, int k, int32_t* ids, float* vals, unsigned long long* rescored, hipStream_t s);
 
+// ---- FP64 exact k-NN and recall (the measurement reference) ------------------
+// sq: FP64 squared norms (COSINE only, launch_sq64).  ids/dist: [nq][kq],
+// sorted by (distance, id), the query excluded.
+void launch_sq64(const float* x, int64_t n, int d, int ld, double* sq, hipStream_t s);
+void launch_exact_knn(int metric, const float* x, int64_t n, int d, int ld, const double* sq, const int64_t* queries,
+                      int64_t nq, int kq, int32_t* ids, double* dist, hipStream_t s);
+// hits += graph neighbours (ids: [n][k]) of each query within its k-th exact distance kth[nq]
+void launch_recall(int metric, const float* x, int d, int ld, const double* sq, const int64_t* queries, int64_t nq,
+                   const double* kth, const int32_t* ids, int k, unsigned long long* hits, hipStream_t s);
+
 // COSINE parity initial pass: ascending-id chunks of `cap` candidates over the
 // union of all trees' start ranges, with the store-time flip after each chunk.
 struct CosineParityScratch {

@@ -32,50 +32,6 @@ def _load(name, path):
     return mod
 
 
-def exact_recall(xn, ids, k, metric, queries, device="cuda"):
-    """recall@k against FP64 exact k-NN (self excluded, ties at the k-th
-    distance count as hits).  Candidates come from an FP64 GEMM-form pass
-    (k + 32 best), their distances are then recomputed directly in FP64."""
-    import torch
-
-    X = torch.from_numpy(xn).to(device=device, dtype=torch.float64)
-    n = X.shape[0]
-    sq = (X * X).sum(1)
-    nrm = sq.sqrt()
-    hits = 0
-    extra = min(n - 1, k + 32)
-
-    def direct(q, cand):  # q: [b, d], cand: [b, m, d]
-        diff = cand - q[:, None, :]
-        if metric == 0:
-            return (diff * diff).sum(-1)
-        if metric == 1:
-            return diff.abs().sum(-1)
-        dots = (cand * q[:, None, :]).sum(-1)
-        return 1.0 - dots / (cand.norm(dim=-1) * q.norm(dim=-1)[:, None])
-
-    qs = torch.from_numpy(np.asarray(queries)).to(device)
-    for s in range(0, len(qs), 256):
-        qi = qs[s:s + 256]
-        Q = X[qi]
-        if metric == 0:
-            approx = sq[qi][:, None] + sq[None, :] - 2.0 * (Q @ X.T)
-        elif metric == 1:
-            approx = torch.cdist(Q, X, p=1)
-        else:
-            approx = 1.0 - (Q @ X.T) / (nrm[qi][:, None] * nrm[None, :])
-        approx[torch.arange(len(qi), device=device), qi] = float("inf")
-        cand = approx.topk(extra, dim=1, largest=False).indices
-        exact = direct(Q, X[cand])
-        exact[cand == qi[:, None]] = float("inf")
-        kth = exact.sort(dim=1).values[:, k - 1]
-        ret = torch.from_numpy(ids[queries[s:s + 256]].astype(np.int64)).to(device)
-        valid = (ret >= 0) & (ret != qi[:, None])
-        dret = direct(Q, X[ret.clamp(min=0)])
-        hits += int(((dret <= kth[:, None]) & valid).sum().item())
-    return hits / (len(queries) * k)
-
-
 def cpu_baseline(datasets, config, cfg, sample_n):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
@@ -245,15 +201,16 @@ def main():
                                        (ga[1].view(np.uint32) == gb[1].view(np.uint32)).all())
         print(f"[bench] chosen/plain {line['chosen_vs_plain']:.3f}, identical graphs: {line['forms_identical']}",
               file=sys.stderr, flush=True)
-    if not args.no_recall and torch is not None and torch.cuda.is_available():
+    if not args.no_recall:
+        # FP64 exact k-NN of seeded queries on the GPU (the library's own
+        # kernel); the graph is the last form's, identical for both forms
         rng = np.random.default_rng(1234)
         q = np.sort(rng.choice(n, size=min(args.queries, n), replace=False))
-        ids, _ = graphs[head]
-        xn = g.normalized()
         t1 = time.perf_counter()
-        line["recall_at_k"] = exact_recall(xn, ids, cfg["k"], metric, q)
+        line["recall_at_k"] = g.recall(q)
         line["recall_queries"] = int(len(q))
-        print(f"[bench] recall {line['recall_at_k']:.4f} ({time.perf_counter() - t1:.1f} s)", file=sys.stderr,
+        line["recall_reference"] = "FP64 exact k-NN (aknn_exact_knn), self excluded, ties at the k-th distance count"
+        print(f"[bench] recall {line['recall_at_k']:.4f} ({time.perf_counter() - t1:.2f} s)", file=sys.stderr,
               flush=True)
     g.close()
     if not args.no_cpu:

@@ -147,6 +147,20 @@ int aknn_get_kernel_stats(aknn_handle* h, const char* name, int64_t* launches, d
  * kernel statistics above. */
 int aknn_get_candidate_counts(aknn_handle* h, const char* stage, double* examined, double* rescored);
 
+/* Exact k-NN in FP64 of nq query points (indices into the data) against all
+ * points, on the normalized points of the last calculate(), with the handle's
+ * distance: EUCLIDEAN the squared L2 distance, TAXICAB L1, COSINE 1 - cos;
+ * dimensions summed in order in FP64.  The query itself is excluded.  ids and
+ * dist are [nq][kq], sorted by (distance, id); 1 <= kq <= 64.  No reference
+ * counterpart: the measurement reference for recall. */
+int aknn_exact_knn(aknn_handle* h, const int64_t* queries, int64_t nq, int kq, int32_t* ids, double* dist);
+
+/* recall@k of the current graph over nq query points: the fraction of the
+ * queries' graph neighbours whose FP64 distance (as aknn_exact_knn) is at most
+ * the query's k-th exact distance (ties at the k-th distance count as hits).
+ * No reference counterpart. */
+int aknn_recall(aknn_handle* h, const int64_t* queries, int64_t nq, double* recall);
+
 /* Message of the last failed call on h (empty if none). */
 const char* aknn_last_error(aknn_handle* h);
 

@@ -295,3 +295,26 @@ def test_sketch_precisions_same_graph(aknn, oracle, datasets, int8):
     print(f"[gpu] int8={int8} examined {ex:.0f} rescored {rs:.0f}", flush=True)
     assert rs < ex
     check_graph(oracle, a, x, 12, 3, 40, 4, 0)
+
+
+@pytest.mark.parametrize("metric,d", [(0, 24), (1, 24), (2, 24), (0, 300), (2, 130)])
+def test_exact_knn_matches_oracle(aknn, oracle, datasets, metric, d):
+    """FP64 exact k-NN on the GPU: k-th distances and every returned distance bit-equal to the oracle's FP64."""
+    x = datasets.gmm_c1(3000, d, clusters=5, seed=70 + d)[0]
+    g = gpu_run(aknn, x, 10, 2, 32, 1, metric, cosine_fixed=1)
+    xn = g.normalized()
+    q = np.random.default_rng(d).choice(len(x), 150, replace=False)
+    ids, dist = g.exact_knn(q, 10)
+    kth = oracle.exact_kth(xn, 10, metric, q)
+    assert (dist[:, -1].view(np.uint64) == kth.view(np.uint64)).all()
+    lib = oracle.lib()
+    for r, qq in enumerate(q[:40]):
+        assert qq not in ids[r]
+        ref = [lib.orc_distance64(np.ascontiguousarray(xn[qq]), np.ascontiguousarray(xn[c]), d, metric)
+               for c in ids[r]]
+        assert (np.array(ref).view(np.uint64) == dist[r].view(np.uint64)).all()
+        order = sorted(zip(dist[r].tolist(), ids[r].tolist()))
+        assert [c for _, c in order] == ids[r].tolist()
+    rec = g.recall(q)
+    gi, _ = g.graph()
+    assert rec == oracle.recall(xn, 10, metric, q, kth, gi)
