@@ -21,7 +21,51 @@ namespace aknn {
 
 namespace {
 constexpr int kNumEvents = 8;
+constexpr int kItemOwners = 64;  // owners per leaf k-NN work item (one wave)
+
+// Work items of one tree's leaf k-NN from its leaves: {first owner position,
+// owners (<= 64), candidate range start, count}.  A leaf with more than 64
+// points gives one item per 64 owners over its own start range.  Consecutive
+// small leaves are grouped while the item keeps at most 64 owners and the
+// union of their start ranges stays within 1.25 times the largest one plus a
+// tile (nested or shared start nodes: a cluster peeled off one point per
+// level leaves a chain of one-point leaves whose ranges are nested); each
+// pair is then masked by its owner's own range.
+void build_work_items(std::vector<Leaf>& leaves, std::vector<Leaf>& items) {
+  std::sort(leaves.begin(), leaves.end(), [](const Leaf& a, const Leaf& b) { return a.start < b.start; });
+  items.clear();
+  Leaf cur{0, 0, 0, 0};
+  int32_t cur_max = 0;
+  bool open = false;
+  for (const Leaf& l : leaves) {
+    if (l.size > kItemOwners) {
+      if (open) items.push_back(cur);
+      open = false;
+      for (int32_t o = 0; o < l.size; o += kItemOwners)
+        items.push_back(Leaf{l.start + o, std::min<int32_t>(kItemOwners, l.size - o), l.rstart, l.rcount});
+      continue;
+    }
+    if (open && cur.size + l.size <= kItemOwners) {
+      const int64_t u0 = std::min(cur.rstart, l.rstart);
+      const int64_t u1 = std::max<int64_t>(static_cast<int64_t>(cur.rstart) + cur.rcount,
+                                           static_cast<int64_t>(l.rstart) + l.rcount);
+      const int32_t mx = std::max(cur_max, l.rcount);
+      if (u1 - u0 <= static_cast<int64_t>(mx) + mx / 4 + 64) {
+        cur.size += l.size;
+        cur.rstart = static_cast<int32_t>(u0);
+        cur.rcount = static_cast<int32_t>(u1 - u0);
+        cur_max = mx;
+        continue;
+      }
+    }
+    if (open) items.push_back(cur);
+    cur = l;
+    cur_max = l.rcount;
+    open = true;
+  }
+  if (open) items.push_back(cur);
 }
+}  // namespace
 
 Engine::Engine(int device) : device_(device) {
   int count = 0;
@@ -80,6 +124,7 @@ void Engine::release() {
   perm_ = nullptr;
   leaves_ = nullptr;
   rs_ = rc_ = nullptr;
+  items_ = nullptr;
   ncand_ = nullptr;
   changed_[0] = changed_[1] = nullptr;
   fb_ = ForestBuffers{};
@@ -99,6 +144,7 @@ void Engine::set_data(const float* x_rowmajor, int64_t n, int d, const Params& p
   AKNN_HIP_TRY(hipStreamSynchronize(stream_));
   release();
   p_ = p;
+  prop_mode_ = -1;
   n_ = n;
   d_ = d;
   ld_ = (d + 3) / 4 * 4;
@@ -196,6 +242,7 @@ void Engine::build_forest() {
 
 void Engine::initial_knn() {
   stats_leaf_knn = KernelStats{};
+  tree_screened_.assign(p_.trees, 0);
   int32_t* ids = gids_[cur_];
   float* vals = gvals_[cur_];
   launch_stub_init(ids, vals, n_ * p_.k, stream_);
@@ -227,17 +274,56 @@ void Engine::initial_knn() {
     stats_leaf_knn.launches = 1;
     return;
   }
-  const bool mfma = leaf_mfma_active();
-  if (mfma) AKNN_HIP_TRY(hipMemsetAsync(leaf_rescored_, 0, sizeof(unsigned long long), stream_));
+  // Work items per tree (host, from the leaf lists): leaves in position order,
+  // big ones cut into 64-owner items, runs of small ones whose start ranges
+  // mostly overlap grouped into one item scanning the union of the ranges.
+  if (!rs_) {
+    rs_ = alloc<int32_t>(static_cast<size_t>(n_) * p_.trees);
+    rc_ = alloc<int32_t>(static_cast<size_t>(n_) * p_.trees);
+  }
+  if (!items_) items_ = alloc<Leaf>(static_cast<size_t>(n_) + n_ / 64 + 16);
+  // (a member: asynchronous copies read it after this function returns)
+  std::vector<std::vector<Leaf>>& items = items_host_;
+  items.assign(p_.trees, {});
+  {
+    std::vector<Leaf> host;
+    for (int t = 0; t < p_.trees; ++t) {
+      host.resize(nleaves_[t]);
+      AKNN_HIP_TRY(hipMemcpy(host.data(), leaves_ + t * n_, sizeof(Leaf) * nleaves_[t], hipMemcpyDeviceToHost));
+      build_work_items(host, items[t]);
+      launch_leaf_ranges(perm_ + t * n_, leaves_ + t * n_, nleaves_[t], rs_ + t * n_, rc_ + t * n_, stream_);
+    }
+  }
+  bool mfma = leaf_mfma_active();
+  if (mfma) {
+    AKNN_HIP_TRY(hipMemsetAsync(leaf_rescored_, 0, sizeof(unsigned long long), stream_));
+    AKNN_HIP_TRY(hipMemsetAsync(leaf_rescored_ + 2, 0, sizeof(unsigned long long), stream_));
+  }
+  unsigned long long prev[2] = {0, 0};
   for (int t = 0; t < p_.trees; ++t) {
+    // The screen pays when few pairs survive it: after a screened tree that
+    // re-scored more than 30% of its pairs, the remaining trees take the
+    // plain kernel (same graph).
+    if (mfma && t > 1) {
+      unsigned long long c[3] = {0, 0, 0};
+      AKNN_HIP_TRY(hipMemcpyAsync(c, leaf_rescored_, sizeof(c), hipMemcpyDeviceToHost, stream_));
+      AKNN_HIP_TRY(hipStreamSynchronize(stream_));
+      const unsigned long long dres = c[0] - prev[0], dscr = c[2] - prev[1];
+      prev[0] = c[0];
+      prev[1] = c[2];
+      if (dscr > 0 && 10 * dres > 3 * dscr) mfma = false;
+    }
+    tree_screened_[t] = mfma && t > 0 ? 1 : 0;
+    const int nit = static_cast<int>(items[t].size());
+    AKNN_HIP_TRY(hipMemcpyAsync(items_, items[t].data(), sizeof(Leaf) * nit, hipMemcpyHostToDevice, stream_));
     AKNN_HIP_TRY(hipEventRecord(launch_ev(2 * t), stream_));
     // the first tree meets stub rows (nothing to screen against): plain kernel
     if (mfma && t > 0)
-      launch_leaf_mfma(p_.metric, x_, d_, ld_, xh_, ldh_, eps_, nh_, norms_, perm_ + t * n_, leaves_ + t * n_,
-                       nleaves_[t], leaf_eff_, p_.k, ids, vals, leaf_rescored_, stream_);
+      launch_leaf_mfma(p_.metric, x_, d_, ld_, xh_, ldh_, eps_, nh_, norms_, perm_ + t * n_, items_, nit, kItemOwners,
+                       p_.k, ids, vals, rs_ + t * n_, rc_ + t * n_, leaf_rescored_, leaf_rescored_ + 2, stream_);
     else
-      launch_leaf_knn(p_.metric, x_, d_, ld_, norms_, perm_ + t * n_, leaves_ + t * n_, nleaves_[t], leaf_eff_,
-                      p_.k, ids, vals, stream_);
+      launch_leaf_knn(p_.metric, x_, d_, ld_, norms_, perm_ + t * n_, items_, nit, kItemOwners, p_.k, ids, vals,
+                      rs_ + t * n_, rc_ + t * n_, stream_);
     AKNN_HIP_TRY(hipEventRecord(launch_ev(2 * t + 1), stream_));
   }
   stats_leaf_knn.launches = p_.trees;
@@ -248,14 +334,44 @@ void Engine::propagate() {
   const bool flip = p_.metric == kCosine && !p_.cosine_fixed;
   const int base = 2 * p_.trees;
   AKNN_HIP_TRY(hipMemsetAsync(ncand_, 0, sizeof(unsigned long long) * 2 * std::max(p_.props, 1), stream_));
+  round_screened_.assign(std::max(p_.props, 0), 0);
+  // Screened or exact rounds (the graph is the same either way).  The first
+  // calculate() on a data set runs round 0 screened and round 1 exact and
+  // keeps the mode with the lower time per examined candidate for the other
+  // rounds and later calls (clustered data can make the exact path's row
+  // reuse win).  A round where more than a quarter of the candidates survived
+  // the screen also switches the remaining rounds to the exact path.
+  const bool can_screen = propagation_screened();
+  bool screen = can_screen && prop_mode_ != 1;
+  const bool probe = can_screen && prop_mode_ < 0 && p_.props >= 3;
   for (int r = 0; r < p_.props; ++r) {
+    if (can_screen && r > 0) {
+      unsigned long long c[2] = {0, 0};
+      AKNN_HIP_TRY(hipMemcpyAsync(c, ncand_ + 2 * (r - 1), sizeof(c), hipMemcpyDeviceToHost, stream_));
+      AKNN_HIP_TRY(hipStreamSynchronize(stream_));
+      if (probe && r == 2) {
+        float ms[2] = {0, 0};
+        unsigned long long c0[2] = {0, 0};
+        AKNN_HIP_TRY(hipMemcpy(c0, ncand_, sizeof(c0), hipMemcpyDeviceToHost));
+        AKNN_HIP_TRY(hipEventElapsedTime(&ms[0], launch_ev(base), launch_ev(base + 1)));
+        AKNN_HIP_TRY(hipEventElapsedTime(&ms[1], launch_ev(base + 2), launch_ev(base + 3)));
+        const double per_s = ms[0] / std::max<double>(1.0, static_cast<double>(c0[0]));
+        const double per_e = ms[1] / std::max<double>(1.0, static_cast<double>(c[0]));
+        prop_mode_ = per_s <= per_e ? 0 : 1;
+        screen = prop_mode_ == 0;
+      } else if (probe && r == 1) {
+        screen = false;  // the exact probe round
+      }
+      if (round_screened_[r - 1] && c[0] > 0 && 4 * c[1] > c[0]) screen = false;
+    }
+    round_screened_[r] = screen ? 1 : 0;
     PropagateArgs a;
     a.metric = p_.metric;
     a.flip = flip;
     a.x = x_;
     a.d = d_;
     a.ld = ld_;
-    if (propagation_screened()) {
+    if (screen) {
       a.xh = xh_;
       a.ldh = ldh_;
       a.eps = eps_;
@@ -298,7 +414,7 @@ void Engine::calculate() {
     xh_ = alloc<_Float16>(static_cast<size_t>(n_) * ldh_);
     eps_ = alloc<float>(n_);
     nh_ = alloc<float>(n_);
-    leaf_rescored_ = alloc<unsigned long long>(2);
+    leaf_rescored_ = alloc<unsigned long long>(3);  // leaf re-scored, forest undecided, leaf screened
   }
   const bool sketch8 = propagation_screened() && p_.sketch_int8 && p_.metric == kEuclidean && p_.props > 0;
   if (sketch8 && !q8_) {
@@ -338,7 +454,7 @@ void Engine::calculate() {
     float ms = 0;
     AKNN_HIP_TRY(hipEventElapsedTime(&ms, launch_ev(2 * l), launch_ev(2 * l + 1)));
     stats_leaf_knn.total_ms += ms;
-    if (mfma && l > 0) {
+    if (mfma && tree_screened_[l]) {
       stats_leaf_knn.screened_ms += ms;
       stats_leaf_knn.screened_launches += 1;
     }
@@ -356,7 +472,7 @@ void Engine::calculate() {
     // sketch row and radius (2 ldh + 4) plus 4D for each exactly re-scored one
     double per_point = 16.0 * k + 4.0 * k * k + 4.0 * d_;
     double cand = 4.0 * d_ * nc[2 * r];
-    if (propagation_screened()) {
+    if (round_screened_[r]) {
       const bool i8 = p_.sketch_int8 && p_.metric == kEuclidean;
       const double row = i8 ? ld8_ + 12.0 : 2.0 * ldh_ + 4.0;  // sketch row + per-row scalars
       per_point += row;
@@ -364,7 +480,7 @@ void Engine::calculate() {
     }
     stats_propagate.units += static_cast<double>(n_) * per_point + cand;
     stats_propagate.examined += static_cast<double>(nc[2 * r]);
-    stats_propagate.rescored += static_cast<double>(propagation_screened() ? nc[2 * r + 1] : nc[2 * r]);
+    stats_propagate.rescored += static_cast<double>(round_screened_[r] ? nc[2 * r + 1] : nc[2 * r]);
   }
   stats_forest.launches = 0;
   for (int t = 0; t < p_.trees; ++t) stats_forest.launches += iters_[t];
@@ -379,7 +495,7 @@ void Engine::calculate() {
       AKNN_HIP_TRY(hipMemcpy(host.data(), leaves_ + t * n_, sizeof(Leaf) * nleaves_[t], hipMemcpyDeviceToHost));
       for (const Leaf& l : host) {
         pairs += static_cast<double>(l.size) * (l.rcount - 1);
-        if (mfma && t > 0) screened += static_cast<double>(l.size) * (l.rcount - 1);
+        if (mfma && tree_screened_[t]) screened += static_cast<double>(l.size) * (l.rcount - 1);
       }
     }
     stats_leaf_knn.units = pairs * 3.0 * d_;

@@ -64,6 +64,7 @@ class Engine {
     p_.cosine_fixed = p.cosine_fixed;
     p_.level_batch = p.level_batch;
     p_.skip_settled = p.skip_settled;
+    if (p_.sketch != p.sketch || p_.sketch_int8 != p.sketch_int8) prop_mode_ = -1;
     p_.sketch = p.sketch;
     p_.leaf_mfma = p.leaf_mfma;
     p_.sketch_int8 = p.sketch_int8;
@@ -137,10 +138,14 @@ class Engine {
   ForestCtl* ctl_host_ = nullptr;
   unsigned long long* ncand_ = nullptr;
   uint8_t* changed_[2] = {nullptr, nullptr};
-  int32_t* rs_ = nullptr;  // [trees][n], allocated on demand
+  int32_t* rs_ = nullptr;  // [trees][n] per-point start ranges, allocated on demand
+  Leaf* items_ = nullptr;  // leaf k-NN work items of one tree
+  std::vector<std::vector<Leaf>> items_host_;
   int32_t* rc_ = nullptr;
   CosineParityScratch cps_{};
   int64_t active_sum_ = 0;
+  std::vector<uint8_t> round_screened_, tree_screened_;  // which rounds / trees used the screens
+  int prop_mode_ = -1;  // propagation rounds learned on this data: -1 unknown, 0 screened, 1 exact
   DrawStream draws_;
   StageTimes times_;
 };

@@ -47,10 +47,13 @@ void launch_leaf_ranges(const int32_t* perm, const Leaf* leaves, int nleaves, in
 void launch_stub_init(int32_t* ids, float* vals, int64_t count, hipStream_t s);
 
 // Initial leaf k-NN for one tree, merged into the graph rows in place
-// (EUCLIDEAN / TAXICAB / COSINE-fixed; exact top-k by (key, id)).
+// (EUCLIDEAN / TAXICAB / COSINE-fixed; exact top-k by (key, id)).  leaves:
+// work items {first owner position, owners, candidate range start, count};
+// with rsp/rcp (per-point start-node ranges, nullable) an item may group the
+// owners of several leaves, each pair masked by its owner's own range.
 void launch_leaf_knn(int metric, const float* x, int d, int ld, const float* norms, const int32_t* perm,
                      const Leaf* leaves, int nleaves, int max_leaf, int k, int32_t* ids, float* vals,
-                     hipStream_t s);
+                     const int32_t* rsp, const int32_t* rcp, hipStream_t s);
 
 // What the previous round left behind, so a round can skip neighbours whose
 // rows were already fully examined (results are unchanged by the skip).
@@ -111,11 +114,12 @@ void launch_sketch8(const float* x, int64_t n, int d, int ld, int8_t* q8, int ld
 // Initial leaf k-NN for one tree, EUCLIDEAN or COSINE with carried distances,
 // with the candidate pairs screened by half-precision MFMA dot products of the
 // sketch and proven lower bounds; the same graph as launch_leaf_knn.
-// norms: the COSINE squared-norm chains.  rescored: optional device counter
-// of pairs whose exact distance was computed.
+// norms: the COSINE squared-norm chains.  rescored, screened: optional device
+// counters of pairs whose exact distance was computed and of pairs screened.
 void launch_leaf_mfma(int metric, const float* x, int d, int ld, const _Float16* xh, int ldh, const float* eps,
                       const float* nh, const float* norms, const int32_t* perm, const Leaf* leaves, int nleaves,
-                      int max_leaf, int k, int32_t* ids, float* vals, unsigned long long* rescored, hipStream_t s);
+                      int max_leaf, int k, int32_t* ids, float* vals, const int32_t* rsp, const int32_t* rcp,
+                      unsigned long long* rescored, unsigned long long* screened, hipStream_t s);
 
 // ---- FP64 exact k-NN and recall (the measurement reference) ------------------
 // sq: FP64 squared norms (COSINE only, launch_sq64).  ids/dist: [nq][kq],

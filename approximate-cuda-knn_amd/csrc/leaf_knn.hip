@@ -60,6 +60,7 @@ template <int METRIC, int KMAX, bool BIGD>
 __global__ __launch_bounds__(64) void leaf_knn_kernel(const float* __restrict__ x, int d, int ld,
                                                       const float* __restrict__ norms, const int32_t* __restrict__ perm,
                                                       const Leaf* __restrict__ leaves, int nleaves, int chunks_per_leaf,
+                                                      const int32_t* __restrict__ rsp, const int32_t* __restrict__ rcp,
                                                       int k, int32_t* __restrict__ gids, float* __restrict__ gvals) {
   // The transposed staging tiles (dimension-major) and the 64x64 distance
   // block are never live at the same time and share one LDS buffer.
@@ -81,6 +82,10 @@ __global__ __launch_bounds__(64) void leaf_knn_kernel(const float* __restrict__ 
   const int nown = L.size - o0 < kTile ? L.size - o0 : kTile;
   const bool has_owner = lane < nown;
   const int32_t my_id = has_owner ? perm[L.start + o0 + lane] : -1;
+  // this owner's own start-node positions; a work item may group owners of
+  // several leaves and scan the union of their ranges
+  const int32_t r_lo = rsp != nullptr && has_owner ? rsp[my_id] : L.rstart;
+  const int32_t r_hi = rsp != nullptr && has_owner ? r_lo + rcp[my_id] : L.rstart + L.rcount;
   if constexpr (METRIC == kCosine) onorm[lane] = has_owner ? norms[my_id] : 1.0f;
 
   // this owner's running top-k, seeded with its graph row (earlier trees)
@@ -188,7 +193,9 @@ __global__ __launch_bounds__(64) void leaf_knn_kernel(const float* __restrict__ 
       uint64_t surv = 0;
       for (int j = 0; j < ncand; ++j) {
         const int32_t cj = cid[j];
-        surv |= static_cast<uint64_t>(cj != my_id && key_less(Dt[lane][j], cj, thk, thi)) << j;
+        const int32_t pos = L.rstart + cb + j;
+        surv |= static_cast<uint64_t>(cj != my_id && pos >= r_lo && pos < r_hi && key_less(Dt[lane][j], cj, thk, thi))
+                << j;
       }
       while (surv) {
         const int j = __ffsll(static_cast<long long>(surv)) - 1;
@@ -222,27 +229,28 @@ __global__ __launch_bounds__(64) void leaf_knn_kernel(const float* __restrict__ 
 
 template <int METRIC, int KMAX, bool BIGD>
 void launch_t(const float* x, int d, int ld, const float* norms, const int32_t* perm, const Leaf* leaves, int nleaves,
-              int max_leaf, int k, int32_t* ids, float* vals, hipStream_t s) {
+              int max_leaf, int k, int32_t* ids, float* vals, const int32_t* rsp, const int32_t* rcp, hipStream_t s) {
   const int chunks = (max_leaf + kTile - 1) / kTile;
   const int64_t grid = xcd_grid(static_cast<int64_t>(nleaves) * chunks);
   hipLaunchKernelGGL((leaf_knn_kernel<METRIC, KMAX, BIGD>), dim3(static_cast<unsigned>(grid)), dim3(kTile), 0, s, x,
-                     d, ld, norms, perm, leaves, nleaves, chunks, k, ids, vals);
+                     d, ld, norms, perm, leaves, nleaves, chunks, rsp, rcp, k, ids, vals);
   AKNN_HIP_TRY(hipGetLastError());
 }
 
 template <int METRIC, bool BIGD>
 void launch_k(int k, const float* x, int d, int ld, const float* norms, const int32_t* perm, const Leaf* leaves,
-              int nleaves, int max_leaf, int32_t* ids, float* vals, hipStream_t s) {
+              int nleaves, int max_leaf, int32_t* ids, float* vals, const int32_t* rsp, const int32_t* rcp,
+              hipStream_t s) {
   if (k <= 8)
-    launch_t<METRIC, 8, BIGD>(x, d, ld, norms, perm, leaves, nleaves, max_leaf, k, ids, vals, s);
+    launch_t<METRIC, 8, BIGD>(x, d, ld, norms, perm, leaves, nleaves, max_leaf, k, ids, vals, rsp, rcp, s);
   else if (k <= 16)
-    launch_t<METRIC, 16, BIGD>(x, d, ld, norms, perm, leaves, nleaves, max_leaf, k, ids, vals, s);
+    launch_t<METRIC, 16, BIGD>(x, d, ld, norms, perm, leaves, nleaves, max_leaf, k, ids, vals, rsp, rcp, s);
   else if (k <= 24)
-    launch_t<METRIC, 24, BIGD>(x, d, ld, norms, perm, leaves, nleaves, max_leaf, k, ids, vals, s);
+    launch_t<METRIC, 24, BIGD>(x, d, ld, norms, perm, leaves, nleaves, max_leaf, k, ids, vals, rsp, rcp, s);
   else if (k <= 32)
-    launch_t<METRIC, 32, BIGD>(x, d, ld, norms, perm, leaves, nleaves, max_leaf, k, ids, vals, s);
+    launch_t<METRIC, 32, BIGD>(x, d, ld, norms, perm, leaves, nleaves, max_leaf, k, ids, vals, rsp, rcp, s);
   else if (k <= 64)
-    launch_t<METRIC, 64, BIGD>(x, d, ld, norms, perm, leaves, nleaves, max_leaf, k, ids, vals, s);
+    launch_t<METRIC, 64, BIGD>(x, d, ld, norms, perm, leaves, nleaves, max_leaf, k, ids, vals, rsp, rcp, s);
   else
     throw HipError("leaf k-NN supports k <= 64");
 }
@@ -267,21 +275,22 @@ void launch_stub_init(int32_t* ids, float* vals, int64_t count, hipStream_t s) {
 }
 
 void launch_leaf_knn(int metric, const float* x, int d, int ld, const float* norms, const int32_t* perm,
-                     const Leaf* leaves, int nleaves, int max_leaf, int k, int32_t* ids, float* vals, hipStream_t s) {
+                     const Leaf* leaves, int nleaves, int max_leaf, int k, int32_t* ids, float* vals,
+                     const int32_t* rsp, const int32_t* rcp, hipStream_t s) {
   if (nleaves <= 0) return;
   const bool big = d > 256;
   switch (metric) {
     case kEuclidean:
-      big ? launch_k<kEuclidean, true>(k, x, d, ld, norms, perm, leaves, nleaves, max_leaf, ids, vals, s)
-          : launch_k<kEuclidean, false>(k, x, d, ld, norms, perm, leaves, nleaves, max_leaf, ids, vals, s);
+      big ? launch_k<kEuclidean, true>(k, x, d, ld, norms, perm, leaves, nleaves, max_leaf, ids, vals, rsp, rcp, s)
+          : launch_k<kEuclidean, false>(k, x, d, ld, norms, perm, leaves, nleaves, max_leaf, ids, vals, rsp, rcp, s);
       break;
     case kTaxicab:
-      big ? launch_k<kTaxicab, true>(k, x, d, ld, norms, perm, leaves, nleaves, max_leaf, ids, vals, s)
-          : launch_k<kTaxicab, false>(k, x, d, ld, norms, perm, leaves, nleaves, max_leaf, ids, vals, s);
+      big ? launch_k<kTaxicab, true>(k, x, d, ld, norms, perm, leaves, nleaves, max_leaf, ids, vals, rsp, rcp, s)
+          : launch_k<kTaxicab, false>(k, x, d, ld, norms, perm, leaves, nleaves, max_leaf, ids, vals, rsp, rcp, s);
       break;
     case kCosine:
-      big ? launch_k<kCosine, true>(k, x, d, ld, norms, perm, leaves, nleaves, max_leaf, ids, vals, s)
-          : launch_k<kCosine, false>(k, x, d, ld, norms, perm, leaves, nleaves, max_leaf, ids, vals, s);
+      big ? launch_k<kCosine, true>(k, x, d, ld, norms, perm, leaves, nleaves, max_leaf, ids, vals, rsp, rcp, s)
+          : launch_k<kCosine, false>(k, x, d, ld, norms, perm, leaves, nleaves, max_leaf, ids, vals, rsp, rcp, s);
       break;
     default:
       throw HipError("unknown distance");

@@ -96,6 +96,8 @@ struct MfmaLeafArgs {
   const float* norms;  // COSINE: the exact squared-norm chains
   const int32_t* perm;
   const Leaf* leaves;
+  const int32_t* rsp;  // per-point start-node ranges (nullable: the item's range)
+  const int32_t* rcp;
   int nleaves, chunks_per_leaf, k;
   int32_t* gids;
   float* gvals;
@@ -104,6 +106,7 @@ struct MfmaLeafArgs {
   float kmf;     // COSINE: MFMA accumulation error relative to ||xh|| ||yh||
   float kabf;    // COSINE: kap + 4u
   unsigned long long* rescored;  // optional: exactly re-scored pairs
+  unsigned long long* screened;  // optional: screened pairs (owners x candidates)
 };
 
 // COSINE pair terms: up(G) = G + kM (A+eo)(B+ec) + eo B + (A+eo) ec
@@ -126,6 +129,8 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
   __shared__ __attribute__((aligned(16))) float o_e[kTile];  // C: sketch radius
   __shared__ __attribute__((aligned(16))) float o_s[kTile];  // C: sqrtf(aa), as the exact key uses
   __shared__ float c_n[kTile], c_e[kTile], c_s[kTile], kb[kTile];
+  __shared__ __attribute__((aligned(16))) int32_t o_lo[kTile];  // owner's start-node positions [lo, hi)
+  __shared__ __attribute__((aligned(16))) int32_t o_hi[kTile];
   __shared__ uint64_t survm[kTile];
   __shared__ uint16_t plist[kTile * kTile];
 
@@ -161,6 +166,8 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
   const float my_eps = has_owner ? a.eps[my_id] : 0.0f;
   const double tiny = 1e-40;
   oid[lane] = my_id;
+  o_lo[lane] = a.rsp != nullptr && has_owner ? a.rsp[my_id] : L.rstart;
+  o_hi[lane] = a.rsp != nullptr && has_owner ? a.rsp[my_id] + a.rcp[my_id] : L.rstart + L.rcount;
   if constexpr (METRIC == kEuclidean) {
     o_n[lane] = has_owner ? a.nh[my_id] : 0.0f;
   } else {
@@ -194,44 +201,39 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
     survm[lane] = 0;
     __syncthreads();
 
-    // 1. dot products of the sketches: acc[ot][ct] is the 32x32 block of
-    //    owners 32*ot.. against candidates 32*ct..
-    const int32_t ca = cid[col], cbb = cid[32 + col];
-    const _Float16* brow0 = ca >= 0 ? a.xh + static_cast<int64_t>(ca) * a.ldh : nullptr;
-    const _Float16* brow1 = cbb >= 0 ? a.xh + static_cast<int64_t>(cbb) * a.ldh : nullptr;
-    f32x16 acc00, acc01, acc10, acc11;
+    // 1-2. one 32x32 quadrant (owners 32*ot.., candidates 32*ct..) at a
+    //      time: dot products of the sketches on the matrix cores, then the
+    //      screen of the quadrant.  This lane holds candidate column
+    //      j = 32*ct + col against owner rows i = 32*ot + 8*g + 4*half + q
+    //      (register 4*g + q).  One quadrant's accumulators are live at a time.
+#pragma unroll 1
+    for (int quad = 0; quad < 4; ++quad) {
+      const int ot = quad >> 1, ct = quad & 1;
+      const int j = 32 * ct + col;
+      const int32_t cj = cid[j];
+      const float cn = c_n[j], ce = c_e[j];
+      const float cs = METRIC == kCosine ? c_s[j] : 0.0f;
+      const int32_t pos = L.rstart + cb + j;
+      const _Float16* arow = ot == 0 ? arow0 : arow1;
+      const _Float16* brow = cj >= 0 ? a.xh + static_cast<int64_t>(cj) * a.ldh : nullptr;
+      f32x16 acc;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc00[r] = acc01[r] = acc10[r] = acc11[r] = 0.0f;
-#pragma unroll 2
-    for (int s = 0; s < steps; ++s) {
-      const int kk = 16 * s + 8 * half;
-      const half8 A0 = piece(arow0, kk, a.ldh), A1 = piece(arow1, kk, a.ldh);
-      const half8 B0 = piece(brow0, kk, a.ldh), B1 = piece(brow1, kk, a.ldh);
-      acc00 = __builtin_amdgcn_mfma_f32_32x32x16_f16(A0, B0, acc00, 0, 0, 0);
-      acc01 = __builtin_amdgcn_mfma_f32_32x32x16_f16(A0, B1, acc01, 0, 0, 0);
-      acc10 = __builtin_amdgcn_mfma_f32_32x32x16_f16(A1, B0, acc10, 0, 0, 0);
-      acc11 = __builtin_amdgcn_mfma_f32_32x32x16_f16(A1, B1, acc11, 0, 0, 0);
-    }
-
-    // 2. screen: this lane holds candidate columns j = 32*ct + col against
-    //    owner rows i = 32*ot + 8*g + 4*half + q (register 4*g + q)
-    int32_t cjv[2];
-    float cnv[2], cev[2], csv[2];
-#pragma unroll
-    for (int ct = 0; ct < 2; ++ct) {
-      cjv[ct] = cid[32 * ct + col];
-      cnv[ct] = c_n[32 * ct + col];
-      cev[ct] = c_e[32 * ct + col];
-      csv[ct] = METRIC == kCosine ? c_s[32 * ct + col] : 0.0f;
-    }
-#pragma unroll
-    for (int ot = 0; ot < 2; ++ot) {
+      for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+#pragma unroll 4
+      for (int s = 0; s < steps; ++s) {
+        const int kk = 16 * s + 8 * half;
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(piece(arow, kk, a.ldh), piece(brow, kk, a.ldh), acc, 0, 0, 0);
+      }
 #pragma unroll
       for (int g = 0; g < 4; ++g) {
         const int i0 = 32 * ot + 8 * g + 4 * half;
         const float4 on4 = *reinterpret_cast<const float4*>(&o_n[i0]);
         const float4 to4 = *reinterpret_cast<const float4*>(&o_t[i0]);
         const int4 oi4 = *reinterpret_cast<const int4*>(&oid[i0]);
+        const int4 lo4 = *reinterpret_cast<const int4*>(&o_lo[i0]);
+        const int4 hi4 = *reinterpret_cast<const int4*>(&o_hi[i0]);
+        const int32_t olo[4] = {lo4.x, lo4.y, lo4.z, lo4.w};
+        const int32_t ohi[4] = {hi4.x, hi4.y, hi4.z, hi4.w};
         const float on[4] = {on4.x, on4.y, on4.z, on4.w};
         const float to[4] = {to4.x, to4.y, to4.z, to4.w};
         const int32_t oi[4] = {oi4.x, oi4.y, oi4.z, oi4.w};
@@ -242,37 +244,32 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
           oe[0] = oe4.x; oe[1] = oe4.y; oe[2] = oe4.z; oe[3] = oe4.w;
           os[0] = os4.x; os[1] = os4.y; os[2] = os4.z; os[3] = os4.w;
         }
+        uint32_t hit = 0;
 #pragma unroll
-        for (int ct = 0; ct < 2; ++ct) {
-          const f32x16& acc = ot == 0 ? (ct == 0 ? acc00 : acc01) : (ct == 0 ? acc10 : acc11);
-          uint32_t hit = 0;
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            bool keep;
-            if constexpr (METRIC == kEuclidean) {
-              const float s_lo = __builtin_fmaf(-2.0f, acc[4 * g + q], (on[q] + cnv[ct]) * a.c1m);
-              const float t = (to[q] + cev[ct]) * 1.0000002384185791f;                // (1 + 4u)
-              const float t2 = __builtin_fmaf(t, t, 1e-30f) * 1.0000004768371582f;   // (1 + 8u)
-              keep = !(s_lo > t2);
-            } else {
-              // reject when up(G) + rounding margin < T_o * den * (1 - 4u)
-              const float G = acc[4 * g + q];
-              const float A = on[q], B = cnv[ct], eo = oe[q], ec = cev[ct];
-              const float Ao = A + eo;
-              const float P = a.kmf * Ao * (B + ec) + eo * B + Ao * ec + a.kabf * A * B;
-              const float lhs = G + P + 4.76837158203125e-07f * (__builtin_fabsf(G) + P + A * B) + 1e-30f;
-              const float den = os[q] * csv[ct];
-              const float rhs = to[q] * den * 0.99999976158142090f - 1e-30f;  // (1 - 4u)
-              keep = !(den > 0.0f && den <= 3.4e38f && lhs < rhs);
-            }
-            hit |= static_cast<uint32_t>(oi[q] >= 0 && cjv[ct] != oi[q] && keep) << q;
+        for (int q = 0; q < 4; ++q) {
+          bool keep;
+          if constexpr (METRIC == kEuclidean) {
+            const float s_lo = __builtin_fmaf(-2.0f, acc[4 * g + q], (on[q] + cn) * a.c1m);
+            const float t = (to[q] + ce) * 1.0000002384185791f;                    // (1 + 4u)
+            const float t2 = __builtin_fmaf(t, t, 1e-30f) * 1.0000004768371582f;   // (1 + 8u)
+            keep = !(s_lo > t2);
+          } else {
+            // reject when up(G) + rounding margin < T_o * den * (1 - 4u)
+            const float G = acc[4 * g + q];
+            const float A = on[q], B = cn, eo = oe[q];
+            const float Ao = A + eo;
+            const float P = a.kmf * Ao * (B + ce) + eo * B + Ao * ce + a.kabf * A * B;
+            const float lhs = G + P + 4.76837158203125e-07f * (__builtin_fabsf(G) + P + A * B) + 1e-30f;
+            const float den = os[q] * cs;
+            const float rhs = to[q] * den * 0.99999976158142090f - 1e-30f;  // (1 - 4u)
+            keep = !(den > 0.0f && den <= 3.4e38f && lhs < rhs);
           }
-          if (cjv[ct] >= 0 && hit != 0) {
+          hit |= static_cast<uint32_t>(oi[q] >= 0 && cj != oi[q] && pos >= olo[q] && pos < ohi[q] && keep) << q;
+        }
+        if (cj >= 0 && hit != 0) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-              if (hit & (1u << q))
-                atomicOr(reinterpret_cast<unsigned long long*>(&survm[i0 + q]), 1ull << (32 * ct + col));
-          }
+          for (int q = 0; q < 4; ++q)
+            if (hit & (1u << q)) atomicOr(reinterpret_cast<unsigned long long*>(&survm[i0 + q]), 1ull << j);
         }
       }
     }
@@ -356,6 +353,11 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
     if (has_owner && cnt > 0) o_t[lane] = owner_threshold<METRIC>(thk, my_eps, a.kap);
   }
   if (a.rescored != nullptr && lane == 0 && nres > 0) atomicAdd(a.rescored, nres);
+  if (a.screened != nullptr && lane == 0) {
+    unsigned long long pairs = 0;  // the owners' own ranges
+    for (int o = 0; o < nown; ++o) pairs += static_cast<unsigned long long>(o_hi[o] - o_lo[o]);
+    atomicAdd(a.screened, pairs);
+  }
   if (has_owner) {
 #pragma unroll
     for (int p = 0; p < KMAX; ++p)
@@ -379,6 +381,8 @@ void launch_m(const MfmaLeafArgs& a, int k, hipStream_t s) {
     launch_k<METRIC, 8>(a, s);
   else if (k <= 16)
     launch_k<METRIC, 16>(a, s);
+  else if (k <= 20)
+    launch_k<METRIC, 20>(a, s);
   else if (k <= 24)
     launch_k<METRIC, 24>(a, s);
   else if (k <= 32)
@@ -393,7 +397,8 @@ void launch_m(const MfmaLeafArgs& a, int k, hipStream_t s) {
 
 void launch_leaf_mfma(int metric, const float* x, int d, int ld, const _Float16* xh, int ldh, const float* eps,
                       const float* nh, const float* norms, const int32_t* perm, const Leaf* leaves, int nleaves,
-                      int max_leaf, int k, int32_t* ids, float* vals, unsigned long long* rescored, hipStream_t s) {
+                      int max_leaf, int k, int32_t* ids, float* vals, const int32_t* rsp, const int32_t* rcp,
+                      unsigned long long* rescored, unsigned long long* screened, hipStream_t s) {
   if (nleaves <= 0) return;
   if (metric != kEuclidean && metric != kCosine) throw HipError("MFMA leaf screening supports EUCLIDEAN and COSINE");
   if (metric == kCosine && norms == nullptr) throw HipError("MFMA leaf screening: COSINE needs the norms");
@@ -409,6 +414,8 @@ void launch_leaf_mfma(int metric, const float* x, int d, int ld, const _Float16*
   a.norms = norms;
   a.perm = perm;
   a.leaves = leaves;
+  a.rsp = rsp;
+  a.rcp = rcp;
   a.nleaves = nleaves;
   a.chunks_per_leaf = (max_leaf + kTile - 1) / kTile;
   a.k = k;
@@ -425,6 +432,7 @@ void launch_leaf_mfma(int metric, const float* x, int d, int ld, const _Float16*
   a.kmf = static_cast<float>(km * (1.0 + 1e-6));
   a.kabf = static_cast<float>((a.kap + 4.0 * u) * (1.0 + 1e-6));
   a.rescored = rescored;
+  a.screened = screened;
   if (metric == kEuclidean)
     launch_m<kEuclidean>(a, k, s);
   else

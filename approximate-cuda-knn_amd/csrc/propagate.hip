@@ -106,11 +106,7 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
     for (int l = 0; l < k; ++l) was_prev |= __shfl(pid, l) == id;
     if (lane < k && id >= 0 && was_prev && changed_prev[id] == 0) expand = false;
   }
-  if constexpr (!SK) {
-    for (int s = lane; s < hs_size; s += 64) hs[s] = -1;
-  } else {
-    if (lane < k) hs[lane] = id;  // i's row, for the screen's membership test
-  }
+  for (int s = lane; s < hs_size; s += 64) hs[s] = -1;
   const float* xi = SK ? xf : xrow;
   const float* xs = xrow;
   if constexpr (SKM == 1) {
@@ -131,7 +127,7 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
   float thk = __shfl(key, k - 1);
   int32_t thi = __shfl(id, k - 1);
   const float ni = METRIC == kCosine ? norms[i] : 1.0f;
-  if constexpr (!SK) {
+  {
     // two-hop ids (.cu:1078-1085), deduplicated in the LDS hash set that
     // holds i and i's row
     if (lane == 0) *cnt = 0;
@@ -157,50 +153,18 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
     m = *cnt;
     nnew = m;
     if (ncand != nullptr && lane == 0) atomicAdd(ncand, static_cast<unsigned long long>(m));
-  } else {
-    // Two-hop ids gathered and screened in one pass, without a set: an id
-    // equal to i or in i's row is dropped by a compare against the row; a
-    // repeated id is screened again and, if it survives twice, refused at
-    // insertion (it is then in the list, or not below the k-th entry).  The
-    // pool's top-k is the same as with deduplication.  Survivors are
-    // appended to cand[] in ballot order.
-    {
+  }
+  if constexpr (SK) {
+    if (m > 0) {
+      // screen: survivors are written back to cand[] in order (slot
+      // w + rank <= base + lane was already read by this batch)
       const float ei = SKM == 2 ? e8[i] : eps[i];
       const double si = SKM == 2 ? static_cast<double>(s8[i]) : 0.0;
       const double ti = SKM == 2 ? si * si * static_cast<double>(a8[i]) : 0.0;
-      int w = 0, nex = 0;
-      // the ids of up to eight batches are loaded first (all in flight), so
-      // each batch's sketch loads do not wait behind its id load
-      constexpr int kIdBatches = 8;
-      for (int f1 = 0; f1 < kk; f1 += 64 * kIdBatches) {
-      int32_t cids[kIdBatches];
-#pragma unroll
-      for (int b = 0; b < kIdBatches; ++b) {
-        const int f = f1 + 64 * b + lane;
-        const int hi = f / k, lo = f - hi * k;
-        const int src = hi < 64 ? hi : 0;
-        const int32_t nb = __shfl(id, src);
-        const bool ex = __shfl(static_cast<int>(expand), src) != 0;
-        cids[b] = -1;
-        if (f < kk && nb >= 0 && ex) cids[b] = ids_in[static_cast<int64_t>(nb) * k + lo];
-      }
-#pragma unroll
-      for (int b = 0; b < kIdBatches; ++b) {
-        int32_t c = cids[b];
-        if (c == static_cast<int32_t>(i)) c = -1;
-        if (c >= 0) {
-          bool in_row = false;
-          for (int l = 0; l < k; ++l) in_row |= hs[l] == c;
-          if (in_row) c = -1;
-        }
-        cids[b] = c;
-      }
-      for (int b = 0; b < kIdBatches && f1 + 64 * b < kk; ++b) {
-        int32_t c = -1;
-#pragma unroll
-        for (int bb = 0; bb < kIdBatches; ++bb)
-          if (bb == b) c = cids[bb];
-        nex += __popcll(__ballot(c >= 0));
+      int w = 0;
+      for (int base = 0; base < m; base += 64) {
+        const int ci = base + lane;
+        const int32_t c = ci < m ? cand[ci] : -1;
         bool pass = false;
         if (c >= 0) {
           double lb;
@@ -251,15 +215,10 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
         if (pass) cand[w + __popcll(bal & ((1ull << lane) - 1))] = c;
         w += __popcll(bal);
       }
-      }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
       m = w;
-      nnew = nex;
-      if (ncand != nullptr && lane == 0) {
-        atomicAdd(ncand, static_cast<unsigned long long>(nex));
-        atomicAdd(ncand + 1, static_cast<unsigned long long>(m));
-      }
+      if (ncand != nullptr && lane == 0) atomicAdd(ncand + 1, static_cast<unsigned long long>(m));
     }
   }
 
@@ -303,7 +262,7 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
         surv &= surv - 1;
         const float nk = __shfl(dk, src);
         const int32_t nid = __shfl(c, src);
-        if (key_less(nk, nid, thk, thi) && !(SK && __ballot(lane < k && id == nid) != 0)) {
+        if (key_less(nk, nid, thk, thi)) {
           changed = true;
           wave_list_insert(key, id, k, nk, nid);
           thk = __shfl(key, k - 1);
@@ -329,7 +288,6 @@ void launch_w(const PropagateArgs& a, hipStream_t s) {
   // at most 1 + k + k*k ids enter the set: at least twice that many slots
   int hs_size = 128;
   while (hs_size < 2 * (a.k * a.k + a.k + 1)) hs_size *= 2;
-  if (SKM > 0) hs_size = 64;  // i's row only: the screened path has no set
   const int cand_cap = a.k * a.k;
   const int xi_stride = SKM == 1 ? a.ldh + 3 * a.ld : (SKM == 2 ? a.ld8 / 4 + 3 * a.ld : (a.d + 3) / 4 * 4);
   const size_t per_wave = (static_cast<size_t>(xi_stride) + hs_size + cand_cap + 4) * 4;

@@ -190,9 +190,7 @@ def test_sketch_changes_nothing(aknn, oracle, datasets, metric, fixed):
     if metric == 2 and not fixed:
         assert (ex, rs) == (ex0, rs0)  # COSINE parity keeps the exact path
     else:
-        # the screened pass examines repeated two-hop ids again (no set), so
-        # it counts at least the plain pass's unique candidates
-        assert ex >= ex0 and 0 < rs < ex0
+        assert ex == ex0 and 0 < rs < ex
     check_graph(oracle, a, x, 8, 2, 24, 4, metric, cosine_fixed=fixed)
 
 
