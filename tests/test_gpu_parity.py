@@ -176,10 +176,11 @@ def _same_graph(a, b):
     assert (ga[0] == gb[0]).all() and (bits(ga[1]) == bits(gb[1])).all()
 
 
-@pytest.mark.parametrize("metric,fixed", [(0, 0), (1, 0), (2, 1), (2, 0)])
-def test_sketch_changes_nothing(aknn, oracle, datasets, metric, fixed):
+@pytest.mark.parametrize("metric,fixed,d", [(0, 0, 40), (1, 0, 40), (2, 1, 40), (2, 0, 40), (0, 0, 128), (1, 0, 96),
+                                            (2, 1, 128)])
+def test_sketch_changes_nothing(aknn, oracle, datasets, metric, fixed, d):
     """The FP16 sketch screen of propagation gives the same graph (and the oracle's), re-scoring fewer candidates."""
-    x = datasets.gmm_c1(4000, 40, clusters=6, seed=31)[0]
+    x = datasets.gmm_c1(4000, d, clusters=6, seed=31)[0]
     a = gpu_run(aknn, x, 8, 2, 24, 4, metric, cosine_fixed=fixed, sketch=1)
     b = gpu_run(aknn, x, 8, 2, 24, 4, metric, cosine_fixed=fixed, sketch=0)
     _same_graph(a, b)
