@@ -21,7 +21,11 @@ EXPORTS = (
     "aknn_create", "aknn_destroy", "aknn_set_data_file", "aknn_set_results_file", "aknn_initialize",
     "aknn_initialize_host", "aknn_calculate", "aknn_save_result", "aknn_deinitialize", "aknn_set_option",
     "aknn_get_graph", "aknn_get_normalized", "aknn_get_forest", "aknn_get_stage_times",
-    "aknn_get_kernel_stats", "aknn_get_candidate_counts", "aknn_exact_knn", "aknn_recall", "aknn_last_error",
+    "aknn_get_kernel_stats", "aknn_get_candidate_counts", "aknn_exact_knn", "aknn_recall", "aknn_set_bins",
+    "aknn_last_error", "aknn_router_create", "aknn_router_destroy", "aknn_router_labels", "aknn_router_fit",
+    "aknn_router_gradients", "aknn_router_logits", "aknn_router_param_count", "aknn_router_get_params",
+    "aknn_router_set_params", "aknn_router_save", "aknn_router_load", "aknn_router_stats",
+    "aknn_router_last_error", "aknn_set_router",
 )
 
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -67,10 +71,30 @@ def load():
     _i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
     L.aknn_exact_knn.argtypes = [H, _i64p, ctypes.c_int64, ctypes.c_int, _i32p, _f64p]
     L.aknn_recall.argtypes = [H, _i64p, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)]
+    L.aknn_set_bins.argtypes = [H, _i32p, ctypes.c_int64, ctypes.c_int]
     L.aknn_last_error.argtypes = [H]
     L.aknn_last_error.restype = ctypes.c_char_p
+    R = ctypes.c_void_p
+    L.aknn_router_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint32]
+    L.aknn_router_create.restype = R
+    L.aknn_router_destroy.argtypes = [R]
+    L.aknn_router_destroy.restype = None
+    L.aknn_router_labels.argtypes = [R, H, ctypes.c_int, _i32p, _f64p]
+    L.aknn_router_fit.argtypes = [R, H, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float]
+    L.aknn_router_gradients.argtypes = [R, _f32p, ctypes.c_int64, _i32p, _f32p, ctypes.POINTER(ctypes.c_double)]
+    L.aknn_router_logits.argtypes = [R, _f32p, ctypes.c_int64, _f32p]
+    L.aknn_router_param_count.argtypes = [R, ctypes.POINTER(ctypes.c_int64)]
+    L.aknn_router_get_params.argtypes = [R, _f32p, _f32p, ctypes.POINTER(ctypes.c_float)]
+    L.aknn_router_set_params.argtypes = [R, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float]
+    L.aknn_router_save.argtypes = [R, ctypes.c_char_p]
+    L.aknn_router_load.argtypes = [R, ctypes.c_char_p]
+    L.aknn_router_stats.argtypes = [R, _f64p, ctypes.c_int]
+    L.aknn_router_last_error.argtypes = [R]
+    L.aknn_router_last_error.restype = ctypes.c_char_p
+    L.aknn_set_router.argtypes = [H, R]
     for name in EXPORTS:
-        if name not in ("aknn_create", "aknn_destroy", "aknn_last_error"):
+        if name not in ("aknn_create", "aknn_destroy", "aknn_last_error", "aknn_router_create",
+                        "aknn_router_destroy", "aknn_router_last_error"):
             getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -187,8 +211,117 @@ class Graph:
         self._check(self._L.aknn_recall(self._h, q, len(q), ctypes.byref(r)))
         return r.value
 
+    def set_bins(self, bins, bin_bits):
+        """Router bins [n] of the points (values < 2^bin_bits) for the routed
+        trees of the median-split forest."""
+        b = np.ascontiguousarray(bins, dtype=np.int32)
+        self._check(self._L.aknn_set_bins(self._h, b, len(b), bin_bits))
+
+    def set_router(self, router):
+        """Attaches a trained Router (None detaches); routed trees (option
+        router_trees, split=1) then take its bins inside calculate()."""
+        self._check(self._L.aknn_set_router(self._h, router._r if router is not None else None))
+        self._router = router
+
     def deinitialize(self):
         self._check(self._L.aknn_deinitialize(self._h))
+
+
+class Router:
+    """The learned router (MLP d -> hidden -> hidden -> 2^bin_bits bins) of the
+    median-split forest, trained on the device on a graph this library built."""
+
+    def __init__(self, d, hidden=256, bin_bits=8, seed=1, device=0):
+        self._L = load()
+        self._r = self._L.aknn_router_create(device, d, hidden, bin_bits, seed)
+        if not self._r:
+            raise AknnError("aknn_router_create failed (device, D, hidden or bin_bits out of range)")
+        self.d, self.hidden, self.bin_bits, self.bins = d, hidden, bin_bits, 1 << bin_bits
+        self.dp = (d + 15) // 16 * 16
+
+    def _check(self, rc):
+        if rc != 0:
+            raise AknnError(self._L.aknn_router_last_error(self._r).decode(errors="replace") or f"status {rc}")
+
+    def close(self):
+        if self._r:
+            self._L.aknn_router_destroy(self._r)
+            self._r = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def labels(self, graph, sweeps=10):
+        """Balanced graph labels of graph's points and the (before, after) edge-cut fractions."""
+        lab = np.empty(graph.n, np.int32)
+        cut = np.zeros(2, np.float64)
+        self._check(self._L.aknn_router_labels(self._r, graph._h, sweeps, lab, cut))
+        return lab, (float(cut[0]), float(cut[1]))
+
+    def fit(self, graph, labels=None, sweeps=10, epochs=10, batch=1024, lr=2e-3):
+        lab = None if labels is None else np.ascontiguousarray(labels, dtype=np.int32)
+        ptr = None if lab is None else lab.ctypes.data_as(ctypes.c_void_p)
+        self._check(self._L.aknn_router_fit(self._r, graph._h, ptr, sweeps, epochs, batch, lr))
+        return self.stats()
+
+    def gradients(self, x, labels):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        lab = np.ascontiguousarray(labels, dtype=np.int32)
+        g = np.empty(self.param_count(), np.float32)
+        loss = ctypes.c_double()
+        self._check(self._L.aknn_router_gradients(self._r, x, len(x), lab, g, ctypes.byref(loss)))
+        return g, loss.value
+
+    def logits(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        z = np.empty((len(x), self.bins), np.float32)
+        self._check(self._L.aknn_router_logits(self._r, x, len(x), z))
+        return z
+
+    def param_count(self):
+        c = ctypes.c_int64()
+        self._check(self._L.aknn_router_param_count(self._r, ctypes.byref(c)))
+        return c.value
+
+    def get_params(self):
+        p = np.empty(self.param_count(), np.float32)
+        mu = np.empty(self.d, np.float32)
+        inv = ctypes.c_float()
+        self._check(self._L.aknn_router_get_params(self._r, p, mu, ctypes.byref(inv)))
+        return p, mu, inv.value
+
+    def set_params(self, params=None, mu=None, inv_sigma=1.0):
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float32)
+        m = None if mu is None else np.ascontiguousarray(mu, dtype=np.float32)
+        self._check(self._L.aknn_router_set_params(
+            self._r, None if p is None else p.ctypes.data_as(ctypes.c_void_p),
+            None if m is None else m.ctypes.data_as(ctypes.c_void_p), inv_sigma))
+
+    def split_params(self, p):
+        """Views W1 [hidden][dp], b1, W2, b2, W3 [bins][hidden], b3 of a flat parameter array."""
+        h, dp, nb = self.hidden, self.dp, self.bins
+        sizes = [h * dp, h, h * h, h, nb * h, nb]
+        shapes = [(h, dp), (h,), (h, h), (h,), (nb, h), (nb,)]
+        out, o = [], 0
+        for sz, sh in zip(sizes, shapes):
+            out.append(p[o:o + sz].reshape(sh))
+            o += sz
+        return out
+
+    def save(self, path):
+        self._check(self._L.aknn_router_save(self._r, path.encode()))
+
+    def load(self, path):
+        self._check(self._L.aknn_router_load(self._r, path.encode()))
+
+    def stats(self):
+        v = np.zeros(7, np.float64)
+        self._check(self._L.aknn_router_stats(self._r, v, 7))
+        keys = ("epochs", "loss", "accuracy", "train_ms", "cut_before", "cut_after", "assign_ms")
+        return dict(zip(keys, v.tolist()))
 
 
 def build_graph(x, k, trees, leaf, props, metric, device=0, **options):

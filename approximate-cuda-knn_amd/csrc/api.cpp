@@ -11,6 +11,12 @@
 
 #include "engine.hpp"
 #include "io.hpp"
+#include "router.hpp"
+
+struct aknn_router {
+  std::unique_ptr<aknn::Router> router;
+  std::string error;
+};
 
 struct aknn_handle {
   bool debug = false;
@@ -65,6 +71,30 @@ int setup(aknn_handle* h, const float* x, int64_t n, int d, int k, int device, i
   h->params = p;
   h->calculated = false;
   return AKNN_OK;
+}
+
+int rfail(aknn_router* r, int code, const std::string& msg) {
+  if (r) r->error = msg;
+  return code;
+}
+
+template <typename F>
+int rguarded(aknn_router* r, F&& f) {
+  if (!r || !r->router) return AKNN_ERR_ARGUMENT;
+  try {
+    r->error.clear();
+    return f();
+  } catch (const aknn::HipError& e) {
+    return rfail(r, AKNN_ERR_DEVICE, e.what);
+  } catch (const std::invalid_argument& e) {
+    return rfail(r, AKNN_ERR_ARGUMENT, e.what());
+  } catch (const std::bad_alloc&) {
+    return rfail(r, AKNN_ERR_DEVICE, "out of host memory");
+  } catch (const std::exception& e) {
+    return rfail(r, AKNN_ERR_STATE, e.what());
+  } catch (...) {
+    return rfail(r, AKNN_ERR_STATE, "unknown error");
+  }
 }
 
 }  // namespace
@@ -179,6 +209,12 @@ int aknn_set_option(aknn_handle* h, const char* name, int64_t value) {
       h->params.leaf_mfma = value != 0;
     } else if (n == "sketch_int8") {
       h->params.sketch_int8 = value != 0;
+    } else if (n == "split") {
+      if (value != 0 && value != 1) return fail(h, AKNN_ERR_ARGUMENT, "split is 0 (sign) or 1 (median)");
+      h->params.split = static_cast<int>(value);
+    } else if (n == "router_trees") {
+      if (value < 0 || value > 1 << 20) return fail(h, AKNN_ERR_ARGUMENT, "router_trees must be >= 0");
+      h->params.router_trees = static_cast<int>(value);
     } else {
       return fail(h, AKNN_ERR_ARGUMENT, "unknown option " + n);
     }
@@ -290,6 +326,150 @@ int aknn_recall(aknn_handle* h, const int64_t* queries, int64_t nq, double* reca
     if (!h->engine || !h->calculated) return fail(h, AKNN_ERR_STATE, "nothing calculated");
     if (nq < 0 || !recall || (nq > 0 && !queries)) return fail(h, AKNN_ERR_ARGUMENT, "null array");
     *recall = h->engine->recall(queries, nq);
+    return AKNN_OK;
+  });
+}
+
+int aknn_set_bins(aknn_handle* h, const int32_t* bins, int64_t n, int bin_bits) {
+  return guarded(h, [&] {
+    if (!h->engine) return fail(h, AKNN_ERR_STATE, "not initialized");
+    if (!bins) return fail(h, AKNN_ERR_ARGUMENT, "null bins");
+    if (n != h->engine->n()) return fail(h, AKNN_ERR_ARGUMENT, "bins must hold one entry per point");
+    h->engine->set_bins(bins, bin_bits);
+    return AKNN_OK;
+  });
+}
+
+aknn_router* aknn_router_create(int device, int d, int hidden, int bin_bits, uint32_t seed) {
+  try {
+    auto* r = new aknn_router();
+    r->router = std::make_unique<aknn::Router>(device, d, hidden, bin_bits, seed);
+    return r;
+  } catch (...) {
+    return nullptr;
+  }
+}
+
+void aknn_router_destroy(aknn_router* r) {
+  try {
+    delete r;
+  } catch (...) {
+  }
+}
+
+int aknn_router_labels(aknn_router* r, aknn_handle* h, int sweeps, int32_t* labels, double* cut) {
+  return rguarded(r, [&] {
+    if (!h || !h->engine || !h->calculated) return rfail(r, AKNN_ERR_STATE, "the graph handle has no graph");
+    if (!labels || sweeps < 0) return rfail(r, AKNN_ERR_ARGUMENT, "bad arguments");
+    std::vector<int32_t> lab;
+    r->router->graph_labels(*h->engine, sweeps, lab);
+    std::memcpy(labels, lab.data(), sizeof(int32_t) * lab.size());
+    if (cut) {
+      cut[0] = r->router->train_stats().cut_before;
+      cut[1] = r->router->train_stats().cut_after;
+    }
+    return AKNN_OK;
+  });
+}
+
+int aknn_router_fit(aknn_router* r, aknn_handle* h, const int32_t* labels, int sweeps, int epochs, int batch,
+                    float lr) {
+  return rguarded(r, [&] {
+    if (!h || !h->engine || !h->calculated) return rfail(r, AKNN_ERR_STATE, "the graph handle has no graph");
+    std::vector<int32_t> lab;
+    if (labels)
+      lab.assign(labels, labels + h->engine->n());
+    else
+      r->router->graph_labels(*h->engine, sweeps, lab);
+    r->router->fit(*h->engine, lab, epochs, batch, lr);
+    return AKNN_OK;
+  });
+}
+
+int aknn_router_gradients(aknn_router* r, const float* x, int64_t m, const int32_t* labels, float* grads,
+                          double* loss) {
+  return rguarded(r, [&] {
+    if (!x || !labels || !grads) return rfail(r, AKNN_ERR_ARGUMENT, "null array");
+    const double l = r->router->gradients(x, m, labels, grads);
+    if (loss) *loss = l;
+    return AKNN_OK;
+  });
+}
+
+int aknn_router_logits(aknn_router* r, const float* x, int64_t m, float* z) {
+  return rguarded(r, [&] {
+    if (!x || !z) return rfail(r, AKNN_ERR_ARGUMENT, "null array");
+    r->router->logits(x, m, z);
+    return AKNN_OK;
+  });
+}
+
+int aknn_router_param_count(aknn_router* r, int64_t* count) {
+  return rguarded(r, [&] {
+    if (!count) return rfail(r, AKNN_ERR_ARGUMENT, "null output");
+    *count = static_cast<int64_t>(r->router->param_count());
+    return AKNN_OK;
+  });
+}
+
+int aknn_router_get_params(aknn_router* r, float* params, float* mu, float* inv_sigma) {
+  return rguarded(r, [&] {
+    r->router->get_params(params, mu, inv_sigma);
+    return AKNN_OK;
+  });
+}
+
+int aknn_router_set_params(aknn_router* r, const float* params, const float* mu, float inv_sigma) {
+  return rguarded(r, [&] {
+    r->router->set_params(params, mu, inv_sigma);
+    return AKNN_OK;
+  });
+}
+
+int aknn_router_save(aknn_router* r, const char* path) {
+  return rguarded(r, [&] {
+    if (!path) return rfail(r, AKNN_ERR_ARGUMENT, "null path");
+    try {
+      r->router->save(path);
+    } catch (const std::runtime_error& e) {
+      return rfail(r, AKNN_ERR_IO, e.what());
+    }
+    return AKNN_OK;
+  });
+}
+
+int aknn_router_load(aknn_router* r, const char* path) {
+  return rguarded(r, [&] {
+    if (!path) return rfail(r, AKNN_ERR_ARGUMENT, "null path");
+    try {
+      r->router->load(path);
+    } catch (const std::runtime_error& e) {
+      return rfail(r, AKNN_ERR_IO, e.what());
+    }
+    return AKNN_OK;
+  });
+}
+
+int aknn_router_stats(aknn_router* r, double* out, int n) {
+  return rguarded(r, [&] {
+    if (!out) return rfail(r, AKNN_ERR_ARGUMENT, "null output");
+    const aknn::RouterTrainStats& t = r->router->train_stats();
+    const double v[7] = {static_cast<double>(t.epochs), t.loss, t.accuracy, t.train_ms, t.cut_before, t.cut_after,
+                         r->router->last_assign_ms()};
+    for (int i = 0; i < n && i < 7; ++i) out[i] = v[i];
+    return AKNN_OK;
+  });
+}
+
+const char* aknn_router_last_error(aknn_router* r) { return r ? r->error.c_str() : "null router"; }
+
+int aknn_set_router(aknn_handle* h, aknn_router* r) {
+  return guarded(h, [&] {
+    if (!h->engine) return fail(h, AKNN_ERR_STATE, "not initialized");
+    if (r && !r->router) return fail(h, AKNN_ERR_ARGUMENT, "empty router");
+    if (r && (r->router->device() != h->device || r->router->d() != h->engine->d()))
+      return fail(h, AKNN_ERR_ARGUMENT, "the router must have the data's D and live on the handle's device");
+    h->engine->set_router(r ? r->router.get() : nullptr);
     return AKNN_OK;
   });
 }

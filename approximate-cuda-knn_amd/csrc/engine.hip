@@ -13,6 +13,9 @@
 //   * one stream, no per-launch synchronisation.
 #include "engine.hpp"
 
+#include "directions.hpp"
+#include "router.hpp"
+
 #include <algorithm>
 #include <cstring>
 #include <stdexcept>
@@ -129,6 +132,33 @@ void Engine::release() {
   changed_[0] = changed_[1] = nullptr;
   fb_ = ForestBuffers{};
   cps_ = CosineParityScratch{};
+  dirs_ = proj_ = nullptr;
+  nd_ = 0;
+  dirs_key_ = -1;
+  bins_ = nullptr;
+  bin_bits_ = 0;
+  bins_set_ = false;
+}
+
+void Engine::set_bins(const int32_t* bins, int bin_bits) {
+  if (n_ <= 0) throw std::invalid_argument("set the points first");
+  if (bin_bits < 1 || bin_bits > 20) throw std::invalid_argument("bin_bits must be in [1, 20]");
+  for (int64_t i = 0; i < n_; ++i)
+    if (bins[i] < 0 || bins[i] >= (1 << bin_bits)) throw std::invalid_argument("bin out of range");
+  AKNN_HIP_TRY(hipSetDevice(device_));
+  if (!bins_) bins_ = alloc<int32_t>(n_);
+  AKNN_HIP_TRY(hipMemcpy(bins_, bins, sizeof(int32_t) * n_, hipMemcpyHostToDevice));
+  bin_bits_ = bin_bits;
+  bins_set_ = true;
+}
+
+void Engine::set_bins_device(const int32_t* bins, int bin_bits) {
+  if (bin_bits < 1 || bin_bits > 20) throw std::invalid_argument("bin_bits must be in [1, 20]");
+  if (!bins_) bins_ = alloc<int32_t>(n_);
+  if (bins != bins_)
+    AKNN_HIP_TRY(hipMemcpyAsync(bins_, bins, sizeof(int32_t) * n_, hipMemcpyDeviceToDevice, stream_));
+  bin_bits_ = bin_bits;
+  bins_set_ = true;
 }
 
 void Engine::set_data(const float* x_rowmajor, int64_t n, int d, const Params& p) {
@@ -140,6 +170,7 @@ void Engine::set_data(const float* x_rowmajor, int64_t n, int d, const Params& p
   if (p.trees < 1) throw std::invalid_argument("the number of trees must be at least 1");
   if (p.leaf < 0 || p.props < 0) throw std::invalid_argument("leaf size and propagations must be >= 0");
   if (p.metric < kEuclidean || p.metric > kCosine) throw std::invalid_argument("unknown distance");
+  if (p.split < 0 || p.split > 1) throw std::invalid_argument("split must be 0 (sign) or 1 (median)");
   AKNN_HIP_TRY(hipSetDevice(device_));
   AKNN_HIP_TRY(hipStreamSynchronize(stream_));
   release();
@@ -238,6 +269,76 @@ void Engine::build_forest() {
     stats_forest.units = static_cast<double>(active_sum_) * (2.0 * ldh_ + 8.0 + 16.0) + 4.0 * d_ * amb +
                          8.0 * n_ * p_.trees;
   }
+}
+
+// Median-split forest: per tree, the projections on its seeded directions
+// (one pass over the points), then the level loop with the bin / median /
+// position rules; no rand() draws.  The leaf list, start ranges and
+// partition machinery are the sign-split forest's.
+void Engine::build_forest_median() {
+  nleaves_.assign(p_.trees, 0);
+  iters_.assign(p_.trees, 0);
+  active_sum_ = 0;
+  stats_forest = KernelStats{};
+  const int nd = median_depth(n_, leaf_eff_);
+  const int64_t key = (static_cast<int64_t>(p_.seed) << 20) ^ (static_cast<int64_t>(p_.trees) << 8) ^ nd;
+  if (!dirs_ || nd != nd_ || key != dirs_key_) {
+    if (!dirs_ || nd > nd_) {
+      dirs_ = alloc<float>(static_cast<size_t>(p_.trees) * nd * ld_);
+      proj_ = alloc<float>(static_cast<size_t>(nd) * n_);
+    }
+    nd_ = nd;
+    std::vector<float> all(static_cast<size_t>(p_.trees) * nd * ld_), one;
+    for (int t = 0; t < p_.trees; ++t) {
+      tree_directions(p_.seed, t, nd, d_, ld_, one);
+      std::copy(one.begin(), one.end(), all.begin() + static_cast<size_t>(t) * nd * ld_);
+    }
+    AKNN_HIP_TRY(hipMemcpy(dirs_, all.data(), sizeof(float) * all.size(), hipMemcpyHostToDevice));
+    dirs_key_ = key;
+  }
+  if (router_ && p_.router_trees > 0) {
+    // learned leaf assignments of this data set (inside the timed bracket)
+    if (router_->device() != device_ || router_->d() != d_) throw std::invalid_argument("router does not fit the data");
+    if (!bins_) bins_ = alloc<int32_t>(n_);
+    router_->assign(x_raw_, n_, ld_, bins_, stream_);
+    bin_bits_ = router_->bin_bits();
+    bins_set_ = true;
+  }
+  const int routed_trees = bins_set_ ? std::min(p_.router_trees, p_.trees) : 0;
+  for (int t = 0; t < p_.trees; ++t) {
+    int32_t* perm = perm_ + static_cast<int64_t>(t) * n_;
+    Leaf* leaves = leaves_ + static_cast<int64_t>(t) * n_;
+    const bool routed = t < routed_trees;
+    const int bb = routed ? bin_bits_ : 0;
+    launch_project(x_, n_, d_, ld_, dirs_ + static_cast<size_t>(t) * nd * ld_, nd, proj_, stream_);
+    launch_forest_init(fb_, perm, leaves, n_, 0, 0, leaf_eff_, p_.k + 1, stream_);
+    int level = 0;
+    for (;;) {
+      for (int b = 0; b < p_.level_batch; ++b, ++level) {
+        if (level < bb) {
+          launch_forest_level_rule(fb_, 0, nullptr, bins_, bb - 1 - level, perm, leaves, level, leaf_eff_, p_.k + 1,
+                                   stream_);
+        } else if (level - bb < nd) {
+          launch_forest_level_rule(fb_, 1, proj_ + static_cast<int64_t>(level - bb) * n_, nullptr, 0, perm, leaves,
+                                   level, leaf_eff_, p_.k + 1, stream_);
+        } else {
+          launch_forest_level_rule(fb_, 2, nullptr, nullptr, 0, perm, leaves, level, leaf_eff_, p_.k + 1, stream_);
+        }
+      }
+      AKNN_HIP_TRY(hipMemcpyAsync(ctl_host_, fb_.ctl, sizeof(ForestCtl), hipMemcpyDeviceToHost, stream_));
+      AKNN_HIP_TRY(hipStreamSynchronize(stream_));
+      if (ctl_host_->done) break;
+      if (level > n_ + bb + 2) throw HipError("forest level loop did not terminate");
+    }
+    iters_[t] = ctl_host_->iterations;
+    nleaves_[t] = ctl_host_->nleaves;
+    active_sum_ += ctl_host_->active_sum;
+  }
+  // algorithmic bytes: one pass over the rows and the projection writes per
+  // tree, then per level and active point its projection, id and bucket
+  // bookkeeping, plus the leaf array write
+  stats_forest.units = static_cast<double>(p_.trees) * n_ * (4.0 * d_ + 4.0 * nd) +
+                       static_cast<double>(active_sum_) * 20.0 + 8.0 * n_ * p_.trees;
 }
 
 void Engine::initial_knn() {
@@ -435,7 +536,10 @@ void Engine::calculate() {
   if (sketch) launch_sketch(x_, n_, d_, ld_, p_.metric, xh_, ldh_, eps_, nh_, stream_);
   if (sketch8) launch_sketch8(x_, n_, d_, ld_, q8_, ld8_, s8_, a8_, e8_, stream_);
   AKNN_HIP_TRY(hipEventRecord(ev(1), stream_));
-  build_forest();
+  if (p_.split == 1)
+    build_forest_median();
+  else
+    build_forest();
   AKNN_HIP_TRY(hipEventRecord(ev(2), stream_));
   initial_knn();
   AKNN_HIP_TRY(hipEventRecord(ev(3), stream_));

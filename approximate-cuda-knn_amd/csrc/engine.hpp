@@ -28,6 +28,8 @@ struct Params {
   int sketch = 0;         // propagation screens candidates with the FP16 sketch bound (same results)
   int leaf_mfma = 0;      // leaf k-NN of trees after the first screens pairs on the matrix cores (same results)
   int sketch_int8 = 0;    // with sketch, EUCLIDEAN propagation screens with the int8 sketch (same results)
+  int split = 0;          // forest rule: 0 the reference's sign split, 1 median split on seeded directions
+  int router_trees = 0;   // median split: trees 0..router_trees-1 take the router bins as their first levels
 };
 
 // Per-stage times of the last calculate(), milliseconds.
@@ -68,7 +70,21 @@ class Engine {
     p_.sketch = p.sketch;
     p_.leaf_mfma = p.leaf_mfma;
     p_.sketch_int8 = p.sketch_int8;
+    if (p_.split != p.split) prop_mode_ = -1;
+    p_.split = p.split;
+    p_.router_trees = p.router_trees;
   }
+  // Router bins of the points (host array [n], values < 2^bin_bits): the
+  // first bin_bits levels of routed median-split trees split on their bits.
+  void set_bins(const int32_t* bins, int bin_bits);
+  // Bins already on the device (router inference), same contract.
+  void set_bins_device(const int32_t* bins, int bin_bits);
+  int32_t* bins_device() { return bins_; }
+  // A trained router (nullable: detach) that assigns the bins of routed
+  // trees inside calculate(); it must live on this engine's device.
+  void set_router(class Router* r) { router_ = r; }
+  const float* raw_points() const { return x_raw_; }
+  int ld() const { return ld_; }
 
   void get_graph(int32_t* ids, float* vals) const;
   void get_normalized(float* x) const;
@@ -98,6 +114,7 @@ class Engine {
     return p_.leaf_mfma && p_.trees > 1 && (p_.metric == kEuclidean || (p_.metric == kCosine && p_.cosine_fixed));
   }
   void build_forest();
+  void build_forest_median();
   void initial_knn();
   void propagate();
 
@@ -147,6 +164,16 @@ class Engine {
   std::vector<uint8_t> round_screened_, tree_screened_;  // which rounds / trees used the screens
   int prop_mode_ = -1;  // propagation rounds learned on this data: -1 unknown, 0 screened, 1 exact
   DrawStream draws_;
+  // median-split forest: directions [trees][nd][ld] (cached per seed),
+  // projections of one tree [nd][n], router bins [n]
+  float* dirs_ = nullptr;
+  float* proj_ = nullptr;
+  int nd_ = 0;
+  int64_t dirs_key_ = -1;
+  int32_t* bins_ = nullptr;
+  int bin_bits_ = 0;
+  bool bins_set_ = false;
+  class Router* router_ = nullptr;
   StageTimes times_;
 };
 

@@ -387,8 +387,12 @@ __global__ __launch_bounds__(1024) void forest_level_kernel(Bucket* __restrict__
           ++cidx;
           cast += size;
         } else {
-          const int slot = atomicAdd(&ctl->nleaves, 1);
-          leaves[slot] = Leaf{fst, size, rstart, rcount};
+          // an empty side (possible only under the bin rule of routed trees)
+          // is not a leaf
+          if (size > 0) {
+            const int slot = atomicAdd(&ctl->nleaves, 1);
+            leaves[slot] = Leaf{fst, size, rstart, rcount};
+          }
           b.child[c] = -1;
           b.cast[c] = 0;
         }
@@ -429,6 +433,174 @@ __global__ __launch_bounds__(256) void forest_scatter_kernel(const int32_t* __re
     } else {
       perm[b.cfs[bit] + rank] = id;
     }
+  }
+}
+
+// ---- median-split rule (beside the reference's sign split) -------------------
+//
+// A level of a median-split tree splits each bucket by one of three rules:
+//   bins:       bit `shift` of the point's router bin (routed trees' first
+//               levels), RIGHT when set;
+//   projection: the point's projection p = x . r_j (project_kernel) against
+//               the bucket's pivot (forest_median_kernel): RIGHT when
+//               (ord(p), rel) >= (ord(p*), rel*), rel = position in the bucket;
+//   position:   RIGHT when rel >= size / 2 (directions exhausted).
+// Both sides of a projection or position split are non-empty for size >= 2.
+
+// Unsigned key whose order is the float order (-0 < +0, NaNs at the ends).
+__device__ __forceinline__ uint32_t ord_u32(float f) {
+  const uint32_t b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+constexpr int kProjGroup = 16;   // directions per pass over a row
+constexpr int kProjChunk = 256;  // dimensions staged in LDS per step
+
+// P[j * n + i] = x_i . r_j as one FMA chain in dimension order (FP32, the
+// oracle's chain), for j < nd.  One lane per point; the directions of a group
+// are staged through LDS a chunk of dimensions at a time and read as
+// broadcasts; each lane keeps kProjGroup running chains.
+__global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ x, int64_t n, int d, int ld,
+                                                      const float* __restrict__ dirs, int nd,
+                                                      float* __restrict__ proj) {
+  __shared__ __attribute__((aligned(16))) float rs[kProjGroup][kProjChunk];
+  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+  const bool live = i < n;
+  const float* xr = x + (live ? i : 0) * ld;
+  for (int g0 = 0; g0 < nd; g0 += kProjGroup) {
+    float acc[kProjGroup];
+#pragma unroll
+    for (int g = 0; g < kProjGroup; ++g) acc[g] = 0.0f;
+    for (int c0 = 0; c0 < d; c0 += kProjChunk) {
+      const int cw = d - c0 < kProjChunk ? d - c0 : kProjChunk;
+      __syncthreads();
+      for (int e = threadIdx.x; e < kProjGroup * kProjChunk; e += 256) {
+        const int g = e / kProjChunk, c = e % kProjChunk;
+        rs[g][c] = (g0 + g < nd && c < cw) ? dirs[static_cast<int64_t>(g0 + g) * ld + c0 + c] : 0.0f;
+      }
+      __syncthreads();
+      int c = 0;
+      for (; c + 4 <= cw; c += 4) {
+        const float4 v = *reinterpret_cast<const float4*>(xr + c0 + c);
+#pragma unroll
+        for (int g = 0; g < kProjGroup; ++g) {
+          const float4 r = *reinterpret_cast<const float4*>(&rs[g][c]);
+          acc[g] = __builtin_fmaf(v.x, r.x, acc[g]);
+          acc[g] = __builtin_fmaf(v.y, r.y, acc[g]);
+          acc[g] = __builtin_fmaf(v.z, r.z, acc[g]);
+          acc[g] = __builtin_fmaf(v.w, r.w, acc[g]);
+        }
+      }
+      for (; c < cw; ++c) {
+        const float v = xr[c0 + c];
+#pragma unroll
+        for (int g = 0; g < kProjGroup; ++g) acc[g] = __builtin_fmaf(v, rs[g][c], acc[g]);
+      }
+    }
+    if (live) {
+#pragma unroll
+      for (int g = 0; g < kProjGroup; ++g)
+        if (g0 + g < nd) proj[static_cast<int64_t>(g0 + g) * n + i] = acc[g];
+    }
+  }
+}
+
+constexpr int kMedianSamples = 511;
+
+// Pivot of every active bucket: m = min(size, 511) samples at relative
+// positions q * size / m, ordered by (ord(p), rel) with a bitonic sort in LDS;
+// the sample at index m / 2 is stored as (q1 = rel*, q2 = ord(p*)).
+__global__ __launch_bounds__(256) void forest_median_kernel(const float* __restrict__ pj,
+                                                            const int32_t* __restrict__ act_ids,
+                                                            Bucket* __restrict__ bkt, const ForestCtl* ctl,
+                                                            int parity) {
+  __shared__ uint64_t key[512];
+  const int nb = ctl->nbkt[parity];
+  for (int bi = blockIdx.x; bi < nb; bi += gridDim.x) {
+    const int32_t astart = bkt[bi].astart, size = bkt[bi].size;
+    const int m = size < kMedianSamples ? size : kMedianSamples;
+    for (int q = threadIdx.x; q < 512; q += 256) {
+      uint64_t kv = ~0ull;
+      if (q < m) {
+        const int32_t rel = static_cast<int32_t>(static_cast<int64_t>(q) * size / m);
+        kv = (static_cast<uint64_t>(ord_u32(pj[act_ids[astart + rel]])) << 32) | static_cast<uint32_t>(rel);
+      }
+      key[q] = kv;
+    }
+    __syncthreads();
+    for (int kk = 2; kk <= 512; kk <<= 1) {
+      for (int j = kk >> 1; j > 0; j >>= 1) {
+        const int t = threadIdx.x;
+        const int lo = 2 * j * (t / j) + (t % j);
+        const int hi = lo + j;
+        const bool up = (lo & kk) == 0;
+        const uint64_t a = key[lo], c = key[hi];
+        if ((a > c) == up) {
+          key[lo] = c;
+          key[hi] = a;
+        }
+        __syncthreads();
+      }
+    }
+    if (threadIdx.x == 0) {
+      const uint64_t pv = key[m / 2];
+      bkt[bi].q1 = static_cast<int32_t>(static_cast<uint32_t>(pv));
+      bkt[bi].q2 = static_cast<int32_t>(static_cast<uint32_t>(pv >> 32));
+    }
+    __syncthreads();
+  }
+}
+
+// Side pass of a median-split level (RULE 0 bins, 1 projection, 2 position),
+// with the same ballot words and prefixes as forest_side_kernel.
+template <int RULE>
+__global__ __launch_bounds__(256) void forest_side_rule_kernel(const int32_t* __restrict__ bins, int shift,
+                                                               const float* __restrict__ pj,
+                                                               const int32_t* __restrict__ act_ids,
+                                                               const int32_t* __restrict__ act_bkt,
+                                                               const Bucket* __restrict__ bkt, const ForestCtl* ctl,
+                                                               int parity, uint64_t* __restrict__ rmask,
+                                                               int32_t* __restrict__ wpre, int32_t* __restrict__ blk_r) {
+  __shared__ int wcnt[kWordsPerBlock];
+  const int nact = ctl->nact[parity];
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  for (int64_t blk = blockIdx.x; blk * kSideBlock < nact; blk += gridDim.x) {
+    for (int sub = 0; sub < kSideBlock / 256; ++sub) {
+      const int64_t m = blk * kSideBlock + sub * 256 + threadIdx.x;
+      bool right = false;
+      if (m < nact) {
+        const int32_t id = act_ids[m];
+        if constexpr (RULE == 0) {
+          right = ((bins[id] >> shift) & 1) != 0;
+        } else {
+          const Bucket& b = bkt[act_bkt[m]];
+          const int32_t rel = static_cast<int32_t>(m) - b.astart;
+          if constexpr (RULE == 1) {
+            const uint32_t u = ord_u32(pj[id]), pu = static_cast<uint32_t>(b.q2);
+            right = u > pu || (u == pu && rel >= b.q1);
+          } else {
+            right = rel >= b.size / 2;
+          }
+        }
+      }
+      const uint64_t bal = __ballot(right);
+      const int w = sub * 4 + wave;
+      if (lane == 0) {
+        rmask[blk * kWordsPerBlock + w] = bal;
+        wcnt[w] = __popcll(bal);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int acc = 0;
+      for (int w = 0; w < kWordsPerBlock; ++w) {
+        wpre[blk * kWordsPerBlock + w] = acc;
+        acc += wcnt[w];
+      }
+      blk_r[blk] = acc;
+    }
+    __syncthreads();
   }
 }
 
@@ -491,6 +663,45 @@ void launch_forest_level(const ForestBuffers& fb, const float* x, int d, int ld,
   AKNN_HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(forest_level_kernel, dim3(1), dim3(1024), 0, s, fb.bkt[par], fb.bkt[par ^ 1], fb.blk_r,
                      fb.blk_roff, fb.rmask, fb.wpre, fb.ctl, par, level, v1, v2, leaf_eff, kp1, leaves);
+  AKNN_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(forest_scatter_kernel, dim3(clamp_grid((fb.n + 255) / 256, 8192)), dim3(256), 0, s,
+                     fb.act_ids[par], fb.act_bkt[par], fb.bkt[par], fb.rmask, fb.wpre, fb.blk_roff, fb.ctl, par,
+                     fb.act_ids[par ^ 1], fb.act_bkt[par ^ 1], perm);
+  AKNN_HIP_TRY(hipGetLastError());
+}
+
+void launch_project(const float* x, int64_t n, int d, int ld, const float* dirs, int nd, float* proj,
+                    hipStream_t s) {
+  if (nd <= 0 || n <= 0) return;
+  if (ld % 4 != 0 || ld < d) throw HipError("project: malformed row width");
+  hipLaunchKernelGGL(project_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, x, n, d, ld, dirs,
+                     nd, proj);
+  AKNN_HIP_TRY(hipGetLastError());
+}
+
+void launch_forest_level_rule(const ForestBuffers& fb, int rule, const float* pj, const int32_t* bins, int shift,
+                              int32_t* perm, Leaf* leaves, int level, int leaf_eff, int kp1, hipStream_t s) {
+  const int par = level & 1;
+  const dim3 sg(clamp_grid((fb.n + kSideBlock - 1) / kSideBlock, 2048));
+  if (rule == 0) {
+    if (bins == nullptr || shift < 0 || shift > 30) throw HipError("forest: bin rule without bins");
+    hipLaunchKernelGGL(forest_side_rule_kernel<0>, sg, dim3(256), 0, s, bins, shift, pj, fb.act_ids[par],
+                       fb.act_bkt[par], fb.bkt[par], fb.ctl, par, fb.rmask, fb.wpre, fb.blk_r);
+  } else if (rule == 1) {
+    if (pj == nullptr) throw HipError("forest: projection rule without projections");
+    hipLaunchKernelGGL(forest_median_kernel, dim3(clamp_grid(fb.max_buckets, 4096)), dim3(256), 0, s, pj,
+                       fb.act_ids[par], fb.bkt[par], fb.ctl, par);
+    AKNN_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(forest_side_rule_kernel<1>, sg, dim3(256), 0, s, bins, shift, pj, fb.act_ids[par],
+                       fb.act_bkt[par], fb.bkt[par], fb.ctl, par, fb.rmask, fb.wpre, fb.blk_r);
+  } else {
+    hipLaunchKernelGGL(forest_side_rule_kernel<2>, sg, dim3(256), 0, s, bins, shift, pj, fb.act_ids[par],
+                       fb.act_bkt[par], fb.bkt[par], fb.ctl, par, fb.rmask, fb.wpre, fb.blk_r);
+  }
+  AKNN_HIP_TRY(hipGetLastError());
+  // the level kernel's pivot draw (v1, v2) is unused by these rules
+  hipLaunchKernelGGL(forest_level_kernel, dim3(1), dim3(1024), 0, s, fb.bkt[par], fb.bkt[par ^ 1], fb.blk_r,
+                     fb.blk_roff, fb.rmask, fb.wpre, fb.ctl, par, level, 0, 0, leaf_eff, kp1, leaves);
   AKNN_HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(forest_scatter_kernel, dim3(clamp_grid((fb.n + 255) / 256, 8192)), dim3(256), 0, s,
                      fb.act_ids[par], fb.act_bkt[par], fb.bkt[par], fb.rmask, fb.wpre, fb.blk_roff, fb.ctl, par,

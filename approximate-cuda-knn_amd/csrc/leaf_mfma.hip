@@ -177,7 +177,6 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
     o_e[lane] = my_eps;
   }
   o_t[lane] = owner_threshold<METRIC>(thk, my_eps, a.kap);
-  const float my_aa = METRIC == kCosine && has_owner ? a.norms[my_id] : 1.0f;
   __syncthreads();
   const int32_t oa = oid[col], ob = oid[32 + col];
   const _Float16* arow0 = oa >= 0 ? a.xh + static_cast<int64_t>(oa) * a.ldh : nullptr;

@@ -71,11 +71,16 @@ def sphere_c4(n=2_000_000, d=96, clusters=1000, seed=4):
     return x[rng.permutation(n)].astype(np.float32)
 
 
-def embedded_c5(n=4_000_000, d=128, latent=16, clusters=64, seed=5):
+def embedded_c5(n=4_000_000, d=128, latent=16, clusters=64, seed=5, sample_seed=None):
+    """seed fixes the distribution (centres, covariances, embedding) and, when
+    sample_seed is None, also the sample; a sample_seed draws another sample
+    of the same distribution (the router's training and test sets)."""
     rng = np.random.default_rng(seed)
     centres = rng.normal(0.0, 5.0, (clusters, latent))
     a = rng.normal(0.0, 0.25, (clusters, latent, latent))
     w = rng.normal(0.0, 0.25, (d, latent)).astype(np.float32)
+    if sample_seed is not None:
+        rng = np.random.default_rng(sample_seed)
     labels = rng.integers(0, clusters, n)
     z = np.empty((n, latent), np.float32)
     for c in range(clusters):
@@ -85,11 +90,16 @@ def embedded_c5(n=4_000_000, d=128, latent=16, clusters=64, seed=5):
     return x.astype(np.float32)
 
 
-def generate(config, n=None, seed=None):
-    """Points of configuration `config` ("C1".."C5"), optionally at another size."""
+def generate(config, n=None, seed=None, sample_seed=None):
+    """Points of configuration `config` ("C1".."C5"), optionally at another
+    size; sample_seed (C5) draws another sample of the same distribution."""
     c = CONFIGS[config]
     n = c["n"] if n is None else n
     kw = {} if seed is None else {"seed": seed}
+    if sample_seed is not None:
+        if config != "C5":
+            raise ValueError("sample_seed applies to C5")
+        kw["sample_seed"] = sample_seed
     if config == "C1":
         return gmm_c1(n, c["d"], **kw)[0]
     if config == "C2":

@@ -51,12 +51,44 @@ def cpu_baseline(datasets, config, cfg, sample_n):
     }
 
 
-# The two forms the metric compares.  Both give the same graph bit for bit
-# (checked on every run below); the chosen form only computes less.
+# The two forms the metric compares.  On C1-C4 both give the same graph bit
+# for bit (checked on every run below); the chosen form only computes less.
 FORMS = {
     "plain": {"sketch": 0},
     "chosen": {"sketch": 1, "leaf_mfma": 1},
 }
+
+# C5: the plain form is the reference's forest with exact FP32 everywhere;
+# the chosen form is the learned one (router bins as the first levels of
+# routed median-split trees, FP16 sketch and matrix-core screens), with its
+# own trees and rounds.  The graphs differ, so each form's recall is measured.
+C5_FORMS = {
+    "plain": {"opts": {"sketch": 0, "leaf_mfma": 0, "split": 0}},
+    "chosen": {"opts": {"sketch": 1, "leaf_mfma": 1, "split": 1, "router_trees": 6}, "trees": 6, "leaf": 256,
+               "props": 2, "router": True},
+}
+ROUTER = {"n": 200_000, "sample_seed": 105, "bin_bits": 8, "hidden": 256, "epochs": 10, "batch": 1024, "lr": 2e-3,
+          "sweeps": 10, "trees": 8, "leaf": 256, "props": 3}
+
+
+def train_router(aknn, datasets, config, cfg):
+    """The router of the learned form, trained on a graph the method builds
+    for a separate sample of the same distribution (outside the timed region)."""
+    rc = ROUTER
+    xt = datasets.generate(config, n=rc["n"], sample_seed=rc["sample_seed"])
+    t0 = time.perf_counter()
+    gt = aknn.Graph()
+    gt.initialize(xt, cfg["k"], rc["trees"], rc["leaf"], rc["props"], aknn.METRICS[cfg["metric"]], split=1)
+    gt.calculate()
+    r = aknn.Router(xt.shape[1], rc["hidden"], rc["bin_bits"], seed=1)
+    labels, cut = r.labels(gt, sweeps=rc["sweeps"])
+    st = r.fit(gt, labels, epochs=rc["epochs"], batch=rc["batch"], lr=rc["lr"])
+    gt.close()
+    st.update({"cut_before": cut[0], "cut_after": cut[1], "wall_s": time.perf_counter() - t0,
+               "train_points": rc["n"], "sample_seed": rc["sample_seed"], "bins": 1 << rc["bin_bits"],
+               "hidden": rc["hidden"]})
+    print(f"[bench] router trained: {st}", file=sys.stderr, flush=True)
+    return r, st
 
 
 def roofline(stats, config, form):
@@ -161,13 +193,38 @@ def main():
         torch = None
         sync = lambda: None  # noqa: E731
 
-    g = aknn.Graph()
-    g.initialize(x, cfg["k"], cfg["trees"], cfg["leaf"], cfg["props"], metric)
     names = ["plain", "chosen"] if args.form == "both" else [args.form]
-    results, graphs = {}, {}
-    for name in names:
-        results[name] = run_form(g, name, FORMS[name], args, n, sync)
-        graphs[name] = g.graph()
+    results, graphs, recalls, shapes = {}, {}, {}, {}
+    rng = np.random.default_rng(1234)
+    q = np.sort(rng.choice(n, size=min(args.queries, n), replace=False))
+    router, router_stats = None, None
+    if args.config == "C5":
+        if "chosen" in names:
+            router, router_stats = train_router(aknn, datasets, args.config, cfg)
+        g = None
+        for name in names:
+            spec = C5_FORMS[name]
+            shape = {kk: spec.get(kk, cfg[kk]) for kk in ("trees", "leaf", "props")}
+            g = aknn.Graph()
+            g.initialize(x, cfg["k"], shape["trees"], shape["leaf"], shape["props"], metric, **spec["opts"])
+            if spec.get("router"):
+                g.set_router(router)
+            results[name] = run_form(g, name, spec["opts"], args, n, sync)
+            shapes[name] = shape
+            if not args.no_recall:
+                t1 = time.perf_counter()
+                recalls[name] = g.recall(q)
+                print(f"[bench] {name} recall {recalls[name]:.4f} ({time.perf_counter() - t1:.2f} s)",
+                      file=sys.stderr, flush=True)
+            g.set_router(None)
+            g.close()
+    else:
+        g = aknn.Graph()
+        g.initialize(x, cfg["k"], cfg["trees"], cfg["leaf"], cfg["props"], metric)
+        for name in names:
+            results[name] = run_form(g, name, FORMS[name], args, n, sync)
+            graphs[name] = g.graph()
+            shapes[name] = {kk: cfg[kk] for kk in ("trees", "leaf", "props")}
     head = names[-1]
     r = results[head]
     cfg_name = args.config if args.n is None else f"{args.config}_n{n}"
@@ -184,8 +241,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": args.config, "n": n, "d": cfg["d"], "k": cfg["k"], "trees": cfg["trees"],
-                   "leaf": cfg["leaf"], "propagations": cfg["props"], "distance": cfg["metric"], "form": head},
+        "config": {"workload": args.config, "n": n, "d": cfg["d"], "k": cfg["k"], "trees": shapes[head]["trees"],
+                   "leaf": shapes[head]["leaf"], "propagations": shapes[head]["props"], "distance": cfg["metric"],
+                   "form": head},
         "build_s": r["build_s"],
         "stages_ms": r["stages_ms"],
         "kernels": r["kernels"],
@@ -194,25 +252,33 @@ def main():
     if len(names) > 1:
         other = results["plain"]
         other["roofline"] = roofline(other["kernels"], cfg_name, "plain")
+        other["config"] = shapes["plain"]
         line["plain"] = other
         line["chosen_vs_plain"] = r["value"] / other["value"]
-        ga, gb = graphs["plain"], graphs["chosen"]
-        line["forms_identical"] = bool((ga[0] == gb[0]).all() and
-                                       (ga[1].view(np.uint32) == gb[1].view(np.uint32)).all())
-        print(f"[bench] chosen/plain {line['chosen_vs_plain']:.3f}, identical graphs: {line['forms_identical']}",
+        if graphs:
+            ga, gb = graphs["plain"], graphs["chosen"]
+            line["forms_identical"] = bool((ga[0] == gb[0]).all() and
+                                           (ga[1].view(np.uint32) == gb[1].view(np.uint32)).all())
+        print(f"[bench] chosen/plain {line['chosen_vs_plain']:.3f}, identical graphs: {line.get('forms_identical')}",
               file=sys.stderr, flush=True)
-    if not args.no_recall:
+    if router_stats is not None:
+        line["router"] = router_stats
+    if recalls:
+        line["recall_at_k"] = recalls[head]
+        if "plain" in recalls:
+            line["plain"]["recall_at_k"] = recalls["plain"]
+        line["recall_queries"] = int(len(q))
+        line["recall_reference"] = "FP64 exact k-NN (aknn_exact_knn), self excluded, ties at the k-th distance count"
+    elif not args.no_recall:
         # FP64 exact k-NN of seeded queries on the GPU (the library's own
         # kernel); the graph is the last form's, identical for both forms
-        rng = np.random.default_rng(1234)
-        q = np.sort(rng.choice(n, size=min(args.queries, n), replace=False))
         t1 = time.perf_counter()
         line["recall_at_k"] = g.recall(q)
         line["recall_queries"] = int(len(q))
         line["recall_reference"] = "FP64 exact k-NN (aknn_exact_knn), self excluded, ties at the k-th distance count"
         print(f"[bench] recall {line['recall_at_k']:.4f} ({time.perf_counter() - t1:.2f} s)", file=sys.stderr,
               flush=True)
-    g.close()
+        g.close()
     if not args.no_cpu:
         print(f"[bench] CPU baseline on {args.cpu_sample} points", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(datasets, args.config, cfg, min(args.cpu_sample, n))

@@ -105,7 +105,17 @@ int aknn_deinitialize(aknn_handle* h);
  *                   screens its pairs with FP16 dot products on the matrix
  *                   cores and proven lower bounds, computing exact FP32 keys
  *                   only for pairs the bound does not exclude (the results are
- *                   the same); 0 (default): exact keys for every pair */
+ *                   the same); 0 (default): exact keys for every pair
+ *   "split"         forest rule.  0 (default): the reference's sign of
+ *                   x . (x_p2 - x_p1) with forced pivots (.cu:2138-2181).
+ *                   1: median split, a rule of this library that replaces the
+ *                   reference's: tree t draws seeded Gaussian directions (seed
+ *                   option, tree index), each level projects its buckets on
+ *                   one direction and splits each bucket at a sampled median
+ *                   (balanced leaves of leaf/2..leaf points, no rand() use)
+ *   "router_trees"  with split 1: trees 0..router_trees-1 take the router bins
+ *                   (aknn_set_bins or an attached router) as their first
+ *                   levels, one bit per level, before the median levels */
 int aknn_set_option(aknn_handle* h, const char* name, int64_t value);
 
 /* The graph as the result file holds it before sorting: ids[n*k], vals[n*k]
@@ -161,8 +171,75 @@ int aknn_exact_knn(aknn_handle* h, const int64_t* queries, int64_t nq, int kq, i
  * No reference counterpart. */
 int aknn_recall(aknn_handle* h, const int64_t* queries, int64_t nq, double* recall);
 
+/* Router bins of the points for the median-split forest's routed trees:
+ * bins[n] with values in [0, 2^bin_bits), 1 <= bin_bits <= 20, n the number
+ * of points.  The first bin_bits levels of a routed tree split a bucket by
+ * the bits of its points' bins, most significant first.  No reference
+ * counterpart: the reference's trees split only on pivot differences
+ * (.cu:2138-2164); this feeds learned leaf assignments to the same forest. */
+int aknn_set_bins(aknn_handle* h, const int32_t* bins, int64_t n, int bin_bits);
+
 /* Message of the last failed call on h (empty if none). */
 const char* aknn_last_error(aknn_handle* h);
+
+/* ---- Learned router of the median-split forest ------------------------------
+ * No reference counterpart: the reference draws split directions from random
+ * pivot pairs (.cu:2138-2164, :2291-2365).  The router is an MLP
+ *   u = (x - mu) * inv_sigma, h1 = relu(W1 u + b1), h2 = relu(W2 h1 + b2),
+ *   logits = W3 h2 + b3 (2^bin_bits of them), bin = argmax (lowest on ties)
+ * over the raw points, with BF16 weights and activations, FP32 accumulation,
+ * trained (Adam, FP32 master weights) on the device by the library's own
+ * forward and backward kernels.  Its labels are a balanced partition of a
+ * k-NN graph the library built on a training set, so its bins keep graph
+ * neighbours together.  Attached to a handle with option split = 1 and
+ * router_trees > 0, it assigns each point's bin inside aknn_calculate and the
+ * routed trees split on the bin bits before their median levels. */
+typedef struct aknn_router aknn_router;
+
+/* d: point dimension; hidden: width of both hidden layers (multiple of 32);
+ * bin_bits in [5, 12]; seed: initial weights and batch order.  NULL on error. */
+aknn_router* aknn_router_create(int device, int d, int hidden, int bin_bits, uint32_t seed);
+void aknn_router_destroy(aknn_router* r);
+
+/* Balanced graph labels of the points of `trained` (a handle after
+ * aknn_calculate): equal runs of tree 0's leaf order, then `sweeps` sweeps of
+ * capacity-bounded label propagation over the symmetrised graph (parts within
+ * 0.95..1.05 n/bins).  labels[n]; cut (may be NULL) receives the fraction of
+ * graph edges cut before and after the sweeps. */
+int aknn_router_labels(aknn_router* r, aknn_handle* trained, int sweeps, int32_t* labels, double* cut);
+
+/* Trains on the raw points of `trained` with labels[n] (NULL: the graph
+ * labels of aknn_router_labels with `sweeps`): standardisation from those
+ * points, then `epochs` of Adam (learning rate lr) over seeded shuffled batches
+ * of `batch` points minimising the softmax cross-entropy. */
+int aknn_router_fit(aknn_router* r, aknn_handle* trained, const int32_t* labels, int sweeps, int epochs, int batch,
+                    float lr);
+
+/* One forward and backward pass over m raw points x[m][d] with labels[m]
+ * (label < 0: row ignored): gradients of the mean cross-entropy in the flat
+ * parameter layout W1 [hidden][dp], b1, W2 [hidden][hidden], b2,
+ * W3 [2^bin_bits][hidden], b3 (dp = d rounded up to 16, padding columns
+ * zero), and the loss. */
+int aknn_router_gradients(aknn_router* r, const float* x, int64_t m, const int32_t* labels, float* grads,
+                          double* loss);
+/* FP32 logits z[m][2^bin_bits] of m raw points. */
+int aknn_router_logits(aknn_router* r, const float* x, int64_t m, float* z);
+int aknn_router_param_count(aknn_router* r, int64_t* count);
+/* Parameters (flat layout above), mu[d] and inv_sigma; any pointer may be NULL. */
+int aknn_router_get_params(aknn_router* r, float* params, float* mu, float* inv_sigma);
+/* params or mu may be NULL (kept); inv_sigma must be positive. */
+int aknn_router_set_params(aknn_router* r, const float* params, const float* mu, float inv_sigma);
+int aknn_router_save(aknn_router* r, const char* path);
+int aknn_router_load(aknn_router* r, const char* path);
+/* out[0..6]: epochs trained, last epoch's loss and accuracy, training ms, graph
+ * edges cut by the labels before and after refinement, ms of the last bin
+ * assignment. */
+int aknn_router_stats(aknn_router* r, double* out, int n);
+const char* aknn_router_last_error(aknn_router* r);
+
+/* Attaches a router to a handle (NULL detaches); detach before destroying the
+ * router.  It must have the handle's D and device. */
+int aknn_set_router(aknn_handle* h, aknn_router* r);
 
 #ifdef __cplusplus
 }

@@ -320,6 +320,158 @@ int orc_build_tree(const float* x, int64_t n, int d, int leaf_max, int k,
     return 0;
 }
 
+/* ---------------------------------------------------------------------------
+ * Median-split forest (not in the reference: its split is the sign of
+ * x . (x_p2 - x_p1), .cu:2151-2162; this is the balanced rule the rebuild
+ * offers beside it, with its own semantics fixed here).
+ *
+ * Directions: tree t's level directions are seeded Gaussian vectors, drawn as
+ * splitmix64 uniforms (0, 1) -> Box-Muller pairs (cos, then sin) in double,
+ * rounded to float, level-major: r[j][d].
+ * ------------------------------------------------------------------------- */
+static uint64_t sm64_next(uint64_t* s) {
+    uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+static double sm64_open01(uint64_t* s) {
+    return ((double)(sm64_next(s) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+void orc_directions(uint32_t seed, int tree, int count, int d, float* r) {
+    uint64_t s = ((uint64_t)seed << 32) ^ ((uint64_t)(uint32_t)(tree + 1) * 0xD1B54A32D192ED03ull);
+    int64_t total = (int64_t)count * d, i = 0;
+    while (i < total) {
+        double u1 = sm64_open01(&s), u2 = sm64_open01(&s);
+        double rad = sqrt(-2.0 * log(u1)), th = 6.283185307179586 * u2;
+        r[i++] = (float)(rad * cos(th));
+        if (i < total) r[i++] = (float)(rad * sin(th));
+    }
+}
+
+/* Ordered key of a float: unsigned order equals the float order (-0 < +0,
+ * NaNs at the ends), so every projection has a place in a total order. */
+static uint32_t ord_u32(float f) {
+    uint32_t b;
+    memcpy(&b, &f, 4);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+typedef struct { uint32_t u; int32_t rel; } orc_mkey;
+
+static int mkey_cmp(const void* pa, const void* pb) {
+    const orc_mkey* a = (const orc_mkey*)pa;
+    const orc_mkey* b = (const orc_mkey*)pb;
+    if (a->u != b->u) return a->u < b->u ? -1 : 1;
+    return (a->rel > b->rel) - (a->rel < b->rel);
+}
+
+typedef struct {
+    const float* x;
+    int64_t n;
+    int d, leaf_max, k, ndirs, bin_bits;
+    const float* dirs;
+    const int32_t* bins;
+    int32_t *perm, *rs, *rc, *tmp;
+    float* proj;
+    uint8_t* side;
+    int32_t levels;
+} orc_mtree;
+
+/* One bucket [s, s+c) at level lv; (ps, pc) is the range of its deepest
+ * ancestor with at least k+1 points.  The side of position p:
+ *   lv < bin_bits (routed trees): bit (bin_bits-1-lv) of the point's bin;
+ *   else j = lv - bin_bits < ndirs: proj = fmaf chain of x . r[j] in dimension
+ *     order; m = min(c, 511) samples at relative positions q*c/m (q < m) are
+ *     ordered by (ord_u32(proj), rel); the one at index m/2 is the pivot; a
+ *     position is RIGHT when (ord_u32(proj), rel) >= the pivot's (both sides
+ *     non-empty for c >= 2);
+ *   else (directions exhausted): RIGHT when rel >= c/2.
+ * The partition is stable; an empty child is dropped. */
+static void mtree_bucket(orc_mtree* T, int64_t s, int64_t c, int lv, int32_t ps, int32_t pc) {
+    int64_t p, nl = 0, w;
+    int32_t rs0 = c >= T->k + 1 ? (int32_t)s : ps;
+    int32_t rc0 = c >= T->k + 1 ? (int32_t)c : pc;
+    if (lv + 1 > T->levels) T->levels = lv + 1;
+    if (c <= T->leaf_max) {
+        for (p = s; p < s + c; ++p) { T->rs[T->perm[p]] = rs0; T->rc[T->perm[p]] = rc0; }
+        return;
+    }
+    if (T->bins && lv < T->bin_bits) {
+        int sh = T->bin_bits - 1 - lv;
+        for (p = s; p < s + c; ++p) T->side[p] = (uint8_t)((T->bins[T->perm[p]] >> sh) & 1);
+    } else {
+        int j = lv - (T->bins ? T->bin_bits : 0);
+        if (j >= T->ndirs) {
+            for (p = s; p < s + c; ++p) T->side[p] = (uint8_t)((p - s) >= c / 2);
+        } else {
+            const float* r = T->dirs + (int64_t)j * T->d;
+            int64_t m = c < 511 ? c : 511, q;
+            orc_mkey* smp = (orc_mkey*)malloc(sizeof(orc_mkey) * m);
+            orc_mkey piv;
+            for (p = s; p < s + c; ++p) {
+                const float* xr = T->x + (int64_t)T->perm[p] * T->d;
+                float acc = 0.0f;
+                int dd;
+                for (dd = 0; dd < T->d; ++dd) acc = fmaf(xr[dd], r[dd], acc);
+                T->proj[p] = acc;
+            }
+            for (q = 0; q < m; ++q) {
+                int64_t rel = q * c / m;
+                smp[q].u = ord_u32(T->proj[s + rel]);
+                smp[q].rel = (int32_t)rel;
+            }
+            qsort(smp, m, sizeof(orc_mkey), mkey_cmp);
+            piv = smp[m / 2];
+            free(smp);
+            for (p = s; p < s + c; ++p) {
+                uint32_t u = ord_u32(T->proj[p]);
+                int32_t rel = (int32_t)(p - s);
+                T->side[p] = (uint8_t)(u > piv.u || (u == piv.u && rel >= piv.rel));
+            }
+        }
+    }
+    for (p = s; p < s + c; ++p) nl += T->side[p] == 0;
+    w = s;
+    for (p = s; p < s + c; ++p) if (T->side[p] == 0) T->tmp[w++] = T->perm[p];
+    for (p = s; p < s + c; ++p) if (T->side[p] != 0) T->tmp[w++] = T->perm[p];
+    memcpy(T->perm + s, T->tmp + s, sizeof(int32_t) * c);
+    if (nl > 0) mtree_bucket(T, s, nl, lv + 1, rs0, rc0);
+    if (c - nl > 0) mtree_bucket(T, s + nl, c - nl, lv + 1, rs0, rc0);
+}
+
+int orc_build_tree_median(const float* x, int64_t n, int d, int leaf_max, int k,
+                          const float* dirs, int ndirs, const int32_t* bins, int bin_bits,
+                          int32_t* perm, int32_t* range_start, int32_t* range_count,
+                          int32_t* n_levels) {
+    orc_mtree T;
+    int64_t p;
+    if (n < 1 || d < 1 || leaf_max < 0 || k < 0 || n < (int64_t)k + 1 || n > 0x7fffffffLL) return -1;
+    if (ndirs < 0 || bin_bits < 0 || bin_bits > 20 || (ndirs > 0 && !dirs)) return -1;
+    memset(&T, 0, sizeof(T));
+    T.x = x; T.n = n; T.d = d; T.leaf_max = leaf_max < 1 ? 1 : leaf_max; T.k = k;
+    T.dirs = dirs; T.ndirs = ndirs; T.bins = bins; T.bin_bits = bins ? bin_bits : 0;
+    T.perm = perm; T.rs = range_start; T.rc = range_count;
+    T.tmp = (int32_t*)malloc(sizeof(int32_t) * n);
+    T.proj = (float*)malloc(sizeof(float) * n);
+    T.side = (uint8_t*)malloc(n);
+    if (!T.tmp || !T.proj || !T.side) { free(T.tmp); free(T.proj); free(T.side); return -1; }
+    for (p = 0; p < n; ++p) perm[p] = (int32_t)p;
+    mtree_bucket(&T, 0, n, 0, 0, (int32_t)n);
+    if (n_levels) *n_levels = T.levels;
+    free(T.tmp); free(T.proj); free(T.side);
+    return 0;
+}
+
+int orc_median_depth(int64_t n, int leaf) {
+    int lv = 0;
+    int64_t c = n, l = leaf < 1 ? 1 : leaf;
+    while (c > l) { c = (c + 1) / 2; ++lv; }
+    return lv + 8;
+}
+
 /* ------------------------------------------------------------------------- */
 typedef struct { float key; int32_t id; } orc_ent;
 
@@ -469,29 +621,45 @@ int orc_propagate(const float* x, int64_t n, int d, int k, int metric,
     return err ? -1 : 0;
 }
 
-/* calculate() (.cu:2861-2896): normalize, forest, stub + initial, rounds. */
-int orc_run(float* x, int64_t n, int d, int k, int trees, int leaf, int props, int metric,
-            uint32_t seed, int rng_order, int cosine_fixed,
-            int32_t* ids, float* vals, double* stage_ms) {
+/* calculate() (.cu:2861-2896): normalize, forest, stub + initial, rounds.
+ * split 0: the reference's sign-split forest on the rand() stream; split 1:
+ * the median-split forest (orc_build_tree_median) with tree t's directions
+ * orc_directions(seed, t, orc_median_depth(n, leaf), d), and for t <
+ * router_trees the shared bins[n] (bin_bits bits) as its first levels. */
+int orc_run_ex(float* x, int64_t n, int d, int k, int trees, int leaf, int props, int metric,
+               uint32_t seed, int rng_order, int cosine_fixed, int split,
+               const int32_t* bins, int bin_bits, int router_trees,
+               int32_t* ids, float* vals, double* stage_ms) {
     orc_rng g;
     int32_t *perm, *rs, *rc, *id2;
-    float* v2;
+    float *v2, *dirs = NULL;
     double t0, t1, t2, t3, t4;
-    int t, p, rcode = 0;
-    if (n < k + 1 || k < 1 || trees < 1 || d < 1) return -1;
+    int t, p, rcode = 0, ndirs = 0;
+    if (n < k + 1 || k < 1 || trees < 1 || d < 1 || split < 0 || split > 1) return -1;
     perm = (int32_t*)malloc(sizeof(int32_t) * n * trees * 3);
     id2 = (int32_t*)malloc(sizeof(int32_t) * n * k);
     v2 = (float*)malloc(sizeof(float) * n * k);
-    if (!perm || !id2 || !v2) { free(perm); free(id2); free(v2); return -1; }
+    if (split) {
+        ndirs = orc_median_depth(n, leaf);
+        dirs = (float*)malloc(sizeof(float) * (size_t)ndirs * d);
+    }
+    if (!perm || !id2 || !v2 || (split && !dirs)) { free(perm); free(id2); free(v2); free(dirs); return -1; }
     rs = perm + n * trees;
     rc = rs + n * trees;
     orc_rng_seed(&g, seed);
     t0 = now_ms();
     orc_normalize(x, n, d);
     t1 = now_ms();
-    for (t = 0; t < trees && !rcode; ++t)
-        rcode = orc_build_tree(x, n, d, leaf, k, &g, rng_order, perm + (int64_t)t * n,
-                               rs + (int64_t)t * n, rc + (int64_t)t * n, NULL, NULL, NULL);
+    for (t = 0; t < trees && !rcode; ++t) {
+        if (split) {
+            orc_directions(seed, t, ndirs, d, dirs);
+            rcode = orc_build_tree_median(x, n, d, leaf, k, dirs, ndirs, t < router_trees ? bins : NULL, bin_bits,
+                                          perm + (int64_t)t * n, rs + (int64_t)t * n, rc + (int64_t)t * n, NULL);
+        } else {
+            rcode = orc_build_tree(x, n, d, leaf, k, &g, rng_order, perm + (int64_t)t * n,
+                                   rs + (int64_t)t * n, rc + (int64_t)t * n, NULL, NULL, NULL);
+        }
+    }
     t2 = now_ms();
     if (!rcode) rcode = orc_initial_knn(x, n, d, k, trees, metric, perm, rs, rc, ids, vals, cosine_fixed);
     t3 = now_ms();
@@ -504,8 +672,145 @@ int orc_run(float* x, int64_t n, int d, int k, int trees, int leaf, int props, i
     if (stage_ms) {
         stage_ms[0] = t1 - t0; stage_ms[1] = t2 - t1; stage_ms[2] = t3 - t2; stage_ms[3] = t4 - t3;
     }
-    free(perm); free(id2); free(v2);
+    free(perm); free(id2); free(v2); free(dirs);
     return rcode;
+}
+
+int orc_run(float* x, int64_t n, int d, int k, int trees, int leaf, int props, int metric,
+            uint32_t seed, int rng_order, int cosine_fixed,
+            int32_t* ids, float* vals, double* stage_ms) {
+    return orc_run_ex(x, n, d, k, trees, leaf, props, metric, seed, rng_order, cosine_fixed, 0, NULL, 0, 0,
+                      ids, vals, stage_ms);
+}
+
+/* ---------------------------------------------------------------------------
+ * Router MLP (not in the reference): forward and backward in FP64.  With
+ * round_bf16 the values the device rounds to BF16 are rounded here at the
+ * same points (inputs, weights, hidden activations, the logit gradient and
+ * the hidden gradients that feed the products) and the FP32 steps (bias add,
+ * standardisation) are done in float; without it everything is FP64.
+ * params: W1 [h][dp], b1 [h], W2 [h][h], b2 [h], W3 [nb][h], b3 [nb].
+ * ------------------------------------------------------------------------- */
+static float bf16_round(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) u |= 0x00400000u;
+    else u += 0x7fffu + ((u >> 16) & 1u);
+    u &= 0xffff0000u;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+double orc_mlp(const float* x, int64_t m, int d, int dp, int h, int nb,
+               const float* params, const float* mu, float inv_sigma,
+               const int32_t* labels, int round_bf16, float* grads, float* logits) {
+    const float *W1 = params, *b1 = W1 + (int64_t)h * dp, *W2 = b1 + h, *b2 = W2 + (int64_t)h * h;
+    const float *W3 = b2 + h, *b3 = W3 + (int64_t)nb * h;
+    double *w1, *w2, *w3, *u, *h1, *h2, *dz3, *dz3b, *dz2, *dz2b, *dz1, *dz1b, loss = 0.0;
+    double *g = NULL;
+    int64_t r, count = 0, np = (int64_t)h * dp + h + (int64_t)h * h + h + (int64_t)nb * h + nb;
+    int o, c, i;
+#define RB(v) (round_bf16 ? (double)bf16_round((float)(v)) : (double)(v))
+    w1 = (double*)malloc(sizeof(double) * ((int64_t)h * dp + (int64_t)h * h + (int64_t)nb * h));
+    u = (double*)malloc(sizeof(double) * m * (dp + 2 * h + 2 * nb + 4 * h));
+    if (!w1 || !u) { free(w1); free(u); return NAN; }
+    w2 = w1 + (int64_t)h * dp; w3 = w2 + (int64_t)h * h;
+    h1 = u + m * dp; h2 = h1 + m * h; dz3 = h2 + m * h; dz3b = dz3 + m * nb;
+    dz2 = dz3b + m * nb; dz2b = dz2 + m * h; dz1 = dz2b + m * h; dz1b = dz1 + m * h;
+    for (i = 0; i < h * dp; ++i) w1[i] = RB(W1[i]);
+    for (i = 0; i < h * h; ++i) w2[i] = RB(W2[i]);
+    for (i = 0; i < nb * h; ++i) w3[i] = RB(W3[i]);
+    for (r = 0; r < m; ++r)
+        for (c = 0; c < dp; ++c) {
+            double v = 0.0;
+            if (c < d) {
+                if (round_bf16) v = RB((x[r * d + c] - mu[c]) * inv_sigma);
+                else v = ((double)x[r * d + c] - mu[c]) * (double)inv_sigma;
+            }
+            u[r * dp + c] = v;
+        }
+    for (r = 0; r < m; ++r) {
+        for (o = 0; o < h; ++o) {
+            double s = 0.0, v;
+            for (c = 0; c < dp; ++c) s += w1[(int64_t)o * dp + c] * u[r * dp + c];
+            v = round_bf16 ? (double)((float)s + b1[o]) : s + b1[o];
+            h1[r * h + o] = RB(v > 0 ? v : 0.0);
+        }
+        for (o = 0; o < h; ++o) {
+            double s = 0.0, v;
+            for (c = 0; c < h; ++c) s += w2[(int64_t)o * h + c] * h1[r * h + c];
+            v = round_bf16 ? (double)((float)s + b2[o]) : s + b2[o];
+            h2[r * h + o] = RB(v > 0 ? v : 0.0);
+        }
+        for (o = 0; o < nb; ++o) {
+            double s = 0.0, v;
+            for (c = 0; c < h; ++c) s += w3[(int64_t)o * h + c] * h2[r * h + c];
+            v = round_bf16 ? (double)((float)s + b3[o]) : s + b3[o];
+            dz3[r * nb + o] = v;  /* logits for now */
+            if (logits) logits[r * nb + o] = (float)v;
+        }
+    }
+    if (!labels) { free(w1); free(u); return 0.0; }
+    for (r = 0; r < m; ++r) count += labels[r] >= 0 && labels[r] < nb;
+    for (r = 0; r < m; ++r) {
+        double* z = dz3 + r * nb;
+        double mx = -INFINITY, s = 0.0;
+        int y = labels[r];
+        if (y < 0 || y >= nb) {
+            for (o = 0; o < nb; ++o) { z[o] = 0.0; dz3b[r * nb + o] = 0.0; }
+            continue;
+        }
+        for (o = 0; o < nb; ++o) if (z[o] > mx) mx = z[o];
+        for (o = 0; o < nb; ++o) s += exp(z[o] - mx);
+        loss += log(s) + mx - z[y];
+        for (o = 0; o < nb; ++o) {
+            double gv = (exp(z[o] - mx) / s - (o == y ? 1.0 : 0.0)) / (double)(count ? count : 1);
+            if (round_bf16) gv = (double)(float)gv;
+            z[o] = gv;
+            dz3b[r * nb + o] = RB(gv);
+        }
+    }
+    if (!grads) { free(w1); free(u); return loss / (double)(count ? count : 1); }
+    g = (double*)calloc(np, sizeof(double));
+    if (!g) { free(w1); free(u); return NAN; }
+    {
+        double *gW1 = g, *gb1 = gW1 + (int64_t)h * dp, *gW2 = gb1 + h, *gb2 = gW2 + (int64_t)h * h;
+        double *gW3 = gb2 + h, *gb3 = gW3 + (int64_t)nb * h;
+        for (r = 0; r < m; ++r) {
+            for (o = 0; o < nb; ++o) {
+                gb3[o] += dz3[r * nb + o];
+                for (c = 0; c < h; ++c) gW3[(int64_t)o * h + c] += dz3b[r * nb + o] * h2[r * h + c];
+            }
+            for (c = 0; c < h; ++c) {
+                double s = 0.0;
+                for (o = 0; o < nb; ++o) s += dz3b[r * nb + o] * w3[(int64_t)o * h + c];
+                if (!(h2[r * h + c] > 0)) s = 0.0;
+                if (round_bf16) s = (double)(float)s;
+                dz2[r * h + c] = s;
+                dz2b[r * h + c] = RB(s);
+            }
+            for (o = 0; o < h; ++o) {
+                gb2[o] += dz2[r * h + o];
+                for (c = 0; c < h; ++c) gW2[(int64_t)o * h + c] += dz2b[r * h + o] * h1[r * h + c];
+            }
+            for (c = 0; c < h; ++c) {
+                double s = 0.0;
+                for (o = 0; o < h; ++o) s += dz2b[r * h + o] * w2[(int64_t)o * h + c];
+                if (!(h1[r * h + c] > 0)) s = 0.0;
+                if (round_bf16) s = (double)(float)s;
+                dz1[r * h + c] = s;
+                dz1b[r * h + c] = RB(s);
+            }
+            for (o = 0; o < h; ++o) {
+                gb1[o] += dz1[r * h + o];
+                for (c = 0; c < dp; ++c) gW1[(int64_t)o * dp + c] += dz1b[r * h + o] * u[r * dp + c];
+            }
+        }
+    }
+    for (r = 0; r < np; ++r) grads[r] = (float)g[r];
+    free(g); free(w1); free(u);
+#undef RB
+    return loss / (double)(count ? count : 1);
 }
 
 /* --------------------------------------------------------------------------

@@ -69,6 +69,20 @@ int orc_build_tree(const float* x, int64_t n, int d, int leaf_max, int k,
                    int32_t* perm, int32_t* range_start, int32_t* range_count,
                    int32_t* n_iterations, int64_t* n_nodes, int64_t* active_elems);
 
+/* Median-split forest (a rule of this rebuild, not the reference's).
+ * orc_directions: `count` seeded Gaussian directions of length d for tree
+ * `tree` (splitmix64 + Box-Muller in double, rounded to float), level-major.
+ * orc_median_depth: directions per tree, ceil(log2(n / leaf)) + 8.
+ * orc_build_tree_median: bins (nullable) routes the first bin_bits levels by
+ * the bits of each point's bin, most significant first; the later levels
+ * split at a sampled median of x . r[j]; see aknn_oracle.c for the rule. */
+void orc_directions(uint32_t seed, int tree, int count, int d, float* r);
+int orc_median_depth(int64_t n, int leaf);
+int orc_build_tree_median(const float* x, int64_t n, int d, int leaf_max, int k,
+                          const float* dirs, int ndirs, const int32_t* bins, int bin_bits,
+                          int32_t* perm, int32_t* range_start, int32_t* range_count,
+                          int32_t* n_levels);
+
 /* A.6.1.  perm/rs/rc are [trees][n].  Output rows [n][k] sorted by
  * (stored value, id).  cosine_fixed=1 carries distances instead of the
  * reference's per-store 1-key flip. */
@@ -87,6 +101,21 @@ int orc_propagate(const float* x, int64_t n, int d, int k, int metric,
 int orc_run(float* x, int64_t n, int d, int k, int trees, int leaf, int props, int metric,
             uint32_t seed, int rng_order, int cosine_fixed,
             int32_t* ids, float* vals, double* stage_ms);
+
+/* orc_run with the forest rule: split 0 (reference sign split) or 1
+ * (median split; trees t < router_trees take bins[n] as their first levels). */
+int orc_run_ex(float* x, int64_t n, int d, int k, int trees, int leaf, int props, int metric,
+               uint32_t seed, int rng_order, int cosine_fixed, int split,
+               const int32_t* bins, int bin_bits, int router_trees,
+               int32_t* ids, float* vals, double* stage_ms);
+
+/* Router MLP forward (and backward when labels and grads are given): returns
+ * the mean softmax cross-entropy over rows with a label in [0, nb), writes
+ * grads (flat parameter layout) and logits[m][nb] (nullable).  round_bf16
+ * rounds where the device rounds to BF16; 0 computes in FP64 throughout. */
+double orc_mlp(const float* x, int64_t m, int d, int dp, int h, int nb,
+               const float* params, const float* mu, float inv_sigma,
+               const int32_t* labels, int round_bf16, float* grads, float* logits);
 
 /* FP64 exact k-th neighbour distance (self excluded) for each query. */
 void orc_exact_kth(const float* x, int64_t n, int d, int k, int metric,

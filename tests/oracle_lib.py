@@ -54,6 +54,19 @@ def lib():
         L.orc_run.argtypes = [_f32p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                               ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_int,
                               ctypes.c_int, _i32p, _f32p, _f64p]
+        L.orc_directions.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p]
+        L.orc_median_depth.argtypes = [ctypes.c_int64, ctypes.c_int]
+        L.orc_build_tree_median.argtypes = [_f32p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            _f32p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, _i32p, _i32p,
+                                            _i32p, ctypes.POINTER(ctypes.c_int32)]
+        L.orc_run_ex.argtypes = [_f32p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_int,
+                                 ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                 _i32p, _f32p, _f64p]
+        L.orc_mlp.argtypes = [_f32p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                              _f32p, _f32p, ctypes.c_float, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                              ctypes.c_void_p]
+        L.orc_mlp.restype = ctypes.c_double
         L.orc_exact_kth.argtypes = [_f32p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     _i64p, ctypes.c_int64, _f64p]
         L.orc_distance64.argtypes = [_f32p, _f32p, ctypes.c_int, ctypes.c_int]
@@ -107,6 +120,55 @@ def build_forest(x, trees, leaf, k, seed=1, rng_order=0, g=None):
     return perm, rs, rc, iters
 
 
+def directions(seed, tree, count, d):
+    r = np.zeros((count, d), np.float32)
+    lib().orc_directions(seed, tree, count, d, r)
+    return r
+
+
+def median_depth(n, leaf):
+    return int(lib().orc_median_depth(n, leaf))
+
+
+def _bins_ptr(bins):
+    if bins is None:
+        return None, None
+    b = np.ascontiguousarray(bins, dtype=np.int32)
+    return b, b.ctypes.data_as(ctypes.c_void_p)
+
+
+def build_tree_median(x, leaf, k, dirs, bins=None, bin_bits=0):
+    """One median-split tree: (perm, rs, rc, levels)."""
+    n, d = x.shape
+    perm = np.zeros(n, np.int32)
+    rs = np.zeros(n, np.int32)
+    rc = np.zeros(n, np.int32)
+    lv = ctypes.c_int32()
+    dirs = np.ascontiguousarray(dirs, dtype=np.float32)
+    keep, bp = _bins_ptr(bins)
+    r = lib().orc_build_tree_median(x, n, d, leaf, k, dirs, dirs.shape[0], bp, bin_bits, perm, rs, rc,
+                                    ctypes.byref(lv))
+    if r != 0:
+        raise ValueError("orc_build_tree_median failed")
+    return perm, rs, rc, lv.value
+
+
+def build_forest_median(x, trees, leaf, k, seed=1, bins=None, bin_bits=0, router_trees=0):
+    """Median-split forest: perm, rs, rc as [trees][n] and per-tree level counts."""
+    n, d = x.shape
+    nd = median_depth(n, leaf)
+    perm = np.zeros((trees, n), np.int32)
+    rs = np.zeros((trees, n), np.int32)
+    rc = np.zeros((trees, n), np.int32)
+    levels = []
+    for t in range(trees):
+        p, a, c, lv = build_tree_median(x, leaf, k, directions(seed, t, nd, d),
+                                        bins if t < router_trees else None, bin_bits)
+        perm[t], rs[t], rc[t] = p, a, c
+        levels.append(lv)
+    return perm, rs, rc, levels
+
+
 def initial_knn(x, k, metric, perm, rs, rc, cosine_fixed=0):
     n, d = x.shape
     ids = np.zeros((n, k), np.int32)
@@ -141,6 +203,38 @@ def run(x_raw, k, trees, leaf, props, metric, seed=1, rng_order=0, cosine_fixed=
     if r != 0:
         raise ValueError("orc_run failed")
     return x, ids, vals, ms
+
+
+def run_ex(x_raw, k, trees, leaf, props, metric, seed=1, rng_order=0, cosine_fixed=0, split=0, bins=None,
+           bin_bits=0, router_trees=0):
+    """Whole pipeline with the forest rule (split 0 reference, 1 median)."""
+    x = np.ascontiguousarray(x_raw, dtype=np.float32).copy()
+    n, d = x.shape
+    ids = np.zeros((n, k), np.int32)
+    vals = np.zeros((n, k), np.float32)
+    ms = np.zeros(4, np.float64)
+    keep, bp = _bins_ptr(bins)
+    r = lib().orc_run_ex(x, n, d, k, trees, leaf, props, metric, seed, rng_order, cosine_fixed, split, bp,
+                         bin_bits, router_trees, ids, vals, ms)
+    if r != 0:
+        raise ValueError("orc_run_ex failed")
+    return x, ids, vals, ms
+
+
+def mlp(x, dims, params, mu, inv_sigma, labels=None, round_bf16=True, want_grads=True):
+    """Router MLP in FP64: (loss, grads or None, logits).  dims = (d, dp, hidden, bins)."""
+    d, dp, h, nb = dims
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    m = len(x)
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    mu = np.ascontiguousarray(mu, dtype=np.float32)
+    z = np.zeros((m, nb), np.float32)
+    g = np.zeros(len(params), np.float32) if (labels is not None and want_grads) else None
+    lab = None if labels is None else np.ascontiguousarray(labels, dtype=np.int32)
+    loss = lib().orc_mlp(x, m, d, dp, h, nb, params, mu, inv_sigma,
+                         None if lab is None else lab.ctypes.data_as(ctypes.c_void_p), 1 if round_bf16 else 0,
+                         None if g is None else g.ctypes.data_as(ctypes.c_void_p), z.ctypes.data_as(ctypes.c_void_p))
+    return loss, g, z
 
 
 def exact_kth(x, k, metric, queries):

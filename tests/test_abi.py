@@ -4,6 +4,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 from conftest import PKG_DIR, ROOT
@@ -47,9 +48,22 @@ def test_handle_errors_without_device(aknn):
         assert L.aknn_set_option(h, b"no_such_option", 1) != 0
         assert L.aknn_set_option(h, b"rng_order", 3) != 0
         assert L.aknn_set_option(h, b"seed", 7) == 0
+        assert L.aknn_set_option(h, b"split", 2) != 0
+        assert L.aknn_set_option(h, b"split", 1) == 0
+        assert L.aknn_set_option(h, b"router_trees", -1) != 0
+        assert L.aknn_set_bins(h, np.zeros(4, np.int32), 4, 2) != 0  # no data yet
+        assert L.aknn_set_router(h, None) != 0
         assert L.aknn_save_result(h) != 0
     finally:
         L.aknn_destroy(h)
+
+
+def test_router_rejects_bad_shapes_and_missing_device(aknn):
+    L = aknn.load()
+    assert not L.aknn_router_create(0, 16, 30, 8, 1)   # hidden not a multiple of 32
+    assert not L.aknn_router_create(0, 16, 64, 4, 1)   # too few bins for the matrix-core tiles
+    assert not L.aknn_router_create(97, 16, 64, 8, 1)  # no such device
+    assert L.aknn_router_stats(None, np.zeros(7), 7) != 0
 
 
 def _cli(*args):
