@@ -25,7 +25,7 @@ EXPORTS = (
     "aknn_last_error", "aknn_router_create", "aknn_router_destroy", "aknn_router_labels", "aknn_router_fit",
     "aknn_router_gradients", "aknn_router_logits", "aknn_router_param_count", "aknn_router_get_params",
     "aknn_router_set_params", "aknn_router_save", "aknn_router_load", "aknn_router_stats",
-    "aknn_router_last_error", "aknn_set_router",
+    "aknn_router_last_error", "aknn_set_router", "aknn_read_points", "aknn_router_bins", "aknn_add_router",
 )
 
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -83,6 +83,7 @@ def load():
     L.aknn_router_fit.argtypes = [R, H, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float]
     L.aknn_router_gradients.argtypes = [R, _f32p, ctypes.c_int64, _i32p, _f32p, ctypes.POINTER(ctypes.c_double)]
     L.aknn_router_logits.argtypes = [R, _f32p, ctypes.c_int64, _f32p]
+    L.aknn_router_bins.argtypes = [R, _f32p, ctypes.c_int64, ctypes.c_int, _i32p]
     L.aknn_router_param_count.argtypes = [R, ctypes.POINTER(ctypes.c_int64)]
     L.aknn_router_get_params.argtypes = [R, _f32p, _f32p, ctypes.POINTER(ctypes.c_float)]
     L.aknn_router_set_params.argtypes = [R, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float]
@@ -92,6 +93,9 @@ def load():
     L.aknn_router_last_error.argtypes = [R]
     L.aknn_router_last_error.restype = ctypes.c_char_p
     L.aknn_set_router.argtypes = [H, R]
+    L.aknn_add_router.argtypes = [H, R]
+    L.aknn_read_points.argtypes = [ctypes.c_char_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   _f32p, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
     for name in EXPORTS:
         if name not in ("aknn_create", "aknn_destroy", "aknn_last_error", "aknn_router_create",
                         "aknn_router_destroy", "aknn_router_last_error"):
@@ -221,10 +225,28 @@ class Graph:
         """Attaches a trained Router (None detaches); routed trees (option
         router_trees, split=1) then take its bins inside calculate()."""
         self._check(self._L.aknn_set_router(self._h, router._r if router is not None else None))
-        self._router = router
+        self._routers = [] if router is None else [router]
+
+    def add_router(self, router):
+        """Appends a router: routed tree t takes router t mod R."""
+        self._check(self._L.aknn_add_router(self._h, router._r))
+        self._routers = getattr(self, "_routers", []) + [router]
 
     def deinitialize(self):
         self._check(self._L.aknn_deinitialize(self._h))
+
+
+def read_points(path, n, d, klaster=False, threads=0):
+    """The library's input loader without a device: (x [n][d] float32, labels or None)."""
+    L = load()
+    x = np.empty((n, d), np.float32)
+    lab = np.empty(n, np.int32) if klaster else None
+    err = ctypes.create_string_buffer(512)
+    rc = L.aknn_read_points(path.encode(), n, d, 1 if klaster else 0, threads, x,
+                            None if lab is None else lab.ctypes.data_as(ctypes.c_void_p), err, 512)
+    if rc != 0:
+        raise AknnError(err.value.decode(errors="replace"))
+    return x, lab
 
 
 class Router:
@@ -280,6 +302,13 @@ class Router:
         z = np.empty((len(x), self.bins), np.float32)
         self._check(self._L.aknn_router_logits(self._r, x, len(x), z))
         return z
+
+    def bins_of(self, x, fused=True):
+        """Bins of raw points through the assignment path calculate() uses."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        b = np.empty(len(x), np.int32)
+        self._check(self._L.aknn_router_bins(self._r, x, len(x), 1 if fused else 0, b))
+        return b
 
     def param_count(self):
         c = ctypes.c_int64()

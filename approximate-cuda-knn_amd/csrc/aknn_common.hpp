@@ -13,6 +13,10 @@ namespace aknn {
 constexpr int kEuclidean = 0;
 constexpr int kTaxicab = 1;
 constexpr int kCosine = 2;
+// Internal code of TAXICAB without the square root (option taxicab_l1): the
+// reference stores sqrtf of the L1 total (euclideandistancematrixgpu.cu:568,
+// :1477); this fixed mode keys and stores the L1 total itself.
+constexpr int kTaxicabL1 = 3;
 
 // Stub neighbour written before the initial pass (euclideandistancematrixgpu.cu:16-17).
 constexpr int kStubId = -1;

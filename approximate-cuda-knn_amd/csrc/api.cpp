@@ -26,6 +26,7 @@ struct aknn_handle {
   int device = -1;
   std::unique_ptr<aknn::Engine> engine;
   bool calculated = false;
+  int io_threads = 0;
 };
 
 namespace {
@@ -140,7 +141,13 @@ int aknn_initialize(aknn_handle* h, int n, int d, int k, int device, int distanc
     if (n < 1 || d < 1) return fail(h, AKNN_ERR_ARGUMENT, "N and D must be positive");
     std::vector<float> x;
     std::vector<int32_t> labels;
-    const std::string err = aknn::load_csv(h->data_file, n, d, klaster != 0, x, labels);
+    // *.fbin: binary points; otherwise the reference's CSV, parsed by
+    // io_threads host threads (0: up to 16), the same values as one thread
+    const std::string err = aknn::is_fbin(h->data_file)
+                                ? aknn::load_fbin(h->data_file, n, d, x)
+                                : (h->io_threads == 1 ? aknn::load_csv(h->data_file, n, d, klaster != 0, x, labels)
+                                                      : aknn::load_csv_parallel(h->data_file, n, d, klaster != 0,
+                                                                                h->io_threads, x, labels));
     if (!err.empty()) return fail(h, AKNN_ERR_IO, err);
     return setup(h, x.data(), n, d, k, device, distance, trees, propagations, min_size);
   });
@@ -212,6 +219,20 @@ int aknn_set_option(aknn_handle* h, const char* name, int64_t value) {
     } else if (n == "split") {
       if (value != 0 && value != 1) return fail(h, AKNN_ERR_ARGUMENT, "split is 0 (sign) or 1 (median)");
       h->params.split = static_cast<int>(value);
+    } else if (n == "taxicab_l1") {
+      h->params.taxicab_l1 = value != 0;
+    } else if (n == "scale_all") {
+      h->params.scale_all = value != 0;
+    } else if (n == "fixed") {
+      // every fixed-mode semantic at once
+      h->params.cosine_fixed = value != 0;
+      h->params.taxicab_l1 = value != 0;
+      h->params.scale_all = value != 0;
+    } else if (n == "io_threads") {
+      if (value < 0 || value > 256) return fail(h, AKNN_ERR_ARGUMENT, "io_threads must be in [0, 256]");
+      h->io_threads = static_cast<int>(value);
+    } else if (n == "graph_rounds") {
+      h->params.graph_rounds = value != 0;
     } else if (n == "router_trees") {
       if (value < 0 || value > 1 << 20) return fail(h, AKNN_ERR_ARGUMENT, "router_trees must be >= 0");
       h->params.router_trees = static_cast<int>(value);
@@ -275,7 +296,14 @@ int aknn_get_kernel_stats(aknn_handle* h, const char* name, int64_t* launches, d
       s = &h->engine->stats_propagate;
     else if (n == "forest")
       s = &h->engine->stats_forest;
-    else if (n == "leaf_mfma") {
+    else if (n == "propagate_graph") {
+      // the rounds of the last calculate() replayed from the captured graph
+      const bool gr = h->engine->rounds_from_graph();
+      if (launches) *launches = gr ? h->engine->stats_propagate.launches : 0;
+      if (total_ms) *total_ms = gr ? h->engine->stats_propagate.total_ms : 0.0;
+      if (units) *units = gr ? h->engine->stats_propagate.units : 0.0;
+      return AKNN_OK;
+    } else if (n == "leaf_mfma") {
       // the screened (matrix-core) launches of the leaf k-NN: GEMM-form flops
       const aknn::KernelStats& l = h->engine->stats_leaf_knn;
       const int ldh = (h->engine->d() + 7) / 8 * 8;
@@ -340,6 +368,36 @@ int aknn_set_bins(aknn_handle* h, const int32_t* bins, int64_t n, int bin_bits) 
   });
 }
 
+int aknn_read_points(const char* path, int64_t n, int d, int klaster, int threads, float* x, int32_t* labels,
+                     char* error, int error_len) {
+  std::string err;
+  try {
+    if (!path || !x || n < 1 || d < 1 || (klaster && !labels) || threads < 0) {
+      err = "bad arguments";
+    } else {
+      std::vector<float> xs;
+      std::vector<int32_t> ls;
+      const std::string p(path);
+      err = aknn::is_fbin(p) ? aknn::load_fbin(p, n, d, xs)
+                             : (threads == 1 ? aknn::load_csv(p, n, d, klaster != 0, xs, ls)
+                                             : aknn::load_csv_parallel(p, n, d, klaster != 0, threads, xs, ls));
+      if (err.empty()) {
+        std::memcpy(x, xs.data(), sizeof(float) * xs.size());
+        if (klaster && !aknn::is_fbin(p)) std::memcpy(labels, ls.data(), sizeof(int32_t) * ls.size());
+      }
+    }
+  } catch (const std::exception& e) {
+    err = e.what();
+  } catch (...) {
+    err = "unknown error";
+  }
+  if (error && error_len > 0) {
+    std::strncpy(error, err.c_str(), static_cast<size_t>(error_len) - 1);
+    error[error_len - 1] = '\0';
+  }
+  return err.empty() ? AKNN_OK : AKNN_ERR_IO;
+}
+
 aknn_router* aknn_router_create(int device, int d, int hidden, int bin_bits, uint32_t seed) {
   try {
     auto* r = new aknn_router();
@@ -400,6 +458,21 @@ int aknn_router_logits(aknn_router* r, const float* x, int64_t m, float* z) {
   return rguarded(r, [&] {
     if (!x || !z) return rfail(r, AKNN_ERR_ARGUMENT, "null array");
     r->router->logits(x, m, z);
+    return AKNN_OK;
+  });
+}
+
+int aknn_router_bins(aknn_router* r, const float* x, int64_t m, int fused, int32_t* bins) {
+  return rguarded(r, [&] {
+    if (!x || !bins) return rfail(r, AKNN_ERR_ARGUMENT, "null array");
+    r->router->set_fused(fused != 0);
+    try {
+      r->router->bins_of(x, m, bins);
+    } catch (...) {
+      r->router->set_fused(true);
+      throw;
+    }
+    r->router->set_fused(true);
     return AKNN_OK;
   });
 }
@@ -470,6 +543,17 @@ int aknn_set_router(aknn_handle* h, aknn_router* r) {
     if (r && (r->router->device() != h->device || r->router->d() != h->engine->d()))
       return fail(h, AKNN_ERR_ARGUMENT, "the router must have the data's D and live on the handle's device");
     h->engine->set_router(r ? r->router.get() : nullptr);
+    return AKNN_OK;
+  });
+}
+
+int aknn_add_router(aknn_handle* h, aknn_router* r) {
+  return guarded(h, [&] {
+    if (!h->engine) return fail(h, AKNN_ERR_STATE, "not initialized");
+    if (!r || !r->router) return fail(h, AKNN_ERR_ARGUMENT, "empty router");
+    if (r->router->device() != h->device || r->router->d() != h->engine->d())
+      return fail(h, AKNN_ERR_ARGUMENT, "the router must have the data's D and live on the handle's device");
+    h->engine->add_router(r->router.get());
     return AKNN_OK;
   });
 }

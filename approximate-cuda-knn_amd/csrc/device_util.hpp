@@ -33,6 +33,8 @@ __device__ __forceinline__ float finish_key(float tot, float aa, float bb) {
   if constexpr (METRIC == kCosine) {
     const float c = tot / (sqrtf(aa) * sqrtf(bb));
     return static_cast<float>(1.0 - static_cast<double>(c));
+  } else if constexpr (METRIC == kTaxicabL1) {
+    return tot;
   } else {
     return sqrtf(tot);
   }
@@ -44,7 +46,7 @@ __device__ __forceinline__ float chain_step(float a, float b, float p) {
   if constexpr (METRIC == kEuclidean) {
     const float t = a - b;
     return __builtin_fmaf(t, t, p);
-  } else if constexpr (METRIC == kTaxicab) {
+  } else if constexpr (METRIC == kTaxicab || METRIC == kTaxicabL1) {
     return p + __builtin_fabsf(a - b);
   } else {
     return __builtin_fmaf(a, b, p);
@@ -183,7 +185,8 @@ inline SketchBound sketch_bound(int d) {
 // (L2; L1 for TAXICAB) and, for COSINE, the exact squared-norm chains aa, bb.
 //   EUCLIDEAN: ||x-y|| >= sqrt(p (1-kap)) - ex - ey (triangle inequality), and
 //     the exact key >= ||x-y|| (1-kap).
-//   TAXICAB: |x-y|_1 >= p (1-kap) - ex - ey, and key >= sqrt(|x-y|_1 (1-kap)).
+//   TAXICAB: |x-y|_1 >= p (1-kap) - ex - ey, and key >= sqrt(|x-y|_1 (1-kap));
+//     without the root (kTaxicabL1) key >= |x-y|_1 (1-kap).
 //   COSINE: the exact key is 1 - fl(tot / den) with den the same FP32 product
 //     of roots as finish_key, and tot <= p + kap (A+ex)(B+ey) + ex B
 //     + (A+ex) ey + kap A B, with A >= ||x||, B >= ||y|| from aa, bb.
@@ -201,6 +204,11 @@ __device__ __forceinline__ double sketch_lower_bound(float p, float ex, float ey
     const double s =
         fmax(0.0, static_cast<double>(p) * k1 - sb.tiny - static_cast<double>(ex) - static_cast<double>(ey));
     lb = sqrt(fmax(0.0, s * k1 - sb.tiny)) * k1;
+  } else if constexpr (METRIC == kTaxicabL1) {
+    // the exact L1 chain is at least |x-y|_1 (1-kap) - tiny
+    const double s =
+        fmax(0.0, static_cast<double>(p) * k1 - sb.tiny - static_cast<double>(ex) - static_cast<double>(ey));
+    lb = fmax(0.0, s * k1 - sb.tiny);
   } else {
     const float den = sqrtf(aa) * sqrtf(bb);
     if (!(den > 0.0f) || !(den <= 3.4e38f)) return -__builtin_inf();

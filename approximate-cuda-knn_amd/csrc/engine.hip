@@ -87,6 +87,7 @@ Engine::Engine(int device) : device_(device) {
 Engine::~Engine() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
+  drop_round_graph();
   release();
   for (auto e : events_) (void)hipEventDestroy(e);
   for (auto e : launch_events_) (void)hipEventDestroy(e);
@@ -113,7 +114,14 @@ T* Engine::alloc(size_t count) {
   return static_cast<T*>(p);
 }
 
+void Engine::drop_round_graph() {
+  if (round_exec_) (void)hipGraphExecDestroy(round_exec_);
+  round_exec_ = nullptr;
+  round_graph_mode_ = -1;
+}
+
 void Engine::release() {
+  drop_round_graph();
   for (void* p : allocs_) (void)hipFree(p);
   allocs_.clear();
   x_ = x_raw_ = norms_ = mean_ = eps_ = nh_ = s8_ = e8_ = nullptr;
@@ -136,6 +144,7 @@ void Engine::release() {
   nd_ = 0;
   dirs_key_ = -1;
   bins_ = nullptr;
+  bins_cap_ = 0;
   bin_bits_ = 0;
   bins_set_ = false;
 }
@@ -146,7 +155,10 @@ void Engine::set_bins(const int32_t* bins, int bin_bits) {
   for (int64_t i = 0; i < n_; ++i)
     if (bins[i] < 0 || bins[i] >= (1 << bin_bits)) throw std::invalid_argument("bin out of range");
   AKNN_HIP_TRY(hipSetDevice(device_));
-  if (!bins_) bins_ = alloc<int32_t>(n_);
+  if (!bins_) {
+    bins_ = alloc<int32_t>(n_);
+    bins_cap_ = 1;
+  }
   AKNN_HIP_TRY(hipMemcpy(bins_, bins, sizeof(int32_t) * n_, hipMemcpyHostToDevice));
   bin_bits_ = bin_bits;
   bins_set_ = true;
@@ -154,7 +166,10 @@ void Engine::set_bins(const int32_t* bins, int bin_bits) {
 
 void Engine::set_bins_device(const int32_t* bins, int bin_bits) {
   if (bin_bits < 1 || bin_bits > 20) throw std::invalid_argument("bin_bits must be in [1, 20]");
-  if (!bins_) bins_ = alloc<int32_t>(n_);
+  if (!bins_) {
+    bins_ = alloc<int32_t>(n_);
+    bins_cap_ = 1;
+  }
   if (bins != bins_)
     AKNN_HIP_TRY(hipMemcpyAsync(bins_, bins, sizeof(int32_t) * n_, hipMemcpyDeviceToDevice, stream_));
   bin_bits_ = bin_bits;
@@ -296,13 +311,23 @@ void Engine::build_forest_median() {
     AKNN_HIP_TRY(hipMemcpy(dirs_, all.data(), sizeof(float) * all.size(), hipMemcpyHostToDevice));
     dirs_key_ = key;
   }
-  if (router_ && p_.router_trees > 0) {
-    // learned leaf assignments of this data set (inside the timed bracket)
-    if (router_->device() != device_ || router_->d() != d_) throw std::invalid_argument("router does not fit the data");
-    if (!bins_) bins_ = alloc<int32_t>(n_);
-    router_->assign(x_raw_, n_, ld_, bins_, stream_);
-    bin_bits_ = router_->bin_bits();
+  int nbins_sets = bins_set_ ? 1 : 0;
+  if (!routers_.empty() && p_.router_trees > 0) {
+    // learned leaf assignments of this data set (inside the timed bracket),
+    // one bin array per router
+    const int nr = static_cast<int>(routers_.size());
+    for (Router* r : routers_) {
+      if (r->device() != device_ || r->d() != d_) throw std::invalid_argument("router does not fit the data");
+      if (r->bin_bits() != routers_[0]->bin_bits()) throw std::invalid_argument("routers must share bin_bits");
+    }
+    if (!bins_ || bins_cap_ < nr) {
+      bins_ = alloc<int32_t>(static_cast<size_t>(n_) * nr);
+      bins_cap_ = nr;
+    }
+    for (int i = 0; i < nr; ++i) routers_[i]->assign(x_raw_, n_, ld_, bins_ + static_cast<int64_t>(i) * n_, stream_);
+    bin_bits_ = routers_[0]->bin_bits();
     bins_set_ = true;
+    nbins_sets = nr;
   }
   const int routed_trees = bins_set_ ? std::min(p_.router_trees, p_.trees) : 0;
   for (int t = 0; t < p_.trees; ++t) {
@@ -316,7 +341,8 @@ void Engine::build_forest_median() {
     for (;;) {
       for (int b = 0; b < p_.level_batch; ++b, ++level) {
         if (level < bb) {
-          launch_forest_level_rule(fb_, 0, nullptr, bins_, bb - 1 - level, perm, leaves, level, leaf_eff_, p_.k + 1,
+          const int32_t* tb = bins_ + static_cast<int64_t>(t % std::max(nbins_sets, 1)) * n_;
+          launch_forest_level_rule(fb_, 0, nullptr, tb, bb - 1 - level, perm, leaves, level, leaf_eff_, p_.k + 1,
                                    stream_);
         } else if (level - bb < nd) {
           launch_forest_level_rule(fb_, 1, proj_ + static_cast<int64_t>(level - bb) * n_, nullptr, 0, perm, leaves,
@@ -420,14 +446,56 @@ void Engine::initial_knn() {
     AKNN_HIP_TRY(hipEventRecord(launch_ev(2 * t), stream_));
     // the first tree meets stub rows (nothing to screen against): plain kernel
     if (mfma && t > 0)
-      launch_leaf_mfma(p_.metric, x_, d_, ld_, xh_, ldh_, eps_, nh_, norms_, perm_ + t * n_, items_, nit, kItemOwners,
+      launch_leaf_mfma(kmetric(), x_, d_, ld_, xh_, ldh_, eps_, nh_, norms_, perm_ + t * n_, items_, nit, kItemOwners,
                        p_.k, ids, vals, rs_ + t * n_, rc_ + t * n_, leaf_rescored_, leaf_rescored_ + 2, stream_);
     else
-      launch_leaf_knn(p_.metric, x_, d_, ld_, norms_, perm_ + t * n_, items_, nit, kItemOwners, p_.k, ids, vals,
+      launch_leaf_knn(kmetric(), x_, d_, ld_, norms_, perm_ + t * n_, items_, nit, kItemOwners, p_.k, ids, vals,
                       rs_ + t * n_, rc_ + t * n_, stream_);
     AKNN_HIP_TRY(hipEventRecord(launch_ev(2 * t + 1), stream_));
   }
   stats_leaf_knn.launches = p_.trees;
+}
+
+// Arguments of propagation round r, which reads graph buffer `cur` and
+// writes the other one.
+static PropagateArgs round_args(const Params& p, int kmetric, bool flip, bool screen, float* x, int d, int ld,
+                                _Float16* xh, int ldh, float* eps, int8_t* q8, int ld8, float* s8, int32_t* a8,
+                                float* e8, float* norms, int32_t* order, int64_t n, int32_t* const gids[2],
+                                float* const gvals[2], int cur, int r, unsigned long long* ncand,
+                                uint8_t* const changed[2]) {
+  PropagateArgs a;
+  a.metric = kmetric;
+  a.flip = flip;
+  a.x = x;
+  a.d = d;
+  a.ld = ld;
+  if (screen) {
+    a.xh = xh;
+    a.ldh = ldh;
+    a.eps = eps;
+    if (p.sketch_int8 && p.metric == kEuclidean) {
+      a.q8 = q8;
+      a.ld8 = ld8;
+      a.s8 = s8;
+      a.a8 = a8;
+      a.e8 = e8;
+    }
+  }
+  a.norms = norms;
+  a.order = order;
+  a.n = n;
+  a.k = p.k;
+  a.ids_in = gids[cur];
+  a.vals_in = gvals[cur];
+  a.ids_out = gids[cur ^ 1];
+  a.vals_out = gvals[cur ^ 1];
+  a.ncand = ncand + 2 * r;
+  if (p.skip_settled) {
+    a.hist.ids_prev = r > 0 ? gids[cur ^ 1] : nullptr;
+    a.hist.changed_prev = changed[(r + 1) & 1];
+    a.hist.changed_out = changed[r & 1];
+  }
+  return a;
 }
 
 void Engine::propagate() {
@@ -443,8 +511,10 @@ void Engine::propagate() {
   // reuse win).  A round where more than a quarter of the candidates survived
   // the screen also switches the remaining rounds to the exact path.
   const bool can_screen = propagation_screened();
+  rounds_from_graph_ = false;
+  if (p_.graph_rounds && p_.props > 0 && (!can_screen || prop_mode_ >= 0) && propagate_graph(base)) return;
   bool screen = can_screen && prop_mode_ != 1;
-  const bool probe = can_screen && prop_mode_ < 0 && p_.props >= 3;
+  const bool probe = can_screen && prop_mode_ < 0 && p_.props >= 2;
   for (int r = 0; r < p_.props; ++r) {
     if (can_screen && r > 0) {
       unsigned long long c[2] = {0, 0};
@@ -466,44 +536,71 @@ void Engine::propagate() {
       if (round_screened_[r - 1] && c[0] > 0 && 4 * c[1] > c[0]) screen = false;
     }
     round_screened_[r] = screen ? 1 : 0;
-    PropagateArgs a;
-    a.metric = p_.metric;
-    a.flip = flip;
-    a.x = x_;
-    a.d = d_;
-    a.ld = ld_;
-    if (screen) {
-      a.xh = xh_;
-      a.ldh = ldh_;
-      a.eps = eps_;
-      if (p_.sketch_int8 && p_.metric == kEuclidean) {
-        a.q8 = q8_;
-        a.ld8 = ld8_;
-        a.s8 = s8_;
-        a.a8 = a8_;
-        a.e8 = e8_;
-      }
-    }
-    a.norms = norms_;
-    a.order = perm_;
-    a.n = n_;
-    a.k = p_.k;
-    a.ids_in = gids_[cur_];
-    a.vals_in = gvals_[cur_];
-    a.ids_out = gids_[cur_ ^ 1];
-    a.vals_out = gvals_[cur_ ^ 1];
-    a.ncand = ncand_ + 2 * r;
-    if (p_.skip_settled) {
-      a.hist.ids_prev = r > 0 ? gids_[cur_ ^ 1] : nullptr;
-      a.hist.changed_prev = changed_[(r + 1) & 1];
-      a.hist.changed_out = changed_[r & 1];
-    }
+    // buffers alternate each round: round r reads buffer cur_ (= c0 ^ (r & 1))
+    const PropagateArgs a = round_args(p_, kmetric(), flip, screen, x_, d_, ld_, xh_, ldh_, eps_, q8_, ld8_, s8_,
+                                       a8_, e8_, norms_, perm_, n_, gids_, gvals_, cur_, r, ncand_, changed_);
     AKNN_HIP_TRY(hipEventRecord(launch_ev(base + 2 * r), stream_));
     launch_propagate(a, stream_);
     AKNN_HIP_TRY(hipEventRecord(launch_ev(base + 2 * r + 1), stream_));
     cur_ ^= 1;  // buffer swap (.cu:1969-1975)
   }
+  if (probe && prop_mode_ < 0) {
+    // two rounds only: decide from the screened round 0 and the exact round 1
+    unsigned long long c[4] = {0, 0, 0, 0};
+    float ms[2] = {0, 0};
+    AKNN_HIP_TRY(hipMemcpyAsync(c, ncand_, sizeof(c), hipMemcpyDeviceToHost, stream_));
+    AKNN_HIP_TRY(hipStreamSynchronize(stream_));
+    AKNN_HIP_TRY(hipEventElapsedTime(&ms[0], launch_ev(base), launch_ev(base + 1)));
+    AKNN_HIP_TRY(hipEventElapsedTime(&ms[1], launch_ev(base + 2), launch_ev(base + 3)));
+    const double per_s = ms[0] / std::max<double>(1.0, static_cast<double>(c[0]));
+    const double per_e = ms[1] / std::max<double>(1.0, static_cast<double>(c[2]));
+    prop_mode_ = per_s <= per_e ? 0 : 1;
+  }
   stats_propagate.launches = p_.props;
+}
+
+// The P rounds replayed from one captured graph: the round mode is fixed (the
+// probe of the first calculate() decided it, or there is no screen), so the
+// launches and their buffers are known in advance.  The counters are cleared
+// before the launch, outside the graph.  Returns false if capture is not
+// possible (the caller then launches the rounds one by one).
+bool Engine::propagate_graph(int base) {
+  const bool can_screen = propagation_screened();
+  const bool screen = can_screen && prop_mode_ == 0;
+  const int mode = screen ? 1 : 0;
+  const bool flip = p_.metric == kCosine && !p_.cosine_fixed;
+  if (!round_exec_ || round_graph_mode_ != mode) {
+    drop_round_graph();
+    hipGraph_t graph = nullptr;
+    AKNN_HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    try {
+      for (int r = 0; r < p_.props; ++r)
+        launch_propagate(round_args(p_, kmetric(), flip, screen, x_, d_, ld_, xh_, ldh_, eps_, q8_, ld8_, s8_, a8_,
+                                    e8_, norms_, perm_, n_, gids_, gvals_, (cur_ + r) & 1, r, ncand_, changed_),
+                         stream_);
+    } catch (...) {
+      (void)hipStreamEndCapture(stream_, &graph);
+      if (graph) (void)hipGraphDestroy(graph);
+      throw;
+    }
+    AKNN_HIP_TRY(hipStreamEndCapture(stream_, &graph));
+    const hipError_t ie = hipGraphInstantiate(&round_exec_, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+      (void)hipGetLastError();  // the rounds run one by one instead
+      round_exec_ = nullptr;
+      return false;
+    }
+    round_graph_mode_ = mode;
+  }
+  AKNN_HIP_TRY(hipEventRecord(launch_ev(base), stream_));
+  AKNN_HIP_TRY(hipGraphLaunch(round_exec_, stream_));
+  AKNN_HIP_TRY(hipEventRecord(launch_ev(base + 1), stream_));
+  round_screened_.assign(p_.props, screen ? 1 : 0);
+  cur_ ^= p_.props & 1;
+  rounds_from_graph_ = true;
+  stats_propagate.launches = p_.props;
+  return true;
 }
 
 void Engine::calculate() {
@@ -531,9 +628,9 @@ void Engine::calculate() {
   fb_.amb_count = p_.sketch ? leaf_rescored_ + 1 : nullptr;
   AKNN_HIP_TRY(hipMemcpyAsync(x_, x_raw_, sizeof(float) * n_ * ld_, hipMemcpyDeviceToDevice, stream_));
   AKNN_HIP_TRY(hipEventRecord(ev(0), stream_));
-  launch_normalize(x_, n_, d_, ld_, mean_, gmax_, stream_);
+  launch_normalize(x_, n_, d_, ld_, mean_, gmax_, p_.scale_all != 0, stream_);
   if (norms_) launch_norms(x_, n_, d_, ld_, norms_, stream_);
-  if (sketch) launch_sketch(x_, n_, d_, ld_, p_.metric, xh_, ldh_, eps_, nh_, stream_);
+  if (sketch) launch_sketch(x_, n_, d_, ld_, kmetric(), xh_, ldh_, eps_, nh_, stream_);
   if (sketch8) launch_sketch8(x_, n_, d_, ld_, q8_, ld8_, s8_, a8_, e8_, stream_);
   AKNN_HIP_TRY(hipEventRecord(ev(1), stream_));
   if (p_.split == 1)
@@ -566,9 +663,16 @@ void Engine::calculate() {
   std::vector<unsigned long long> nc(2 * static_cast<size_t>(std::max(p_.props, 1)));
   AKNN_HIP_TRY(hipMemcpy(nc.data(), ncand_, sizeof(unsigned long long) * nc.size(), hipMemcpyDeviceToHost));
   stats_propagate.examined = stats_propagate.rescored = 0;
+  if (rounds_from_graph_ && p_.props > 0) {
+    float ms = 0;
+    AKNN_HIP_TRY(hipEventElapsedTime(&ms, launch_ev(2 * p_.trees), launch_ev(2 * p_.trees + 1)));
+    stats_propagate.total_ms = ms;
+  }
   for (int r = 0; r < p_.props; ++r) {
     float ms = 0;
-    AKNN_HIP_TRY(hipEventElapsedTime(&ms, launch_ev(2 * p_.trees + 2 * r), launch_ev(2 * p_.trees + 2 * r + 1)));
+    if (!rounds_from_graph_) {
+      AKNN_HIP_TRY(hipEventElapsedTime(&ms, launch_ev(2 * p_.trees + 2 * r), launch_ev(2 * p_.trees + 2 * r + 1)));
+    }
     stats_propagate.total_ms += ms;
     const double k = p_.k;
     // per point: own row (8k), k neighbour id rows (4k^2), x_i (4D), output

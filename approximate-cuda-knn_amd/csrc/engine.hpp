@@ -30,6 +30,9 @@ struct Params {
   int sketch_int8 = 0;    // with sketch, EUCLIDEAN propagation screens with the int8 sketch (same results)
   int split = 0;          // forest rule: 0 the reference's sign split, 1 median split on seeded directions
   int router_trees = 0;   // median split: trees 0..router_trees-1 take the router bins as their first levels
+  int taxicab_l1 = 0;     // TAXICAB keys and stores the L1 total instead of its square root
+  int scale_all = 0;      // normalization scales every dimension, not only the first 256
+  int graph_rounds = 1;   // once the round mode is fixed, replay the propagation rounds as one captured hipGraph
 };
 
 // Per-stage times of the last calculate(), milliseconds.
@@ -73,6 +76,11 @@ class Engine {
     if (p_.split != p.split) prop_mode_ = -1;
     p_.split = p.split;
     p_.router_trees = p.router_trees;
+    if (p_.taxicab_l1 != p.taxicab_l1 || p_.scale_all != p.scale_all) prop_mode_ = -1;
+    p_.taxicab_l1 = p.taxicab_l1;
+    p_.scale_all = p.scale_all;
+    p_.graph_rounds = p.graph_rounds;
+    drop_round_graph();
   }
   // Router bins of the points (host array [n], values < 2^bin_bits): the
   // first bin_bits levels of routed median-split trees split on their bits.
@@ -80,9 +88,16 @@ class Engine {
   // Bins already on the device (router inference), same contract.
   void set_bins_device(const int32_t* bins, int bin_bits);
   int32_t* bins_device() { return bins_; }
-  // A trained router (nullable: detach) that assigns the bins of routed
-  // trees inside calculate(); it must live on this engine's device.
-  void set_router(class Router* r) { router_ = r; }
+  // Trained routers that assign the bins of routed trees inside calculate()
+  // (routed tree t takes router t mod R); they must live on this engine's
+  // device.  set_router replaces the list (nullptr: empty), add_router appends.
+  void set_router(class Router* r) {
+    routers_.clear();
+    if (r) routers_.push_back(r);
+  }
+  void add_router(class Router* r) {
+    if (r) routers_.push_back(r);
+  }
   const float* raw_points() const { return x_raw_; }
   int ld() const { return ld_; }
 
@@ -93,6 +108,7 @@ class Engine {
   void exact_knn(const int64_t* queries, int64_t nq, int kq, int32_t* ids, double* dist);
   // recall@k of the current graph against the FP64 exact k-NN of the queries
   double recall(const int64_t* queries, int64_t nq);
+  bool rounds_from_graph() const { return rounds_from_graph_; }
   int tree_iterations(int tree) const { return tree < static_cast<int>(iters_.size()) ? iters_[tree] : -1; }
 
   const StageTimes& times() const { return times_; }
@@ -109,12 +125,18 @@ class Engine {
   T* alloc(size_t count);
   // propagation screens with the sketch (not in COSINE parity mode, whose
   // flipped stored values give no distance threshold)
+  // metric code of the distance kernels
+  int kmetric() const { return p_.metric == kTaxicab && p_.taxicab_l1 ? kTaxicabL1 : p_.metric; }
   bool propagation_screened() const { return p_.sketch && !(p_.metric == kCosine && !p_.cosine_fixed); }
   bool leaf_mfma_active() const {
     return p_.leaf_mfma && p_.trees > 1 && (p_.metric == kEuclidean || (p_.metric == kCosine && p_.cosine_fixed));
   }
   void build_forest();
   void build_forest_median();
+  // the propagation rounds as one hipGraph (captured once the round mode is
+  // fixed; every option change drops it)
+  void drop_round_graph();
+  bool propagate_graph(int base);
   void initial_knn();
   void propagate();
 
@@ -172,8 +194,12 @@ class Engine {
   int64_t dirs_key_ = -1;
   int32_t* bins_ = nullptr;
   int bin_bits_ = 0;
+  int bins_cap_ = 0;  // bin arrays bins_ holds
   bool bins_set_ = false;
-  class Router* router_ = nullptr;
+  std::vector<class Router*> routers_;
+  hipGraphExec_t round_exec_ = nullptr;
+  int round_graph_mode_ = -1;
+  bool rounds_from_graph_ = false;
   StageTimes times_;
 };
 

@@ -11,7 +11,10 @@ namespace aknn {
 
 // ---- normalization (euclideandistancematrixgpu.cu:19-85) -------------------
 // x: [n][ld] row-major, d <= ld.  mean: [d] scratch, gmax: [1] scratch.
-void launch_normalize(float* x, int64_t n, int d, int ld, float* mean, uint32_t* gmax, hipStream_t s);
+// scale_all: every dimension enters the max-abs and is scaled (the reference
+// does so for the first 256 only).
+void launch_normalize(float* x, int64_t n, int d, int ld, float* mean, uint32_t* gmax, bool scale_all,
+                      hipStream_t s);
 // Per-point COSINE squared norms (the chain of .cu:849-865 for one operand).
 void launch_norms(const float* x, int64_t n, int d, int ld, float* norms, hipStream_t s);
 

@@ -288,6 +288,10 @@ void launch_leaf_knn(int metric, const float* x, int d, int ld, const float* nor
       big ? launch_k<kTaxicab, true>(k, x, d, ld, norms, perm, leaves, nleaves, max_leaf, ids, vals, rsp, rcp, s)
           : launch_k<kTaxicab, false>(k, x, d, ld, norms, perm, leaves, nleaves, max_leaf, ids, vals, rsp, rcp, s);
       break;
+    case kTaxicabL1:
+      big ? launch_k<kTaxicabL1, true>(k, x, d, ld, norms, perm, leaves, nleaves, max_leaf, ids, vals, rsp, rcp, s)
+          : launch_k<kTaxicabL1, false>(k, x, d, ld, norms, perm, leaves, nleaves, max_leaf, ids, vals, rsp, rcp, s);
+      break;
     case kCosine:
       big ? launch_k<kCosine, true>(k, x, d, ld, norms, perm, leaves, nleaves, max_leaf, ids, vals, rsp, rcp, s)
           : launch_k<kCosine, false>(k, x, d, ld, norms, perm, leaves, nleaves, max_leaf, ids, vals, rsp, rcp, s);

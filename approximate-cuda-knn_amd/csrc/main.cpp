@@ -69,16 +69,26 @@ int main(int argc, char** argv) {
   const bool klaster = std::string(argv[9]) == "T";
   const bool debugMode = argc == 11 && std::string(argv[10]) == "--debug";
 
-  std::ifstream header(inputFile.c_str());
+  std::ifstream header(inputFile.c_str(), std::ios::binary);
   if (!header.is_open()) {
     std::cout << "Error opening the file\n";
     return 7;
   }
-  std::string line;
-  std::getline(header, line);
-  const int numberOfPoints = std::atoi(line.c_str());
-  std::getline(header, line);
-  const int dimentionOfData = std::atoi(line.c_str());
+  int numberOfPoints = 0, dimentionOfData = 0;
+  const bool fbin = inputFile.size() >= 5 && inputFile.compare(inputFile.size() - 5, 5, ".fbin") == 0;
+  if (fbin) {
+    // binary input (no reference counterpart): uint32 n, uint32 d, then the points
+    uint32_t hd[2] = {0, 0};
+    header.read(reinterpret_cast<char*>(hd), sizeof(hd));
+    numberOfPoints = static_cast<int>(hd[0] > 0x7fffffffu ? 0x7fffffffu : hd[0]);
+    dimentionOfData = static_cast<int>(hd[1] > 0x7fffffffu ? 0x7fffffffu : hd[1]);
+  } else {
+    std::string line;
+    std::getline(header, line);
+    numberOfPoints = std::atoi(line.c_str());
+    std::getline(header, line);
+    dimentionOfData = std::atoi(line.c_str());
+  }
   header.close();
 
   EuclideanDistanceMatrixGPU edm(debugMode);

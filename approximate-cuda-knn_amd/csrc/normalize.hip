@@ -60,10 +60,11 @@ __global__ __launch_bounds__(1024) void mean_kernel(const float* __restrict__ x,
   if (wave == 0 && dd < d) mean[dd] = m / static_cast<float>(n);
 }
 
-__global__ __launch_bounds__(256) void center_kernel(float* __restrict__ x, int64_t n, int d, int ld,
+// dm: dimensions that enter the max-abs and get scaled (the reference:
+// min(D, 256), .cu:57-65, :76-84; option scale_all: D)
+__global__ __launch_bounds__(256) void center_kernel(float* __restrict__ x, int64_t n, int d, int ld, int dm,
                                                      const float* __restrict__ mean, uint32_t* __restrict__ gmax) {
   __shared__ float red[256];
-  const int dm = d < 256 ? d : 256;
   float local = 0.0f;
   const int64_t total = n * ld;
   for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < total; e += gridDim.x * 256ll) {
@@ -87,9 +88,8 @@ __global__ __launch_bounds__(256) void center_kernel(float* __restrict__ x, int6
   if (threadIdx.x == 0) atomicMax(gmax, __float_as_uint(red[0]));
 }
 
-__global__ __launch_bounds__(256) void scale_kernel(float* __restrict__ x, int64_t n, int d, int ld,
+__global__ __launch_bounds__(256) void scale_kernel(float* __restrict__ x, int64_t n, int d, int ld, int dm,
                                                     const uint32_t* __restrict__ gmax) {
-  const int dm = d < 256 ? d : 256;
   const float g = __uint_as_float(*gmax);
   const int64_t total = n * ld;
   for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < total; e += gridDim.x * 256ll) {
@@ -112,13 +112,15 @@ int grid_for(int64_t work, int64_t per_block, int cap) {
 
 }  // namespace
 
-void launch_normalize(float* x, int64_t n, int d, int ld, float* mean, uint32_t* gmax, hipStream_t s) {
+void launch_normalize(float* x, int64_t n, int d, int ld, float* mean, uint32_t* gmax, bool scale_all,
+                      hipStream_t s) {
+  const int dm = scale_all ? d : (d < 256 ? d : 256);
   AKNN_HIP_TRY(hipMemsetAsync(gmax, 0, sizeof(uint32_t), s));
   hipLaunchKernelGGL(mean_kernel, dim3((d + 63) / 64), dim3(1024), 0, s, x, n, d, ld, mean);
   AKNN_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(center_kernel, dim3(grid_for(n * ld, 256, 8192)), dim3(256), 0, s, x, n, d, ld, mean, gmax);
+  hipLaunchKernelGGL(center_kernel, dim3(grid_for(n * ld, 256, 8192)), dim3(256), 0, s, x, n, d, ld, dm, mean, gmax);
   AKNN_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(scale_kernel, dim3(grid_for(n * ld, 256, 8192)), dim3(256), 0, s, x, n, d, ld, gmax);
+  hipLaunchKernelGGL(scale_kernel, dim3(grid_for(n * ld, 256, 8192)), dim3(256), 0, s, x, n, d, ld, dm, gmax);
   AKNN_HIP_TRY(hipGetLastError());
 }
 

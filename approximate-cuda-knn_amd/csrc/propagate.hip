@@ -342,6 +342,9 @@ void launch_propagate(const PropagateArgs& a, hipStream_t s) {
     case kTaxicab:
       launch_t<kTaxicab, false>(a, s);
       break;
+    case kTaxicabL1:
+      launch_t<kTaxicabL1, false>(a, s);
+      break;
     case kCosine:
       if (a.flip)
         launch_t<kCosine, true>(a, s);

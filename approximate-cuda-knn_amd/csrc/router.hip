@@ -231,6 +231,147 @@ __global__ __launch_bounds__(256) void to_bf16_kernel(const float* __restrict__ 
   if (dstt) dstt[static_cast<int64_t>(c) * rows + r] = b;
 }
 
+// ---- fused inference: 64 rows per workgroup through all three layers -------
+//
+// The rows are standardised into LDS as BF16, each layer's activations stay
+// in LDS (row stride padded by 8 halves so the 32 rows of an A fragment fall
+// in different banks), the weights stream from L2 as B fragments, and the
+// logits never leave registers: each lane's (value, column) pairs reduce
+// across its 32-lane half by shuffles, then across waves through LDS, to the
+// lowest column holding the row maximum.  Wave w owns output column tiles
+// [w*cpw, (w+1)*cpw) of a layer for both 32-row tiles, so a B fragment feeds
+// two MFMAs.  Per output tile the K steps and lane layouts are the ones of
+// gemm_nt_kernel, so the logits, and the bins, equal the unfused path's.
+constexpr int kFuseRows = 64;
+constexpr int kFuseH = 256;       // hidden width of the fused path
+constexpr int kFuseLd = 256 + 8;  // LDS row stride (halves)
+
+template <int CPW>
+__device__ __forceinline__ void fused_layer(const uint16_t* __restrict__ src, const uint16_t* __restrict__ w, int k,
+                                            int n, int wave, int lane, f32x16 (&acc)[2][2]) {
+  const int li = lane & 31, kh = (lane >> 5) * 8;
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt) acc[c][rt] = f32x16{};
+  const int nt = n / 32;
+  for (int k0 = 0; k0 < k; k0 += 16) {
+    const uint4 a0 = *reinterpret_cast<const uint4*>(src + li * kFuseLd + k0 + kh);
+    const uint4 a1 = *reinterpret_cast<const uint4*>(src + (32 + li) * kFuseLd + k0 + kh);
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+      const int ct = wave * CPW + c;
+      if (ct < nt) {
+        const uint4 bv = *reinterpret_cast<const uint4*>(w + static_cast<int64_t>(ct * 32 + li) * k + k0 + kh);
+        const bf16x8 b = __builtin_bit_cast(bf16x8, bv);
+        acc[c][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a0), b, acc[c][0], 0, 0, 0);
+        acc[c][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a1), b, acc[c][1], 0, 0, 0);
+      }
+    }
+  }
+}
+
+template <int CPW>
+__device__ __forceinline__ void fused_hidden_store(const f32x16 (&acc)[2][2], const float* __restrict__ bias,
+                                                   int n, int wave, int lane, uint16_t* __restrict__ dst) {
+#pragma unroll
+  for (int c = 0; c < CPW; ++c) {
+    const int ct = wave * CPW + c;
+    if (ct >= n / 32) continue;
+    const int col = ct * 32 + (lane & 31);
+    const float bv = bias[col];
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int row = rt * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        float v = acc[c][rt][r] + bv;
+        v = v > 0.0f ? v : 0.0f;
+        dst[row * kFuseLd + col] = f2bf(v);
+      }
+  }
+}
+
+template <int CPW3>
+__global__ __launch_bounds__(256) void router_fused_kernel(const float* __restrict__ x, int ld, int d, int dp,
+                                                           int64_t n, const float* __restrict__ mu, float inv_sigma,
+                                                           const uint16_t* __restrict__ w1, const float* __restrict__ b1,
+                                                           const uint16_t* __restrict__ w2, const float* __restrict__ b2,
+                                                           const uint16_t* __restrict__ w3, const float* __restrict__ b3,
+                                                           int nb, int32_t* __restrict__ bins) {
+  __shared__ __attribute__((aligned(16))) uint16_t bufa[kFuseRows * kFuseLd];
+  __shared__ __attribute__((aligned(16))) uint16_t bufb[kFuseRows * kFuseLd];
+  __shared__ float bval[4][kFuseRows];
+  __shared__ int bidx[4][kFuseRows];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t row0 = static_cast<int64_t>(blockIdx.x) * kFuseRows;
+  // standardised rows into bufb (BF16), zero past d and past n
+  for (int e = threadIdx.x; e < kFuseRows * dp; e += 256) {
+    const int r = e / dp, c = e % dp;
+    const int64_t row = row0 + r;
+    uint16_t b = 0;
+    if (row < n && c < d) b = f2bf((x[row * ld + c] - mu[c]) * inv_sigma);
+    bufb[r * kFuseLd + c] = b;
+  }
+  __syncthreads();
+  f32x16 acc[2][2];
+  fused_layer<2>(bufb, w1, dp, kFuseH, wave, lane, acc);
+  fused_hidden_store<2>(acc, b1, kFuseH, wave, lane, bufa);
+  __syncthreads();
+  fused_layer<2>(bufa, w2, kFuseH, kFuseH, wave, lane, acc);
+  fused_hidden_store<2>(acc, b2, kFuseH, wave, lane, bufb);
+  __syncthreads();
+  fused_layer<CPW3>(bufb, w3, kFuseH, nb, wave, lane, acc);
+  // argmax: per (row tile, register) the 32 lanes of a half share a row
+#pragma unroll
+  for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      float best = -__builtin_inff();
+      int bi = nb;
+#pragma unroll
+      for (int c = 0; c < CPW3; ++c) {
+        const int ct = wave * CPW3 + c;
+        if (ct < nb / 32) {
+          const int col = ct * 32 + (lane & 31);
+          const float v = acc[c][rt][r] + b3[col];
+          if (v > best) {
+            best = v;
+            bi = col;
+          }
+        }
+      }
+      for (int o = 1; o < 32; o <<= 1) {
+        const float ob = __shfl_xor(best, o);
+        const int oi = __shfl_xor(bi, o);
+        if (ob > best || (ob == best && oi < bi)) {
+          best = ob;
+          bi = oi;
+        }
+      }
+      if ((lane & 31) == 0) {
+        const int row = rt * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        bval[wave][row] = best;
+        bidx[wave][row] = bi;
+      }
+    }
+  __syncthreads();
+  if (threadIdx.x < kFuseRows) {
+    const int row = threadIdx.x;
+    float best = -__builtin_inff();
+    int bi = nb;
+    for (int w = 0; w < 4; ++w) {
+      const float v = bval[w][row];
+      const int i = bidx[w][row];
+      if (v > best || (v == best && i < bi)) {
+        best = v;
+        bi = i;
+      }
+    }
+    if (row0 + row < n) bins[row0 + row] = bi >= nb ? 0 : bi;
+  }
+}
+
 unsigned grid_of(int64_t work, int per) { return static_cast<unsigned>((work + per - 1) / per); }
 
 void gemm(const GemmArgs& g, hipStream_t s) {
@@ -432,7 +573,8 @@ void Router::graph_labels(Engine& e, int sweeps, std::vector<int32_t>& labels) {
   std::vector<int32_t> ids(static_cast<size_t>(n) * k), perm(n), rs(n), rc(n);
   std::vector<float> vals(static_cast<size_t>(n) * k);
   e.get_graph(ids.data(), vals.data());
-  e.get_forest(0, perm.data(), rs.data(), rc.data());
+  // routers with different seeds start from different trees' leaf orders
+  e.get_forest(static_cast<int>(seed_ % static_cast<uint32_t>(e.params().trees)), perm.data(), rs.data(), rc.data());
   labels.assign(n, 0);
   for (int64_t p = 0; p < n; ++p) labels[perm[p]] = static_cast<int32_t>(p * nb_ / n);
   // symmetrised adjacency (CSR): every graph edge in both directions
@@ -638,6 +780,20 @@ void Router::logits(const float* x_raw, int64_t m, float* z) {
 
 void Router::assign(const float* x_raw, int64_t n, int ld, int32_t* bins, hipStream_t s) {
   AKNN_HIP_TRY(hipSetDevice(device_));
+  if (fused_ && h_ == kFuseH && dp_ <= kFuseH && (nb_ == 256 || nb_ == 128 || nb_ == 64 || nb_ == 32)) {
+    AKNN_HIP_TRY(hipEventRecord(ev_[0], s));
+    const unsigned grid = grid_of(n, kFuseRows);
+    if (nb_ == 256)
+      hipLaunchKernelGGL(router_fused_kernel<2>, dim3(grid), dim3(256), 0, s, x_raw, ld, d_, dp_, n, mu_, inv_sigma_,
+                         w1b_, p_ + off_[1], w2b_, p_ + off_[3], w3b_, p_ + off_[5], nb_, bins);
+    else
+      hipLaunchKernelGGL(router_fused_kernel<1>, dim3(grid), dim3(256), 0, s, x_raw, ld, d_, dp_, n, mu_, inv_sigma_,
+                         w1b_, p_ + off_[1], w2b_, p_ + off_[3], w3b_, p_ + off_[5], nb_, bins);
+    AKNN_HIP_TRY(hipGetLastError());
+    AKNN_HIP_TRY(hipEventRecord(ev_[1], s));
+    assigned_ = true;
+    return;
+  }
   const int64_t chunk = std::min<int64_t>(kAssignChunk, (n + 63) / 64 * 64);
   ensure_batch(chunk);
   AKNN_HIP_TRY(hipEventRecord(ev_[0], s));
@@ -654,11 +810,41 @@ void Router::assign(const float* x_raw, int64_t n, int ld, int32_t* bins, hipStr
     AKNN_HIP_TRY(hipGetLastError());
   }
   AKNN_HIP_TRY(hipEventRecord(ev_[1], s));
+  assigned_ = true;
+}
+
+void Router::bins_of(const float* x_raw, int64_t m, int32_t* bins) {
+  if (m < 1 || m > (1 << 24)) throw std::invalid_argument("router: 1..2^24 rows");
+  AKNN_HIP_TRY(hipSetDevice(device_));
+  float* xd = nullptr;
+  int32_t* bd = nullptr;
+  AKNN_HIP_TRY(hipMalloc(&xd, sizeof(float) * m * d_));
+  if (hipMalloc(&bd, sizeof(int32_t) * m) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(xd);
+    throw HipError("router: out of device memory");
+  }
+  struct Free {
+    void* a;
+    void* b;
+    ~Free() {
+      (void)hipFree(a);
+      (void)hipFree(b);
+    }
+  } fr{xd, bd};
+  AKNN_HIP_TRY(hipMemcpy(xd, x_raw, sizeof(float) * m * d_, hipMemcpyHostToDevice));
+  assign(xd, m, d_, bd, stream_);
+  AKNN_HIP_TRY(hipMemcpyAsync(bins, bd, sizeof(int32_t) * m, hipMemcpyDeviceToHost, stream_));
+  AKNN_HIP_TRY(hipStreamSynchronize(stream_));
 }
 
 double Router::last_assign_ms() const {
+  if (!assigned_) return 0.0;
   float ms = 0.0f;
-  if (hipEventSynchronize(ev_[1]) != hipSuccess || hipEventElapsedTime(&ms, ev_[0], ev_[1]) != hipSuccess) return 0.0;
+  if (hipEventSynchronize(ev_[1]) != hipSuccess || hipEventElapsedTime(&ms, ev_[0], ev_[1]) != hipSuccess) {
+    (void)hipGetLastError();  // leave no error behind for the next launch check
+    return 0.0;
+  }
   return ms;
 }
 

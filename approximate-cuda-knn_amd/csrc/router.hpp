@@ -48,8 +48,8 @@ class Router {
   size_t offset(int i) const { return off_[i]; }
 
   // Balanced partition of the engine's graph (host): initial parts are equal
-  // runs of tree 0's leaf order, refined by `sweeps` capacity-bounded label
-  // propagation sweeps over the symmetrised graph.
+  // runs of the leaf order of tree (seed mod trees), refined by `sweeps`
+  // capacity-bounded label propagation sweeps over the symmetrised graph.
   void graph_labels(Engine& e, int sweeps, std::vector<int32_t>& labels);
   // Standardisation (mu, inv_sigma) from the engine's raw points, then
   // `epochs` of Adam over shuffled batches of the engine's raw points.
@@ -63,6 +63,11 @@ class Router {
   void assign(const float* x_raw, int64_t n, int ld, int32_t* bins, hipStream_t s);
   // Logits of m raw host points (m x bins), FP32 (testing).
   void logits(const float* x_raw, int64_t m, float* z);
+  // Bins of m raw host points through assign() (testing and host callers).
+  void bins_of(const float* x_raw, int64_t m, int32_t* bins);
+  // assign() through the fused one-kernel path where the shape allows it
+  // (hidden 256, dp <= 256, bins <= 256; default on) or the layer-by-layer one
+  void set_fused(bool f) { fused_ = f; }
 
   void get_params(float* p, float* mu, float* inv_sigma) const;
   void set_params(const float* p, const float* mu, float inv_sigma);
@@ -106,6 +111,8 @@ class Router {
   int64_t step_ = 0;
   RouterTrainStats stats_;
   hipEvent_t ev_[2] = {nullptr, nullptr};
+  bool assigned_ = false;
+  bool fused_ = true;
 };
 
 }  // namespace aknn

@@ -136,7 +136,7 @@ void launch_sketch(const float* x, int64_t n, int d, int ld, int metric, _Float1
   if (ldh % 8 != 0 || ldh < d) throw HipError("sketch: row stride must be a multiple of 8 covering d");
   const int64_t blocks = (n + kSketchWaves - 1) / kSketchWaves;
   hipLaunchKernelGGL(sketch_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, x, n, d, ld,
-                     metric == kTaxicab ? 1 : 0, xh, ldh, eps, nh);
+                     (metric == kTaxicab || metric == kTaxicabL1) ? 1 : 0, xh, ldh, eps, nh);
   AKNN_HIP_TRY(hipGetLastError());
 }
 

@@ -51,10 +51,20 @@ int aknn_set_results_file(aknn_handle* h, const char* path);
  * minSize), euclideandistancematrixgpu.cu:2759-2819: selects the GPU, reads
  * the CSV data file (two header lines, then N lines of D comma-separated
  * values, plus a trailing integer label when klaster != 0, .cu:2712-2757) and
- * uploads it.  Rejects inputs the reference cannot run correctly: N < k+1,
+ * uploads it.  A data file named *.fbin is read as binary instead (uint32 N,
+ * uint32 D, then N*D float32, row-major; no reference counterpart).  Rejects inputs the reference cannot run correctly: N < k+1,
  * trees < 1, missing or short rows. */
 int aknn_initialize(aknn_handle* h, int n, int d, int k, int device, int distance, int klaster, int trees,
                     int propagations, int min_size);
+
+/* loadData (euclideandistancematrixgpu.cu:2712-2757) on its own, without a
+ * device: reads n points of d values from the CSV (or *.fbin) file at path
+ * into x[n*d] (row-major) and, with klaster, the labels into labels[n].
+ * threads: CSV parsing threads (0: up to 16, 1: sequential; the values are
+ * the same).  On failure returns AKNN_ERR_IO or AKNN_ERR_ARGUMENT with the
+ * message in error[error_len] (may be NULL). */
+int aknn_read_points(const char* path, int64_t n, int d, int klaster, int threads, float* x, int32_t* labels,
+                     char* error, int error_len);
 
 /* Same as aknn_initialize with the points given in memory instead of a file
  * (x: n*d floats, row-major, point-major).  No reference counterpart: it is
@@ -113,6 +123,19 @@ int aknn_deinitialize(aknn_handle* h);
  *                   option, tree index), each level projects its buckets on
  *                   one direction and splits each bucket at a sampled median
  *                   (balanced leaves of leaf/2..leaf points, no rand() use)
+ *   "taxicab_l1"    1: TAXICAB keys and stores the L1 total instead of its
+ *                   square root (the reference stores sqrtf, .cu:568, :1477)
+ *   "scale_all"     1: normalization divides every dimension by the max-abs
+ *                   over all dimensions (the reference: the first 256 only,
+ *                   .cu:57-65, :76-84)
+ *   "fixed"         sets cosine_fixed, taxicab_l1 and scale_all together
+ *   "io_threads"    host threads parsing the CSV in aknn_initialize (0, the
+ *                   default: up to 16; 1: the sequential loader); the values
+ *                   are the same for any count
+ *   "graph_rounds"  1 (default): once the propagation round mode is fixed (the
+ *                   first calculate() probes it, or nothing is screened), the
+ *                   rounds are captured into one hipGraph and replayed by later
+ *                   calculate() calls (same results); 0: launch round by round
  *   "router_trees"  with split 1: trees 0..router_trees-1 take the router bins
  *                   (aknn_set_bins or an attached router) as their first
  *                   levels, one bit per level, before the median levels */
@@ -140,8 +163,9 @@ int aknn_get_forest(aknn_handle* h, int tree, int32_t* perm, int32_t* range_star
 int aknn_get_stage_times(aknn_handle* h, double* ms, int n);
 
 /* Kernel statistics of the last calculate(): name is "leaf_knn",
- * "propagate", "forest" or "leaf_mfma" (the screened leaf k-NN launches
- * alone).  launches, summed launch time (HIP events on the launch stream), and
+ * "propagate", "forest", "leaf_mfma" (the screened leaf k-NN launches
+ * alone) or "propagate_graph" (the rounds replayed from the captured graph;
+ * zero launches when they were launched one by one).  launches, summed launch time (HIP events on the launch stream), and
  * algorithmic units summed over the launches (flops for leaf_knn: 3D per
  * pair; GEMM-form flops for leaf_mfma: 2 * padded D per pair; bytes for
  * propagate and forest). */
@@ -202,7 +226,8 @@ aknn_router* aknn_router_create(int device, int d, int hidden, int bin_bits, uin
 void aknn_router_destroy(aknn_router* r);
 
 /* Balanced graph labels of the points of `trained` (a handle after
- * aknn_calculate): equal runs of tree 0's leaf order, then `sweeps` sweeps of
+ * aknn_calculate): equal runs of the leaf order of tree (router seed mod
+ * trees), then `sweeps` sweeps of
  * capacity-bounded label propagation over the symmetrised graph (parts within
  * 0.95..1.05 n/bins).  labels[n]; cut (may be NULL) receives the fraction of
  * graph edges cut before and after the sweeps. */
@@ -224,6 +249,10 @@ int aknn_router_gradients(aknn_router* r, const float* x, int64_t m, const int32
                           double* loss);
 /* FP32 logits z[m][2^bin_bits] of m raw points. */
 int aknn_router_logits(aknn_router* r, const float* x, int64_t m, float* z);
+/* Bins of m raw points x[m][d], as aknn_calculate assigns them: fused 1 runs
+ * the one-kernel inference (hidden 256, d <= 256, up to 256 bins; other
+ * shapes fall back), 0 the layer-by-layer products; both give the same bins. */
+int aknn_router_bins(aknn_router* r, const float* x, int64_t m, int fused, int32_t* bins);
 int aknn_router_param_count(aknn_router* r, int64_t* count);
 /* Parameters (flat layout above), mu[d] and inv_sigma; any pointer may be NULL. */
 int aknn_router_get_params(aknn_router* r, float* params, float* mu, float* inv_sigma);
@@ -237,9 +266,12 @@ int aknn_router_load(aknn_router* r, const char* path);
 int aknn_router_stats(aknn_router* r, double* out, int n);
 const char* aknn_router_last_error(aknn_router* r);
 
-/* Attaches a router to a handle (NULL detaches); detach before destroying the
- * router.  It must have the handle's D and device. */
+/* Attaches a router to a handle (NULL detaches every router); detach before
+ * destroying the router.  It must have the handle's D and device. */
 int aknn_set_router(aknn_handle* h, aknn_router* r);
+/* Appends another router (same bin_bits): routed tree t takes router t mod R,
+ * so routers trained on different labels give routed trees different tops. */
+int aknn_add_router(aknn_handle* h, aknn_router* r);
 
 #ifdef __cplusplus
 }

@@ -83,9 +83,12 @@ int32_t orc_pick(int32_t r, int32_t count) {
  * only, and only those dimensions are divided by it (.cu:57-65, :76-84 use
  * `if (tid < D)`, not a loop).
  * ------------------------------------------------------------------------- */
-void orc_normalize(float* x, int64_t n, int d) {
+void orc_normalize(float* x, int64_t n, int d) { orc_normalize_ex(x, n, d, 0); }
+
+/* scale_all (fixed mode): every dimension enters the max-abs and is scaled. */
+void orc_normalize_ex(float* x, int64_t n, int d, int scale_all) {
     int dd;
-    int dm = d < 256 ? d : 256;
+    int dm = scale_all ? d : (d < 256 ? d : 256);
     float g = 0.0f;
 #pragma omp parallel for schedule(static)
     for (dd = 0; dd < d; ++dd) {
@@ -148,7 +151,8 @@ float orc_distance(const float* a, const float* b, int d, int metric) {
             }
             tot = tot + p;
         }
-        return sqrtf(tot);
+        /* ORC_TAXICAB_L1 (fixed mode): the L1 total itself */
+        return metric == ORC_TAXICAB_L1 ? tot : sqrtf(tot);
     }
 }
 
@@ -628,7 +632,7 @@ int orc_propagate(const float* x, int64_t n, int d, int k, int metric,
  * router_trees the shared bins[n] (bin_bits bits) as its first levels. */
 int orc_run_ex(float* x, int64_t n, int d, int k, int trees, int leaf, int props, int metric,
                uint32_t seed, int rng_order, int cosine_fixed, int split,
-               const int32_t* bins, int bin_bits, int router_trees,
+               const int32_t* bins, int bin_bits, int router_trees, int scale_all,
                int32_t* ids, float* vals, double* stage_ms) {
     orc_rng g;
     int32_t *perm, *rs, *rc, *id2;
@@ -648,7 +652,7 @@ int orc_run_ex(float* x, int64_t n, int d, int k, int trees, int leaf, int props
     rc = rs + n * trees;
     orc_rng_seed(&g, seed);
     t0 = now_ms();
-    orc_normalize(x, n, d);
+    orc_normalize_ex(x, n, d, scale_all);
     t1 = now_ms();
     for (t = 0; t < trees && !rcode; ++t) {
         if (split) {
@@ -679,7 +683,7 @@ int orc_run_ex(float* x, int64_t n, int d, int k, int trees, int leaf, int props
 int orc_run(float* x, int64_t n, int d, int k, int trees, int leaf, int props, int metric,
             uint32_t seed, int rng_order, int cosine_fixed,
             int32_t* ids, float* vals, double* stage_ms) {
-    return orc_run_ex(x, n, d, k, trees, leaf, props, metric, seed, rng_order, cosine_fixed, 0, NULL, 0, 0,
+    return orc_run_ex(x, n, d, k, trees, leaf, props, metric, seed, rng_order, cosine_fixed, 0, NULL, 0, 0, 0,
                       ids, vals, stage_ms);
 }
 

@@ -35,6 +35,9 @@ extern "C" {
 #define ORC_EUCLIDEAN 0
 #define ORC_TAXICAB 1
 #define ORC_COSINE 2
+/* fixed mode of TAXICAB: keys and stored values are the L1 total itself (the
+ * reference stores its sqrtf, .cu:568, :1477) */
+#define ORC_TAXICAB_L1 3
 
 #define ORC_STUB_ID (-1)
 #define ORC_STUB_DIST 100000.0f
@@ -53,6 +56,9 @@ int32_t orc_pick(int32_t r, int32_t count);
 
 /* A.2: in place on a row-major [n][d] array. */
 void orc_normalize(float* x, int64_t n, int d);
+/* scale_all 1 (fixed mode): the max-abs and the division cover every
+ * dimension instead of the first 256. */
+void orc_normalize_ex(float* x, int64_t n, int d, int scale_all);
 
 /* A.7: the FP32 key of a pair (EUCLIDEAN sqrt(L2^2), TAXICAB sqrt(L1),
  * COSINE float(1 - (double)cos)). */
@@ -103,10 +109,11 @@ int orc_run(float* x, int64_t n, int d, int k, int trees, int leaf, int props, i
             int32_t* ids, float* vals, double* stage_ms);
 
 /* orc_run with the forest rule: split 0 (reference sign split) or 1
- * (median split; trees t < router_trees take bins[n] as their first levels). */
+ * (median split; trees t < router_trees take bins[n] as their first levels),
+ * and scale_all for the normalization; metric may be ORC_TAXICAB_L1. */
 int orc_run_ex(float* x, int64_t n, int d, int k, int trees, int leaf, int props, int metric,
                uint32_t seed, int rng_order, int cosine_fixed, int split,
-               const int32_t* bins, int bin_bits, int router_trees,
+               const int32_t* bins, int bin_bits, int router_trees, int scale_all,
                int32_t* ids, float* vals, double* stage_ms);
 
 /* Router MLP forward (and backward when labels and grads are given): returns

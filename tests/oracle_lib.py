@@ -14,6 +14,7 @@ ORACLE_DIR = os.path.join(ROOT, "oracle")
 ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
 
 EUCLIDEAN, TAXICAB, COSINE = 0, 1, 2
+TAXICAB_L1 = 3  # fixed mode: the L1 total without the square root
 METRICS = {"EUCLIDEAN": EUCLIDEAN, "TAXICAB": TAXICAB, "COSINE": COSINE}
 
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -62,7 +63,8 @@ def lib():
         L.orc_run_ex.argtypes = [_f32p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_int,
                                  ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
-                                 _i32p, _f32p, _f64p]
+                                 ctypes.c_int, _i32p, _f32p, _f64p]
+        L.orc_normalize_ex.argtypes = [_f32p, ctypes.c_int64, ctypes.c_int, ctypes.c_int]
         L.orc_mlp.argtypes = [_f32p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                               _f32p, _f32p, ctypes.c_float, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                               ctypes.c_void_p]
@@ -87,9 +89,9 @@ def rng(seed=1):
     return g
 
 
-def normalize(x):
+def normalize(x, scale_all=0):
     x = np.ascontiguousarray(x, dtype=np.float32).copy()
-    lib().orc_normalize(x, x.shape[0], x.shape[1])
+    lib().orc_normalize_ex(x, x.shape[0], x.shape[1], scale_all)
     return x
 
 
@@ -206,7 +208,7 @@ def run(x_raw, k, trees, leaf, props, metric, seed=1, rng_order=0, cosine_fixed=
 
 
 def run_ex(x_raw, k, trees, leaf, props, metric, seed=1, rng_order=0, cosine_fixed=0, split=0, bins=None,
-           bin_bits=0, router_trees=0):
+           bin_bits=0, router_trees=0, scale_all=0):
     """Whole pipeline with the forest rule (split 0 reference, 1 median)."""
     x = np.ascontiguousarray(x_raw, dtype=np.float32).copy()
     n, d = x.shape
@@ -215,7 +217,7 @@ def run_ex(x_raw, k, trees, leaf, props, metric, seed=1, rng_order=0, cosine_fix
     ms = np.zeros(4, np.float64)
     keep, bp = _bins_ptr(bins)
     r = lib().orc_run_ex(x, n, d, k, trees, leaf, props, metric, seed, rng_order, cosine_fixed, split, bp,
-                         bin_bits, router_trees, ids, vals, ms)
+                         bin_bits, router_trees, scale_all, ids, vals, ms)
     if r != 0:
         raise ValueError("orc_run_ex failed")
     return x, ids, vals, ms

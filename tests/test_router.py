@@ -144,3 +144,27 @@ def test_gpu_router_trains_and_routed_trees_gain(aknn, oracle, datasets):
     print(f"[gpu] recall unrouted {rec[0]:.4f} routed {rec[4]:.4f}", flush=True)
     assert rec[4] > rec[0] + 0.02
     r.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("d,hidden,bin_bits", [(128, 256, 8), (20, 256, 5), (96, 256, 6), (256, 256, 7),
+                                               (300, 256, 8), (37, 64, 6)])
+def test_gpu_fused_bins_equal_layerwise(aknn, d, hidden, bin_bits):
+    """The one-kernel inference calculate() uses gives the layer-by-layer
+    path's bins, which are the argmax of its logits (lowest bin on ties);
+    shapes outside the fused kernel fall back to the layer-by-layer path."""
+    print(f"[gpu] fused bins d={d} h={hidden} bits={bin_bits}", flush=True)
+    rng = np.random.default_rng(d + bin_bits)
+    r = aknn.Router(d, hidden, bin_bits, seed=5)
+    p, mu, inv = r.get_params()
+    r.set_params(p, rng.normal(0, 0.3, d).astype(np.float32), 0.7)
+    m = 5000 + d  # not a multiple of the 64-row tile
+    x = rng.normal(0, 1.5, (m, d)).astype(np.float32)
+    x[7] = x[3]
+    bf = r.bins_of(x, fused=True)
+    bl = r.bins_of(x, fused=False)
+    z = r.logits(x)
+    assert (bf == bl).all()
+    assert (bl == np.argmax(z, axis=1)).all()
+    assert 0 <= bf.min() and bf.max() < (1 << bin_bits)
+    r.close()
