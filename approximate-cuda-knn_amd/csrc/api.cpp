@@ -231,6 +231,8 @@ int aknn_set_option(aknn_handle* h, const char* name, int64_t value) {
     } else if (n == "io_threads") {
       if (value < 0 || value > 256) return fail(h, AKNN_ERR_ARGUMENT, "io_threads must be in [0, 256]");
       h->io_threads = static_cast<int>(value);
+    } else if (n == "pca_screen") {
+      h->params.pca_screen = value != 0;
     } else if (n == "graph_rounds") {
       h->params.graph_rounds = value != 0;
     } else if (n == "router_trees") {
@@ -326,6 +328,13 @@ int aknn_get_candidate_counts(aknn_handle* h, const char* stage, double* examine
     if (!stage) return fail(h, AKNN_ERR_ARGUMENT, "null stage");
     const std::string n(stage);
     const aknn::KernelStats* s = nullptr;
+    if (n == "pca") {
+      // the principal-subspace screen of the last build: its width (0: unused)
+      // and the share of the sample's second moment in the top 16 directions
+      if (examined) *examined = h->engine->pca_rank();
+      if (rescored) *rescored = h->engine->pca_retained16();
+      return AKNN_OK;
+    }
     if (n == "propagate")
       s = &h->engine->stats_propagate;
     else if (n == "leaf_knn")

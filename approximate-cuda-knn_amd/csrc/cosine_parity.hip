@@ -11,7 +11,10 @@
 // the chunk loop: pool = carried k (key = stored) + next chunk (key = distance),
 // first k by (key, id), store 1 - key, re-sort by (stored, id).
 //
-// MI355X mapping: one workgroup per point (persistent grid).  The union is a
+// MI355X mapping: one workgroup per point (persistent grid), points taken in
+// the first tree's leaf order with each XCD's groups on a contiguous stretch
+// of it, so the groups running together on one XCD read overlapping candidate
+// rows through its L2.  The union is a
 // two-level bitmap in a per-workgroup global scratch (one bit per point plus
 // one summary bit per 32-bit word, both L2-resident), enumerated in id order
 // with a block scan and cleared as it is read.  Keys are computed by all four
@@ -49,7 +52,8 @@ __global__ __launch_bounds__(256) void cosine_parity_kernel(const float* __restr
                                                             const int32_t* __restrict__ rc,
                                                             const int32_t* __restrict__ perm, int64_t cap,
                                                             CosineParityScratch sc, int32_t* __restrict__ ids,
-                                                            float* __restrict__ vals) {
+                                                            float* __restrict__ vals,
+                                                            const int32_t* __restrict__ order) {
   extern __shared__ __attribute__((aligned(16))) float xi[];
   __shared__ int sh[4];
   uint32_t* bits = sc.bits + static_cast<int64_t>(blockIdx.x) * sc.words;
@@ -58,7 +62,11 @@ __global__ __launch_bounds__(256) void cosine_parity_kernel(const float* __restr
   float* ckey = sc.key + static_cast<int64_t>(blockIdx.x) * sc.max_cand;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
-  for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
+  const int64_t g = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+  // logical group g takes the stretch [g*per, (g+1)*per) of the leaf order
+  for (int64_t pidx = g * per; pidx < n && pidx < (g + 1) * per; ++pidx) {
+    const int64_t i = order != nullptr ? order[pidx] : pidx;
     for (int dd = tid; dd < d; dd += 256) xi[dd] = x[i * ld + dd];
     // union of the start-node ranges (.cu:160-191)
     for (int t = 0; t < trees; ++t) {
@@ -148,8 +156,9 @@ void launch_cosine_parity_initial(const float* x, int d, int ld, const float* no
   if (k > 64) throw HipError("COSINE parity path supports k <= 64");
   const size_t shmem = static_cast<size_t>((d + 3) / 4 * 4) * sizeof(float);
   if (shmem > 64 * 1024) throw HipError("COSINE parity path: d too large");
+  if (sc.groups % 8 != 0) throw HipError("COSINE parity path: groups must be a multiple of 8");
   hipLaunchKernelGGL(cosine_parity_kernel, dim3(sc.groups), dim3(256), shmem, s, x, d, ld, norms, n, k, trees, rs, rc,
-                     perm, cap, sc, ids, vals);
+                     perm, cap, sc, ids, vals, perm);
   AKNN_HIP_TRY(hipGetLastError());
 }
 

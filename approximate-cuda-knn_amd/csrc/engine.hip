@@ -141,6 +141,8 @@ void Engine::release() {
   fb_ = ForestBuffers{};
   cps_ = CosineParityScratch{};
   dirs_ = proj_ = nullptr;
+  pca_p_ = pca_mom_ = pca_proj_ = pca_e_ = nullptr;
+  pca_ = PcaBasis{};
   nd_ = 0;
   dirs_key_ = -1;
   bins_ = nullptr;
@@ -381,11 +383,30 @@ void Engine::initial_knn() {
     }
     for (int t = 0; t < p_.trees; ++t)
       launch_leaf_ranges(perm_ + t * n_, leaves_ + t * n_, nleaves_[t], rs_ + t * n_, rc_ + t * n_, stream_);
+    // a point's union holds at most the sum over trees of the largest start
+    // range: that bounds each group's candidate scratch, and the scratch
+    // budget (4 GB) sets how many groups run at once (256..2048)
+    int64_t bound = 0;
+    {
+      std::vector<Leaf> host;
+      for (int t = 0; t < p_.trees; ++t) {
+        host.resize(nleaves_[t]);
+        AKNN_HIP_TRY(hipMemcpy(host.data(), leaves_ + t * n_, sizeof(Leaf) * nleaves_[t], hipMemcpyDeviceToHost));
+        int32_t mx = 0;
+        for (const Leaf& l : host) mx = std::max(mx, l.rcount);
+        bound += mx;
+      }
+    }
+    const int64_t need = std::min<int64_t>(n_, std::max<int64_t>(bound, 1));
+    if (cps_.bits && need > cps_.max_cand) cps_ = CosineParityScratch{};  // grow (old buffers freed at release)
     if (!cps_.bits) {
-      cps_.groups = static_cast<int>(std::min<int64_t>(n_, 256));
-      cps_.words = (n_ + 31) / 32;
-      cps_.summary_words = (cps_.words + 31) / 32;
-      cps_.max_cand = n_;
+      const int64_t words = (n_ + 31) / 32, summary = (words + 31) / 32;
+      const int64_t per_group = 4 * (words + summary) + 8 * need;
+      const int64_t fit = (int64_t{4} << 30) / std::max<int64_t>(per_group, 1);
+      cps_.groups = static_cast<int>(std::max<int64_t>(8, std::min<int64_t>({2048, fit / 8 * 8, (n_ + 7) / 8 * 8})));
+      cps_.words = words;
+      cps_.summary_words = summary;
+      cps_.max_cand = need;
       cps_.bits = alloc<uint32_t>(static_cast<size_t>(cps_.groups) * cps_.words);
       cps_.summary = alloc<uint32_t>(static_cast<size_t>(cps_.groups) * cps_.summary_words);
       cps_.cand = alloc<int32_t>(static_cast<size_t>(cps_.groups) * cps_.max_cand);
@@ -458,7 +479,8 @@ void Engine::initial_knn() {
 
 // Arguments of propagation round r, which reads graph buffer `cur` and
 // writes the other one.
-static PropagateArgs round_args(const Params& p, int kmetric, bool flip, bool screen, float* x, int d, int ld,
+static PropagateArgs round_args(const Params& p, int kmetric, bool flip, bool screen, const PcaBasis& pb,
+                                float* pca_proj, float* pca_e, float* x, int d, int ld,
                                 _Float16* xh, int ldh, float* eps, int8_t* q8, int ld8, float* s8, int32_t* a8,
                                 float* e8, float* norms, int32_t* order, int64_t n, int32_t* const gids[2],
                                 float* const gvals[2], int cur, int r, unsigned long long* ncand,
@@ -480,6 +502,16 @@ static PropagateArgs round_args(const Params& p, int kmetric, bool flip, bool sc
       a.a8 = a8;
       a.e8 = e8;
     }
+  }
+  if (pb.r > 0 && pca_proj != nullptr) {
+    // the principal-subspace screen replaces the sketch screens
+    a.xh = nullptr;
+    a.eps = nullptr;
+    a.q8 = nullptr;
+    a.pca = pca_proj;
+    a.pca_r = pb.r;
+    a.pca_e = pca_e;
+    a.pca_sigma = pb.sigma;
   }
   a.norms = norms;
   a.order = order;
@@ -510,7 +542,9 @@ void Engine::propagate() {
   // rounds and later calls (clustered data can make the exact path's row
   // reuse win).  A round where more than a quarter of the candidates survived
   // the screen also switches the remaining rounds to the exact path.
-  const bool can_screen = propagation_screened();
+  // with the principal-subspace screen every round screens with it (no probe)
+  const bool pca = pca_.r > 0;
+  const bool can_screen = propagation_screened() && !pca;
   rounds_from_graph_ = false;
   if (p_.graph_rounds && p_.props > 0 && (!can_screen || prop_mode_ >= 0) && propagate_graph(base)) return;
   bool screen = can_screen && prop_mode_ != 1;
@@ -535,9 +569,9 @@ void Engine::propagate() {
       }
       if (round_screened_[r - 1] && c[0] > 0 && 4 * c[1] > c[0]) screen = false;
     }
-    round_screened_[r] = screen ? 1 : 0;
+    round_screened_[r] = pca ? 2 : (screen ? 1 : 0);
     // buffers alternate each round: round r reads buffer cur_ (= c0 ^ (r & 1))
-    const PropagateArgs a = round_args(p_, kmetric(), flip, screen, x_, d_, ld_, xh_, ldh_, eps_, q8_, ld8_, s8_,
+    const PropagateArgs a = round_args(p_, kmetric(), flip, screen, pca_, pca_proj_, pca_e_, x_, d_, ld_, xh_, ldh_, eps_, q8_, ld8_, s8_,
                                        a8_, e8_, norms_, perm_, n_, gids_, gvals_, cur_, r, ncand_, changed_);
     AKNN_HIP_TRY(hipEventRecord(launch_ev(base + 2 * r), stream_));
     launch_propagate(a, stream_);
@@ -565,17 +599,19 @@ void Engine::propagate() {
 // before the launch, outside the graph.  Returns false if capture is not
 // possible (the caller then launches the rounds one by one).
 bool Engine::propagate_graph(int base) {
-  const bool can_screen = propagation_screened();
+  const bool pca = pca_.r > 0;
+  const bool can_screen = propagation_screened() && !pca;
   const bool screen = can_screen && prop_mode_ == 0;
-  const int mode = screen ? 1 : 0;
+  const int mode = pca ? 2 + pca_.r : (screen ? 1 : 0);
   const bool flip = p_.metric == kCosine && !p_.cosine_fixed;
+  if (pca && round_graph_sigma_ != pca_.sigma) drop_round_graph();  // sigma is a kernel argument
   if (!round_exec_ || round_graph_mode_ != mode) {
     drop_round_graph();
     hipGraph_t graph = nullptr;
     AKNN_HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     try {
       for (int r = 0; r < p_.props; ++r)
-        launch_propagate(round_args(p_, kmetric(), flip, screen, x_, d_, ld_, xh_, ldh_, eps_, q8_, ld8_, s8_, a8_,
+        launch_propagate(round_args(p_, kmetric(), flip, screen, pca_, pca_proj_, pca_e_, x_, d_, ld_, xh_, ldh_, eps_, q8_, ld8_, s8_, a8_,
                                     e8_, norms_, perm_, n_, gids_, gvals_, (cur_ + r) & 1, r, ncand_, changed_),
                          stream_);
     } catch (...) {
@@ -592,11 +628,12 @@ bool Engine::propagate_graph(int base) {
       return false;
     }
     round_graph_mode_ = mode;
+    round_graph_sigma_ = pca ? pca_.sigma : 0.0;
   }
   AKNN_HIP_TRY(hipEventRecord(launch_ev(base), stream_));
   AKNN_HIP_TRY(hipGraphLaunch(round_exec_, stream_));
   AKNN_HIP_TRY(hipEventRecord(launch_ev(base + 1), stream_));
-  round_screened_.assign(p_.props, screen ? 1 : 0);
+  round_screened_.assign(p_.props, pca ? 2 : (screen ? 1 : 0));
   cur_ ^= p_.props & 1;
   rounds_from_graph_ = true;
   stats_propagate.launches = p_.props;
@@ -632,6 +669,23 @@ void Engine::calculate() {
   if (norms_) launch_norms(x_, n_, d_, ld_, norms_, stream_);
   if (sketch) launch_sketch(x_, n_, d_, ld_, kmetric(), xh_, ldh_, eps_, nh_, stream_);
   if (sketch8) launch_sketch8(x_, n_, d_, ld_, q8_, ld8_, s8_, a8_, e8_, stream_);
+  pca_ = PcaBasis{};
+  if (p_.pca_screen && p_.metric == kEuclidean && p_.props > 0 && d_ >= 16 && d_ <= 256) {
+    // this build's principal subspace (a stream sync and a host eigen-solve
+    // inside the bracket), then every point's projection and radius
+    if (!pca_mom_) {
+      pca_mom_ = alloc<float>(static_cast<size_t>(d_) * d_);
+      pca_p_ = alloc<float>(static_cast<size_t>(32) * d_);
+      pca_proj_ = alloc<float>(static_cast<size_t>(n_) * 32);
+      pca_e_ = alloc<float>(n_);
+    }
+    pca_ = compute_pca_basis(x_, n_, d_, ld_, pca_mom_, stream_);
+    if (pca_.r > 0) {
+      AKNN_HIP_TRY(hipMemcpyAsync(pca_p_, pca_.p.data(), sizeof(float) * pca_.p.size(), hipMemcpyHostToDevice,
+                                  stream_));
+      launch_pca_project(x_, n_, d_, ld_, pca_p_, pca_.r, pca_proj_, pca_e_, stream_);
+    }
+  }
   AKNN_HIP_TRY(hipEventRecord(ev(1), stream_));
   if (p_.split == 1)
     build_forest_median();
@@ -682,7 +736,8 @@ void Engine::calculate() {
     double cand = 4.0 * d_ * nc[2 * r];
     if (round_screened_[r]) {
       const bool i8 = p_.sketch_int8 && p_.metric == kEuclidean;
-      const double row = i8 ? ld8_ + 12.0 : 2.0 * ldh_ + 4.0;  // sketch row + per-row scalars
+      const double row = round_screened_[r] == 2 ? 4.0 * pca_.r + 4.0  // projection + radius
+                                                 : (i8 ? ld8_ + 12.0 : 2.0 * ldh_ + 4.0);  // sketch row + scalars
       per_point += row;
       cand = row * nc[2 * r] + 4.0 * d_ * nc[2 * r + 1];
     }

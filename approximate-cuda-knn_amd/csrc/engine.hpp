@@ -33,6 +33,7 @@ struct Params {
   int taxicab_l1 = 0;     // TAXICAB keys and stores the L1 total instead of its square root
   int scale_all = 0;      // normalization scales every dimension, not only the first 256
   int graph_rounds = 1;   // once the round mode is fixed, replay the propagation rounds as one captured hipGraph
+  int pca_screen = 0;     // EUCLIDEAN propagation screens with a principal-subspace bound when the data allow (same results)
 };
 
 // Per-stage times of the last calculate(), milliseconds.
@@ -80,6 +81,7 @@ class Engine {
     p_.taxicab_l1 = p.taxicab_l1;
     p_.scale_all = p.scale_all;
     p_.graph_rounds = p.graph_rounds;
+    p_.pca_screen = p.pca_screen;
     drop_round_graph();
   }
   // Router bins of the points (host array [n], values < 2^bin_bits): the
@@ -109,6 +111,9 @@ class Engine {
   // recall@k of the current graph against the FP64 exact k-NN of the queries
   double recall(const int64_t* queries, int64_t nq);
   bool rounds_from_graph() const { return rounds_from_graph_; }
+  // basis width of the last build's principal-subspace screen (0: not used)
+  int pca_rank() const { return pca_.r; }
+  double pca_retained16() const { return pca_.retained16; }
   int tree_iterations(int tree) const { return tree < static_cast<int>(iters_.size()) ? iters_[tree] : -1; }
 
   const StageTimes& times() const { return times_; }
@@ -197,8 +202,16 @@ class Engine {
   int bins_cap_ = 0;  // bin arrays bins_ holds
   bool bins_set_ = false;
   std::vector<class Router*> routers_;
+  // principal-subspace screen: basis [32][d] and moments scratch [d][d],
+  // projections [n][32] and radii [n]; pca_ is this build's basis
+  float* pca_p_ = nullptr;
+  float* pca_mom_ = nullptr;
+  float* pca_proj_ = nullptr;
+  float* pca_e_ = nullptr;
+  PcaBasis pca_;
   hipGraphExec_t round_exec_ = nullptr;
   int round_graph_mode_ = -1;
+  double round_graph_sigma_ = 0.0;
   bool rounds_from_graph_ = false;
   StageTimes times_;
 };

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "aknn_common.hpp"
 
@@ -99,6 +100,11 @@ struct PropagateArgs {
   const int32_t* a8 = nullptr;
   const float* e8 = nullptr;
   const float* norms = nullptr;
+  // principal-subspace screen (EUCLIDEAN; used instead of the sketches when set)
+  const float* pca = nullptr;
+  int pca_r = 0;
+  const float* pca_e = nullptr;
+  double pca_sigma = 1.0;
   const int32_t* order = nullptr;
   int64_t n = 0;
   int k = 0;
@@ -110,6 +116,22 @@ struct PropagateArgs {
   PropagateHistory hist;
 };
 void launch_propagate(const PropagateArgs& a, hipStream_t s);
+
+// ---- principal-subspace screen (pca.hip) ------------------------------------
+// r (16 or 32, 0: the data is not close enough to a 16- or 32-dimensional
+// subspace, or D is outside [16, 256]) FP32 rows p [r][d] of top sample
+// eigenvectors, sigma >= the largest singular value of p.
+struct PcaBasis {
+  int r = 0;
+  std::vector<float> p;
+  double sigma = 1.0;
+  double retained16 = 0.0;  // share of the sample's second moment in the top 16 directions
+};
+// moments_dev: device scratch of d*d floats.  Synchronises the stream.
+PcaBasis compute_pca_basis(const float* x, int64_t n, int d, int ld, float* moments_dev, hipStream_t s);
+// proj [n][r] = fl(p x_i), rad[n] >= ||proj_i - p x_i||.
+void launch_pca_project(const float* x, int64_t n, int d, int ld, const float* p, int r, float* proj, float* rad,
+                        hipStream_t s);
 
 // ---- FP16 sketch of the normalized points -----------------------------------
 // xh: [n][ldh] halves (ldh % 8 == 0, ldh >= d); eps[n]: upper bounds of the
@@ -144,6 +166,8 @@ void launch_recall(int metric, const float* x, int d, int ld, const double* sq, 
 
 // COSINE parity initial pass: ascending-id chunks of `cap` candidates over the
 // union of all trees' start ranges, with the store-time flip after each chunk.
+// Points are taken in the first tree's leaf order (perm); groups is a
+// multiple of 8 and max_cand bounds every point's union.
 struct CosineParityScratch {
   uint32_t* bits;     // [groups][words]
   uint32_t* summary;  // [groups][summary_words]

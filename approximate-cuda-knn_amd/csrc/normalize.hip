@@ -15,47 +15,78 @@ namespace aknn {
 namespace {
 
 constexpr int kMeanRows = 256;  // rows per LDS tile (64 dimensions wide)
+constexpr int kMeanPad = 4;     // floats of padding per LDS column (bank spread)
+constexpr int kMeanLoaders = 15;
+constexpr int kMeanPer = (kMeanRows + kMeanLoaders - 1) / kMeanLoaders;  // rows per loader lane per tile
 
 // One workgroup per 64 dimensions.  Wave 0 holds the 64 sequential sums (one
-// lane per dimension) and walks the rows of a tile in order; waves 1..15
-// stream the next tile into the other half of an LDS ring meanwhile, so a
-// whole tile of rows is in flight per memory latency instead of the handful
-// one lane could keep outstanding.
+// lane per dimension) and walks the rows of a tile in order; waves 1..15 load
+// the rows.  The add chain is the reference's sequential order and cannot be
+// split, so the aim is to keep it fed: each loader wave keeps four tiles of
+// loads in flight in registers (slot u % 4 holds tile u) and writes the next
+// tile into the free half of a two-tile LDS ring while wave 0 adds the other.
+// The ring is column-major (one padded column of rows per dimension), so the
+// adding lane reads four consecutive rows with one 16-byte LDS load.
 __global__ __launch_bounds__(1024) void mean_kernel(const float* __restrict__ x, int64_t n, int d, int ld,
                                                     float* __restrict__ mean) {
-  __shared__ float ring[2][kMeanRows][64];
+  __shared__ __attribute__((aligned(16))) float ring[2][64][kMeanRows + kMeanPad];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int dd = blockIdx.x * 64 + lane;
   const int64_t ntiles = (n + kMeanRows - 1) / kMeanRows;
-  constexpr int kLoaders = 15;
-  constexpr int kPer = (kMeanRows + kLoaders - 1) / kLoaders;
-  auto load_tile = [&](int buf, int64_t t) {
-    float v[kPer];
+  float v[4][kMeanPer];
+  auto issue = [&](float (&dst)[kMeanPer], int64_t t) {
 #pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-      const int r = wave - 1 + kLoaders * j;
+    for (int j = 0; j < kMeanPer; ++j) {
+      const int r = wave - 1 + kMeanLoaders * j;
       const int64_t row = t * kMeanRows + r;
-      v[j] = (r < kMeanRows && row < n && dd < d) ? x[row * ld + dd] : 0.0f;
-    }
-#pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-      const int r = wave - 1 + kLoaders * j;
-      if (r < kMeanRows) ring[buf][r][lane] = v[j];
+      dst[j] = (t < ntiles && r < kMeanRows && row < n && dd < d) ? x[row * ld + dd] : 0.0f;
     }
   };
-  if (wave > 0) load_tile(0, 0);
+  auto store = [&](const float (&src)[kMeanPer], int buf) {
+#pragma unroll
+    for (int j = 0; j < kMeanPer; ++j) {
+      const int r = wave - 1 + kMeanLoaders * j;
+      if (r < kMeanRows) ring[buf][lane][r] = src[j];
+    }
+  };
+  if (wave > 0) {
+    issue(v[0], 0);
+    issue(v[1], 1);
+    issue(v[2], 2);
+    issue(v[3], 3);
+    store(v[0], 0);
+    issue(v[0], 4);
+  }
   __syncthreads();
   float m = 0.0f;
-  for (int64_t t = 0; t < ntiles; ++t) {
-    if (wave > 0) {
-      if (t + 1 < ntiles) load_tile(static_cast<int>((t + 1) & 1), t + 1);
-    } else {
-      const int64_t left = n - t * kMeanRows;
-      const int rows = left < kMeanRows ? static_cast<int>(left) : kMeanRows;
-      const float* col = &ring[t & 1][0][lane];
-      for (int r = 0; r < rows; ++r) m = m + col[r * 64];
+  for (int64_t t0 = 0; t0 < ntiles; t0 += 4) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int64_t t = t0 + s;
+      if (t < ntiles) {
+        if (wave > 0) {
+          // tile t+1 from its slot into the free buffer, then tile t+5 into that slot
+          if (t + 1 < ntiles) {
+            store(v[(s + 1) & 3], static_cast<int>((t + 1) & 1));
+            issue(v[(s + 1) & 3], t + 5);
+          }
+        } else {
+          const int64_t left = n - t * kMeanRows;
+          const int rows = left < kMeanRows ? static_cast<int>(left) : kMeanRows;
+          const float* col = &ring[t & 1][lane][0];
+          int r = 0;
+          for (; r + 4 <= rows; r += 4) {
+            const float4 q = *reinterpret_cast<const float4*>(col + r);
+            m = m + q.x;
+            m = m + q.y;
+            m = m + q.z;
+            m = m + q.w;
+          }
+          for (; r < rows; ++r) m = m + col[r];
+        }
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
   if (wave == 0 && dd < d) mean[dd] = m / static_cast<float>(n);
 }

@@ -48,7 +48,8 @@ __device__ __forceinline__ bool hash_insert(int32_t* hs, uint32_t size, int32_t 
   }
 }
 
-// SKM: 0 exact pass only, 1 FP16 sketch screen, 2 int8 sketch screen.
+// SKM: 0 exact pass only, 1 FP16 sketch screen, 2 int8 sketch screen,
+// 3 principal-subspace screen (pca.hip).
 template <int METRIC, bool FLIP, int SKM>
 __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict__ x, int d, int ld,
                                                         const _Float16* __restrict__ xh, int ldh,
@@ -56,6 +57,8 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
                                                         const int8_t* __restrict__ q8, int ld8,
                                                         const float* __restrict__ s8, const int32_t* __restrict__ a8,
                                                         const float* __restrict__ e8,
+                                                        const float* __restrict__ pca, int pca_r,
+                                                        const float* __restrict__ pca_e, double pca_sigma,
                                                         const float* __restrict__ norms,
                                                         const int32_t* __restrict__ order, int64_t n, int k,
                                                         int hs_size, int cand_cap, int xi_stride,
@@ -74,7 +77,7 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
   const size_t per_wave_words = static_cast<size_t>(xi_stride) + hs_size + cand_cap + 4;
   float* xrow = reinterpret_cast<float*>(smem) + per_wave_words * wave;  // 16-byte aligned
   constexpr bool SK = SKM > 0;
-  float* xf = xrow + (SKM == 1 ? ldh : (SKM == 2 ? ld8 / 4 : 0));
+  float* xf = xrow + (SKM == 1 ? ldh : (SKM == 2 ? ld8 / 4 : (SKM == 3 ? pca_r : 0)));
   float* stg = xf + ld;
   int32_t* hs = reinterpret_cast<int32_t*>(xrow + xi_stride);
   int32_t* cand = hs + hs_size;
@@ -115,6 +118,9 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
   } else if constexpr (SKM == 2) {
     for (int dd = lane; dd < ld8 / 4; dd += 64)
       reinterpret_cast<int32_t*>(xrow)[dd] = reinterpret_cast<const int32_t*>(q8 + i * ld8)[dd];
+    for (int dd = lane; dd < ld; dd += 64) xf[dd] = x[i * ld + dd];
+  } else if constexpr (SKM == 3) {
+    for (int dd = lane; dd < pca_r; dd += 64) xrow[dd] = pca[i * pca_r + dd];
     for (int dd = lane; dd < ld; dd += 64) xf[dd] = x[i * ld + dd];
   } else {
     for (int dd = lane; dd < d; dd += 64) xrow[dd] = x[i * ld + dd];
@@ -158,7 +164,7 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
     if (m > 0) {
       // screen: survivors are written back to cand[] in order (slot
       // w + rank <= base + lane was already read by this batch)
-      const float ei = SKM == 2 ? e8[i] : eps[i];
+      const float ei = SKM == 2 ? e8[i] : (SKM == 3 ? pca_e[i] : eps[i]);
       const double si = SKM == 2 ? static_cast<double>(s8[i]) : 0.0;
       const double ti = SKM == 2 ? si * si * static_cast<double>(a8[i]) : 0.0;
       int w = 0;
@@ -205,6 +211,28 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
             const double r = fmax(0.0, sqrt(fmax(0.0, slo)) - static_cast<double>(ei) - static_cast<double>(e8[c]));
             const double k1 = 1.0 - sb.kap;
             lb = sqrt(fmax(0.0, r * r * k1 - sb.tiny)) * k1;
+          } else if constexpr (SKM == 3) {
+            // ||x - y|| >= (||p(x) - p(y)|| - e(x) - e(y)) / sigma, with the
+            // r-term chain's rounding taken off; then the exact chain's error
+            const float4* pc = reinterpret_cast<const float4*>(pca + static_cast<int64_t>(c) * pca_r);
+            const float4* pq = reinterpret_cast<const float4*>(xs);
+            float p = 0.0f;
+            for (int g = 0; g < pca_r / 4; ++g) {
+              const float4 u = pc[g], v = pq[g];
+              float t = v.x - u.x;
+              p = __builtin_fmaf(t, t, p);
+              t = v.y - u.y;
+              p = __builtin_fmaf(t, t, p);
+              t = v.z - u.z;
+              p = __builtin_fmaf(t, t, p);
+              t = v.w - u.w;
+              p = __builtin_fmaf(t, t, p);
+            }
+            const double k1 = 1.0 - sb.kap;
+            const double sp = fmax(0.0, static_cast<double>(p) * k1 - sb.tiny);
+            const double r =
+                fmax(0.0, sqrt(sp) - static_cast<double>(ei) - static_cast<double>(pca_e[c])) / pca_sigma;
+            lb = sqrt(fmax(0.0, r * r * k1 - sb.tiny)) * k1;
           } else {
             const float p = sketch_chain<METRIC>(xs, xh + static_cast<int64_t>(c) * ldh, ldh);
             lb = sketch_lower_bound<METRIC>(p, ei, eps[c], ni, METRIC == kCosine ? norms[c] : 1.0f, sb);
@@ -224,15 +252,18 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
 
   bool changed = false;
   if (nnew > 0) {
-    // exact pass: in the plain form one candidate per lane streams its row;
-    // with the sketch the few survivors' rows are first staged in LDS two at
-    // a time by the whole wave (one memory round trip), then chained from LDS
-    const int step = SK ? 2 : 64;
+    // exact pass: in the plain form, and after the principal-subspace screen
+    // (a handful of survivors: one round trip for all of them), one candidate
+    // per lane streams its row; after the sketch screens the survivors' rows
+    // are first staged in LDS two at a time by the whole wave, then chained
+    // from LDS
+    constexpr bool STAGE = SK && SKM != 3;
+    const int step = STAGE ? 2 : 64;
     for (int base = 0; base < m; base += step) {
       const int ci = base + lane;
       int32_t c = -1;
       float dk = __builtin_inff();
-      if constexpr (SK) {
+      if constexpr (STAGE) {
         const int nrow = m - base < 2 ? m - base : 2;
         const int q4 = ld / 4;
         __builtin_amdgcn_wave_barrier();
@@ -288,15 +319,20 @@ void launch_w(const PropagateArgs& a, hipStream_t s) {
   // at most 1 + k + k*k ids enter the set: at least twice that many slots
   int hs_size = 128;
   while (hs_size < 2 * (a.k * a.k + a.k + 1)) hs_size *= 2;
-  const int cand_cap = a.k * a.k;
-  const int xi_stride = SKM == 1 ? a.ldh + 3 * a.ld : (SKM == 2 ? a.ld8 / 4 + 3 * a.ld : (a.d + 3) / 4 * 4);
+  // (a multiple of 4 keeps every wave's LDS block 16-byte aligned)
+  const int cand_cap = (a.k * a.k + 3) / 4 * 4;
+  const int xi_stride = SKM == 1   ? a.ldh + 3 * a.ld
+                        : SKM == 2 ? a.ld8 / 4 + 3 * a.ld
+                        : SKM == 3 ? a.pca_r + a.ld
+                                   : (a.d + 3) / 4 * 4;
   const size_t per_wave = (static_cast<size_t>(xi_stride) + hs_size + cand_cap + 4) * 4;
   const size_t shmem = per_wave * kWavesPerBlock;
   if (shmem > 160 * 1024) throw HipError("propagation: k or d too large for LDS");
   const int64_t blocks = xcd_grid((a.n + kWavesPerBlock - 1) / kWavesPerBlock);
   const SketchBound sb = sketch_bound(a.d);
   hipLaunchKernelGGL((propagate_kernel<METRIC, FLIP, SKM>), dim3(static_cast<unsigned>(blocks)), dim3(256), shmem,
-                     s, a.x, a.d, a.ld, a.xh, a.ldh, a.eps, sb, a.q8, a.ld8, a.s8, a.a8, a.e8, a.norms, a.order, a.n, a.k, hs_size, cand_cap,
+                     s, a.x, a.d, a.ld, a.xh, a.ldh, a.eps, sb, a.q8, a.ld8, a.s8, a.a8, a.e8, a.pca, a.pca_r, a.pca_e,
+                     a.pca_sigma, a.norms, a.order, a.n, a.k, hs_size, cand_cap,
                      xi_stride, a.ids_in, a.vals_in, a.ids_out, a.vals_out, a.ncand,
                      FLIP ? nullptr : a.hist.ids_prev, a.hist.changed_prev, a.hist.changed_out);
   AKNN_HIP_TRY(hipGetLastError());
@@ -315,6 +351,10 @@ void launch_t(const PropagateArgs& a, hipStream_t s) {
     return;
   }
   if constexpr (METRIC == kEuclidean) {
+    if (a.pca != nullptr) {
+      launch_w<METRIC, FLIP, 3>(a, s);
+      return;
+    }
     if (a.q8 != nullptr) {
       launch_w<METRIC, FLIP, 2>(a, s);
       return;
@@ -335,6 +375,8 @@ void launch_propagate(const PropagateArgs& a, hipStream_t s) {
     throw HipError("propagation: malformed int8 sketch");
   if (a.xh != nullptr && (a.eps == nullptr || a.ldh % 8 != 0 || a.ldh < a.d))
     throw HipError("propagation: malformed sketch");
+  if (a.pca != nullptr && (a.pca_e == nullptr || (a.pca_r != 16 && a.pca_r != 32) || !(a.pca_sigma > 0.0)))
+    throw HipError("propagation: malformed principal-subspace screen");
   switch (a.metric) {
     case kEuclidean:
       launch_t<kEuclidean, false>(a, s);

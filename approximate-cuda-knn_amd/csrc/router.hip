@@ -255,17 +255,34 @@ __device__ __forceinline__ void fused_layer(const uint16_t* __restrict__ src, co
 #pragma unroll
     for (int rt = 0; rt < 2; ++rt) acc[c][rt] = f32x16{};
   const int nt = n / 32;
-  for (int k0 = 0; k0 < k; k0 += 16) {
-    const uint4 a0 = *reinterpret_cast<const uint4*>(src + li * kFuseLd + k0 + kh);
-    const uint4 a1 = *reinterpret_cast<const uint4*>(src + (32 + li) * kFuseLd + k0 + kh);
+  // K in groups of four steps: every fragment of a group is loaded (the B
+  // fragments from L2) before its MFMAs, so four steps of L2 latency overlap;
+  // within a tile the K steps keep their order
+  constexpr int kGroup = 4;
+  for (int kb = 0; kb < k; kb += 16 * kGroup) {
+    uint4 a0[kGroup], a1[kGroup], bv[kGroup][CPW];
 #pragma unroll
-    for (int c = 0; c < CPW; ++c) {
-      const int ct = wave * CPW + c;
-      if (ct < nt) {
-        const uint4 bv = *reinterpret_cast<const uint4*>(w + static_cast<int64_t>(ct * 32 + li) * k + k0 + kh);
-        const bf16x8 b = __builtin_bit_cast(bf16x8, bv);
-        acc[c][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a0), b, acc[c][0], 0, 0, 0);
-        acc[c][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a1), b, acc[c][1], 0, 0, 0);
+    for (int g = 0; g < kGroup; ++g) {
+      const int k0 = kb + 16 * g;
+      const bool in = k0 < k;
+      a0[g] = in ? *reinterpret_cast<const uint4*>(src + li * kFuseLd + k0 + kh) : make_uint4(0u, 0u, 0u, 0u);
+      a1[g] = in ? *reinterpret_cast<const uint4*>(src + (32 + li) * kFuseLd + k0 + kh) : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+      for (int c = 0; c < CPW; ++c) {
+        const int ct = wave * CPW + c;
+        bv[g][c] = (in && ct < nt) ? *reinterpret_cast<const uint4*>(w + static_cast<int64_t>(ct * 32 + li) * k + k0 + kh)
+                                   : make_uint4(0u, 0u, 0u, 0u);
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < kGroup; ++g) {
+      if (kb + 16 * g >= k) break;
+#pragma unroll
+      for (int c = 0; c < CPW; ++c) {
+        if (wave * CPW + c >= nt) continue;
+        const bf16x8 b = __builtin_bit_cast(bf16x8, bv[g][c]);
+        acc[c][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a0[g]), b, acc[c][0], 0, 0, 0);
+        acc[c][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a1[g]), b, acc[c][1], 0, 0, 0);
       }
     }
   }

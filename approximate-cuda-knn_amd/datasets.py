@@ -21,7 +21,7 @@ CONFIGS = {
     "C2": dict(n=1_000_000, d=128, k=20, trees=16, leaf=256, props=5, metric="EUCLIDEAN"),
     "C3": dict(n=1_000_000, d=784, k=20, trees=12, leaf=512, props=4, metric="COSINE"),
     "C4": dict(n=2_000_000, d=96, k=30, trees=24, leaf=384, props=6, metric="COSINE"),
-    "C5": dict(n=4_000_000, d=128, k=20, trees=12, leaf=256, props=4, metric="EUCLIDEAN"),
+    "C5": dict(n=4_000_000, d=128, k=20, trees=6, leaf=256, props=2, metric="EUCLIDEAN"),
 }
 
 

@@ -129,6 +129,13 @@ int aknn_deinitialize(aknn_handle* h);
  *                   over all dimensions (the reference: the first 256 only,
  *                   .cu:57-65, :76-84)
  *   "fixed"         sets cosine_fixed, taxicab_l1 and scale_all together
+ *   "pca_screen"    1: EUCLIDEAN propagation screens each candidate with the
+ *                   distance of its projection on the data's top 16 or 32
+ *                   principal directions (recomputed per build from a sample;
+ *                   used when they hold 97% of the second moment, D in
+ *                   [16, 256]), a proven lower bound of the exact key; the
+ *                   exact chain runs where the bound does not exclude a
+ *                   candidate (the results are the same); 0 (default): off
  *   "io_threads"    host threads parsing the CSV in aknn_initialize (0, the
  *                   default: up to 16; 1: the sequential loader); the values
  *                   are the same for any count
@@ -177,7 +184,9 @@ int aknn_get_kernel_stats(aknn_handle* h, const char* name, int64_t* launches, d
  * on the matrix cores (trees after the first, with "leaf_mfma"), and those
  * whose exact distance was computed.  stage "forest": side tests of active
  * points over all levels, and those the sketch left to the exact chain (0
- * without "sketch").  No reference counterpart: the work accounting of the
+ * without "sketch").  stage "pca": the principal-subspace screen's width
+ * (16, 32, or 0 when unused) and the share of the sample's second moment its
+ * top 16 directions hold.  No reference counterpart: the work accounting of the
  * kernel statistics above. */
 int aknn_get_candidate_counts(aknn_handle* h, const char* stage, double* examined, double* rescored);
 

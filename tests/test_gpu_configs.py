@@ -11,7 +11,7 @@ are checked through properties that do not depend on size:
   * the normalized points equal the oracle's normalization (C3: the D > 256
     quirk at full size);
   * recall@k against the FP64 exact k-NN (C5: the configuration's recall
-    target of 0.95 is met by the plain RP forest at T=12, L=256, P=4).
+    target of 0.95 is met by the plain RP forest at T=6, L=256, P=2).
 """
 import numpy as np
 import pytest
@@ -109,3 +109,38 @@ def test_c4_cosine_full_size(aknn, datasets):
     r = g.recall(np.random.default_rng(4).choice(len(x), 1000, replace=False))
     print(f"[gpu] C4 recall@30 {r:.4f}", flush=True)
     assert r > 0.0
+
+
+def test_c5_learned_form_full_size(aknn, oracle, datasets):
+    """The learned form at C5's full size: six routers trained on a separate
+    200,000-point sample's graph, one per median-split tree, FP16 sketch and
+    matrix-core screens; rows valid, sampled values the oracle's FP32
+    distances, recall@20 at the configuration's 0.95 target."""
+    cfg = datasets.CONFIGS["C5"]
+    xt = datasets.generate("C5", n=200000, sample_seed=105)
+    gt = aknn.Graph()
+    gt.initialize(xt, cfg["k"], 8, 256, 3, 0, split=1)
+    gt.calculate()
+    routers = []
+    for seed in range(1, 7):
+        r = aknn.Router(xt.shape[1], 256, 8, seed=seed)
+        lab, _ = r.labels(gt, sweeps=20)
+        r.fit(gt, lab, epochs=10, batch=1024, lr=2e-3)
+        routers.append(r)
+    gt.close()
+    x = datasets.generate("C5")
+    g = aknn.Graph()
+    g.initialize(x, cfg["k"], 6, 256, 2, 0, split=1, router_trees=6, sketch=1, leaf_mfma=1)
+    for r in routers:
+        g.add_router(r)
+    print("[gpu] C5 learned form", flush=True)
+    g.calculate()
+    ids, vals = g.graph()
+    check_rows(ids, vals, cfg["k"])
+    xn = g.normalized()
+    check_values(oracle, xn, ids, vals, 0, np.random.default_rng(8).choice(len(x), 200, replace=False))
+    rec = g.recall(np.random.default_rng(6).choice(len(x), 2000, replace=False))
+    print(f"[gpu] C5 learned recall@20 {rec:.4f} stages {g.stage_times()}", flush=True)
+    assert rec >= 0.95
+    g.set_router(None)
+    g.close()
