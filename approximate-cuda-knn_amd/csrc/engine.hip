@@ -230,7 +230,7 @@ void Engine::set_data(const float* x_rowmajor, int64_t n, int d, const Params& p
   fb_.blk_r = alloc<int32_t>(blocks);
   fb_.blk_roff = alloc<int32_t>(blocks);
   fb_.ctl = alloc<ForestCtl>(1);
-  ncand_ = alloc<unsigned long long>(2 * static_cast<size_t>(std::max(p.props, 1)));
+  ncand_ = alloc<unsigned long long>(static_cast<size_t>(kCountWords) * std::max(p.props, 1));
   changed_[0] = alloc<uint8_t>(n);
   changed_[1] = alloc<uint8_t>(n);
   AKNN_HIP_TRY(hipStreamSynchronize(stream_));
@@ -241,7 +241,7 @@ void Engine::build_forest() {
   iters_.assign(p_.trees, 0);
   active_sum_ = 0;
   stats_forest = KernelStats{};
-  if (fb_.amb_count) AKNN_HIP_TRY(hipMemsetAsync(fb_.amb_count, 0, sizeof(unsigned long long), stream_));
+  if (fb_.amb_count) AKNN_HIP_TRY(hipMemsetAsync(fb_.amb_count, 0, sizeof(unsigned long long) * kSlotWords, stream_));
   const int n32 = static_cast<int>(n_);
   for (int t = 0; t < p_.trees; ++t) {
     const size_t base = draws_.pos();
@@ -280,7 +280,7 @@ void Engine::build_forest() {
   stats_forest.units = static_cast<double>(active_sum_) * (4.0 * d_ + 16.0) + 8.0 * n_ * p_.trees;
   if (fb_.amb_count) {
     unsigned long long amb = 0;
-    AKNN_HIP_TRY(hipMemcpy(&amb, fb_.amb_count, sizeof(amb), hipMemcpyDeviceToHost));
+    slot_sums(fb_.amb_count, 1, &amb);
     stats_forest.examined = static_cast<double>(active_sum_);
     stats_forest.rescored = static_cast<double>(amb);
     stats_forest.units = static_cast<double>(active_sum_) * (2.0 * ldh_ + 8.0 + 16.0) + 4.0 * d_ * amb +
@@ -444,8 +444,8 @@ void Engine::initial_knn() {
   }
   bool mfma = leaf_mfma_active();
   if (mfma) {
-    AKNN_HIP_TRY(hipMemsetAsync(leaf_rescored_, 0, sizeof(unsigned long long), stream_));
-    AKNN_HIP_TRY(hipMemsetAsync(leaf_rescored_ + 2, 0, sizeof(unsigned long long), stream_));
+    AKNN_HIP_TRY(hipMemsetAsync(leaf_rescored_, 0, sizeof(unsigned long long) * kSlotWords, stream_));
+    AKNN_HIP_TRY(hipMemsetAsync(leaf_rescored_ + 2 * kSlotWords, 0, sizeof(unsigned long long) * kSlotWords, stream_));
   }
   unsigned long long prev[2] = {0, 0};
   for (int t = 0; t < p_.trees; ++t) {
@@ -454,8 +454,7 @@ void Engine::initial_knn() {
     // plain kernel (same graph).
     if (mfma && t > 1) {
       unsigned long long c[3] = {0, 0, 0};
-      AKNN_HIP_TRY(hipMemcpyAsync(c, leaf_rescored_, sizeof(c), hipMemcpyDeviceToHost, stream_));
-      AKNN_HIP_TRY(hipStreamSynchronize(stream_));
+      slot_sums(leaf_rescored_, 3, c);
       const unsigned long long dres = c[0] - prev[0], dscr = c[2] - prev[1];
       prev[0] = c[0];
       prev[1] = c[2];
@@ -468,7 +467,8 @@ void Engine::initial_knn() {
     // the first tree meets stub rows (nothing to screen against): plain kernel
     if (mfma && t > 0)
       launch_leaf_mfma(kmetric(), x_, d_, ld_, xh_, ldh_, eps_, nh_, norms_, perm_ + t * n_, items_, nit, kItemOwners,
-                       p_.k, ids, vals, rs_ + t * n_, rc_ + t * n_, leaf_rescored_, leaf_rescored_ + 2, stream_);
+                       p_.k, ids, vals, rs_ + t * n_, rc_ + t * n_, leaf_rescored_, leaf_rescored_ + 2 * kSlotWords,
+                       stream_);
     else
       launch_leaf_knn(kmetric(), x_, d_, ld_, norms_, perm_ + t * n_, items_, nit, kItemOwners, p_.k, ids, vals,
                       rs_ + t * n_, rc_ + t * n_, stream_);
@@ -521,7 +521,7 @@ static PropagateArgs round_args(const Params& p, int kmetric, bool flip, bool sc
   a.vals_in = gvals[cur];
   a.ids_out = gids[cur ^ 1];
   a.vals_out = gvals[cur ^ 1];
-  a.ncand = ncand + 2 * r;
+  a.ncand = ncand + static_cast<size_t>(kCountWords) * r;
   if (p.skip_settled) {
     a.hist.ids_prev = r > 0 ? gids[cur ^ 1] : nullptr;
     a.hist.changed_prev = changed[(r + 1) & 1];
@@ -530,11 +530,36 @@ static PropagateArgs round_args(const Params& p, int kmetric, bool flip, bool sc
   return a;
 }
 
+// Totals of `kinds` consecutive spread counters (kSlotWords u64 each) on the
+// device (synchronises the stream).
+void Engine::slot_sums(const unsigned long long* dev, int kinds, unsigned long long* out) {
+  std::vector<unsigned long long> w(static_cast<size_t>(kSlotWords) * kinds);
+  AKNN_HIP_TRY(hipMemcpyAsync(w.data(), dev, sizeof(unsigned long long) * w.size(), hipMemcpyDeviceToHost, stream_));
+  AKNN_HIP_TRY(hipStreamSynchronize(stream_));
+  for (int k = 0; k < kinds; ++k) {
+    out[k] = 0;
+    for (int s = 0; s < kCountSlots; ++s) out[k] += w[static_cast<size_t>(k) * kSlotWords + s * kCountStride];
+  }
+}
+
+// Sums of round r's counter slots: out[0] examined, out[1] re-scored (synchronises).
+void Engine::round_counts(int r, unsigned long long out[2]) {
+  std::vector<unsigned long long> w(kCountWords);
+  AKNN_HIP_TRY(hipMemcpyAsync(w.data(), ncand_ + static_cast<size_t>(kCountWords) * r, sizeof(unsigned long long) * w.size(),
+                              hipMemcpyDeviceToHost, stream_));
+  AKNN_HIP_TRY(hipStreamSynchronize(stream_));
+  out[0] = out[1] = 0;
+  for (int s = 0; s < kCountSlots; ++s) {
+    out[0] += w[static_cast<size_t>(s) * kCountStride];
+    out[1] += w[static_cast<size_t>(kCountSlots + s) * kCountStride];
+  }
+}
+
 void Engine::propagate() {
   stats_propagate = KernelStats{};
   const bool flip = p_.metric == kCosine && !p_.cosine_fixed;
   const int base = 2 * p_.trees;
-  AKNN_HIP_TRY(hipMemsetAsync(ncand_, 0, sizeof(unsigned long long) * 2 * std::max(p_.props, 1), stream_));
+  AKNN_HIP_TRY(hipMemsetAsync(ncand_, 0, sizeof(unsigned long long) * kCountWords * std::max(p_.props, 1), stream_));
   round_screened_.assign(std::max(p_.props, 0), 0);
   // Screened or exact rounds (the graph is the same either way).  The first
   // calculate() on a data set runs round 0 screened and round 1 exact and
@@ -552,12 +577,11 @@ void Engine::propagate() {
   for (int r = 0; r < p_.props; ++r) {
     if (can_screen && r > 0) {
       unsigned long long c[2] = {0, 0};
-      AKNN_HIP_TRY(hipMemcpyAsync(c, ncand_ + 2 * (r - 1), sizeof(c), hipMemcpyDeviceToHost, stream_));
-      AKNN_HIP_TRY(hipStreamSynchronize(stream_));
+      round_counts(r - 1, c);
       if (probe && r == 2) {
         float ms[2] = {0, 0};
         unsigned long long c0[2] = {0, 0};
-        AKNN_HIP_TRY(hipMemcpy(c0, ncand_, sizeof(c0), hipMemcpyDeviceToHost));
+        round_counts(0, c0);
         AKNN_HIP_TRY(hipEventElapsedTime(&ms[0], launch_ev(base), launch_ev(base + 1)));
         AKNN_HIP_TRY(hipEventElapsedTime(&ms[1], launch_ev(base + 2), launch_ev(base + 3)));
         const double per_s = ms[0] / std::max<double>(1.0, static_cast<double>(c0[0]));
@@ -582,8 +606,8 @@ void Engine::propagate() {
     // two rounds only: decide from the screened round 0 and the exact round 1
     unsigned long long c[4] = {0, 0, 0, 0};
     float ms[2] = {0, 0};
-    AKNN_HIP_TRY(hipMemcpyAsync(c, ncand_, sizeof(c), hipMemcpyDeviceToHost, stream_));
-    AKNN_HIP_TRY(hipStreamSynchronize(stream_));
+    round_counts(0, c);
+    round_counts(1, c + 2);
     AKNN_HIP_TRY(hipEventElapsedTime(&ms[0], launch_ev(base), launch_ev(base + 1)));
     AKNN_HIP_TRY(hipEventElapsedTime(&ms[1], launch_ev(base + 2), launch_ev(base + 3)));
     const double per_s = ms[0] / std::max<double>(1.0, static_cast<double>(c[0]));
@@ -649,7 +673,8 @@ void Engine::calculate() {
     xh_ = alloc<_Float16>(static_cast<size_t>(n_) * ldh_);
     eps_ = alloc<float>(n_);
     nh_ = alloc<float>(n_);
-    leaf_rescored_ = alloc<unsigned long long>(3);  // leaf re-scored, forest undecided, leaf screened
+    // leaf re-scored, forest undecided, leaf screened: kSlotWords each
+    leaf_rescored_ = alloc<unsigned long long>(3 * static_cast<size_t>(kSlotWords));
   }
   const bool sketch8 = propagation_screened() && p_.sketch_int8 && p_.metric == kEuclidean && p_.props > 0;
   if (sketch8 && !q8_) {
@@ -662,7 +687,7 @@ void Engine::calculate() {
   fb_.ldh = ldh_;
   fb_.eps = p_.sketch ? eps_ : nullptr;
   fb_.nh = p_.sketch ? nh_ : nullptr;
-  fb_.amb_count = p_.sketch ? leaf_rescored_ + 1 : nullptr;
+  fb_.amb_count = p_.sketch ? leaf_rescored_ + kSlotWords : nullptr;
   AKNN_HIP_TRY(hipMemcpyAsync(x_, x_raw_, sizeof(float) * n_ * ld_, hipMemcpyDeviceToDevice, stream_));
   AKNN_HIP_TRY(hipEventRecord(ev(0), stream_));
   launch_normalize(x_, n_, d_, ld_, mean_, gmax_, p_.scale_all != 0, stream_);
@@ -674,7 +699,7 @@ void Engine::calculate() {
     // this build's principal subspace (a stream sync and a host eigen-solve
     // inside the bracket), then every point's projection and radius
     if (!pca_mom_) {
-      pca_mom_ = alloc<float>(static_cast<size_t>(d_) * d_);
+      pca_mom_ = alloc<float>(static_cast<size_t>(33) * d_ * d_);
       pca_p_ = alloc<float>(static_cast<size_t>(32) * d_);
       pca_proj_ = alloc<float>(static_cast<size_t>(n_) * 32);
       pca_e_ = alloc<float>(n_);
@@ -715,7 +740,7 @@ void Engine::calculate() {
     }
   }
   std::vector<unsigned long long> nc(2 * static_cast<size_t>(std::max(p_.props, 1)));
-  AKNN_HIP_TRY(hipMemcpy(nc.data(), ncand_, sizeof(unsigned long long) * nc.size(), hipMemcpyDeviceToHost));
+  for (int r = 0; r < p_.props; ++r) round_counts(r, &nc[2 * r]);
   stats_propagate.examined = stats_propagate.rescored = 0;
   if (rounds_from_graph_ && p_.props > 0) {
     float ms = 0;
@@ -765,7 +790,7 @@ void Engine::calculate() {
     stats_leaf_knn.screened = screened;
     if (mfma) {
       unsigned long long r = 0;
-      AKNN_HIP_TRY(hipMemcpy(&r, leaf_rescored_, sizeof(r), hipMemcpyDeviceToHost));
+      slot_sums(leaf_rescored_, 1, &r);
       stats_leaf_knn.rescored = static_cast<double>(r);
     }
   }

@@ -144,6 +144,8 @@ class Engine {
   bool propagate_graph(int base);
   void initial_knn();
   void propagate();
+  void round_counts(int r, unsigned long long out[2]);
+  void slot_sums(const unsigned long long* dev, int kinds, unsigned long long* out);
 
   int device_;
   hipStream_t stream_ = nullptr;

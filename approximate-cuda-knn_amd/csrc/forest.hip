@@ -263,7 +263,8 @@ __global__ __launch_bounds__(256) void forest_side_sketch_kernel(
         acc += wcnt[w];
       }
       blk_r[blk] = acc;
-      if (amb_count != nullptr && na > 0) atomicAdd(amb_count, static_cast<unsigned long long>(na));
+      if (amb_count != nullptr && na > 0)
+        atomicAdd(amb_count + (blk % kCountSlots) * kCountStride, static_cast<unsigned long long>(na));
     }
     __syncthreads();
   }

@@ -33,6 +33,7 @@ struct ForestBuffers {
   int64_t max_buckets;
   // FP16 sketch for the side test (nullable: exact chains for every point;
   // the partition is the same), and a counter of points it left undecided
+  // (kSlotWords u64, spread over kCountSlots slots)
   const _Float16* xh = nullptr;
   int ldh = 0;
   const float* eps = nullptr;
@@ -78,13 +79,23 @@ struct PropagateHistory {
   uint8_t* changed_out = nullptr;
 };
 
+// Candidate counters of one propagation round: two kinds (examined,
+// re-scored), each spread over kCountSlots slots one 128-byte line apart so
+// that millions of per-point additions do not queue on one address; the total
+// is the sum of the slots.
+constexpr int kCountSlots = 64;
+constexpr int kCountStride = 16;                                  // u64 per slot (128 bytes)
+constexpr int kSlotWords = kCountSlots * kCountStride;           // one spread counter
+constexpr int kCountWords = 2 * kSlotWords;                       // per round
+
 // One Jacobi propagation round, ids_in/vals_in -> ids_out/vals_out.
 // flip: the COSINE parity store rule (stored = 1 - key; no skipping then,
 // since any candidate triggers the flip).  order: processing order of the
 // points (nullable: 0..n-1); results do not depend on it.  xh/eps: FP16
 // sketch and radii (nullable: exact distances for every candidate; results do
-// not depend on it).  ncand: optional device counters [examined new
-// candidates, exactly re-scored candidates] (byte accounting).
+// not depend on it).  ncand: optional device counters of the round
+// (kCountWords u64: examined new candidates, exactly re-scored candidates;
+// byte accounting).
 struct PropagateArgs {
   int metric = kEuclidean;
   bool flip = false;
@@ -127,7 +138,7 @@ struct PcaBasis {
   double sigma = 1.0;
   double retained16 = 0.0;  // share of the sample's second moment in the top 16 directions
 };
-// moments_dev: device scratch of d*d floats.  Synchronises the stream.
+// moments_dev: device scratch of 33*d*d floats.  Synchronises the stream.
 PcaBasis compute_pca_basis(const float* x, int64_t n, int d, int ld, float* moments_dev, hipStream_t s);
 // proj [n][r] = fl(p x_i), rad[n] >= ||proj_i - p x_i||.
 void launch_pca_project(const float* x, int64_t n, int d, int ld, const float* p, int r, float* proj, float* rad,

@@ -351,11 +351,13 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
     }
     if (has_owner && cnt > 0) o_t[lane] = owner_threshold<METRIC>(thk, my_eps, a.kap);
   }
-  if (a.rescored != nullptr && lane == 0 && nres > 0) atomicAdd(a.rescored, nres);
+  // counters spread over slots one 128-byte line apart (summed by the host)
+  const int64_t cslot = (blockIdx.x % kCountSlots) * kCountStride;
+  if (a.rescored != nullptr && lane == 0 && nres > 0) atomicAdd(a.rescored + cslot, nres);
   if (a.screened != nullptr && lane == 0) {
     unsigned long long pairs = 0;  // the owners' own ranges
     for (int o = 0; o < nown; ++o) pairs += static_cast<unsigned long long>(o_hi[o] - o_lo[o]);
-    atomicAdd(a.screened, pairs);
+    atomicAdd(a.screened + cslot, pairs);
   }
   if (has_owner) {
 #pragma unroll

@@ -14,81 +14,130 @@ namespace aknn {
 
 namespace {
 
-constexpr int kMeanRows = 256;  // rows per LDS tile (64 dimensions wide)
-constexpr int kMeanPad = 4;     // floats of padding per LDS column (bank spread)
-constexpr int kMeanLoaders = 15;
-constexpr int kMeanPer = (kMeanRows + kMeanLoaders - 1) / kMeanLoaders;  // rows per loader lane per tile
+constexpr int kMeanLoaders = 15;                  // waves 1..15 load, wave 0 adds
+constexpr int kMeanDims = 16;                     // dimensions per workgroup (one chain per lane)
+constexpr int kMeanRowsPerLoad = 64 / kMeanDims;  // rows one 64-lane load covers (4)
+constexpr int kMeanLoadsPerTile = 8;              // loads each loader wave issues per tile
+constexpr int kMeanRows = kMeanLoaders * kMeanLoadsPerTile * kMeanRowsPerLoad;  // rows per tile (480)
+constexpr int kMeanSlots = 4;                     // LDS ring of tiles
+constexpr int kMeanAhead = 3;                     // tiles in flight ahead of the adder
+constexpr int kMeanChunk = 24;                    // rows the adder holds in registers per buffer
+static_assert(kMeanRows % (2 * kMeanChunk) == 0, "the adder's double buffer walks whole tiles");
 
-// One workgroup per 64 dimensions.  Wave 0 holds the 64 sequential sums (one
-// lane per dimension) and walks the rows of a tile in order; waves 1..15 load
-// the rows.  The add chain is the reference's sequential order and cannot be
-// split, so the aim is to keep it fed: each loader wave keeps four tiles of
-// loads in flight in registers (slot u % 4 holds tile u) and writes the next
-// tile into the free half of a two-tile LDS ring while wave 0 adds the other.
-// The ring is column-major (one padded column of rows per dimension), so the
-// adding lane reads four consecutive rows with one 16-byte LDS load.
+// Waits until at most N vector-memory operations of this wave are pending
+// (gfx9 encoding: vmcnt in bits 3:0 and 15:14; expcnt and lgkmcnt left open).
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt is 6 bits");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+// Waits until this wave's LDS operations are done (vmcnt and expcnt left open).
+__device__ __forceinline__ void wait_lgkmcnt0() { __builtin_amdgcn_s_waitcnt(15 | (3 << 14) | (7 << 4)); }
+
+// Workgroup barrier without __syncthreads()'s fence, which would also wait for
+// every load in flight; the callers wait for exactly what the other waves
+// read next.  The empty asm statements keep the compiler from moving memory
+// accesses across it.
+__device__ __forceinline__ void ring_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+typedef __attribute__((address_space(3))) void* lds_void_ptr;
+typedef __attribute__((address_space(1))) const void* global_void_ptr;
+
+// One workgroup per 16 dimensions, so a 128-dimensional matrix spreads its
+// chains over 8 CUs.  Lanes 0..15 of wave 0 hold the sequential sums (one
+// lane per dimension) and add the rows of each tile in order: the
+// reference's sequential FP32 sum, which cannot be split, so one dependent
+// add per row is the bound.  Waves 1..15 keep it fed: each 64-lane
+// global_load_lds moves 4 rows x 16 dimensions (lane i: row i / 16,
+// dimension i % 16) straight from global memory into LDS as [row][16], three
+// tiles ahead in a ring of four tile slots, so no registers bound the rows in
+// flight.  Every load address is clamped into the matrix (rows past n and
+// dimensions past d read valid data the adder ignores), so each wave issues
+// exactly 8 loads per tile and waits with a fixed count.  The adder reads a
+// full tile through two 16-row register buffers, loading one while it adds
+// the other.
 __global__ __launch_bounds__(1024) void mean_kernel(const float* __restrict__ x, int64_t n, int d, int ld,
                                                     float* __restrict__ mean) {
-  __shared__ __attribute__((aligned(16))) float ring[2][64][kMeanRows + kMeanPad];
+  __shared__ __attribute__((aligned(16))) float ring[kMeanSlots][kMeanRows][kMeanDims];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int dd = blockIdx.x * 64 + lane;
+  const int d0 = blockIdx.x * kMeanDims;
   const int64_t ntiles = (n + kMeanRows - 1) / kMeanRows;
-  float v[4][kMeanPer];
-  auto issue = [&](float (&dst)[kMeanPer], int64_t t) {
+  // loader lane: row within a load, dimension (clamped)
+  const int lrow = lane / kMeanDims;
+  const int ldim = d0 + lane % kMeanDims < d ? d0 + lane % kMeanDims : d - 1;
+  auto issue = [&](int64_t t) {
+    float* slot = &ring[t % kMeanSlots][0][0];
 #pragma unroll
-    for (int j = 0; j < kMeanPer; ++j) {
-      const int r = wave - 1 + kMeanLoaders * j;
-      const int64_t row = t * kMeanRows + r;
-      dst[j] = (t < ntiles && r < kMeanRows && row < n && dd < d) ? x[row * ld + dd] : 0.0f;
-    }
-  };
-  auto store = [&](const float (&src)[kMeanPer], int buf) {
-#pragma unroll
-    for (int j = 0; j < kMeanPer; ++j) {
-      const int r = wave - 1 + kMeanLoaders * j;
-      if (r < kMeanRows) ring[buf][lane][r] = src[j];
+    for (int j = 0; j < kMeanLoadsPerTile; ++j) {
+      const int r = ((wave - 1) * kMeanLoadsPerTile + j) * kMeanRowsPerLoad;  // first row of this load
+      int64_t row = t * kMeanRows + r + lrow;
+      row = row < n ? row : n - 1;
+      __builtin_amdgcn_global_load_lds((global_void_ptr)(x + row * ld + ldim),
+                                       (lds_void_ptr)(slot + r * kMeanDims), 4, 0, 0);
     }
   };
   if (wave > 0) {
-    issue(v[0], 0);
-    issue(v[1], 1);
-    issue(v[2], 2);
-    issue(v[3], 3);
-    store(v[0], 0);
-    issue(v[0], 4);
-  }
-  __syncthreads();
-  float m = 0.0f;
-  for (int64_t t0 = 0; t0 < ntiles; t0 += 4) {
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const int64_t t = t0 + s;
-      if (t < ntiles) {
-        if (wave > 0) {
-          // tile t+1 from its slot into the free buffer, then tile t+5 into that slot
-          if (t + 1 < ntiles) {
-            store(v[(s + 1) & 3], static_cast<int>((t + 1) & 1));
-            issue(v[(s + 1) & 3], t + 5);
-          }
-        } else {
-          const int64_t left = n - t * kMeanRows;
-          const int rows = left < kMeanRows ? static_cast<int>(left) : kMeanRows;
-          const float* col = &ring[t & 1][lane][0];
-          int r = 0;
-          for (; r + 4 <= rows; r += 4) {
-            const float4 q = *reinterpret_cast<const float4*>(col + r);
-            m = m + q.x;
-            m = m + q.y;
-            m = m + q.z;
-            m = m + q.w;
-          }
-          for (; r < rows; ++r) m = m + col[r];
-        }
-      }
-      __syncthreads();
-    }
+    for (int t = 0; t < kMeanAhead; ++t) issue(t);
+    wait_vmcnt<(kMeanAhead - 1) * kMeanLoadsPerTile>();  // tile 0 has landed
   }
-  if (wave == 0 && dd < d) mean[dd] = m / static_cast<float>(n);
+  ring_barrier();
+  // no scalar load may still be pending in the loop: LDS waits could not then
+  // count down in order and would all become waits for zero
+  wait_lgkmcnt0();
+  float m = 0.0f;
+  const int dl = lane % kMeanDims;
+  for (int64_t t = 0; t < ntiles; ++t) {
+    if (wave > 0) {
+      // slot (t + 3) % 4 held tile t - 1, consumed before the last barrier
+      issue(t + kMeanAhead);
+      wait_vmcnt<(kMeanAhead - 1) * kMeanLoadsPerTile>();  // tile t + 1 has landed
+    } else {
+      const float* col = &ring[t % kMeanSlots][0][dl];
+      const int64_t left = n - t * kMeanRows;
+      if (left >= kMeanRows) {
+        float a[kMeanChunk], b[kMeanChunk];
+#pragma unroll
+        for (int i = 0; i < kMeanChunk; ++i) a[i] = col[i * kMeanDims];
+        wait_lgkmcnt0();
+        // Each phase issues the other buffer's reads, adds a buffer that has
+        // already landed, then waits: the reads have had the adds' time to
+        // land, and the next phase's adds need no wait.  sched_barrier keeps
+        // the compiler from regrouping the phases.
+#pragma unroll
+        for (int c = 0; c < kMeanRows; c += 2 * kMeanChunk) {
+#pragma unroll
+          for (int i = 0; i < kMeanChunk; ++i) b[i] = col[(c + kMeanChunk + i) * kMeanDims];
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int i = 0; i < kMeanChunk; ++i) m = m + a[i];
+          wait_lgkmcnt0();
+          __builtin_amdgcn_sched_barrier(0);
+          if (c + 2 * kMeanChunk < kMeanRows) {
+#pragma unroll
+            for (int i = 0; i < kMeanChunk; ++i) a[i] = col[(c + 2 * kMeanChunk + i) * kMeanDims];
+          }
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int i = 0; i < kMeanChunk; ++i) m = m + b[i];
+          wait_lgkmcnt0();
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      } else {
+        const int rows = static_cast<int>(left);
+        for (int r = 0; r < rows; ++r) m = m + col[r * kMeanDims];
+      }
+      wait_lgkmcnt0();  // the slot is free once its reads are done
+    }
+    ring_barrier();
+  }
+  if (wave > 0) wait_vmcnt<0>();  // no load may land after the workgroup ends
+  if (wave == 0 && lane < kMeanDims && d0 + lane < d) mean[d0 + lane] = m / static_cast<float>(n);
 }
 
 // dm: dimensions that enter the max-abs and get scaled (the reference:
@@ -147,7 +196,7 @@ void launch_normalize(float* x, int64_t n, int d, int ld, float* mean, uint32_t*
                       hipStream_t s) {
   const int dm = scale_all ? d : (d < 256 ? d : 256);
   AKNN_HIP_TRY(hipMemsetAsync(gmax, 0, sizeof(uint32_t), s));
-  hipLaunchKernelGGL(mean_kernel, dim3((d + 63) / 64), dim3(1024), 0, s, x, n, d, ld, mean);
+  hipLaunchKernelGGL(mean_kernel, dim3((d + kMeanDims - 1) / kMeanDims), dim3(1024), 0, s, x, n, d, ld, mean);
   AKNN_HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(center_kernel, dim3(grid_for(n * ld, 256, 8192)), dim3(256), 0, s, x, n, d, ld, dm, mean, gmax);
   AKNN_HIP_TRY(hipGetLastError());

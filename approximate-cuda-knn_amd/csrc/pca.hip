@@ -25,26 +25,45 @@ namespace aknn {
 
 namespace {
 
-// second moments of a sample (rows q*stride, q < m) for pairs a <= b, FP32
+constexpr int kMomentChunks = 32;
+
+// second moments of a sample (rows q*stride, q < m) for pairs a <= b, FP32:
+// blockIdx.y takes one of kMomentChunks consecutive row ranges and writes its
+// partial sums; moments_reduce_kernel adds the chunks in order
 __global__ __launch_bounds__(256) void moments_kernel(const float* __restrict__ x, int d, int ld, int64_t stride,
-                                                      int64_t m, float* __restrict__ out) {
+                                                      int64_t m, float* __restrict__ part) {
+  const int64_t e = blockIdx.x * 256ll + threadIdx.x;
+  if (e >= static_cast<int64_t>(d) * d) return;
+  const int a = static_cast<int>(e / d), b = static_cast<int>(e % d);
+  if (a > b) return;
+  const int64_t per = (m + kMomentChunks - 1) / kMomentChunks;
+  const int64_t q0 = blockIdx.y * per, q1 = q0 + per < m ? q0 + per : m;
+  float s = 0.0f;
+  for (int64_t q = q0; q < q1; ++q) {
+    const float* r = x + q * stride * ld;
+    s = __builtin_fmaf(r[a], r[b], s);
+  }
+  part[static_cast<int64_t>(blockIdx.y) * d * d + a * d + b] = s;
+}
+
+__global__ __launch_bounds__(256) void moments_reduce_kernel(const float* __restrict__ part, int d,
+                                                             float* __restrict__ out) {
   const int64_t e = blockIdx.x * 256ll + threadIdx.x;
   if (e >= static_cast<int64_t>(d) * d) return;
   const int a = static_cast<int>(e / d), b = static_cast<int>(e % d);
   if (a > b) return;
   float s = 0.0f;
-  for (int64_t q = 0; q < m; ++q) {
-    const float* r = x + q * stride * ld;
-    s = __builtin_fmaf(r[a], r[b], s);
-  }
+  for (int c = 0; c < kMomentChunks; ++c) s += part[static_cast<int64_t>(c) * d * d + a * d + b];
   out[a * d + b] = s;
   out[b * d + a] = s;
 }
 
 // p(x) and its radius: E_j = g * sum_d |P_jd x_d| bounds the rounding of
 // coordinate j's chain (g = 2 (D + 8) 2^-24 covers the D-term FMA chain with a
-// factor 2 to spare), e = ||E|| evaluated in FP64 and rounded up, plus the
-// absolute subnormal term.
+// factor 2 to spare); the sum of magnitudes is itself an FP32 FMA chain of
+// non-negative terms, at least (1 - g/2) of the exact one, so it is scaled by
+// (1 + g); e = ||E|| is evaluated in FP64 and rounded up, plus the absolute
+// subnormal term.
 template <int R>
 __global__ __launch_bounds__(256) void pca_project_kernel(const float* __restrict__ x, int64_t n, int d, int ld,
                                                           const float* __restrict__ p, float* __restrict__ proj,
@@ -52,12 +71,11 @@ __global__ __launch_bounds__(256) void pca_project_kernel(const float* __restric
   __shared__ float ps[R * 256];
   const int64_t i = blockIdx.x * 256ll + threadIdx.x;
   const float* xr = x + (i < n ? i : 0) * ld;
-  float acc[R];
-  double ab[R];
+  float acc[R], ab[R];
 #pragma unroll
   for (int j = 0; j < R; ++j) {
     acc[j] = 0.0f;
-    ab[j] = 0.0;
+    ab[j] = 0.0f;
   }
   for (int c0 = 0; c0 < d; c0 += 256) {
     const int cw = d - c0 < 256 ? d - c0 : 256;
@@ -67,14 +85,35 @@ __global__ __launch_bounds__(256) void pca_project_kernel(const float* __restric
       ps[e] = c < cw ? p[static_cast<int64_t>(j) * d + c0 + c] : 0.0f;
     }
     __syncthreads();
-    for (int c = 0; c < cw; ++c) {
+    // 16-byte row loads, eight in flight per lane before their FMAs
+    int c = 0;
+    for (; c + 32 <= cw; c += 32) {
+      float4 q[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) q[u] = *reinterpret_cast<const float4*>(xr + c0 + c + 4 * u);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const float f[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const float av = __builtin_fabsf(f[t]);
+#pragma unroll
+          for (int j = 0; j < R; ++j) {
+            const float pv = ps[j * 256 + c + 4 * u + t];
+            acc[j] = __builtin_fmaf(f[t], pv, acc[j]);
+            ab[j] = __builtin_fmaf(av, __builtin_fabsf(pv), ab[j]);
+          }
+        }
+      }
+    }
+    for (; c < cw; ++c) {
       const float v = xr[c0 + c];
-      const double av = fabs(static_cast<double>(v));
+      const float av = __builtin_fabsf(v);
 #pragma unroll
       for (int j = 0; j < R; ++j) {
         const float pv = ps[j * 256 + c];
         acc[j] = __builtin_fmaf(v, pv, acc[j]);
-        ab[j] += av * fabs(static_cast<double>(pv));
+        ab[j] = __builtin_fmaf(av, __builtin_fabsf(pv), ab[j]);
       }
     }
   }
@@ -83,7 +122,7 @@ __global__ __launch_bounds__(256) void pca_project_kernel(const float* __restric
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       proj[i * R + j] = acc[j];
-      const double ej = g * ab[j] * (1.0 + 1e-12) + tiny;
+      const double ej = g * static_cast<double>(ab[j]) * (1.0 + g) + tiny;
       e2 += ej * ej;
     }
     const double e = sqrt(e2) * (1.0 + 1e-12);
@@ -137,8 +176,11 @@ PcaBasis compute_pca_basis(const float* x, int64_t n, int d, int ld, float* mome
   if (d < 16 || d > 256 || n < 64) return b;
   const int64_t m = std::min<int64_t>(n, 16384);
   const int64_t stride = n / m;
-  hipLaunchKernelGGL(moments_kernel, dim3(static_cast<unsigned>((static_cast<int64_t>(d) * d + 255) / 256)), dim3(256),
-                     0, s, x, d, ld, stride, m, moments_dev);
+  const unsigned pg = static_cast<unsigned>((static_cast<int64_t>(d) * d + 255) / 256);
+  float* part = moments_dev + static_cast<int64_t>(d) * d;
+  hipLaunchKernelGGL(moments_kernel, dim3(pg, kMomentChunks), dim3(256), 0, s, x, d, ld, stride, m, part);
+  AKNN_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(moments_reduce_kernel, dim3(pg), dim3(256), 0, s, part, d, moments_dev);
   AKNN_HIP_TRY(hipGetLastError());
   std::vector<float> mf(static_cast<size_t>(d) * d);
   AKNN_HIP_TRY(hipMemcpyAsync(mf.data(), moments_dev, sizeof(float) * mf.size(), hipMemcpyDeviceToHost, s));

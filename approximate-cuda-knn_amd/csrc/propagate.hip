@@ -142,12 +142,24 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     if (lane < k && id >= 0) hash_insert(hs, hsz, id);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    for (int f = lane; f < kk + (64 - kk % 64) % 64; f += 64) {
-      const int hi = f / k, lo = f - (f / k) * k;
-      const int32_t nb = __shfl(id, hi < 64 ? hi : 0);
-      const bool ex = __shfl(static_cast<int>(expand), hi < 64 ? hi : 0) != 0;
-      if (f < kk && nb >= 0 && ex) {
-        const int32_t c = ids_in[static_cast<int64_t>(nb) * k + lo];
+    // eight slots per lane are loaded before any of them is inserted, so
+    // their memory round trips overlap instead of queueing behind the LDS
+    // set operations (the set and the final candidate set are the same)
+    constexpr int kHop = 8;
+    const int kk_pad = kk + (64 - kk % 64) % 64;
+    for (int f0 = 0; f0 < kk_pad; f0 += 64 * kHop) {
+      int32_t cv[kHop];
+#pragma unroll
+      for (int u = 0; u < kHop; ++u) {
+        const int f = f0 + 64 * u + lane;
+        const int hi = f / k, lo = f - (f / k) * k;
+        const int32_t nb = __shfl(id, hi < 64 ? hi : 0);
+        const bool ex = __shfl(static_cast<int>(expand), hi < 64 ? hi : 0) != 0;
+        cv[u] = (f < kk && nb >= 0 && ex) ? ids_in[static_cast<int64_t>(nb) * k + lo] : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < kHop; ++u) {
+        const int32_t c = cv[u];
         if (c >= 0 && hash_insert(hs, hsz, c)) {
           const int pos = atomicAdd(cnt, 1);
           cand[pos] = c;
@@ -158,7 +170,8 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
     __builtin_amdgcn_wave_barrier();
     m = *cnt;
     nnew = m;
-    if (ncand != nullptr && lane == 0) atomicAdd(ncand, static_cast<unsigned long long>(m));
+    if (ncand != nullptr && lane == 0)
+      atomicAdd(ncand + (blockIdx.x % kCountSlots) * kCountStride, static_cast<unsigned long long>(m));
   }
   if constexpr (SK) {
     if (m > 0) {
@@ -168,6 +181,54 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
       const double si = SKM == 2 ? static_cast<double>(s8[i]) : 0.0;
       const double ti = SKM == 2 ? si * si * static_cast<double>(a8[i]) : 0.0;
       int w = 0;
+      if constexpr (SKM == 3) {
+        // two batches of candidates per memory round trip: their ids and
+        // projections are loaded before either is tested (compaction writes
+        // slots at or below ones already read)
+        const double k1 = 1.0 - sb.kap;
+        for (int base = 0; base < m; base += 128) {
+          int32_t cc[2];
+          float4 pv[2][8];
+          float ec[2];
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            const int ci = base + 64 * u + lane;
+            cc[u] = ci < m ? cand[ci] : -1;
+            const float4* pc = reinterpret_cast<const float4*>(pca + static_cast<int64_t>(cc[u] >= 0 ? cc[u] : 0) * pca_r);
+#pragma unroll
+            for (int g = 0; g < 8; ++g) pv[u][g] = g < pca_r / 4 ? pc[g] : make_float4(0.f, 0.f, 0.f, 0.f);
+            ec[u] = pca_e[cc[u] >= 0 ? cc[u] : 0];
+          }
+          const float4* pq = reinterpret_cast<const float4*>(xs);
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            float p = 0.0f;
+#pragma unroll
+            for (int g = 0; g < 8; ++g) {
+              if (g < pca_r / 4) {
+                const float4 v = pq[g], q = pv[u][g];
+                float t = v.x - q.x;
+                p = __builtin_fmaf(t, t, p);
+                t = v.y - q.y;
+                p = __builtin_fmaf(t, t, p);
+                t = v.z - q.z;
+                p = __builtin_fmaf(t, t, p);
+                t = v.w - q.w;
+                p = __builtin_fmaf(t, t, p);
+              }
+            }
+            // ||x - y|| >= (||p(x) - p(y)|| - e(x) - e(y)) / sigma, with the
+            // r-term chain's rounding taken off; then the exact chain's error
+            const double sp = fmax(0.0, static_cast<double>(p) * k1 - sb.tiny);
+            const double r = fmax(0.0, sqrt(sp) - static_cast<double>(ei) - static_cast<double>(ec[u])) / pca_sigma;
+            const double lb = sqrt(fmax(0.0, r * r * k1 - sb.tiny)) * k1;
+            const bool pass = cc[u] >= 0 && !(lb > static_cast<double>(thk));
+            const uint64_t bal = __ballot(pass);
+            if (pass) cand[w + __popcll(bal & ((1ull << lane) - 1))] = cc[u];
+            w += __popcll(bal);
+          }
+        }
+      } else
       for (int base = 0; base < m; base += 64) {
         const int ci = base + lane;
         const int32_t c = ci < m ? cand[ci] : -1;
@@ -211,28 +272,6 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
             const double r = fmax(0.0, sqrt(fmax(0.0, slo)) - static_cast<double>(ei) - static_cast<double>(e8[c]));
             const double k1 = 1.0 - sb.kap;
             lb = sqrt(fmax(0.0, r * r * k1 - sb.tiny)) * k1;
-          } else if constexpr (SKM == 3) {
-            // ||x - y|| >= (||p(x) - p(y)|| - e(x) - e(y)) / sigma, with the
-            // r-term chain's rounding taken off; then the exact chain's error
-            const float4* pc = reinterpret_cast<const float4*>(pca + static_cast<int64_t>(c) * pca_r);
-            const float4* pq = reinterpret_cast<const float4*>(xs);
-            float p = 0.0f;
-            for (int g = 0; g < pca_r / 4; ++g) {
-              const float4 u = pc[g], v = pq[g];
-              float t = v.x - u.x;
-              p = __builtin_fmaf(t, t, p);
-              t = v.y - u.y;
-              p = __builtin_fmaf(t, t, p);
-              t = v.z - u.z;
-              p = __builtin_fmaf(t, t, p);
-              t = v.w - u.w;
-              p = __builtin_fmaf(t, t, p);
-            }
-            const double k1 = 1.0 - sb.kap;
-            const double sp = fmax(0.0, static_cast<double>(p) * k1 - sb.tiny);
-            const double r =
-                fmax(0.0, sqrt(sp) - static_cast<double>(ei) - static_cast<double>(pca_e[c])) / pca_sigma;
-            lb = sqrt(fmax(0.0, r * r * k1 - sb.tiny)) * k1;
           } else {
             const float p = sketch_chain<METRIC>(xs, xh + static_cast<int64_t>(c) * ldh, ldh);
             lb = sketch_lower_bound<METRIC>(p, ei, eps[c], ni, METRIC == kCosine ? norms[c] : 1.0f, sb);
@@ -246,7 +285,8 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
       m = w;
-      if (ncand != nullptr && lane == 0) atomicAdd(ncand + 1, static_cast<unsigned long long>(m));
+      if (ncand != nullptr && lane == 0)
+        atomicAdd(ncand + (kCountSlots + blockIdx.x % kCountSlots) * kCountStride, static_cast<unsigned long long>(m));
     }
   }
 

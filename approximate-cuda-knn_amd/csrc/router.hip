@@ -197,14 +197,20 @@ __global__ __launch_bounds__(256) void argmax_kernel(const float* __restrict__ z
   if (lane == 0) bins[row] = am;
 }
 
-// Column sums (bias gradients), one thread per column, rows in order.
+// Column sums (bias gradients): a block per 64 columns, its four waves
+// summing interleaved rows (each lane one column, coalesced rows), then the
+// four partial sums added in wave order.
 __global__ __launch_bounds__(256) void colsum_kernel(const float* __restrict__ a, int64_t m, int n,
                                                      float* __restrict__ out) {
-  const int c = blockIdx.x * 256 + threadIdx.x;
-  if (c >= n) return;
+  __shared__ float part[4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + lane;
   float s = 0.0f;
-  for (int64_t r = 0; r < m; ++r) s += a[r * n + c];
-  out[c] = s;
+  if (c < n)
+    for (int64_t r = wave; r < m; r += 4) s += a[r * n + c];
+  part[wave][lane] = s;
+  __syncthreads();
+  if (wave == 0 && c < n) out[c] = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
 }
 
 __global__ __launch_bounds__(256) void adam_kernel(float* __restrict__ p, const float* __restrict__ g,
@@ -246,6 +252,23 @@ constexpr int kFuseRows = 64;
 constexpr int kFuseH = 256;       // hidden width of the fused path
 constexpr int kFuseLd = 256 + 8;  // LDS row stride (halves)
 
+// A [n][k] BF16 weight matrix in MFMA B-fragment order: fragment (column
+// tile ct, K step ks) is 64 lanes x 8 halves stored contiguously, lane l
+// holding row ct*32 + l%32, columns ks*16 + (l/32)*8 .. +8, so a wave reads
+// one fragment as 1 KB of consecutive bytes.
+__global__ __launch_bounds__(256) void pack_fragments_kernel(const uint16_t* __restrict__ w, int n, int k,
+                                                             uint16_t* __restrict__ out) {
+  const int64_t e = blockIdx.x * 256ll + threadIdx.x;  // one 16-byte piece
+  const int ksn = k / 16;
+  if (e >= static_cast<int64_t>(n / 32) * ksn * 64) return;
+  const int lane = static_cast<int>(e % 64);
+  const int64_t f = e / 64;
+  const int ks = static_cast<int>(f % ksn), ct = static_cast<int>(f / ksn);
+  const uint4 v = *reinterpret_cast<const uint4*>(w + static_cast<int64_t>(ct * 32 + (lane & 31)) * k + ks * 16 +
+                                                  (lane >> 5) * 8);
+  *reinterpret_cast<uint4*>(out + e * 8) = v;
+}
+
 template <int CPW>
 __device__ __forceinline__ void fused_layer(const uint16_t* __restrict__ src, const uint16_t* __restrict__ w, int k,
                                             int n, int wave, int lane, f32x16 (&acc)[2][2]) {
@@ -254,6 +277,7 @@ __device__ __forceinline__ void fused_layer(const uint16_t* __restrict__ src, co
   for (int c = 0; c < 2; ++c)
 #pragma unroll
     for (int rt = 0; rt < 2; ++rt) acc[c][rt] = f32x16{};
+  (void)li;
   const int nt = n / 32;
   // K in groups of four steps: every fragment of a group is loaded (the B
   // fragments from L2) before its MFMAs, so four steps of L2 latency overlap;
@@ -270,8 +294,10 @@ __device__ __forceinline__ void fused_layer(const uint16_t* __restrict__ src, co
 #pragma unroll
       for (int c = 0; c < CPW; ++c) {
         const int ct = wave * CPW + c;
-        bv[g][c] = (in && ct < nt) ? *reinterpret_cast<const uint4*>(w + static_cast<int64_t>(ct * 32 + li) * k + k0 + kh)
-                                   : make_uint4(0u, 0u, 0u, 0u);
+        // w is fragment-packed: fragment (ct, k0/16) is 64 consecutive 16-byte pieces
+        bv[g][c] = (in && ct < nt)
+                       ? *reinterpret_cast<const uint4*>(w + ((static_cast<int64_t>(ct) * (k / 16) + k0 / 16) * 64 + lane) * 8)
+                       : make_uint4(0u, 0u, 0u, 0u);
       }
     }
 #pragma unroll
@@ -322,13 +348,30 @@ __global__ __launch_bounds__(256) void router_fused_kernel(const float* __restri
   __shared__ int bidx[4][kFuseRows];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t row0 = static_cast<int64_t>(blockIdx.x) * kFuseRows;
-  // standardised rows into bufb (BF16), zero past d and past n
-  for (int e = threadIdx.x; e < kFuseRows * dp; e += 256) {
-    const int r = e / dp, c = e % dp;
-    const int64_t row = row0 + r;
-    uint16_t b = 0;
-    if (row < n && c < d) b = f2bf((x[row * ld + c] - mu[c]) * inv_sigma);
-    bufb[r * kFuseLd + c] = b;
+  // standardised rows into bufb (BF16), zero past d and past n: every
+  // thread's loads (up to 16 x 4 floats) are issued before any is converted
+  {
+    const int q4 = dp / 4;  // 4-float groups per row (dp % 16 == 0, ld % 4 == 0)
+    float4 v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int e = threadIdx.x + 256 * u;
+      const int r = e / q4, c = 4 * (e % q4);
+      const int64_t row = row0 + r;
+      v[u] = (e < kFuseRows * q4 && row < n && c < ld) ? *reinterpret_cast<const float4*>(x + row * ld + c)
+                                                        : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int e = threadIdx.x + 256 * u;
+      if (e < kFuseRows * q4) {
+        const int r = e / q4, c = 4 * (e % q4);
+        const float f[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          bufb[r * kFuseLd + c + j] = (row0 + r < n && c + j < d) ? f2bf((f[j] - mu[c + j]) * inv_sigma) : 0;
+      }
+    }
   }
   __syncthreads();
   f32x16 acc[2][2];
@@ -454,6 +497,9 @@ Router::Router(int device, int d, int hidden, int bin_bits, uint32_t seed)
   w2t_ = alloc<uint16_t>(static_cast<size_t>(h_) * h_);
   w3b_ = alloc<uint16_t>(static_cast<size_t>(nb_) * h_);
   w3t_ = alloc<uint16_t>(static_cast<size_t>(nb_) * h_);
+  w1p_ = alloc<uint16_t>(static_cast<size_t>(h_) * dp_);
+  w2p_ = alloc<uint16_t>(static_cast<size_t>(h_) * h_);
+  w3p_ = alloc<uint16_t>(static_cast<size_t>(nb_) * h_);
   mu_ = alloc<float>(dp_);
   acc_ = alloc<double>(2);
   // He-normal weights from the seed (splitmix64 + Box-Muller), zero biases;
@@ -490,6 +536,16 @@ void Router::refresh_weights(hipStream_t s) {
   hipLaunchKernelGGL(to_bf16_kernel, dim3(grid_of(static_cast<int64_t>(nb_) * h_, 256)), dim3(256), 0, s,
                      p_ + off_[4], nb_, h_, w3b_, w3t_);
   AKNN_HIP_TRY(hipGetLastError());
+  if (w1p_) {
+    // fragment-ordered copies for the fused inference kernel
+    hipLaunchKernelGGL(pack_fragments_kernel, dim3(grid_of(static_cast<int64_t>(h_) * dp_ / 8, 256)), dim3(256), 0, s,
+                       w1b_, h_, dp_, w1p_);
+    hipLaunchKernelGGL(pack_fragments_kernel, dim3(grid_of(static_cast<int64_t>(h_) * h_ / 8, 256)), dim3(256), 0, s,
+                       w2b_, h_, h_, w2p_);
+    hipLaunchKernelGGL(pack_fragments_kernel, dim3(grid_of(static_cast<int64_t>(nb_) * h_ / 8, 256)), dim3(256), 0, s,
+                       w3b_, nb_, h_, w3p_);
+    AKNN_HIP_TRY(hipGetLastError());
+  }
 }
 
 void Router::ensure_batch(int64_t mpad) {
@@ -549,7 +605,7 @@ void Router::backward(int64_t mpad, int64_t m, int64_t count, hipStream_t s) {
   g.of = g_ + off_[4];
   g.ldof = h_;
   gemm(g, s);
-  hipLaunchKernelGGL(colsum_kernel, dim3(grid_of(nb_, 256)), dim3(256), 0, s, dz3f_, mpad, nb_, g_ + off_[5]);
+  hipLaunchKernelGGL(colsum_kernel, dim3(grid_of(nb_, 64)), dim3(256), 0, s, dz3f_, mpad, nb_, g_ + off_[5]);
   // dZ2 = (dZ3 W3) masked by H2 > 0
   g = gemm_args(dz3_, nb_, w3t_, nb_, mp, h_, nb_);
   g.mask = h2_;
@@ -565,7 +621,7 @@ void Router::backward(int64_t mpad, int64_t m, int64_t count, hipStream_t s) {
   g.of = g_ + off_[2];
   g.ldof = h_;
   gemm(g, s);
-  hipLaunchKernelGGL(colsum_kernel, dim3(grid_of(h_, 256)), dim3(256), 0, s, dz2f_, mpad, h_, g_ + off_[3]);
+  hipLaunchKernelGGL(colsum_kernel, dim3(grid_of(h_, 64)), dim3(256), 0, s, dz2f_, mpad, h_, g_ + off_[3]);
   // dZ1 = (dZ2 W2) masked by H1 > 0
   g = gemm_args(dz2_, h_, w2t_, h_, mp, h_, h_);
   g.mask = h1_;
@@ -579,7 +635,7 @@ void Router::backward(int64_t mpad, int64_t m, int64_t count, hipStream_t s) {
   g.of = g_ + off_[0];
   g.ldof = dp_;
   gemm(g, s);
-  hipLaunchKernelGGL(colsum_kernel, dim3(grid_of(h_, 256)), dim3(256), 0, s, dz1f_, mpad, h_, g_ + off_[1]);
+  hipLaunchKernelGGL(colsum_kernel, dim3(grid_of(h_, 64)), dim3(256), 0, s, dz1f_, mpad, h_, g_ + off_[1]);
   AKNN_HIP_TRY(hipGetLastError());
 }
 
@@ -802,10 +858,10 @@ void Router::assign(const float* x_raw, int64_t n, int ld, int32_t* bins, hipStr
     const unsigned grid = grid_of(n, kFuseRows);
     if (nb_ == 256)
       hipLaunchKernelGGL(router_fused_kernel<2>, dim3(grid), dim3(256), 0, s, x_raw, ld, d_, dp_, n, mu_, inv_sigma_,
-                         w1b_, p_ + off_[1], w2b_, p_ + off_[3], w3b_, p_ + off_[5], nb_, bins);
+                         w1p_, p_ + off_[1], w2p_, p_ + off_[3], w3p_, p_ + off_[5], nb_, bins);
     else
       hipLaunchKernelGGL(router_fused_kernel<1>, dim3(grid), dim3(256), 0, s, x_raw, ld, d_, dp_, n, mu_, inv_sigma_,
-                         w1b_, p_ + off_[1], w2b_, p_ + off_[3], w3b_, p_ + off_[5], nb_, bins);
+                         w1p_, p_ + off_[1], w2p_, p_ + off_[3], w3p_, p_ + off_[5], nb_, bins);
     AKNN_HIP_TRY(hipGetLastError());
     AKNN_HIP_TRY(hipEventRecord(ev_[1], s));
     assigned_ = true;

@@ -98,6 +98,8 @@ class Router {
   float *p_ = nullptr, *g_ = nullptr, *m1_ = nullptr, *m2_ = nullptr;
   // BF16 weights: W1 [h][dp], W2 [h][h], W3 [nb][h]; transposes W2T [h][h], W3T [h][nb]
   uint16_t *w1b_ = nullptr, *w2b_ = nullptr, *w3b_ = nullptr, *w2t_ = nullptr, *w3t_ = nullptr;
+  // the same weights in MFMA B-fragment order (fused inference)
+  uint16_t *w1p_ = nullptr, *w2p_ = nullptr, *w3p_ = nullptr;
   float* mu_ = nullptr;  // [dp], device
   std::vector<float> mu_host_;
   float inv_sigma_ = 1.0f;

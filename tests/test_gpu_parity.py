@@ -89,6 +89,30 @@ def test_c1_end_to_end_bit_exact(aknn, oracle, datasets):
     assert st["total"] > 0
 
 
+@pytest.mark.parametrize("n,d", [(5, 1), (479, 17), (480, 16), (481, 15), (961, 33), (1441, 64), (20000, 130),
+                                 (3000, 300)])
+def test_normalize_shapes_bit_exact(aknn, oracle, n, d):
+    """The per-dimension sequential mean walks the rows in tiles of 480 and the
+    dimensions in groups of 16: shapes on both sides of those edges.  An
+    offset of 1000 makes the FP32 sum's rounding depend on the order, so a
+    reordered or split sum fails."""
+    rng = np.random.default_rng(n * 1000 + d)
+    x = (1000.0 + rng.standard_normal((n, d)) * rng.uniform(0.01, 10.0, size=d)).astype(np.float32)
+    k = min(3, n - 1)
+    g = gpu_run(aknn, x, k, 1, 16, 0, 0)
+    xn = oracle.normalize(x)
+    assert (bits(g.normalized()) == bits(xn)).all(), "normalized points"
+    if n >= 1000:
+        # the data is order-sensitive: summing two halves separately rounds differently
+        seq = np.zeros(d, np.float32)
+        for r in x:
+            seq = seq + r
+        half = [np.zeros(d, np.float32), np.zeros(d, np.float32)]
+        for i, r in enumerate(x):
+            half[i * 2 // n] += r
+        assert (half[0] + half[1] != seq).any()
+
+
 @pytest.mark.parametrize("metric", [0, 1])
 @pytest.mark.parametrize("props", [0, 2])
 def test_small_bit_exact(aknn, oracle, datasets, metric, props):
