@@ -28,14 +28,17 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-// FP32 -> BF16, round to nearest even (NaN kept quiet).
-__host__ __device__ inline uint16_t f2bf(float f) {
-  uint32_t u;
-  std::memcpy(&u, &f, 4);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return static_cast<uint16_t>((u >> 16) | 0x40u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return static_cast<uint16_t>(u >> 16);
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// FP32 -> BF16 pair on gfx950's converter (v_cvt_pk_bf16_f32, round to
+// nearest even): lo in bits 0..15, hi in bits 16..31.  Every BF16 value the
+// library makes goes through it, so the fused and layer-by-layer paths round
+// alike.
+__device__ __forceinline__ uint32_t bf16_pack2(float lo, float hi) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{lo, hi}, bf16x2));
 }
+__device__ __forceinline__ uint16_t f2bf(float f) { return static_cast<uint16_t>(bf16_pack2(f, 0.0f)); }
 
 __device__ inline float bf2f(uint16_t b) { return __uint_as_float(static_cast<uint32_t>(b) << 16); }
 
@@ -197,20 +200,33 @@ __global__ __launch_bounds__(256) void argmax_kernel(const float* __restrict__ z
   if (lane == 0) bins[row] = am;
 }
 
-// Column sums (bias gradients): a block per 64 columns, its four waves
-// summing interleaved rows (each lane one column, coalesced rows), then the
-// four partial sums added in wave order.
-__global__ __launch_bounds__(256) void colsum_kernel(const float* __restrict__ a, int64_t m, int n,
-                                                     float* __restrict__ out) {
-  __shared__ float part[4][64];
+// Column sums (bias gradients): a block per 64 columns, its sixteen waves
+// summing interleaved rows (each lane one column, coalesced rows, eight loads
+// in flight per lane), then the sixteen partial sums added in wave order.
+__global__ __launch_bounds__(1024) void colsum_kernel(const float* __restrict__ a, int64_t m, int n,
+                                                      float* __restrict__ out) {
+  __shared__ float part[16][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int c = blockIdx.x * 64 + lane;
   float s = 0.0f;
-  if (c < n)
-    for (int64_t r = wave; r < m; r += 4) s += a[r * n + c];
+  if (c < n) {
+    int64_t r = wave;
+    for (; r + 16 * 7 < m; r += 16 * 8) {
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = a[(r + 16 * u) * n + c];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; r < m; r += 16) s += a[r * n + c];
+  }
   part[wave][lane] = s;
   __syncthreads();
-  if (wave == 0 && c < n) out[c] = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+  if (wave == 0 && c < n) {
+    float t = part[0][lane];
+    for (int w = 1; w < 16; ++w) t += part[w][lane];
+    out[c] = t;
+  }
 }
 
 __global__ __launch_bounds__(256) void adam_kernel(float* __restrict__ p, const float* __restrict__ g,
@@ -237,18 +253,21 @@ __global__ __launch_bounds__(256) void to_bf16_kernel(const float* __restrict__ 
   if (dstt) dstt[static_cast<int64_t>(c) * rows + r] = b;
 }
 
-// ---- fused inference: 64 rows per workgroup through all three layers -------
+// ---- fused inference: 128 rows per workgroup through all three layers ------
 //
 // The rows are standardised into LDS as BF16, each layer's activations stay
-// in LDS (row stride padded by 8 halves so the 32 rows of an A fragment fall
-// in different banks), the weights stream from L2 as B fragments, and the
-// logits never leave registers: each lane's (value, column) pairs reduce
-// across its 32-lane half by shuffles, then across waves through LDS, to the
-// lowest column holding the row maximum.  Wave w owns output column tiles
-// [w*cpw, (w+1)*cpw) of a layer for both 32-row tiles, so a B fragment feeds
-// two MFMAs.  Per output tile the K steps and lane layouts are the ones of
-// gemm_nt_kernel, so the logits, and the bins, equal the unfused path's.
-constexpr int kFuseRows = 64;
+// in one LDS buffer (row stride padded by 8 halves so the 32 rows of an A
+// fragment fall in different banks), the weights stream from L2 as B
+// fragments, and the logits never leave registers: each lane's (value,
+// column) pairs reduce across its 32-lane half by shuffles, then across waves
+// through LDS, to the lowest column holding the row maximum.  Wave w owns
+// output column tiles [w*cpw, (w+1)*cpw) of a layer for all four 32-row
+// tiles, so a B fragment feeds four MFMAs: the weights cross from L2 once per
+// 128 rows (2.5 KB per row for a 128-256-256-256 router).  Per output tile the
+// K steps and lane layouts are the ones of gemm_nt_kernel, so the logits, and
+// the bins, equal the unfused path's.
+constexpr int kFuseRows = 128;                 // rows per workgroup of the fused path
+constexpr int kFuseRT = kFuseRows / 32;         // 32-row MFMA tiles per workgroup
 constexpr int kFuseH = 256;       // hidden width of the fused path
 constexpr int kFuseLd = 256 + 8;  // LDS row stride (halves)
 
@@ -269,152 +288,222 @@ __global__ __launch_bounds__(256) void pack_fragments_kernel(const uint16_t* __r
   *reinterpret_cast<uint4*>(out + e * 8) = v;
 }
 
+// One layer of the fused path for a workgroup's kFuseRows rows: wave w
+// computes column tiles w*CPW .. w*CPW+CPW-1 of all kFuseRT row tiles, so each
+// B fragment read from L2 feeds kFuseRT MFMAs.  The B fragments stream in
+// groups of kFuseGroup K steps, double-buffered in registers: a group's loads
+// are issued a whole group of MFMAs ahead of their use.  The A fragments
+// (LDS) of step ks+1 are loaded during the MFMAs of step ks.  Every
+// accumulator still takes its K steps in order.
+constexpr int kFuseGroup = 4;
+
 template <int CPW>
 __device__ __forceinline__ void fused_layer(const uint16_t* __restrict__ src, const uint16_t* __restrict__ w, int k,
-                                            int n, int wave, int lane, f32x16 (&acc)[2][2]) {
+                                            int n, int wave, int lane, f32x16 (&acc)[2][kFuseRT]) {
   const int li = lane & 31, kh = (lane >> 5) * 8;
 #pragma unroll
   for (int c = 0; c < 2; ++c)
 #pragma unroll
-    for (int rt = 0; rt < 2; ++rt) acc[c][rt] = f32x16{};
-  (void)li;
-  const int nt = n / 32;
-  // K in groups of four steps: every fragment of a group is loaded (the B
-  // fragments from L2) before its MFMAs, so four steps of L2 latency overlap;
-  // within a tile the K steps keep their order
-  constexpr int kGroup = 4;
-  for (int kb = 0; kb < k; kb += 16 * kGroup) {
-    uint4 a0[kGroup], a1[kGroup], bv[kGroup][CPW];
+    for (int rt = 0; rt < kFuseRT; ++rt) acc[c][rt] = f32x16{};
+  const int nt = n / 32, ksn = k / 16;
+  auto load_a = [&](uint4 (&a)[kFuseRT], int ks) {
 #pragma unroll
-    for (int g = 0; g < kGroup; ++g) {
-      const int k0 = kb + 16 * g;
-      const bool in = k0 < k;
-      a0[g] = in ? *reinterpret_cast<const uint4*>(src + li * kFuseLd + k0 + kh) : make_uint4(0u, 0u, 0u, 0u);
-      a1[g] = in ? *reinterpret_cast<const uint4*>(src + (32 + li) * kFuseLd + k0 + kh) : make_uint4(0u, 0u, 0u, 0u);
+    for (int rt = 0; rt < kFuseRT; ++rt)
+      a[rt] = *reinterpret_cast<const uint4*>(src + (rt * 32 + li) * kFuseLd + ks * 16 + kh);
+  };
+  // w is fragment-packed: fragment (ct, ks) is 64 consecutive 16-byte pieces.
+  // Steps past k and tiles past n load valid fragments that are never used,
+  // so no load sits behind a branch.
+  auto load_group = [&](uint4 (&b)[kFuseGroup][CPW], int g0) {
+#pragma unroll
+    for (int j = 0; j < kFuseGroup; ++j) {
+      const int ks = g0 + j < ksn ? g0 + j : ksn - 1;
 #pragma unroll
       for (int c = 0; c < CPW; ++c) {
-        const int ct = wave * CPW + c;
-        // w is fragment-packed: fragment (ct, k0/16) is 64 consecutive 16-byte pieces
-        bv[g][c] = (in && ct < nt)
-                       ? *reinterpret_cast<const uint4*>(w + ((static_cast<int64_t>(ct) * (k / 16) + k0 / 16) * 64 + lane) * 8)
-                       : make_uint4(0u, 0u, 0u, 0u);
+        const int ct = wave * CPW + c < nt ? wave * CPW + c : nt - 1;
+        b[j][c] = *reinterpret_cast<const uint4*>(w + ((static_cast<int64_t>(ct) * ksn + ks) * 64 + lane) * 8);
       }
     }
+  };
+  auto mma = [&](const uint4 (&a)[kFuseRT], const uint4 (&b)[CPW]) {
 #pragma unroll
-    for (int g = 0; g < kGroup; ++g) {
-      if (kb + 16 * g >= k) break;
+    for (int c = 0; c < CPW; ++c) {
+      if (wave * CPW + c >= nt) continue;
+      const bf16x8 bb = __builtin_bit_cast(bf16x8, b[c]);
 #pragma unroll
-      for (int c = 0; c < CPW; ++c) {
-        if (wave * CPW + c >= nt) continue;
-        const bf16x8 b = __builtin_bit_cast(bf16x8, bv[g][c]);
-        acc[c][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a0[g]), b, acc[c][0], 0, 0, 0);
-        acc[c][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a1[g]), b, acc[c][1], 0, 0, 0);
-      }
+      for (int rt = 0; rt < kFuseRT; ++rt)
+        acc[c][rt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bb, __builtin_bit_cast(bf16x8, a[rt]), acc[c][rt], 0, 0, 0);
+    }
+  };
+  auto run_group = [&](const uint4 (&b)[kFuseGroup][CPW], int g0) {
+    uint4 a0[kFuseRT], a1[kFuseRT];
+    load_a(a0, g0);
+#pragma unroll
+    for (int j = 0; j < kFuseGroup; j += 2) {
+      if (g0 + j >= ksn) break;
+      if (g0 + j + 1 < ksn) load_a(a1, g0 + j + 1);
+      mma(a0, b[j]);
+      if (g0 + j + 1 >= ksn) break;
+      if (j + 2 < kFuseGroup && g0 + j + 2 < ksn) load_a(a0, g0 + j + 2);
+      mma(a1, b[j + 1]);
+    }
+  };
+  // sched_barrier keeps each group's loads ahead of the previous group's
+  // MFMAs: regrouped, the loads would be waited for as soon as issued
+  uint4 bA[kFuseGroup][CPW], bB[kFuseGroup][CPW];
+  load_group(bA, 0);
+  for (int g0 = 0; g0 < ksn; g0 += 2 * kFuseGroup) {
+    if (g0 + kFuseGroup < ksn) load_group(bB, g0 + kFuseGroup);
+    __builtin_amdgcn_sched_barrier(0);
+    run_group(bA, g0);
+    __builtin_amdgcn_sched_barrier(0);
+    if (g0 + kFuseGroup < ksn) {
+      if (g0 + 2 * kFuseGroup < ksn) load_group(bA, g0 + 2 * kFuseGroup);
+      __builtin_amdgcn_sched_barrier(0);
+      run_group(bB, g0 + kFuseGroup);
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
 }
 
+// acc[c][rt] holds the transposed tile: lane (i + 32*hh) has data row
+// rt*32 + i and, in registers 4g..4g+3, the four consecutive columns
+// ct*32 + 8g + 4hh + 0..3.  Bias, ReLU and BF16 rounding, then one 8-byte
+// LDS store per four columns.
 template <int CPW>
-__device__ __forceinline__ void fused_hidden_store(const f32x16 (&acc)[2][2], const float* __restrict__ bias,
+__device__ __forceinline__ void fused_hidden_store(const f32x16 (&acc)[2][kFuseRT], const float* __restrict__ sbias,
                                                    int n, int wave, int lane, uint16_t* __restrict__ dst) {
+  const int il = lane & 31, hh = lane >> 5;
 #pragma unroll
   for (int c = 0; c < CPW; ++c) {
     const int ct = wave * CPW + c;
     if (ct >= n / 32) continue;
-    const int col = ct * 32 + (lane & 31);
-    const float bv = bias[col];
 #pragma unroll
-    for (int rt = 0; rt < 2; ++rt)
+    for (int g = 0; g < 4; ++g) {
+      const int n0 = ct * 32 + 8 * g + 4 * hh;
+      const float4 b4 = *reinterpret_cast<const float4*>(sbias + n0);
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int row = rt * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        float v = acc[c][rt][r] + bv;
-        v = v > 0.0f ? v : 0.0f;
-        dst[row * kFuseLd + col] = f2bf(v);
+      for (int rt = 0; rt < kFuseRT; ++rt) {
+        // ReLU as gemm_nt_kernel's epilogue writes it (NaN and -0 give +0)
+        const float u0 = acc[c][rt][4 * g + 0] + b4.x, u1 = acc[c][rt][4 * g + 1] + b4.y;
+        const float u2 = acc[c][rt][4 * g + 2] + b4.z, u3 = acc[c][rt][4 * g + 3] + b4.w;
+        const float v0 = u0 > 0.0f ? u0 : 0.0f, v1 = u1 > 0.0f ? u1 : 0.0f;
+        const float v2 = u2 > 0.0f ? u2 : 0.0f, v3 = u3 > 0.0f ? u3 : 0.0f;
+        *reinterpret_cast<uint2*>(dst + (rt * 32 + il) * kFuseLd + n0) =
+            make_uint2(bf16_pack2(v0, v1), bf16_pack2(v2, v3));
       }
+    }
   }
 }
 
+// Router inference for kFuseRows rows per workgroup through all three
+// layers.  The activations live in one LDS buffer that each hidden layer
+// overwrites once every wave has read all of it; the logits stay in
+// registers and reduce to the arg-max (lowest bin on ties).
 template <int CPW3>
-__global__ __launch_bounds__(256) void router_fused_kernel(const float* __restrict__ x, int ld, int d, int dp,
+__global__ __launch_bounds__(256, 2) void router_fused_kernel(const float* __restrict__ x, int ld, int d, int dp,
                                                            int64_t n, const float* __restrict__ mu, float inv_sigma,
                                                            const uint16_t* __restrict__ w1, const float* __restrict__ b1,
                                                            const uint16_t* __restrict__ w2, const float* __restrict__ b2,
                                                            const uint16_t* __restrict__ w3, const float* __restrict__ b3,
                                                            int nb, int32_t* __restrict__ bins) {
-  __shared__ __attribute__((aligned(16))) uint16_t bufa[kFuseRows * kFuseLd];
-  __shared__ __attribute__((aligned(16))) uint16_t bufb[kFuseRows * kFuseLd];
+  __shared__ __attribute__((aligned(16))) uint16_t buf[kFuseRows * kFuseLd];
+  __shared__ __attribute__((aligned(16))) float smu[kFuseH];
+  __shared__ __attribute__((aligned(16))) float sb[3][kFuseH];  // b1, b2, b3 (zero past nb)
   __shared__ float bval[4][kFuseRows];
   __shared__ int bidx[4][kFuseRows];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int il = lane & 31, hh = lane >> 5;
   const int64_t row0 = static_cast<int64_t>(blockIdx.x) * kFuseRows;
-  // standardised rows into bufb (BF16), zero past d and past n: every
-  // thread's loads (up to 16 x 4 floats) are issued before any is converted
+  // the mean (zero past d) and the biases; dp, nb <= kFuseH = blockDim
+  {
+    const int t = threadIdx.x;
+    if (t < dp) smu[t] = t < d ? mu[t] : 0.0f;
+    sb[0][t] = b1[t];
+    sb[1][t] = b2[t];
+    sb[2][t] = t < nb ? b3[t] : 0.0f;
+  }
+  __syncthreads();
+  // standardised rows into buf (BF16), zero past d and past n: in rounds of
+  // 16 float4 per thread, every load of a round issued before any is converted
   {
     const int q4 = dp / 4;  // 4-float groups per row (dp % 16 == 0, ld % 4 == 0)
-    float4 v[16];
+    const int total = kFuseRows * q4;
+    for (int base = 0; base < total; base += 256 * 16) {
+      float4 v[16];
+      // addresses clamped into the matrix (ld % 4 == 0), so no load sits
+      // behind a branch; the conversion zeroes what lies past n or d
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      const int e = threadIdx.x + 256 * u;
-      const int r = e / q4, c = 4 * (e % q4);
-      const int64_t row = row0 + r;
-      v[u] = (e < kFuseRows * q4 && row < n && c < ld) ? *reinterpret_cast<const float4*>(x + row * ld + c)
-                                                        : make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-#pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      const int e = threadIdx.x + 256 * u;
-      if (e < kFuseRows * q4) {
+      for (int u = 0; u < 16; ++u) {
+        const int e = base + threadIdx.x + 256 * u < total ? base + threadIdx.x + 256 * u : total - 1;
         const int r = e / q4, c = 4 * (e % q4);
-        const float f[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+        const int64_t row = row0 + r < n ? row0 + r : n - 1;
+        v[u] = *reinterpret_cast<const float4*>(x + row * ld + (c < ld ? c : ld - 4));
+      }
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-          bufb[r * kFuseLd + c + j] = (row0 + r < n && c + j < d) ? f2bf((f[j] - mu[c + j]) * inv_sigma) : 0;
+      for (int u = 0; u < 16; ++u) {
+        const int e = base + threadIdx.x + 256 * u;
+        if (e < total) {
+          const int r = e / q4, c = 4 * (e % q4);
+          const float4 m4 = *reinterpret_cast<const float4*>(smu + c);
+          const bool live = row0 + r < n;
+          const float f0 = live && c + 0 < d ? (v[u].x - m4.x) * inv_sigma : 0.0f;
+          const float f1 = live && c + 1 < d ? (v[u].y - m4.y) * inv_sigma : 0.0f;
+          const float f2 = live && c + 2 < d ? (v[u].z - m4.z) * inv_sigma : 0.0f;
+          const float f3 = live && c + 3 < d ? (v[u].w - m4.w) * inv_sigma : 0.0f;
+          *reinterpret_cast<uint2*>(buf + r * kFuseLd + c) = make_uint2(bf16_pack2(f0, f1), bf16_pack2(f2, f3));
+        }
       }
     }
   }
   __syncthreads();
-  f32x16 acc[2][2];
-  fused_layer<2>(bufb, w1, dp, kFuseH, wave, lane, acc);
-  fused_hidden_store<2>(acc, b1, kFuseH, wave, lane, bufa);
+  f32x16 acc[2][kFuseRT];
+  fused_layer<2>(buf, w1, dp, kFuseH, wave, lane, acc);
+  __syncthreads();  // every wave has read the layer's input
+  fused_hidden_store<2>(acc, sb[0], kFuseH, wave, lane, buf);
   __syncthreads();
-  fused_layer<2>(bufa, w2, kFuseH, kFuseH, wave, lane, acc);
-  fused_hidden_store<2>(acc, b2, kFuseH, wave, lane, bufb);
+  fused_layer<2>(buf, w2, kFuseH, kFuseH, wave, lane, acc);
   __syncthreads();
-  fused_layer<CPW3>(bufb, w3, kFuseH, nb, wave, lane, acc);
-  // argmax: per (row tile, register) the 32 lanes of a half share a row
+  fused_hidden_store<2>(acc, sb[1], kFuseH, wave, lane, buf);
+  __syncthreads();
+  fused_layer<CPW3>(buf, w3, kFuseH, nb, wave, lane, acc);
+  // argmax.  Lane (i + 32*hh) holds, for data row rt*32 + i, the columns
+  // ct*32 + 8g + 4hh + q: it visits them in increasing order (strict '>', so
+  // the lowest column wins a tie and NaN never wins), then meets lane ^ 32
+  // (the other four of each eight columns), then the other waves in LDS.
 #pragma unroll
-  for (int rt = 0; rt < 2; ++rt)
+  for (int rt = 0; rt < kFuseRT; ++rt) {
+    float best = -__builtin_inff();
+    int bi = nb;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      float best = -__builtin_inff();
-      int bi = nb;
+    for (int c = 0; c < CPW3; ++c) {
+      const int ct = wave * CPW3 + c;
+      if (ct >= nb / 32) continue;
 #pragma unroll
-      for (int c = 0; c < CPW3; ++c) {
-        const int ct = wave * CPW3 + c;
-        if (ct < nb / 32) {
-          const int col = ct * 32 + (lane & 31);
-          const float v = acc[c][rt][r] + b3[col];
-          if (v > best) {
-            best = v;
-            bi = col;
+      for (int g = 0; g < 4; ++g) {
+        const int n0 = ct * 32 + 8 * g + 4 * hh;
+        const float4 b4 = *reinterpret_cast<const float4*>(&sb[2][n0]);
+        const float u[4] = {acc[c][rt][4 * g] + b4.x, acc[c][rt][4 * g + 1] + b4.y, acc[c][rt][4 * g + 2] + b4.z,
+                            acc[c][rt][4 * g + 3] + b4.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (u[q] > best) {
+            best = u[q];
+            bi = n0 + q;
           }
-        }
-      }
-      for (int o = 1; o < 32; o <<= 1) {
-        const float ob = __shfl_xor(best, o);
-        const int oi = __shfl_xor(bi, o);
-        if (ob > best || (ob == best && oi < bi)) {
-          best = ob;
-          bi = oi;
-        }
-      }
-      if ((lane & 31) == 0) {
-        const int row = rt * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        bval[wave][row] = best;
-        bidx[wave][row] = bi;
       }
     }
+    const float ob = __shfl_xor(best, 32);
+    const int oi = __shfl_xor(bi, 32);
+    if (ob > best || (ob == best && oi < bi)) {
+      best = ob;
+      bi = oi;
+    }
+    if (hh == 0) {
+      bval[wave][rt * 32 + il] = best;
+      bidx[wave][rt * 32 + il] = bi;
+    }
+  }
   __syncthreads();
   if (threadIdx.x < kFuseRows) {
     const int row = threadIdx.x;
@@ -605,7 +694,7 @@ void Router::backward(int64_t mpad, int64_t m, int64_t count, hipStream_t s) {
   g.of = g_ + off_[4];
   g.ldof = h_;
   gemm(g, s);
-  hipLaunchKernelGGL(colsum_kernel, dim3(grid_of(nb_, 64)), dim3(256), 0, s, dz3f_, mpad, nb_, g_ + off_[5]);
+  hipLaunchKernelGGL(colsum_kernel, dim3(grid_of(nb_, 64)), dim3(1024), 0, s, dz3f_, mpad, nb_, g_ + off_[5]);
   // dZ2 = (dZ3 W3) masked by H2 > 0
   g = gemm_args(dz3_, nb_, w3t_, nb_, mp, h_, nb_);
   g.mask = h2_;
@@ -621,7 +710,7 @@ void Router::backward(int64_t mpad, int64_t m, int64_t count, hipStream_t s) {
   g.of = g_ + off_[2];
   g.ldof = h_;
   gemm(g, s);
-  hipLaunchKernelGGL(colsum_kernel, dim3(grid_of(h_, 64)), dim3(256), 0, s, dz2f_, mpad, h_, g_ + off_[3]);
+  hipLaunchKernelGGL(colsum_kernel, dim3(grid_of(h_, 64)), dim3(1024), 0, s, dz2f_, mpad, h_, g_ + off_[3]);
   // dZ1 = (dZ2 W2) masked by H1 > 0
   g = gemm_args(dz2_, h_, w2t_, h_, mp, h_, h_);
   g.mask = h1_;
@@ -635,7 +724,7 @@ void Router::backward(int64_t mpad, int64_t m, int64_t count, hipStream_t s) {
   g.of = g_ + off_[0];
   g.ldof = dp_;
   gemm(g, s);
-  hipLaunchKernelGGL(colsum_kernel, dim3(grid_of(h_, 64)), dim3(256), 0, s, dz1f_, mpad, h_, g_ + off_[1]);
+  hipLaunchKernelGGL(colsum_kernel, dim3(grid_of(h_, 64)), dim3(1024), 0, s, dz1f_, mpad, h_, g_ + off_[1]);
   AKNN_HIP_TRY(hipGetLastError());
 }
 

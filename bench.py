@@ -278,7 +278,9 @@ def main():
         print(f"[bench] chosen/plain {line['chosen_vs_plain']:.3f}, identical graphs: {line.get('forms_identical')}",
               file=sys.stderr, flush=True)
     if router_stats is not None:
-        line["router"] = router_stats
+        line["router"] = dict(router_stats)
+        if "router_assign_ms" in r:
+            line["router"]["assign_ms"] = r["router_assign_ms"]
     if recalls:
         line["recall_at_k"] = recalls[head]
         if "plain" in recalls:

@@ -168,7 +168,7 @@ def test_gpu_fused_bins_equal_layerwise(aknn, d, hidden, bin_bits):
     r = aknn.Router(d, hidden, bin_bits, seed=5)
     p, mu, inv = r.get_params()
     r.set_params(p, rng.normal(0, 0.3, d).astype(np.float32), 0.7)
-    m = 5000 + d  # not a multiple of the 64-row tile
+    m = 5000 + d  # not a multiple of the 128-row tile
     x = rng.normal(0, 1.5, (m, d)).astype(np.float32)
     x[7] = x[3]
     bf = r.bins_of(x, fused=True)
@@ -177,4 +177,6 @@ def test_gpu_fused_bins_equal_layerwise(aknn, d, hidden, bin_bits):
     assert (bf == bl).all()
     assert (bl == np.argmax(z, axis=1)).all()
     assert 0 <= bf.min() and bf.max() < (1 << bin_bits)
+    # fewer rows than one tile
+    assert (r.bins_of(x[:77], fused=True) == bl[:77]).all()
     r.close()
