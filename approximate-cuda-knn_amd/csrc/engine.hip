@@ -297,6 +297,8 @@ void Engine::build_forest_median() {
   iters_.assign(p_.trees, 0);
   active_sum_ = 0;
   stats_forest = KernelStats{};
+  proj_dirs_ = 0;
+  proj_launches_ = 0;
   const int nd = median_depth(n_, leaf_eff_);
   const int64_t key = (static_cast<int64_t>(p_.seed) << 20) ^ (static_cast<int64_t>(p_.trees) << 8) ^ nd;
   if (!dirs_ || nd != nd_ || key != dirs_key_) {
@@ -337,7 +339,11 @@ void Engine::build_forest_median() {
     Leaf* leaves = leaves_ + static_cast<int64_t>(t) * n_;
     const bool routed = t < routed_trees;
     const int bb = routed ? bin_bits_ : 0;
-    launch_project(x_, n_, d_, ld_, dirs_ + static_cast<size_t>(t) * nd * ld_, nd, proj_, stream_);
+    // projections are made 16 directions at a time, the first time a level
+    // needs one: routed trees use only the directions below their depth
+    // past the bin levels
+    const float* tdirs = dirs_ + static_cast<size_t>(t) * nd * ld_;
+    int projected = 0;
     launch_forest_init(fb_, perm, leaves, n_, 0, 0, leaf_eff_, p_.k + 1, stream_);
     int level = 0;
     for (;;) {
@@ -347,6 +353,14 @@ void Engine::build_forest_median() {
           launch_forest_level_rule(fb_, 0, nullptr, tb, bb - 1 - level, perm, leaves, level, leaf_eff_, p_.k + 1,
                                    stream_);
         } else if (level - bb < nd) {
+          if (level - bb >= projected) {
+            const int cnt = std::min(kProjectStep, nd - projected);
+            launch_project(x_, n_, d_, ld_, tdirs + static_cast<size_t>(projected) * ld_, cnt,
+                           proj_ + static_cast<int64_t>(projected) * n_, stream_);
+            projected += cnt;
+            proj_dirs_ += cnt;
+            ++proj_launches_;
+          }
           launch_forest_level_rule(fb_, 1, proj_ + static_cast<int64_t>(level - bb) * n_, nullptr, 0, perm, leaves,
                                    level, leaf_eff_, p_.k + 1, stream_);
         } else {
@@ -365,8 +379,9 @@ void Engine::build_forest_median() {
   // algorithmic bytes: one pass over the rows and the projection writes per
   // tree, then per level and active point its projection, id and bucket
   // bookkeeping, plus the leaf array write
-  stats_forest.units = static_cast<double>(p_.trees) * n_ * (4.0 * d_ + 4.0 * nd) +
-                       static_cast<double>(active_sum_) * 20.0 + 8.0 * n_ * p_.trees;
+  stats_forest.units = static_cast<double>(proj_launches_) * n_ * 4.0 * d_ +
+                       static_cast<double>(proj_dirs_) * n_ * 4.0 + static_cast<double>(active_sum_) * 20.0 +
+                       8.0 * n_ * p_.trees;
 }
 
 void Engine::initial_knn() {

@@ -199,6 +199,8 @@ class Engine {
   float* proj_ = nullptr;
   int nd_ = 0;
   int64_t dirs_key_ = -1;
+  int64_t proj_dirs_ = 0;      // directions projected in the last median forest
+  int64_t proj_launches_ = 0;  // project_kernel launches (passes over the rows)
   int32_t* bins_ = nullptr;
   int bin_bits_ = 0;
   int bins_cap_ = 0;  // bin arrays bins_ holds

@@ -454,55 +454,69 @@ __device__ __forceinline__ uint32_t ord_u32(float f) {
   return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 
-constexpr int kProjGroup = 16;   // directions per pass over a row
-constexpr int kProjChunk = 256;  // dimensions staged in LDS per step
+constexpr int kProjGroup = kProjectStep;  // directions per launch (one pass over the rows)
+constexpr int kProjChunk = 32;  // dimensions staged in LDS per step
+constexpr int kProjXs = kProjChunk + 4;  // LDS row stride (floats): 16-byte reads of 8 lanes hit distinct banks
 
 // P[j * n + i] = x_i . r_j as one FMA chain in dimension order (FP32, the
-// oracle's chain), for j < nd.  One lane per point; the directions of a group
-// are staged through LDS a chunk of dimensions at a time and read as
-// broadcasts; each lane keeps kProjGroup running chains.
+// oracle's chain), for j < nd <= kProjGroup.  One lane per point, 256 points
+// per block: each step stages the block's rows for 32 dimensions in LDS with
+// coalesced 16-byte loads (8 lanes per 128-byte row piece), and the
+// directions' 32 dimensions beside them (read as broadcasts); each lane then
+// advances its kProjGroup chains from its own staged row.
 __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ x, int64_t n, int d, int ld,
                                                       const float* __restrict__ dirs, int nd,
                                                       float* __restrict__ proj) {
+  __shared__ __attribute__((aligned(16))) float xs[256 * kProjXs];
   __shared__ __attribute__((aligned(16))) float rs[kProjGroup][kProjChunk];
-  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
-  const bool live = i < n;
-  const float* xr = x + (live ? i : 0) * ld;
-  for (int g0 = 0; g0 < nd; g0 += kProjGroup) {
-    float acc[kProjGroup];
+  const int t = threadIdx.x;
+  const int64_t i0 = blockIdx.x * 256ll;
+  float acc[kProjGroup];
 #pragma unroll
-    for (int g = 0; g < kProjGroup; ++g) acc[g] = 0.0f;
-    for (int c0 = 0; c0 < d; c0 += kProjChunk) {
-      const int cw = d - c0 < kProjChunk ? d - c0 : kProjChunk;
-      __syncthreads();
-      for (int e = threadIdx.x; e < kProjGroup * kProjChunk; e += 256) {
-        const int g = e / kProjChunk, c = e % kProjChunk;
-        rs[g][c] = (g0 + g < nd && c < cw) ? dirs[static_cast<int64_t>(g0 + g) * ld + c0 + c] : 0.0f;
-      }
-      __syncthreads();
-      int c = 0;
-      for (; c + 4 <= cw; c += 4) {
-        const float4 v = *reinterpret_cast<const float4*>(xr + c0 + c);
+  for (int g = 0; g < kProjGroup; ++g) acc[g] = 0.0f;
+  const float* xr = xs + t * kProjXs;
+  for (int c0 = 0; c0 < d; c0 += kProjChunk) {
+    const int cw = d - c0 < kProjChunk ? d - c0 : kProjChunk;
+    __syncthreads();
+    // rows i0.. (clamped into the matrix), float4 pieces past d left unread:
+    // ld % 4 == 0 and ld >= d, so a piece that starts below d stays in the row
 #pragma unroll
-        for (int g = 0; g < kProjGroup; ++g) {
-          const float4 r = *reinterpret_cast<const float4*>(&rs[g][c]);
-          acc[g] = __builtin_fmaf(v.x, r.x, acc[g]);
-          acc[g] = __builtin_fmaf(v.y, r.y, acc[g]);
-          acc[g] = __builtin_fmaf(v.z, r.z, acc[g]);
-          acc[g] = __builtin_fmaf(v.w, r.w, acc[g]);
-        }
-      }
-      for (; c < cw; ++c) {
-        const float v = xr[c0 + c];
+    for (int u = 0; u < 256 * kProjChunk / 4 / 256; ++u) {
+      const int e = t + 256 * u;
+      const int r = e / (kProjChunk / 4), q = e % (kProjChunk / 4);
+      const int64_t row = i0 + r < n ? i0 + r : n - 1;
+      const int c = c0 + 4 * q;
+      const float4 v = c < d ? *reinterpret_cast<const float4*>(x + row * ld + c) : make_float4(0.f, 0.f, 0.f, 0.f);
+      *reinterpret_cast<float4*>(xs + r * kProjXs + 4 * q) = v;
+    }
+    for (int e = t; e < kProjGroup * kProjChunk; e += 256) {
+      const int g = e / kProjChunk, c = e % kProjChunk;
+      rs[g][c] = (g < nd && c < cw) ? dirs[static_cast<int64_t>(g) * ld + c0 + c] : 0.0f;
+    }
+    __syncthreads();
+    int c = 0;
+    for (; c + 4 <= cw; c += 4) {
+      const float4 v = *reinterpret_cast<const float4*>(xr + c);
 #pragma unroll
-        for (int g = 0; g < kProjGroup; ++g) acc[g] = __builtin_fmaf(v, rs[g][c], acc[g]);
+      for (int g = 0; g < kProjGroup; ++g) {
+        const float4 r = *reinterpret_cast<const float4*>(&rs[g][c]);
+        acc[g] = __builtin_fmaf(v.x, r.x, acc[g]);
+        acc[g] = __builtin_fmaf(v.y, r.y, acc[g]);
+        acc[g] = __builtin_fmaf(v.z, r.z, acc[g]);
+        acc[g] = __builtin_fmaf(v.w, r.w, acc[g]);
       }
     }
-    if (live) {
+    for (; c < cw; ++c) {
+      const float v = xr[c];
 #pragma unroll
-      for (int g = 0; g < kProjGroup; ++g)
-        if (g0 + g < nd) proj[static_cast<int64_t>(g0 + g) * n + i] = acc[g];
+      for (int g = 0; g < kProjGroup; ++g) acc[g] = __builtin_fmaf(v, rs[g][c], acc[g]);
     }
+  }
+  const int64_t i = i0 + t;
+  if (i < n) {
+#pragma unroll
+    for (int g = 0; g < kProjGroup; ++g)
+      if (g < nd) proj[static_cast<int64_t>(g) * n + i] = acc[g];
   }
 }
 
@@ -675,6 +689,7 @@ void launch_project(const float* x, int64_t n, int d, int ld, const float* dirs,
                     hipStream_t s) {
   if (nd <= 0 || n <= 0) return;
   if (ld % 4 != 0 || ld < d) throw HipError("project: malformed row width");
+  if (nd > kProjGroup) throw HipError("project: at most 16 directions per launch");
   hipLaunchKernelGGL(project_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, x, n, d, ld, dirs,
                      nd, proj);
   AKNN_HIP_TRY(hipGetLastError());
