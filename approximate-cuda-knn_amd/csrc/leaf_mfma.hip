@@ -179,9 +179,10 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
   o_t[lane] = owner_threshold<METRIC>(thk, my_eps, a.kap);
   __syncthreads();
   const int32_t oa = oid[col], ob = oid[32 + col];
-  const _Float16* arow0 = oa >= 0 ? a.xh + static_cast<int64_t>(oa) * a.ldh : nullptr;
-  const _Float16* arow1 = ob >= 0 ? a.xh + static_cast<int64_t>(ob) * a.ldh : nullptr;
+  const _Float16* arow0s = a.xh + static_cast<int64_t>(oa >= 0 ? oa : 0) * a.ldh;
+  const _Float16* arow1s = a.xh + static_cast<int64_t>(ob >= 0 ? ob : 0) * a.ldh;
   const int steps = (a.ldh + 15) / 16;
+  const bool ktail = (a.ldh % 16) != 0;  // the last step's upper half lies past the row
   unsigned long long nres = 0;
 
   for (int cb = 0; cb < L.rcount; cb += kTile) {
@@ -200,75 +201,108 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
     survm[lane] = 0;
     __syncthreads();
 
-    // 1-2. one 32x32 quadrant (owners 32*ot.., candidates 32*ct..) at a
-    //      time: dot products of the sketches on the matrix cores, then the
-    //      screen of the quadrant.  This lane holds candidate column
-    //      j = 32*ct + col against owner rows i = 32*ot + 8*g + 4*half + q
-    //      (register 4*g + q).  One quadrant's accumulators are live at a time.
+    // 1-2. the owner tiles ot = 0, 1 in turn, each against both candidate
+    //      tiles: dot products of the sketches on the matrix cores, four K
+    //      steps of fragments loaded before their MFMAs (one A fragment feeds
+    //      both candidate tiles), then the screen of the two quadrants.  This
+    //      lane holds candidate column j = 32*ct + col against owner rows
+    //      i = 32*ot + 8*g + 4*half + q (register 4*g + q).  Loads are never
+    //      behind a branch: rows of absent owners or candidates read row 0
+    //      (their sums are never screened in), and the K tail past ldh is
+    //      zeroed.
+    const _Float16* bsafe0 = a.xh + static_cast<int64_t>(cid[col] >= 0 ? cid[col] : 0) * a.ldh;
+    const _Float16* bsafe1 = a.xh + static_cast<int64_t>(cid[32 + col] >= 0 ? cid[32 + col] : 0) * a.ldh;
 #pragma unroll 1
-    for (int quad = 0; quad < 4; ++quad) {
-      const int ot = quad >> 1, ct = quad & 1;
-      const int j = 32 * ct + col;
-      const int32_t cj = cid[j];
-      const float cn = c_n[j], ce = c_e[j];
-      const float cs = METRIC == kCosine ? c_s[j] : 0.0f;
-      const int32_t pos = L.rstart + cb + j;
-      const _Float16* arow = ot == 0 ? arow0 : arow1;
-      const _Float16* brow = cj >= 0 ? a.xh + static_cast<int64_t>(cj) * a.ldh : nullptr;
-      f32x16 acc;
+    for (int ot = 0; ot < 2; ++ot) {
+      const _Float16* asafe = ot == 0 ? arow0s : arow1s;
+      f32x16 acc2[2];
 #pragma unroll
-      for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
-#pragma unroll 4
-      for (int s = 0; s < steps; ++s) {
-        const int kk = 16 * s + 8 * half;
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(piece(arow, kk, a.ldh), piece(brow, kk, a.ldh), acc, 0, 0, 0);
+      for (int r = 0; r < 16; ++r) {
+        acc2[0][r] = 0.0f;
+        acc2[1][r] = 0.0f;
+      }
+      for (int s0 = 0; s0 < steps; s0 += 4) {
+        uint4 av[4], bv0[4], bv1[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int kk = 16 * (s0 + u) + 8 * half;
+          const int kc = kk < a.ldh ? kk : a.ldh - 8;
+          av[u] = *reinterpret_cast<const uint4*>(asafe + kc);
+          bv0[u] = *reinterpret_cast<const uint4*>(bsafe0 + kc);
+          bv1[u] = *reinterpret_cast<const uint4*>(bsafe1 + kc);
+        }
+        if (ktail && s0 + 4 >= steps) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (16 * (s0 + u) + 8 * half >= a.ldh) {
+              av[u] = make_uint4(0u, 0u, 0u, 0u);
+              bv0[u] = av[u];
+              bv1[u] = av[u];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (s0 + u >= steps) break;
+          const half8 ah = __builtin_bit_cast(half8, av[u]);
+          acc2[0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, __builtin_bit_cast(half8, bv0[u]), acc2[0], 0, 0, 0);
+          acc2[1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, __builtin_bit_cast(half8, bv1[u]), acc2[1], 0, 0, 0);
+        }
       }
 #pragma unroll
-      for (int g = 0; g < 4; ++g) {
-        const int i0 = 32 * ot + 8 * g + 4 * half;
-        const float4 on4 = *reinterpret_cast<const float4*>(&o_n[i0]);
-        const float4 to4 = *reinterpret_cast<const float4*>(&o_t[i0]);
-        const int4 oi4 = *reinterpret_cast<const int4*>(&oid[i0]);
-        const int4 lo4 = *reinterpret_cast<const int4*>(&o_lo[i0]);
-        const int4 hi4 = *reinterpret_cast<const int4*>(&o_hi[i0]);
-        const int32_t olo[4] = {lo4.x, lo4.y, lo4.z, lo4.w};
-        const int32_t ohi[4] = {hi4.x, hi4.y, hi4.z, hi4.w};
-        const float on[4] = {on4.x, on4.y, on4.z, on4.w};
-        const float to[4] = {to4.x, to4.y, to4.z, to4.w};
-        const int32_t oi[4] = {oi4.x, oi4.y, oi4.z, oi4.w};
-        float oe[4] = {0.0f, 0.0f, 0.0f, 0.0f}, os[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        if constexpr (METRIC == kCosine) {
-          const float4 oe4 = *reinterpret_cast<const float4*>(&o_e[i0]);
-          const float4 os4 = *reinterpret_cast<const float4*>(&o_s[i0]);
-          oe[0] = oe4.x; oe[1] = oe4.y; oe[2] = oe4.z; oe[3] = oe4.w;
-          os[0] = os4.x; os[1] = os4.y; os[2] = os4.z; os[3] = os4.w;
-        }
-        uint32_t hit = 0;
+      for (int ct = 0; ct < 2; ++ct) {
+        const f32x16& acc = acc2[ct];
+        const int j = 32 * ct + col;
+        const int32_t cj = cid[j];
+        const float cn = c_n[j], ce = c_e[j];
+        const float cs = METRIC == kCosine ? c_s[j] : 0.0f;
+        const int32_t pos = L.rstart + cb + j;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          bool keep;
-          if constexpr (METRIC == kEuclidean) {
-            const float s_lo = __builtin_fmaf(-2.0f, acc[4 * g + q], (on[q] + cn) * a.c1m);
-            const float t = (to[q] + ce) * 1.0000002384185791f;                    // (1 + 4u)
-            const float t2 = __builtin_fmaf(t, t, 1e-30f) * 1.0000004768371582f;   // (1 + 8u)
-            keep = !(s_lo > t2);
-          } else {
-            // reject when up(G) + rounding margin < T_o * den * (1 - 4u)
-            const float G = acc[4 * g + q];
-            const float A = on[q], B = cn, eo = oe[q];
-            const float Ao = A + eo;
-            const float P = a.kmf * Ao * (B + ce) + eo * B + Ao * ce + a.kabf * A * B;
-            const float lhs = G + P + 4.76837158203125e-07f * (__builtin_fabsf(G) + P + A * B) + 1e-30f;
-            const float den = os[q] * cs;
-            const float rhs = to[q] * den * 0.99999976158142090f - 1e-30f;  // (1 - 4u)
-            keep = !(den > 0.0f && den <= 3.4e38f && lhs < rhs);
+        for (int g = 0; g < 4; ++g) {
+          const int i0 = 32 * ot + 8 * g + 4 * half;
+          const float4 on4 = *reinterpret_cast<const float4*>(&o_n[i0]);
+          const float4 to4 = *reinterpret_cast<const float4*>(&o_t[i0]);
+          const int4 oi4 = *reinterpret_cast<const int4*>(&oid[i0]);
+          const int4 lo4 = *reinterpret_cast<const int4*>(&o_lo[i0]);
+          const int4 hi4 = *reinterpret_cast<const int4*>(&o_hi[i0]);
+          const int32_t olo[4] = {lo4.x, lo4.y, lo4.z, lo4.w};
+          const int32_t ohi[4] = {hi4.x, hi4.y, hi4.z, hi4.w};
+          const float on[4] = {on4.x, on4.y, on4.z, on4.w};
+          const float to[4] = {to4.x, to4.y, to4.z, to4.w};
+          const int32_t oi[4] = {oi4.x, oi4.y, oi4.z, oi4.w};
+          float oe[4] = {0.0f, 0.0f, 0.0f, 0.0f}, os[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+          if constexpr (METRIC == kCosine) {
+            const float4 oe4 = *reinterpret_cast<const float4*>(&o_e[i0]);
+            const float4 os4 = *reinterpret_cast<const float4*>(&o_s[i0]);
+            oe[0] = oe4.x; oe[1] = oe4.y; oe[2] = oe4.z; oe[3] = oe4.w;
+            os[0] = os4.x; os[1] = os4.y; os[2] = os4.z; os[3] = os4.w;
           }
-          hit |= static_cast<uint32_t>(oi[q] >= 0 && cj != oi[q] && pos >= olo[q] && pos < ohi[q] && keep) << q;
-        }
-        if (cj >= 0 && hit != 0) {
+          uint32_t hit = 0;
 #pragma unroll
-          for (int q = 0; q < 4; ++q)
-            if (hit & (1u << q)) atomicOr(reinterpret_cast<unsigned long long*>(&survm[i0 + q]), 1ull << j);
+          for (int q = 0; q < 4; ++q) {
+            bool keep;
+            if constexpr (METRIC == kEuclidean) {
+              const float s_lo = __builtin_fmaf(-2.0f, acc[4 * g + q], (on[q] + cn) * a.c1m);
+              const float t = (to[q] + ce) * 1.0000002384185791f;                    // (1 + 4u)
+              const float t2 = __builtin_fmaf(t, t, 1e-30f) * 1.0000004768371582f;   // (1 + 8u)
+              keep = !(s_lo > t2);
+            } else {
+              // reject when up(G) + rounding margin < T_o * den * (1 - 4u)
+              const float G = acc[4 * g + q];
+              const float A = on[q], B = cn, eo = oe[q];
+              const float Ao = A + eo;
+              const float P = a.kmf * Ao * (B + ce) + eo * B + Ao * ce + a.kabf * A * B;
+              const float lhs = G + P + 4.76837158203125e-07f * (__builtin_fabsf(G) + P + A * B) + 1e-30f;
+              const float den = os[q] * cs;
+              const float rhs = to[q] * den * 0.99999976158142090f - 1e-30f;  // (1 - 4u)
+              keep = !(den > 0.0f && den <= 3.4e38f && lhs < rhs);
+            }
+            hit |= static_cast<uint32_t>(oi[q] >= 0 && cj != oi[q] && pos >= olo[q] && pos < ohi[q] && keep) << q;
+          }
+          if (cj >= 0 && hit != 0) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+              if (hit & (1u << q)) atomicOr(reinterpret_cast<unsigned long long*>(&survm[i0 + q]), 1ull << j);
+          }
         }
       }
     }
