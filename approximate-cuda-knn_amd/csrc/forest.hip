@@ -152,16 +152,25 @@ __global__ __launch_bounds__(256) void forest_side_kernel(const float* __restric
 // at most ||w|| (eps + kap (||xh|| + eps)) (the sketch radius and the two
 // chains' rounding), so |s'| above that bound fixes the sign of s: +1 RIGHT
 // (s > 0), -1 LEFT (s < 0).  ||w|| is bounded from the same loop.
-__device__ __forceinline__ int sketch_side(const float* __restrict__ x, int ld, const _Float16* __restrict__ xh,
-                                           int ldh, float eo, float no, int32_t id, int32_t p1, int32_t p2,
-                                           const SketchBound& sb) {
+//
+// The four lanes q of a quad share one point: lane q takes the 8-dimension
+// pieces q, q+4, q+8, ... of the rows, so consecutive lanes read consecutive
+// bytes of the sketch row and of both pivot rows, and the partial sums meet by
+// two xor shuffles (every lane of the quad then holds the same sums and
+// decides alike).  Each partial is a chain of at most ld/4 + 1 steps and the
+// meeting adds two more roundings, within the d + 8 that kap covers, so the
+// bound holds for this order too.
+__device__ __forceinline__ int sketch_side_quad(const float* __restrict__ x, int ld, const _Float16* __restrict__ xh,
+                                                int ldh, float eo, float no, int32_t id, int32_t p1, int32_t p2,
+                                                const SketchBound& sb, int q) {
   const _Float16* hc = xh + static_cast<int64_t>(id) * ldh;
   const float* xa = x + static_cast<int64_t>(p1) * ld;
   const float* xb = x + static_cast<int64_t>(p2) * ld;
   float sp = 0.0f, ww = 0.0f;
-  int dd = 0;
+  const int pieces = ld / 8;
 #pragma unroll 4
-  for (; dd + 8 <= ld; dd += 8) {
+  for (int pc = q; pc < pieces; pc += 4) {
+    const int dd = 8 * pc;
     const uint4 h = *reinterpret_cast<const uint4*>(hc + dd);
     const float4 a0 = *reinterpret_cast<const float4*>(xa + dd);
     const float4 a1 = *reinterpret_cast<const float4*>(xa + dd + 4);
@@ -177,7 +186,8 @@ __device__ __forceinline__ int sketch_side(const float* __restrict__ x, int ld, 
       ww = __builtin_fmaf(w[j], w[j], ww);
     }
   }
-  if (dd < ld) {  // ld % 8 == 4: one group of four
+  if (ld % 8 != 0 && q == pieces % 4) {  // ld % 8 == 4: one group of four, taken by the next lane in turn
+    const int dd = 8 * pieces;
     const uint2 h = *reinterpret_cast<const uint2*>(hc + dd);
     const float4 a0 = *reinterpret_cast<const float4*>(xa + dd);
     const float4 b0 = *reinterpret_cast<const float4*>(xb + dd);
@@ -189,6 +199,10 @@ __device__ __forceinline__ int sketch_side(const float* __restrict__ x, int ld, 
       ww = __builtin_fmaf(w[j], w[j], ww);
     }
   }
+  sp += __shfl_xor(sp, 1);
+  ww += __shfl_xor(ww, 1);
+  sp += __shfl_xor(sp, 2);
+  ww += __shfl_xor(ww, 2);
   const double wn = sqrt(static_cast<double>(ww)) * (1.0 + sb.kap);
   const double xn = sqrt(static_cast<double>(no)) * (1.0 + sb.kap);
   const double e = eo;
@@ -199,8 +213,10 @@ __device__ __forceinline__ int sketch_side(const float* __restrict__ x, int ld, 
 }
 
 // Side kernel with the sketch: one pass decides most points from their half
-// rows; the undecided ones of a 1024-point block are queued in LDS and take
-// the exact chain together, so a straggler does not hold a whole wave.
+// rows, a quad of lanes per point so the rows are read coalesced; the
+// decisions become the block's ballot words through LDS.  The undecided ones
+// of a 1024-point block are queued in LDS and take the exact chain together,
+// so a straggler does not hold a whole wave.
 __global__ __launch_bounds__(256) void forest_side_sketch_kernel(
     const float* __restrict__ x, int d, int ld, const _Float16* __restrict__ xh, int ldh,
     const float* __restrict__ eps, const float* __restrict__ nh, SketchBound sb, const int32_t* __restrict__ act_ids,
@@ -210,35 +226,40 @@ __global__ __launch_bounds__(256) void forest_side_sketch_kernel(
   __shared__ uint64_t bits[kWordsPerBlock];
   __shared__ int wcnt[kWordsPerBlock];
   __shared__ int16_t amb[kSideBlock];
+  __shared__ int8_t dec[kSideBlock];  // 0 LEFT, 1 RIGHT, 2 undecided
   __shared__ int namb;
   const int nact = ctl->nact[parity];
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
+  const int quad = threadIdx.x >> 2, q = threadIdx.x & 3;
   for (int64_t blk = blockIdx.x; blk * kSideBlock < nact; blk += gridDim.x) {
     if (threadIdx.x == 0) namb = 0;
     __syncthreads();
-    for (int sub = 0; sub < kSideBlock / 256; ++sub) {
-      const int ml = sub * 256 + threadIdx.x;
+    // one point per quad at a time (64 quads, 16 points each)
+    for (int t = 0; t < kSideBlock / 64; ++t) {
+      const int ml = t * 64 + quad;
       const int64_t m = blk * kSideBlock + ml;
-      bool right = false;
+      int r = 0;
       if (m < nact) {
         const Bucket& b = bkt[act_bkt[m]];
         const int32_t rel = static_cast<int32_t>(m) - b.astart;
         if (rel == b.q2) {
-          right = true;  // Leaf2 writes RIGHT for p2 last (.cu:2178-2179)
+          r = 1;  // Leaf2 writes RIGHT for p2 last (.cu:2178-2179)
         } else if (rel != b.q1) {
           const int32_t id = act_ids[m];
-          const int r = sketch_side(x, ld, xh, ldh, eps[id], nh[id], id, act_ids[b.astart + b.q1],
-                                    act_ids[b.astart + b.q2], sb);
-          if (r > 0) {
-            right = true;
-          } else if (r == 0) {
-            const int pos = atomicAdd(&namb, 1);
-            amb[pos] = static_cast<int16_t>(ml);
-          }
+          const int sd = sketch_side_quad(x, ld, xh, ldh, eps[id], nh[id], id, act_ids[b.astart + b.q1],
+                                          act_ids[b.astart + b.q2], sb, q);
+          r = sd > 0 ? 1 : (sd == 0 ? 2 : 0);
         }
       }
-      const uint64_t bal = __ballot(right);
+      if (q == 0) {
+        dec[ml] = static_cast<int8_t>(r);
+        if (r == 2) amb[atomicAdd(&namb, 1)] = static_cast<int16_t>(ml);
+      }
+    }
+    __syncthreads();
+    for (int sub = 0; sub < kSideBlock / 256; ++sub) {
+      const uint64_t bal = __ballot(dec[sub * 256 + threadIdx.x] == 1);
       if (lane == 0) bits[sub * 4 + wave] = bal;
     }
     __syncthreads();
@@ -649,7 +670,7 @@ void launch_forest_level(const ForestBuffers& fb, const float* x, int d, int ld,
   const int par = level & 1;
   const dim3 sg(clamp_grid((fb.n + kSideBlock - 1) / kSideBlock, 2048));
   if (fb.xh != nullptr) {
-    if (fb.eps == nullptr || fb.nh == nullptr || fb.ldh % 8 != 0 || fb.ldh < d || ld % 4 != 0)
+    if (fb.eps == nullptr || fb.nh == nullptr || fb.ldh % 8 != 0 || fb.ldh < ld || ld < d || ld % 4 != 0)
       throw HipError("forest: malformed sketch");
     hipLaunchKernelGGL(forest_side_sketch_kernel, sg, dim3(256), 0, s, x, d, ld, fb.xh, fb.ldh, fb.eps, fb.nh,
                        sketch_bound(d), fb.act_ids[par], fb.act_bkt[par], fb.bkt[par], fb.ctl, par, fb.rmask, fb.wpre,
