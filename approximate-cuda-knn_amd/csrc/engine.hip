@@ -686,7 +686,7 @@ void Engine::calculate() {
   const bool sketch = p_.sketch || leaf_mfma_active();
   if (sketch && !xh_) {
     xh_ = alloc<_Float16>(static_cast<size_t>(n_) * ldh_);
-    eps_ = alloc<float>(n_);
+    eps_ = alloc<float>(n_ + 1);  // [n]: the largest radius
     nh_ = alloc<float>(n_);
     // leaf re-scored, forest undecided, leaf screened: kSlotWords each
     leaf_rescored_ = alloc<unsigned long long>(3 * static_cast<size_t>(kSlotWords));
@@ -707,7 +707,10 @@ void Engine::calculate() {
   AKNN_HIP_TRY(hipEventRecord(ev(0), stream_));
   launch_normalize(x_, n_, d_, ld_, mean_, gmax_, p_.scale_all != 0, stream_);
   if (norms_) launch_norms(x_, n_, d_, ld_, norms_, stream_);
-  if (sketch) launch_sketch(x_, n_, d_, ld_, kmetric(), xh_, ldh_, eps_, nh_, stream_);
+  if (sketch) {
+    launch_sketch(x_, n_, d_, ld_, kmetric(), xh_, ldh_, eps_, nh_, stream_);
+    launch_max_nonneg(eps_, n_, eps_ + n_, stream_);
+  }
   if (sketch8) launch_sketch8(x_, n_, d_, ld_, q8_, ld8_, s8_, a8_, e8_, stream_);
   pca_ = PcaBasis{};
   if (p_.pca_screen && p_.metric == kEuclidean && p_.props > 0 && d_ >= 16 && d_ <= 256) {
@@ -717,13 +720,14 @@ void Engine::calculate() {
       pca_mom_ = alloc<float>(static_cast<size_t>(33) * d_ * d_);
       pca_p_ = alloc<float>(static_cast<size_t>(32) * d_);
       pca_proj_ = alloc<float>(static_cast<size_t>(n_) * 32);
-      pca_e_ = alloc<float>(n_);
+      pca_e_ = alloc<float>(n_ + 1);  // [n]: the largest radius
     }
     pca_ = compute_pca_basis(x_, n_, d_, ld_, pca_mom_, stream_);
     if (pca_.r > 0) {
       AKNN_HIP_TRY(hipMemcpyAsync(pca_p_, pca_.p.data(), sizeof(float) * pca_.p.size(), hipMemcpyHostToDevice,
                                   stream_));
       launch_pca_project(x_, n_, d_, ld_, pca_p_, pca_.r, pca_proj_, pca_e_, stream_);
+      launch_max_nonneg(pca_e_, n_, pca_e_ + n_, stream_);
     }
   }
   AKNN_HIP_TRY(hipEventRecord(ev(1), stream_));
@@ -770,15 +774,19 @@ void Engine::calculate() {
     stats_propagate.total_ms += ms;
     const double k = p_.k;
     // per point: own row (8k), k neighbour id rows (4k^2), x_i (4D), output
-    // row (8k); per unique new candidate its row (4D), or with the sketch its
-    // sketch row and radius (2 ldh + 4) plus 4D for each exactly re-scored one
+    // row (8k); per unique new candidate its row (4D), or with a screen its
+    // sketch row (2 ldh; COSINE also its norm) or projection (4r), plus 4D for
+    // each exactly re-scored one.  The screens bound every candidate's radius
+    // by the largest, read once per point; the int8 screen reads three
+    // scalars per candidate.
     double per_point = 16.0 * k + 4.0 * k * k + 4.0 * d_;
     double cand = 4.0 * d_ * nc[2 * r];
     if (round_screened_[r]) {
       const bool i8 = p_.sketch_int8 && p_.metric == kEuclidean;
-      const double row = round_screened_[r] == 2 ? 4.0 * pca_.r + 4.0  // projection + radius
-                                                 : (i8 ? ld8_ + 12.0 : 2.0 * ldh_ + 4.0);  // sketch row + scalars
-      per_point += row;
+      const double row = round_screened_[r] == 2 ? 4.0 * pca_.r
+                                                 : (i8 ? ld8_ + 12.0
+                                                       : 2.0 * ldh_ + (p_.metric == kCosine ? 4.0 : 0.0));
+      per_point += row + 8.0;  // own sketch row or projection, own radius and the radius bound
       cand = row * nc[2 * r] + 4.0 * d_ * nc[2 * r + 1];
     }
     stats_propagate.units += static_cast<double>(n_) * per_point + cand;

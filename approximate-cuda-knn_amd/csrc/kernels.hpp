@@ -145,6 +145,10 @@ PcaBasis compute_pca_basis(const float* x, int64_t n, int d, int ld, float* mome
 // proj [n][r] = fl(p x_i), rad[n] >= ||proj_i - p x_i||.
 void launch_pca_project(const float* x, int64_t n, int d, int ld, const float* p, int r, float* proj, float* rad,
                         hipStream_t s);
+// *out = max of n non-negative floats (+inf allowed).  The screens' radius
+// arrays keep this at index n: one upper bound for every candidate, read
+// once per wave instead of one scattered 4-byte load per candidate.
+void launch_max_nonneg(const float* v, int64_t n, float* out, hipStream_t s);
 
 // ---- FP16 sketch of the normalized points -----------------------------------
 // xh: [n][ldh] halves (ldh % 8 == 0, ldh >= d); eps[n]: upper bounds of the

@@ -178,6 +178,12 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
       // screen: survivors are written back to cand[] in order (slot
       // w + rank <= base + lane was already read by this batch)
       const float ei = SKM == 2 ? e8[i] : (SKM == 3 ? pca_e[i] : eps[i]);
+      // every candidate's radius is bounded by the largest (kept at index n):
+      // one load per wave instead of a scattered 4-byte load per candidate.
+      // An infinite maximum (a row the sketch cannot hold) would switch the
+      // screen off for all; then each candidate's own radius is read.
+      const float ec_max = SKM == 3 ? pca_e[n] : (SKM == 1 ? eps[n] : 0.0f);
+      const bool per_cand = !(ec_max < __builtin_inff());
       const double si = SKM == 2 ? static_cast<double>(s8[i]) : 0.0;
       const double ti = SKM == 2 ? si * si * static_cast<double>(a8[i]) : 0.0;
       int w = 0;
@@ -189,7 +195,6 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
         for (int base = 0; base < m; base += 128) {
           int32_t cc[2];
           float4 pv[2][8];
-          float ec[2];
 #pragma unroll
           for (int u = 0; u < 2; ++u) {
             const int ci = base + 64 * u + lane;
@@ -197,7 +202,6 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
             const float4* pc = reinterpret_cast<const float4*>(pca + static_cast<int64_t>(cc[u] >= 0 ? cc[u] : 0) * pca_r);
 #pragma unroll
             for (int g = 0; g < 8; ++g) pv[u][g] = g < pca_r / 4 ? pc[g] : make_float4(0.f, 0.f, 0.f, 0.f);
-            ec[u] = pca_e[cc[u] >= 0 ? cc[u] : 0];
           }
           const float4* pq = reinterpret_cast<const float4*>(xs);
 #pragma unroll
@@ -220,7 +224,8 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
             // ||x - y|| >= (||p(x) - p(y)|| - e(x) - e(y)) / sigma, with the
             // r-term chain's rounding taken off; then the exact chain's error
             const double sp = fmax(0.0, static_cast<double>(p) * k1 - sb.tiny);
-            const double r = fmax(0.0, sqrt(sp) - static_cast<double>(ei) - static_cast<double>(ec[u])) / pca_sigma;
+            const float ec = per_cand ? pca_e[cc[u] >= 0 ? cc[u] : 0] : ec_max;
+            const double r = fmax(0.0, sqrt(sp) - static_cast<double>(ei) - static_cast<double>(ec)) / pca_sigma;
             const double lb = sqrt(fmax(0.0, r * r * k1 - sb.tiny)) * k1;
             const bool pass = cc[u] >= 0 && !(lb > static_cast<double>(thk));
             const uint64_t bal = __ballot(pass);
@@ -274,7 +279,8 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
             lb = sqrt(fmax(0.0, r * r * k1 - sb.tiny)) * k1;
           } else {
             const float p = sketch_chain<METRIC>(xs, xh + static_cast<int64_t>(c) * ldh, ldh);
-            lb = sketch_lower_bound<METRIC>(p, ei, eps[c], ni, METRIC == kCosine ? norms[c] : 1.0f, sb);
+            lb = sketch_lower_bound<METRIC>(p, ei, per_cand ? eps[c] : ec_max, ni, METRIC == kCosine ? norms[c] : 1.0f,
+                                            sb);
           }
           pass = !(lb > static_cast<double>(thk));
         }

@@ -131,6 +131,37 @@ void launch_sketch8(const float* x, int64_t n, int d, int ld, int8_t* q8, int ld
   AKNN_HIP_TRY(hipGetLastError());
 }
 
+namespace {
+
+// out = max(v[0..n)) for non-negative v (+inf allowed): grid-stride partial
+// maxima, one atomic per block (non-negative floats order like their bits).
+__global__ __launch_bounds__(256) void max_nonneg_kernel(const float* __restrict__ v, int64_t n,
+                                                         uint32_t* __restrict__ out) {
+  __shared__ uint32_t red[256];
+  uint32_t m = 0u;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += gridDim.x * 256ll) {
+    const uint32_t b = __float_as_uint(v[e]);
+    m = b > m ? b : m;
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s && red[threadIdx.x + s] > red[threadIdx.x]) red[threadIdx.x] = red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(out, red[0]);
+}
+
+}  // namespace
+
+void launch_max_nonneg(const float* v, int64_t n, float* out, hipStream_t s) {
+  AKNN_HIP_TRY(hipMemsetAsync(out, 0, sizeof(float), s));
+  const int64_t g = (n + 255) / 256;
+  hipLaunchKernelGGL(max_nonneg_kernel, dim3(static_cast<unsigned>(g < 256 ? (g < 1 ? 1 : g) : 256)), dim3(256), 0, s,
+                     v, n, reinterpret_cast<uint32_t*>(out));
+  AKNN_HIP_TRY(hipGetLastError());
+}
+
 void launch_sketch(const float* x, int64_t n, int d, int ld, int metric, _Float16* xh, int ldh, float* eps,
                    float* nh, hipStream_t s) {
   if (ldh % 8 != 0 || ldh < d) throw HipError("sketch: row stride must be a multiple of 8 covering d");
