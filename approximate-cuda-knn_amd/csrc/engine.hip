@@ -283,7 +283,9 @@ void Engine::build_forest() {
     slot_sums(fb_.amb_count, 1, &amb);
     stats_forest.examined = static_cast<double>(active_sum_);
     stats_forest.rescored = static_cast<double>(amb);
-    stats_forest.units = static_cast<double>(active_sum_) * (2.0 * ldh_ + 8.0 + 16.0) + 4.0 * d_ * amb +
+    // per active point its sketch row and 16 bytes of ids, bucket and output;
+    // the radius and norm bounds are one value each per block
+    stats_forest.units = static_cast<double>(active_sum_) * (2.0 * ldh_ + 16.0) + 4.0 * d_ * amb +
                          8.0 * n_ * p_.trees;
   }
 }
@@ -687,7 +689,7 @@ void Engine::calculate() {
   if (sketch && !xh_) {
     xh_ = alloc<_Float16>(static_cast<size_t>(n_) * ldh_);
     eps_ = alloc<float>(n_ + 1);  // [n]: the largest radius
-    nh_ = alloc<float>(n_);
+    nh_ = alloc<float>(n_ + 1);   // [n]: the largest sketch norm
     // leaf re-scored, forest undecided, leaf screened: kSlotWords each
     leaf_rescored_ = alloc<unsigned long long>(3 * static_cast<size_t>(kSlotWords));
   }
@@ -710,6 +712,7 @@ void Engine::calculate() {
   if (sketch) {
     launch_sketch(x_, n_, d_, ld_, kmetric(), xh_, ldh_, eps_, nh_, stream_);
     launch_max_nonneg(eps_, n_, eps_ + n_, stream_);
+    launch_max_nonneg(nh_, n_, nh_ + n_, stream_);
   }
   if (sketch8) launch_sketch8(x_, n_, d_, ld_, q8_, ld8_, s8_, a8_, e8_, stream_);
   pca_ = PcaBasis{};

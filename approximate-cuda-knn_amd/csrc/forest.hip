@@ -219,7 +219,8 @@ __device__ __forceinline__ int sketch_side_quad(const float* __restrict__ x, int
 // so a straggler does not hold a whole wave.
 __global__ __launch_bounds__(256) void forest_side_sketch_kernel(
     const float* __restrict__ x, int d, int ld, const _Float16* __restrict__ xh, int ldh,
-    const float* __restrict__ eps, const float* __restrict__ nh, SketchBound sb, const int32_t* __restrict__ act_ids,
+    const float* __restrict__ eps, const float* __restrict__ nh, int64_t n, SketchBound sb,
+    const int32_t* __restrict__ act_ids,
     const int32_t* __restrict__ act_bkt, const Bucket* __restrict__ bkt, const ForestCtl* ctl, int parity,
     uint64_t* __restrict__ rmask, int32_t* __restrict__ wpre, int32_t* __restrict__ blk_r,
     unsigned long long* __restrict__ amb_count) {
@@ -232,6 +233,12 @@ __global__ __launch_bounds__(256) void forest_side_sketch_kernel(
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int quad = threadIdx.x >> 2, q = threadIdx.x & 3;
+  // The bound grows with the point's radius and sketch norm, so their
+  // largest values (kept at index n) serve every point without a scattered
+  // load per point and level; an infinite largest radius (a row the sketch
+  // cannot hold) falls back to each point's own values.
+  const float emax = eps[n], nmax = nh[n];
+  const bool per_point = !(emax < __builtin_inff());
   for (int64_t blk = blockIdx.x; blk * kSideBlock < nact; blk += gridDim.x) {
     if (threadIdx.x == 0) namb = 0;
     __syncthreads();
@@ -247,8 +254,8 @@ __global__ __launch_bounds__(256) void forest_side_sketch_kernel(
           r = 1;  // Leaf2 writes RIGHT for p2 last (.cu:2178-2179)
         } else if (rel != b.q1) {
           const int32_t id = act_ids[m];
-          const int sd = sketch_side_quad(x, ld, xh, ldh, eps[id], nh[id], id, act_ids[b.astart + b.q1],
-                                          act_ids[b.astart + b.q2], sb, q);
+          const int sd = sketch_side_quad(x, ld, xh, ldh, per_point ? eps[id] : emax, per_point ? nh[id] : nmax, id,
+                                          act_ids[b.astart + b.q1], act_ids[b.astart + b.q2], sb, q);
           r = sd > 0 ? 1 : (sd == 0 ? 2 : 0);
         }
       }
@@ -673,8 +680,8 @@ void launch_forest_level(const ForestBuffers& fb, const float* x, int d, int ld,
     if (fb.eps == nullptr || fb.nh == nullptr || fb.ldh % 8 != 0 || fb.ldh < ld || ld < d || ld % 4 != 0)
       throw HipError("forest: malformed sketch");
     hipLaunchKernelGGL(forest_side_sketch_kernel, sg, dim3(256), 0, s, x, d, ld, fb.xh, fb.ldh, fb.eps, fb.nh,
-                       sketch_bound(d), fb.act_ids[par], fb.act_bkt[par], fb.bkt[par], fb.ctl, par, fb.rmask, fb.wpre,
-                       fb.blk_r, fb.amb_count);
+                       fb.n, sketch_bound(d), fb.act_ids[par], fb.act_bkt[par], fb.bkt[par], fb.ctl, par, fb.rmask,
+                       fb.wpre, fb.blk_r, fb.amb_count);
   } else switch (row_width_class(d)) {
     case 8:
       hipLaunchKernelGGL(forest_side_kernel<8>, sg, dim3(256), 0, s, x, d, ld, fb.act_ids[par], fb.act_bkt[par],
