@@ -11,8 +11,9 @@
 // buckets that still split are kept, as a compact list of records plus a
 // compact array of their points (ids in bucket order).  Finished buckets are
 // written once, straight into the leaf array and a leaf list.  A level is
-//   side:    one lane per active point: projection chain -> one ballot bit,
-//            per-1024-point block counts and per-wave prefixes;
+//   side:    one lane per active point (a quad of lanes with the sketch
+//            screen): projection chain -> one ballot bit, per-1024-point
+//            block counts and per-wave prefixes;
 //   level:   one workgroup: scan of the block counts, the RIGHT count of
 //            every bucket from popcounts, child records, pivots, compaction;
 //   scatter: one lane per active point: stable rank from popcounts, write to
