@@ -92,7 +92,24 @@ Engine::~Engine() {
   for (auto e : events_) (void)hipEventDestroy(e);
   for (auto e : launch_events_) (void)hipEventDestroy(e);
   if (ctl_host_) (void)hipHostFree(ctl_host_);
+  for (Pinned& b : pinned_)
+    if (b.p) (void)hipHostFree(b.p);
   if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void* Engine::pinned(int slot, size_t bytes) {
+  Pinned& b = pinned_[slot];
+  if (bytes > b.cap) {
+    if (b.p) {
+      AKNN_HIP_TRY(hipStreamSynchronize(stream_));  // no copy may still target the old buffer
+      AKNN_HIP_TRY(hipHostFree(b.p));
+      b.p = nullptr;
+      b.cap = 0;
+    }
+    AKNN_HIP_TRY(hipHostMalloc(&b.p, bytes, hipHostMallocDefault));
+    b.cap = bytes;
+  }
+  return b.p;
 }
 
 hipEvent_t Engine::ev(int i) { return events_.at(i); }
@@ -136,6 +153,7 @@ void Engine::release() {
   leaves_ = nullptr;
   rs_ = rc_ = nullptr;
   items_ = nullptr;
+  items_cap_ = 0;
   ncand_ = nullptr;
   changed_[0] = changed_[1] = nullptr;
   fb_ = ForestBuffers{};
@@ -446,50 +464,85 @@ void Engine::initial_knn() {
     rs_ = alloc<int32_t>(static_cast<size_t>(n_) * p_.trees);
     rc_ = alloc<int32_t>(static_cast<size_t>(n_) * p_.trees);
   }
-  if (!items_) items_ = alloc<Leaf>(static_cast<size_t>(n_) + n_ / 64 + 16);
-  // (a member: asynchronous copies read it after this function returns)
+  // The leaf lists of all trees come over in one round trip (asynchronous
+  // copies into one pinned buffer, one sync).  Each tree's work items are
+  // then built on the host while the device runs the previous tree, and go
+  // up just before its kernel.  A leaf gives at most ceil(size / 64) items,
+  // so every tree's items fit in nleaves + n / 64 + 1 entries.
   std::vector<std::vector<Leaf>>& items = items_host_;
   items.assign(p_.trees, {});
+  std::vector<int64_t> item_off(p_.trees + 1, 0);
+  for (int t = 0; t < p_.trees; ++t) item_off[t + 1] = item_off[t] + nleaves_[t] + n_ / kItemOwners + 1;
+  Leaf* lh = nullptr;
+  std::vector<int64_t> leaf_off(p_.trees + 1, 0);
   {
-    std::vector<Leaf> host;
-    for (int t = 0; t < p_.trees; ++t) {
-      host.resize(nleaves_[t]);
-      AKNN_HIP_TRY(hipMemcpy(host.data(), leaves_ + t * n_, sizeof(Leaf) * nleaves_[t], hipMemcpyDeviceToHost));
-      build_work_items(host, items[t]);
+    for (int t = 0; t < p_.trees; ++t) leaf_off[t + 1] = leaf_off[t] + nleaves_[t];
+    lh = static_cast<Leaf*>(pinned(0, sizeof(Leaf) * static_cast<size_t>(std::max<int64_t>(leaf_off[p_.trees], 1))));
+    for (int t = 0; t < p_.trees; ++t)
+      AKNN_HIP_TRY(hipMemcpyAsync(lh + leaf_off[t], leaves_ + t * n_, sizeof(Leaf) * nleaves_[t],
+                                  hipMemcpyDeviceToHost, stream_));
+    AKNN_HIP_TRY(hipStreamSynchronize(stream_));
+    for (int t = 0; t < p_.trees; ++t)
       launch_leaf_ranges(perm_ + t * n_, leaves_ + t * n_, nleaves_[t], rs_ + t * n_, rc_ + t * n_, stream_);
+    const size_t ni = static_cast<size_t>(item_off[p_.trees]);
+    if (ni > items_cap_) {
+      items_ = alloc<Leaf>(ni);
+      items_cap_ = ni;
     }
   }
+  Leaf* ih = static_cast<Leaf*>(pinned(1, sizeof(Leaf) * static_cast<size_t>(item_off[p_.trees])));
   bool mfma = leaf_mfma_active();
   if (mfma) {
     AKNN_HIP_TRY(hipMemsetAsync(leaf_rescored_, 0, sizeof(unsigned long long) * kSlotWords, stream_));
     AKNN_HIP_TRY(hipMemsetAsync(leaf_rescored_ + 2 * kSlotWords, 0, sizeof(unsigned long long) * kSlotWords, stream_));
   }
-  unsigned long long prev[2] = {0, 0};
+  // leaf-screen counters (re-scored, -, screened slots) copied after each tree
+  const size_t cw = 3 * static_cast<size_t>(kSlotWords);
+  unsigned long long* ch = static_cast<unsigned long long*>(pinned(2, sizeof(unsigned long long) * cw * p_.trees));
+  auto counts = [&](int t, unsigned long long c[2]) {
+    c[0] = c[1] = 0;
+    for (int s = 0; s < kCountSlots; ++s) {
+      c[0] += ch[cw * t + s * kCountStride];
+      c[1] += ch[cw * t + 2 * kSlotWords + s * kCountStride];
+    }
+  };
   for (int t = 0; t < p_.trees; ++t) {
     // The screen pays when few pairs survive it: after a screened tree that
     // re-scored more than 30% of its pairs, the remaining trees take the
-    // plain kernel (same graph).
-    if (mfma && t > 1) {
-      unsigned long long c[3] = {0, 0, 0};
-      slot_sums(leaf_rescored_, 3, c);
-      const unsigned long long dres = c[0] - prev[0], dscr = c[2] - prev[1];
-      prev[0] = c[0];
-      prev[1] = c[2];
+    // plain kernel (same graph).  Tree t - 2's counts decide for tree t: they
+    // were copied behind its kernel, so waiting for them does not let the
+    // device run dry (tree t - 1 is queued behind them).
+    if (mfma && t > 2) {
+      AKNN_HIP_TRY(hipEventSynchronize(launch_ev(2 * (t - 1))));
+      unsigned long long a[2], b[2];
+      counts(t - 2, a);
+      counts(t - 3, b);
+      const unsigned long long dres = a[0] - b[0], dscr = a[1] - b[1];
       if (dscr > 0 && 10 * dres > 3 * dscr) mfma = false;
     }
     tree_screened_[t] = mfma && t > 0 ? 1 : 0;
+    {
+      std::vector<Leaf> host(lh + leaf_off[t], lh + leaf_off[t + 1]);
+      build_work_items(host, items[t]);
+    }
     const int nit = static_cast<int>(items[t].size());
-    AKNN_HIP_TRY(hipMemcpyAsync(items_, items[t].data(), sizeof(Leaf) * nit, hipMemcpyHostToDevice, stream_));
+    if (nit > item_off[t + 1] - item_off[t]) throw HipError("leaf work items exceed their bound");
+    std::copy(items[t].begin(), items[t].end(), ih + item_off[t]);
+    Leaf* it = items_ + item_off[t];
+    AKNN_HIP_TRY(hipMemcpyAsync(it, ih + item_off[t], sizeof(Leaf) * nit, hipMemcpyHostToDevice, stream_));
     AKNN_HIP_TRY(hipEventRecord(launch_ev(2 * t), stream_));
     // the first tree meets stub rows (nothing to screen against): plain kernel
     if (mfma && t > 0)
-      launch_leaf_mfma(kmetric(), x_, d_, ld_, xh_, ldh_, eps_, nh_, norms_, perm_ + t * n_, items_, nit, kItemOwners,
+      launch_leaf_mfma(kmetric(), x_, d_, ld_, xh_, ldh_, eps_, nh_, norms_, perm_ + t * n_, it, nit, kItemOwners,
                        p_.k, ids, vals, rs_ + t * n_, rc_ + t * n_, leaf_rescored_, leaf_rescored_ + 2 * kSlotWords,
                        stream_);
     else
-      launch_leaf_knn(kmetric(), x_, d_, ld_, norms_, perm_ + t * n_, items_, nit, kItemOwners, p_.k, ids, vals,
+      launch_leaf_knn(kmetric(), x_, d_, ld_, norms_, perm_ + t * n_, it, nit, kItemOwners, p_.k, ids, vals,
                       rs_ + t * n_, rc_ + t * n_, stream_);
     AKNN_HIP_TRY(hipEventRecord(launch_ev(2 * t + 1), stream_));
+    if (mfma)
+      AKNN_HIP_TRY(hipMemcpyAsync(ch + cw * t, leaf_rescored_, sizeof(unsigned long long) * cw,
+                                  hipMemcpyDeviceToHost, stream_));
   }
   stats_leaf_knn.launches = p_.trees;
 }

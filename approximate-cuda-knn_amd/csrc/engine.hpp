@@ -185,8 +185,17 @@ class Engine {
   unsigned long long* ncand_ = nullptr;
   uint8_t* changed_[2] = {nullptr, nullptr};
   int32_t* rs_ = nullptr;  // [trees][n] per-point start ranges, allocated on demand
-  Leaf* items_ = nullptr;  // leaf k-NN work items of one tree
+  Leaf* items_ = nullptr;  // leaf k-NN work items of all trees, tree after tree
+  size_t items_cap_ = 0;   // capacity of items_ (entries)
   std::vector<std::vector<Leaf>> items_host_;
+  // pinned host staging, grown on demand: 0 leaf lists, 1 work items,
+  // 2 leaf-screen counters after each tree
+  struct Pinned {
+    void* p = nullptr;
+    size_t cap = 0;
+  };
+  Pinned pinned_[3];
+  void* pinned(int slot, size_t bytes);
   int32_t* rc_ = nullptr;
   CosineParityScratch cps_{};
   int64_t active_sum_ = 0;

@@ -55,13 +55,6 @@ __device__ __forceinline__ void list_insert(float (&lk)[KMAX], int32_t (&li)[KMA
   }
 }
 
-// 8 halves of a sketch row starting at half kk, or zeros past the row.
-__device__ __forceinline__ half8 piece(const _Float16* __restrict__ row, int kk, int ldh) {
-  uint4 v = make_uint4(0u, 0u, 0u, 0u);
-  if (row != nullptr && kk < ldh) v = *reinterpret_cast<const uint4*>(row + kk);
-  return __builtin_bit_cast(half8, v);
-}
-
 // Owner threshold.  EUCLIDEAN: the smallest T such that a pair whose sketch
 // distance exceeds T + eps_candidate has an exact key above thk
 // (T >= thk / (1 - kap) + eps_owner, rounded up).  COSINE: the largest T such
