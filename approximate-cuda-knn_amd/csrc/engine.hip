@@ -360,8 +360,9 @@ void Engine::build_forest_median() {
     const bool routed = t < routed_trees;
     const int bb = routed ? bin_bits_ : 0;
     // projections are made 16 directions at a time, the first time a level
-    // needs one: routed trees use only the directions below their depth
-    // past the bin levels
+    // needs one: routed trees use only the directions below their depth past
+    // the bin levels.  The first batch runs over all rows in order; a later
+    // one only over the points still active then (the few of deep buckets).
     const float* tdirs = dirs_ + static_cast<size_t>(t) * nd * ld_;
     int projected = 0;
     launch_forest_init(fb_, perm, leaves, n_, 0, 0, leaf_eff_, p_.k + 1, stream_);
@@ -376,7 +377,8 @@ void Engine::build_forest_median() {
           if (level - bb >= projected) {
             const int cnt = std::min(kProjectStep, nd - projected);
             launch_project(x_, n_, d_, ld_, tdirs + static_cast<size_t>(projected) * ld_, cnt,
-                           proj_ + static_cast<int64_t>(projected) * n_, stream_);
+                           proj_ + static_cast<int64_t>(projected) * n_, projected > 0 ? &fb_ : nullptr, level,
+                           stream_);
             projected += cnt;
             proj_dirs_ += cnt;
             ++proj_launches_;

@@ -488,18 +488,24 @@ constexpr int kProjChunk = 32;  // dimensions staged in LDS per step
 constexpr int kProjXs = kProjChunk + 4;  // LDS row stride (floats): 16-byte reads of 8 lanes hit distinct banks
 
 // P[j * n + i] = x_i . r_j as one FMA chain in dimension order (FP32, the
-// oracle's chain), for j < nd <= kProjGroup.  One lane per point, 256 points
-// per block: each step stages the block's rows for 32 dimensions in LDS with
+// oracle's chain), for j < nd <= kProjGroup and every point i (act_ids null,
+// rows in order), or every point of a level's active list (act_ids[0 ..
+// ctl->nact[parity]): only they, or their subsets at deeper levels, read
+// these directions).  One lane per point, 256 points per
+// block: each step stages the block's rows for 32 dimensions in LDS with
 // coalesced 16-byte loads (8 lanes per 128-byte row piece), and the
 // directions' 32 dimensions beside them (read as broadcasts); each lane then
 // advances its kProjGroup chains from its own staged row.
 __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ x, int64_t n, int d, int ld,
                                                       const float* __restrict__ dirs, int nd,
-                                                      float* __restrict__ proj) {
+                                                      float* __restrict__ proj, const int32_t* __restrict__ act_ids,
+                                                      const ForestCtl* ctl, int parity) {
   __shared__ __attribute__((aligned(16))) float xs[256 * kProjXs];
   __shared__ __attribute__((aligned(16))) float rs[kProjGroup][kProjChunk];
   const int t = threadIdx.x;
   const int64_t i0 = blockIdx.x * 256ll;
+  const int64_t rows = act_ids != nullptr ? ctl->nact[parity] : n;
+  if (i0 >= rows) return;  // the grid covers n; the list is shorter at deep levels
   float acc[kProjGroup];
 #pragma unroll
   for (int g = 0; g < kProjGroup; ++g) acc[g] = 0.0f;
@@ -513,7 +519,8 @@ __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ 
     for (int u = 0; u < 256 * kProjChunk / 4 / 256; ++u) {
       const int e = t + 256 * u;
       const int r = e / (kProjChunk / 4), q = e % (kProjChunk / 4);
-      const int64_t row = i0 + r < n ? i0 + r : n - 1;
+      const int64_t rr = i0 + r < rows ? i0 + r : rows - 1;
+      const int64_t row = act_ids != nullptr ? act_ids[rr] : rr;
       const int c = c0 + 4 * q;
       const float4 v = c < d ? *reinterpret_cast<const float4*>(x + row * ld + c) : make_float4(0.f, 0.f, 0.f, 0.f);
       *reinterpret_cast<float4*>(xs + r * kProjXs + 4 * q) = v;
@@ -541,8 +548,8 @@ __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ 
       for (int g = 0; g < kProjGroup; ++g) acc[g] = __builtin_fmaf(v, rs[g][c], acc[g]);
     }
   }
-  const int64_t i = i0 + t;
-  if (i < n) {
+  if (i0 + t < rows) {
+    const int64_t i = act_ids != nullptr ? act_ids[i0 + t] : i0 + t;
 #pragma unroll
     for (int g = 0; g < kProjGroup; ++g)
       if (g < nd) proj[static_cast<int64_t>(g) * n + i] = acc[g];
@@ -715,12 +722,13 @@ void launch_forest_level(const ForestBuffers& fb, const float* x, int d, int ld,
 }
 
 void launch_project(const float* x, int64_t n, int d, int ld, const float* dirs, int nd, float* proj,
-                    hipStream_t s) {
+                    const ForestBuffers* fb, int level, hipStream_t s) {
   if (nd <= 0 || n <= 0) return;
   if (ld % 4 != 0 || ld < d) throw HipError("project: malformed row width");
   if (nd > kProjGroup) throw HipError("project: at most 16 directions per launch");
+  const int par = level & 1;
   hipLaunchKernelGGL(project_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, x, n, d, ld, dirs,
-                     nd, proj);
+                     nd, proj, fb ? fb->act_ids[par] : nullptr, fb ? fb->ctl : nullptr, par);
   AKNN_HIP_TRY(hipGetLastError());
 }
 

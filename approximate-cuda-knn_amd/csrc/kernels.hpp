@@ -46,12 +46,14 @@ void launch_forest_level(const ForestBuffers& fb, const float* x, int d, int ld,
                          int level, int32_t v1, int32_t v2, int leaf_eff, int kp1, hipStream_t s);
 // Median-split forest.  launch_project: proj[j * n + i] = x_i . dirs[j] (FP32
 // chain in dimension order), j < nd <= kProjectStep, dirs [nd][ld], one pass
-// over the rows.  launch_forest_level_rule:
+// over all rows in order (fb null) or over the rows of the points active at
+// `level` (fb's list for its parity).
+// launch_forest_level_rule:
 // one level under rule 0 (bit `shift` of bins[id]), 1 (sampled median of the
 // level's projections pj[n]) or 2 (position); the scatter is the same.
 constexpr int kProjectStep = 16;
 void launch_project(const float* x, int64_t n, int d, int ld, const float* dirs, int nd, float* proj,
-                    hipStream_t s);
+                    const ForestBuffers* fb, int level, hipStream_t s);
 void launch_forest_level_rule(const ForestBuffers& fb, int rule, const float* pj, const int32_t* bins, int shift,
                               int32_t* perm, Leaf* leaves, int level, int leaf_eff, int kp1, hipStream_t s);
 // Per-point start ranges from a leaf list (for export and the COSINE parity path).
