@@ -24,6 +24,7 @@
 
 #include "device_util.hpp"
 #include "kernels.hpp"
+#include "topk.hpp"
 
 namespace aknn {
 
@@ -32,26 +33,6 @@ namespace {
 constexpr int kTile = 64;
 constexpr int kDc = 32;   // dimensions per LDS stage (divides 256)
 constexpr int kPad = 4;
-
-template <int KMAX>
-__device__ __forceinline__ void list_insert(float (&lk)[KMAX], int32_t (&li)[KMAX], int k, float key, int32_t id) {
-#pragma unroll
-  for (int p = KMAX - 1; p > 0; --p) {
-    if (p < k) {
-      if (key_less(key, id, lk[p - 1], li[p - 1])) {
-        lk[p] = lk[p - 1];
-        li[p] = li[p - 1];
-      } else if (key_less(key, id, lk[p], li[p])) {
-        lk[p] = key;
-        li[p] = id;
-      }
-    }
-  }
-  if (key_less(key, id, lk[0], li[0])) {
-    lk[0] = key;
-    li[0] = id;
-  }
-}
 
 constexpr int kStageFloats = 2 * kDc * (kTile + kPad);
 static_assert(kStageFloats >= kTile * (kTile + 1), "the distance block reuses the staging tiles");
@@ -68,7 +49,7 @@ __global__ __launch_bounds__(64) void leaf_knn_kernel(const float* __restrict__ 
   float(*As)[kTile + kPad] = reinterpret_cast<float(*)[kTile + kPad]>(smem);
   float(*Bs)[kTile + kPad] = reinterpret_cast<float(*)[kTile + kPad]>(smem + kDc * (kTile + kPad));
   float(*Dt)[kTile + 1] = reinterpret_cast<float(*)[kTile + 1]>(smem);
-  __shared__ int32_t cid[kTile];
+  __shared__ uint32_t cox[kTile];  // candidate ids through order_id
   __shared__ float onorm[kTile];
   __shared__ float cnorm[kTile];
 
@@ -89,22 +70,10 @@ __global__ __launch_bounds__(64) void leaf_knn_kernel(const float* __restrict__ 
   if constexpr (METRIC == kCosine) onorm[lane] = has_owner ? norms[my_id] : 1.0f;
 
   // this owner's running top-k, seeded with its graph row (earlier trees)
-  float lk[KMAX];
-  int32_t li[KMAX];
-#pragma unroll
-  for (int p = 0; p < KMAX; ++p) {
-    const bool v = has_owner && p < k;
-    lk[p] = v ? gvals[static_cast<int64_t>(my_id) * k + p] : __builtin_inff();
-    li[p] = v ? gids[static_cast<int64_t>(my_id) * k + p] : INT_MAX;
-  }
-  float thk = __builtin_inff();
-  int32_t thi = INT_MAX;
-#pragma unroll
-  for (int p = 0; p < KMAX; ++p)
-    if (p == k - 1) {
-      thk = lk[p];
-      thi = li[p];
-    }
+  const int64_t row0 = static_cast<int64_t>(has_owner ? my_id : 0) * k;
+  uint64_t pk[KMAX];
+  topk_load<KMAX>(pk, k, has_owner, gvals + row0, gids + row0);
+  const uint32_t my_ox = order_id(my_id);
 
   const int r = lane >> 3, c = lane & 7;
   const float* orow = x + static_cast<int64_t>(my_id < 0 ? 0 : my_id) * ld;
@@ -112,7 +81,7 @@ __global__ __launch_bounds__(64) void leaf_knn_kernel(const float* __restrict__ 
     const int ncand = L.rcount - cb < kTile ? L.rcount - cb : kTile;
     __syncthreads();
     const int32_t my_c = lane < ncand ? perm[L.rstart + cb + lane] : -1;
-    cid[lane] = my_c;
+    cox[lane] = order_id(my_c);
     if constexpr (METRIC == kCosine) cnorm[lane] = lane < ncand ? norms[my_c] : 1.0f;
 
     float acc[8][8];
@@ -188,43 +157,29 @@ __global__ __launch_bounds__(64) void leaf_knn_kernel(const float* __restrict__ 
       }
     __syncthreads();
     if (has_owner) {
-      // each lane first marks the candidates below its threshold, then
-      // inserts only those: lanes do not wait on each other's insertions
+      // each lane first marks the candidates of its own range that undercut
+      // its threshold, then offers only those: lanes do not wait on each
+      // other's insertions
+      const int base = L.rstart + cb;
+      const int lo = r_lo - base < 0 ? 0 : r_lo - base;
+      const int hi = r_hi - base > ncand ? ncand : r_hi - base;
+      const uint64_t below_hi = hi >= 64 ? ~0ull : (hi > 0 ? (1ull << hi) - 1 : 0ull);
+      const uint64_t below_lo = lo >= 64 ? ~0ull : (1ull << lo) - 1;
+      const uint64_t th = pk[KMAX - 1];
       uint64_t surv = 0;
       for (int j = 0; j < ncand; ++j) {
-        const int32_t cj = cid[j];
-        const int32_t pos = L.rstart + cb + j;
-        surv |= static_cast<uint64_t>(cj != my_id && pos >= r_lo && pos < r_hi && key_less(Dt[lane][j], cj, thk, thi))
-                << j;
+        const uint32_t cj = cox[j];
+        surv |= static_cast<uint64_t>(cj != my_ox && pack_pair(Dt[lane][j], cj) < th) << j;
       }
+      surv &= below_hi & ~below_lo;
       while (surv) {
         const int j = __ffsll(static_cast<long long>(surv)) - 1;
         surv &= surv - 1;
-        const int32_t cj = cid[j];
-        const float key = Dt[lane][j];
-        if (!key_less(key, cj, thk, thi)) continue;
-        bool dup = false;
-#pragma unroll
-        for (int p = 0; p < KMAX; ++p) dup |= (p < k) && li[p] == cj;
-        if (dup) continue;
-        list_insert<KMAX>(lk, li, k, key, cj);
-#pragma unroll
-        for (int p = 0; p < KMAX; ++p)
-          if (p == k - 1) {
-            thk = lk[p];
-            thi = li[p];
-          }
+        topk_offer<KMAX>(pk, pack_pair(Dt[lane][j], cox[j]));
       }
     }
   }
-  if (has_owner) {
-#pragma unroll
-    for (int p = 0; p < KMAX; ++p)
-      if (p < k) {
-        gids[static_cast<int64_t>(my_id) * k + p] = li[p];
-        gvals[static_cast<int64_t>(my_id) * k + p] = lk[p];
-      }
-  }
+  if (has_owner) topk_store<KMAX>(pk, k, gvals + row0, gids + row0);
 }
 
 template <int METRIC, int KMAX, bool BIGD>
@@ -245,6 +200,8 @@ void launch_k(int k, const float* x, int d, int ld, const float* norms, const in
     launch_t<METRIC, 8, BIGD>(x, d, ld, norms, perm, leaves, nleaves, max_leaf, k, ids, vals, rsp, rcp, s);
   else if (k <= 16)
     launch_t<METRIC, 16, BIGD>(x, d, ld, norms, perm, leaves, nleaves, max_leaf, k, ids, vals, rsp, rcp, s);
+  else if (k <= 20)
+    launch_t<METRIC, 20, BIGD>(x, d, ld, norms, perm, leaves, nleaves, max_leaf, k, ids, vals, rsp, rcp, s);
   else if (k <= 24)
     launch_t<METRIC, 24, BIGD>(x, d, ld, norms, perm, leaves, nleaves, max_leaf, k, ids, vals, rsp, rcp, s);
   else if (k <= 32)

@@ -25,6 +25,7 @@
 
 #include "device_util.hpp"
 #include "kernels.hpp"
+#include "topk.hpp"
 
 namespace aknn {
 
@@ -34,26 +35,6 @@ typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kTile = 64;
-
-template <int KMAX>
-__device__ __forceinline__ void list_insert(float (&lk)[KMAX], int32_t (&li)[KMAX], int k, float key, int32_t id) {
-#pragma unroll
-  for (int p = KMAX - 1; p > 0; --p) {
-    if (p < k) {
-      if (key_less(key, id, lk[p - 1], li[p - 1])) {
-        lk[p] = lk[p - 1];
-        li[p] = li[p - 1];
-      } else if (key_less(key, id, lk[p], li[p])) {
-        lk[p] = key;
-        li[p] = id;
-      }
-    }
-  }
-  if (key_less(key, id, lk[0], li[0])) {
-    lk[0] = key;
-    li[0] = id;
-  }
-}
 
 // Owner threshold.  EUCLIDEAN: the smallest T such that a pair whose sketch
 // distance exceeds T + eps_candidate has an exact key above thk
@@ -140,22 +121,9 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
   const int32_t my_id = has_owner ? a.perm[L.start + o0 + lane] : -1;
   const int k = a.k;
 
-  float lk[KMAX];
-  int32_t li[KMAX];
-#pragma unroll
-  for (int p = 0; p < KMAX; ++p) {
-    const bool v = has_owner && p < k;
-    lk[p] = v ? a.gvals[static_cast<int64_t>(my_id) * k + p] : __builtin_inff();
-    li[p] = v ? a.gids[static_cast<int64_t>(my_id) * k + p] : INT_MAX;
-  }
-  float thk = __builtin_inff();
-  int32_t thi = INT_MAX;
-#pragma unroll
-  for (int p = 0; p < KMAX; ++p)
-    if (p == k - 1) {
-      thk = lk[p];
-      thi = li[p];
-    }
+  const int64_t row0 = static_cast<int64_t>(has_owner ? my_id : 0) * k;
+  uint64_t pk[KMAX];
+  topk_load<KMAX>(pk, k, has_owner, a.gvals + row0, a.gids + row0);
   const float my_eps = has_owner ? a.eps[my_id] : 0.0f;
   const double tiny = 1e-40;
   oid[lane] = my_id;
@@ -169,7 +137,7 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
     o_s[lane] = sqrtf(aa);
     o_e[lane] = my_eps;
   }
-  o_t[lane] = owner_threshold<METRIC>(thk, my_eps, a.kap);
+  o_t[lane] = owner_threshold<METRIC>(pair_key(pk[KMAX - 1]), my_eps, a.kap);
   __syncthreads();
   const int32_t oa = oid[col], ob = oid[32 + col];
   const _Float16* arow0s = a.xh + static_cast<int64_t>(oa >= 0 ? oa : 0) * a.ldh;
@@ -311,11 +279,7 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
       while (mm) {
         const int c = __ffsll(static_cast<long long>(mm)) - 1;
         mm &= mm - 1;
-        const int32_t cj = cid[c];
-        bool dup = false;
-#pragma unroll
-        for (int pp = 0; pp < KMAX; ++pp) dup |= (pp < k) && li[pp] == cj;
-        if (dup) m &= ~(1ull << c);
+        if (topk_has<KMAX>(pk, order_id(cid[c]))) m &= ~(1ull << c);
       }
     }
     const int cnt = __popcll(m);
@@ -358,25 +322,12 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
         const int lo = off > base ? off : base;
         const int hi = off + cnt < base + kTile ? off + cnt : base + kTile;
         for (int q = lo; q < hi; ++q) {
-          const int32_t cj = cid[plist[q] & 63];
-          const float kq = kb[q - base];
-          if (!key_less(kq, cj, thk, thi)) continue;
-          bool dup = false;
-#pragma unroll
-          for (int pp = 0; pp < KMAX; ++pp) dup |= (pp < k) && li[pp] == cj;
-          if (dup) continue;
-          list_insert<KMAX>(lk, li, k, kq, cj);
-#pragma unroll
-          for (int pp = 0; pp < KMAX; ++pp)
-            if (pp == k - 1) {
-              thk = lk[pp];
-              thi = li[pp];
-            }
+          topk_offer<KMAX>(pk, pack_pair(kb[q - base], order_id(cid[plist[q] & 63])));
         }
       }
       __syncthreads();
     }
-    if (has_owner && cnt > 0) o_t[lane] = owner_threshold<METRIC>(thk, my_eps, a.kap);
+    if (has_owner && cnt > 0) o_t[lane] = owner_threshold<METRIC>(pair_key(pk[KMAX - 1]), my_eps, a.kap);
   }
   // counters spread over slots one 128-byte line apart (summed by the host)
   const int64_t cslot = (blockIdx.x % kCountSlots) * kCountStride;
@@ -386,14 +337,7 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
     for (int o = 0; o < nown; ++o) pairs += static_cast<unsigned long long>(o_hi[o] - o_lo[o]);
     atomicAdd(a.screened + cslot, pairs);
   }
-  if (has_owner) {
-#pragma unroll
-    for (int p = 0; p < KMAX; ++p)
-      if (p < k) {
-        a.gids[static_cast<int64_t>(my_id) * k + p] = li[p];
-        a.gvals[static_cast<int64_t>(my_id) * k + p] = lk[p];
-      }
-  }
+  if (has_owner) topk_store<KMAX>(pk, k, a.gvals + row0, a.gids + row0);
 }
 
 template <int METRIC, int KMAX>
