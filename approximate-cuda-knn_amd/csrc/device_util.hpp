@@ -78,15 +78,15 @@ __device__ __forceinline__ float row_chain(const float* a, const float* __restri
 }
 
 // Same chain with both rows in global memory (16-byte aligned, padded to a
-// multiple of 4): 16-byte loads from both, eight in flight per lane.
-template <int METRIC>
+// multiple of 4): 16-byte loads from both, 2 * UNROLL in flight per lane.
+template <int METRIC, int UNROLL = 4>
 __device__ __forceinline__ float row_chain_global(const float* __restrict__ a, const float* __restrict__ b, int d) {
   float tot = 0.0f;
   for (int c0 = 0; c0 < d; c0 += 256) {
     const int c1 = c0 + 256 < d ? c0 + 256 : d;
     float p = 0.0f;
     int dd = c0;
-#pragma unroll 4
+#pragma unroll UNROLL
     for (; dd + 4 <= c1; dd += 4) {
       const float4 av = *reinterpret_cast<const float4*>(a + dd);
       const float4 bv = *reinterpret_cast<const float4*>(b + dd);

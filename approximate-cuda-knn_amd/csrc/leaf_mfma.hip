@@ -35,6 +35,9 @@ typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kTile = 64;
+// 16-byte steps of the re-score chain unrolled together: 2 * 8 row loads in
+// flight per lane (the MFMA phase's registers are free by then)
+constexpr int kRescoreUnroll = 8;
 
 // Owner threshold.  EUCLIDEAN: the smallest T such that a pair whose sketch
 // distance exceeds T + eps_candidate has an exact key above thk
@@ -102,7 +105,8 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
   __shared__ __attribute__((aligned(16))) float o_t[kTile];  // owner threshold
   __shared__ __attribute__((aligned(16))) float o_e[kTile];  // C: sketch radius
   __shared__ __attribute__((aligned(16))) float o_s[kTile];  // C: sqrtf(aa), as the exact key uses
-  __shared__ float c_n[kTile], c_e[kTile], c_s[kTile], kb[kTile];
+  __shared__ float c_n[kTile], c_e[kTile], c_s[kTile];
+  __shared__ uint64_t kp[kTile];  // re-scored pairs, packed for the owners' top-k
   __shared__ __attribute__((aligned(16))) int32_t o_lo[kTile];  // owner's start-node positions [lo, hi)
   __shared__ __attribute__((aligned(16))) int32_t o_hi[kTile];
   __shared__ uint64_t survm[kTile];
@@ -306,24 +310,23 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
     for (int base = 0; base < total; base += kTile) {
       const int p = base + lane;
       float key = 0.0f;
+      int e = 0;
       if (p < total) {
-        const int e = plist[p];
+        e = plist[p];
         const float* xo = a.x + static_cast<int64_t>(oid[e >> 6]) * a.ld;
         const float* xc = a.x + static_cast<int64_t>(cid[e & 63]) * a.ld;
         if constexpr (METRIC == kEuclidean)
-          key = finish_key<kEuclidean>(row_chain_global<kEuclidean>(xo, xc, a.d), 1.0f, 1.0f);
+          key = finish_key<kEuclidean>(row_chain_global<kEuclidean, kRescoreUnroll>(xo, xc, a.d), 1.0f, 1.0f);
         else
-          key = finish_key<kCosine>(row_chain_global<kCosine>(xo, xc, a.d), a.norms[oid[e >> 6]],
+          key = finish_key<kCosine>(row_chain_global<kCosine, kRescoreUnroll>(xo, xc, a.d), a.norms[oid[e >> 6]],
                                     a.norms[cid[e & 63]]);
       }
-      kb[lane] = key;
+      kp[lane] = pack_pair(key, order_id(cid[e & 63]));
       __syncthreads();
       if (has_owner) {
         const int lo = off > base ? off : base;
         const int hi = off + cnt < base + kTile ? off + cnt : base + kTile;
-        for (int q = lo; q < hi; ++q) {
-          topk_offer<KMAX>(pk, pack_pair(kb[q - base], order_id(cid[plist[q] & 63])));
-        }
+        for (int q = lo; q < hi; ++q) topk_offer<KMAX>(pk, kp[q - base]);
       }
       __syncthreads();
     }
