@@ -264,6 +264,17 @@ def test_leaf_mfma_changes_nothing(aknn, oracle, datasets, n, d, k, trees, leaf,
     check_graph(oracle, a, x, k, trees, leaf, 0, 0)
 
 
+@pytest.mark.parametrize("k", [23, 40, 64])
+@pytest.mark.parametrize("leaf_mfma", [0, 1])
+def test_leaf_list_sizes(aknn, oracle, datasets, k, leaf_mfma):
+    """Owner lists of 24 and 64 slots holding 23, 40 and 64 pairs: the row sits
+    in the top slots of the list with empty slots below it.  No propagation
+    round: its LDS-resident lists stop short of k=64."""
+    x = datasets.gmm_c1(3000, 40, clusters=5, seed=50 + k)[0]
+    g = gpu_run(aknn, x, k, 3, 80, 0, 0, leaf_mfma=leaf_mfma)
+    check_graph(oracle, g, x, k, 3, 80, 0, 0)
+
+
 @pytest.mark.parametrize("n,d,k,trees,leaf,seed", [(4000, 40, 8, 4, 24, 45), (2500, 96, 30, 3, 100, 46),
                                                    (1500, 300, 10, 3, 60, 47)])
 def test_leaf_mfma_cosine_fixed(aknn, oracle, datasets, n, d, k, trees, leaf, seed):
