@@ -43,12 +43,36 @@ __device__ __forceinline__ int32_t pair_id(uint64_t v) {
   return static_cast<int32_t>(static_cast<uint32_t>(v) ^ 0x80000000u);
 }
 
+// True when a row of KMAX values at both addresses can move in 16-byte
+// pieces: the list is full (k == KMAX, a multiple of 4) and both rows are
+// 16-byte aligned (the graph arrays are; row r starts r * k values in).
+template <int KMAX>
+__device__ __forceinline__ bool topk_row_vector(int k, const void* vals, const void* ids) {
+  static_assert(KMAX % 4 == 0, "list sizes are multiples of 4");
+  return k == KMAX && ((reinterpret_cast<uintptr_t>(vals) | reinterpret_cast<uintptr_t>(ids)) & 15) == 0;
+}
+
 // Loads a graph row of k pairs (k <= KMAX) into the right-aligned list.  A
-// lane without a row holds (inf, INT_MAX) pairs.
+// lane without a row holds (inf, INT_MAX) pairs; its row pointers must still
+// point at a readable row.  Each lane reads its own row, so a full list is
+// read in 16-byte pieces: a quarter of the accesses, each to a row that
+// other lanes' accesses do not share.
 template <int KMAX>
 __device__ __forceinline__ void topk_load(uint64_t (&pk)[KMAX], int k, bool valid, const float* __restrict__ vals,
                                           const int32_t* __restrict__ ids) {
   const uint64_t none = pack_pair(__builtin_inff(), order_id(0x7fffffff));
+  if (topk_row_vector<KMAX>(k, vals, ids)) {
+#pragma unroll
+    for (int p = 0; p < KMAX; p += 4) {
+      const float4 v = *reinterpret_cast<const float4*>(vals + p);
+      const int4 i = *reinterpret_cast<const int4*>(ids + p);
+      pk[p + 0] = valid ? pack_pair(v.x, order_id(i.x)) : none;
+      pk[p + 1] = valid ? pack_pair(v.y, order_id(i.y)) : none;
+      pk[p + 2] = valid ? pack_pair(v.z, order_id(i.z)) : none;
+      pk[p + 3] = valid ? pack_pair(v.w, order_id(i.w)) : none;
+    }
+    return;
+  }
 #pragma unroll
   for (int p = 0; p < KMAX; ++p) {
     const int q = p - (KMAX - k);
@@ -62,6 +86,16 @@ __device__ __forceinline__ void topk_load(uint64_t (&pk)[KMAX], int k, bool vali
 template <int KMAX>
 __device__ __forceinline__ void topk_store(const uint64_t (&pk)[KMAX], int k, float* __restrict__ vals,
                                            int32_t* __restrict__ ids) {
+  if (topk_row_vector<KMAX>(k, vals, ids)) {
+#pragma unroll
+    for (int p = 0; p < KMAX; p += 4) {
+      *reinterpret_cast<float4*>(vals + p) =
+          make_float4(pair_key(pk[p]), pair_key(pk[p + 1]), pair_key(pk[p + 2]), pair_key(pk[p + 3]));
+      *reinterpret_cast<int4*>(ids + p) = make_int4(pair_id(pk[p]), pair_id(pk[p + 1]), pair_id(pk[p + 2]),
+                                                    pair_id(pk[p + 3]));
+    }
+    return;
+  }
 #pragma unroll
   for (int p = 0; p < KMAX; ++p) {
     const int q = p - (KMAX - k);
