@@ -26,6 +26,7 @@ EXPORTS = (
     "aknn_router_gradients", "aknn_router_logits", "aknn_router_param_count", "aknn_router_get_params",
     "aknn_router_set_params", "aknn_router_save", "aknn_router_load", "aknn_router_stats",
     "aknn_router_last_error", "aknn_set_router", "aknn_read_points", "aknn_router_bins", "aknn_add_router",
+    "aknn_recall_noalloc",
 )
 
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -71,6 +72,7 @@ def load():
     _i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
     L.aknn_exact_knn.argtypes = [H, _i64p, ctypes.c_int64, ctypes.c_int, _i32p, _f64p]
     L.aknn_recall.argtypes = [H, _i64p, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)]
+    L.aknn_recall_noalloc.argtypes = [H, _i64p, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)]
     L.aknn_set_bins.argtypes = [H, _i32p, ctypes.c_int64, ctypes.c_int]
     L.aknn_last_error.argtypes = [H]
     L.aknn_last_error.restype = ctypes.c_char_p
@@ -213,6 +215,14 @@ class Graph:
         q = np.ascontiguousarray(queries, dtype=np.int64)
         r = ctypes.c_double()
         self._check(self._L.aknn_recall(self._h, q, len(q), ctypes.byref(r)))
+        return r.value
+
+    def recall_noalloc(self, queries):
+        """recall() without device allocation (buffers from the library's
+        static device scratch; the same result)."""
+        q = np.ascontiguousarray(queries, dtype=np.int64)
+        r = ctypes.c_double()
+        self._check(self._L.aknn_recall_noalloc(self._h, q, len(q), ctypes.byref(r)))
         return r.value
 
     def set_bins(self, bins, bin_bits):

@@ -367,6 +367,15 @@ int aknn_recall(aknn_handle* h, const int64_t* queries, int64_t nq, double* reca
   });
 }
 
+int aknn_recall_noalloc(aknn_handle* h, const int64_t* queries, int64_t nq, double* recall) {
+  return guarded(h, [&] {
+    if (!h->engine || !h->calculated) return fail(h, AKNN_ERR_STATE, "nothing calculated");
+    if (nq < 0 || !recall || (nq > 0 && !queries)) return fail(h, AKNN_ERR_ARGUMENT, "null array");
+    *recall = aknn::recall_no_alloc(*h->engine, queries, nq);
+    return AKNN_OK;
+  });
+}
+
 int aknn_set_bins(aknn_handle* h, const int32_t* bins, int64_t n, int bin_bits) {
   return guarded(h, [&] {
     if (!h->engine) return fail(h, AKNN_ERR_STATE, "not initialized");

@@ -102,6 +102,11 @@ class Engine {
   }
   const float* raw_points() const { return x_raw_; }
   int ld() const { return ld_; }
+  // normalized points of the last calculate(), the current graph's ids, and
+  // the stream every launch of this engine goes to
+  const float* points() const { return x_; }
+  const int32_t* graph_ids() const { return gids_[cur_]; }
+  hipStream_t stream() const { return stream_; }
 
   void get_graph(int32_t* ids, float* vals) const;
   void get_normalized(float* x) const;
@@ -228,5 +233,13 @@ class Engine {
   bool rounds_from_graph_ = false;
   StageTimes times_;
 };
+
+// recall@k of e's current graph over host query indices, computed with the
+// kernels of Engine::recall on buffers carved from a static device array
+// (no device allocation); rejects queries whose buffers exceed that array.
+// Defined in noalloc.hip.
+double recall_no_alloc(Engine& e, const int64_t* queries, int64_t nq);
+// bytes of that array a call with these shapes uses
+size_t recall_scratch_bytes(int64_t n, int k, int64_t nq, bool cosine);
 
 }  // namespace aknn

@@ -204,6 +204,20 @@ int aknn_exact_knn(aknn_handle* h, const int64_t* queries, int64_t nq, int kq, i
  * No reference counterpart. */
 int aknn_recall(aknn_handle* h, const int64_t* queries, int64_t nq, double* recall);
 
+/* aknn_recall without device allocation: the same result, bit for bit, from
+ * the same kernels, whose buffers (query indices, exact ids and distances
+ * [nq][k], k-th distances, hit counter, and for COSINE the FP64 squared norms
+ * of all n points) come from one static device array of 48 MiB per GPU that
+ * the library holds until the process ends; the call makes no hipMalloc or
+ * hipFree.  They need 8 + 12k + 8 bytes per query (rounded up per buffer to
+ * 256 bytes), plus 8n for COSINE; a call needing more than the array returns
+ * AKNN_ERR_ARGUMENT.  The array is shared by every call on a GPU and keeps
+ * the last call's contents: calls that overlap on one GPU (several host
+ * threads or handles on the same device) race and must be serialised.
+ * No reference counterpart: for callers that cannot allocate device memory
+ * (tight serving loops, harnesses that forbid hipMalloc). */
+int aknn_recall_noalloc(aknn_handle* h, const int64_t* queries, int64_t nq, double* recall);
+
 /* Router bins of the points for the median-split forest's routed trees:
  * bins[n] with values in [0, 2^bin_bits), 1 <= bin_bits <= 20, n the number
  * of points.  The first bin_bits levels of a routed tree split a bucket by
