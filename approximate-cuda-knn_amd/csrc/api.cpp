@@ -313,6 +313,15 @@ int aknn_get_kernel_stats(aknn_handle* h, const char* name, int64_t* launches, d
       if (total_ms) *total_ms = l.screened_ms;
       if (units) *units = 2.0 * ldh * l.screened;
       return AKNN_OK;
+    } else if (n == "leaf_exact") {
+      // the leaf k-NN launches that ran the exact FP32 chain on every pair
+      // (all trees without the screen): 3D flops per pair
+      const aknn::KernelStats& l = h->engine->stats_leaf_knn;
+      const double per_pair = 3.0 * h->engine->d();
+      if (launches) *launches = l.launches - l.screened_launches;
+      if (total_ms) *total_ms = l.total_ms - l.screened_ms;
+      if (units) *units = l.units - per_pair * l.screened;
+      return AKNN_OK;
     } else
       return fail(h, AKNN_ERR_ARGUMENT, "unknown kernel " + n);
     if (launches) *launches = s->launches;

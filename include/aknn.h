@@ -171,11 +171,13 @@ int aknn_get_stage_times(aknn_handle* h, double* ms, int n);
 
 /* Kernel statistics of the last calculate(): name is "leaf_knn",
  * "propagate", "forest", "leaf_mfma" (the screened leaf k-NN launches
- * alone) or "propagate_graph" (the rounds replayed from the captured graph;
- * zero launches when they were launched one by one).  launches, summed launch time (HIP events on the launch stream), and
- * algorithmic units summed over the launches (flops for leaf_knn: 3D per
- * pair; GEMM-form flops for leaf_mfma: 2 * padded D per pair; bytes for
- * propagate and forest). */
+ * alone), "leaf_exact" (the leaf k-NN launches that ran the exact chain on
+ * every pair) or "propagate_graph" (the rounds replayed from the captured
+ * graph; zero launches when they were launched one by one).  launches,
+ * summed launch time (HIP events on the launch stream), and algorithmic units
+ * summed over the launches (flops for leaf_knn and leaf_exact: 3D per pair;
+ * GEMM-form flops for leaf_mfma: 2 * padded D per pair; bytes for propagate
+ * and forest). */
 int aknn_get_kernel_stats(aknn_handle* h, const char* name, int64_t* launches, double* total_ms, double* units);
 
 /* Candidate counts of the last calculate().  stage "propagate": new
