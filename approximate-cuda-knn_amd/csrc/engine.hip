@@ -70,6 +70,17 @@ void build_work_items(std::vector<Leaf>& leaves, std::vector<Leaf>& items) {
 }
 }  // namespace
 
+// Shapes a stage cannot run, refused before any work: a propagation round
+// keeps one wave's hash set, candidate list and x_i in LDS; the COSINE parity
+// pass keeps x_i in 64 KB of LDS.
+static void check_shapes(const Params& p, int d) {
+  if (p.props > 0 && !propagation_fits(p.k, d))
+    throw std::invalid_argument("propagation: k = " + std::to_string(p.k) + " with D = " + std::to_string(d) +
+                                " needs more than 160 KB of LDS for one wave ((2(k^2+k+1) + k^2 + D) * 4 bytes)");
+  if (p.metric == kCosine && !p.cosine_fixed && (d + 3) / 4 * 4 * 4 > 64 * 1024)
+    throw std::invalid_argument("COSINE (parity mode) supports D <= 16384");
+}
+
 Engine::Engine(int device) : device_(device) {
   int count = 0;
   AKNN_HIP_TRY(hipGetDeviceCount(&count));
@@ -206,6 +217,7 @@ void Engine::set_data(const float* x_rowmajor, int64_t n, int d, const Params& p
   if (p.leaf < 0 || p.props < 0) throw std::invalid_argument("leaf size and propagations must be >= 0");
   if (p.metric < kEuclidean || p.metric > kCosine) throw std::invalid_argument("unknown distance");
   if (p.split < 0 || p.split > 1) throw std::invalid_argument("split must be 0 (sign) or 1 (median)");
+  check_shapes(p, d);
   AKNN_HIP_TRY(hipSetDevice(device_));
   AKNN_HIP_TRY(hipStreamSynchronize(stream_));
   release();
@@ -737,6 +749,7 @@ bool Engine::propagate_graph(int base) {
 }
 
 void Engine::calculate() {
+  check_shapes(p_, d_);  // options may have changed the path since set_data
   AKNN_HIP_TRY(hipSetDevice(device_));
   cur_ = 0;
   draws_.reset(p_.seed);

@@ -131,6 +131,10 @@ struct PropagateArgs {
   PropagateHistory hist;
 };
 void launch_propagate(const PropagateArgs& a, hipStream_t s);
+// whether a propagation round with k neighbours over d dimensions fits the
+// LDS of at least one wave (the exact pass; screens that do not fit give way
+// to it)
+bool propagation_fits(int k, int d);
 
 // ---- principal-subspace screen (pca.hip) ------------------------------------
 // r (16 or 32, 0: the data is not close enough to a 16- or 32-dimensional

@@ -32,7 +32,8 @@ namespace aknn {
 
 namespace {
 
-constexpr int kWavesPerBlock = 4;
+constexpr int kWavesPerBlock = 4;  // waves per block when their LDS fits (fewer otherwise)
+constexpr size_t kLdsBytes = 160 * 1024;
 
 
 // Open-addressing set of ids in LDS (-1 = empty), linear probing.  The table
@@ -86,7 +87,7 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
   // Points are taken in `order` (the first tree's leaf order): waves running at
   // the same time then own nearby points whose candidate rows overlap, and
   // each XCD walks a contiguous part of that order, so its L2 sees the reuse.
-  const int64_t pidx = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+  const int64_t pidx = xcd_block(blockIdx.x, gridDim.x) * static_cast<int64_t>(blockDim.x >> 6) + wave;
   if (pidx >= n) return;
   const int64_t i = order != nullptr ? order[pidx] : pidx;
 
@@ -360,23 +361,46 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
   }
 }
 
+// LDS layout of one wave: x_i (and with a screen its screening row and two
+// staged rows), the hash set, the candidate list and its counter.
+struct WaveLds {
+  int hs_size = 0, cand_cap = 0, xi_stride = 0, waves = 0;
+  size_t bytes() const { return (static_cast<size_t>(xi_stride) + hs_size + cand_cap + 4) * 4 * waves; }
+};
+
+// Largest block (4, 2 or 1 waves) whose LDS fits; waves == 0 if none does.
+// At most 1 + k + k*k ids enter the set, which has at least twice as many
+// slots: a power of two when four waves fit with it (the measured layout),
+// else the smallest multiple of 64.
+WaveLds plan_lds(int k, int d, int ld, int ldh, int ld8, int pca_r, int skm) {
+  WaveLds w;
+  const int need = 2 * (k * k + k + 1);
+  // (multiples of 4 keep every wave's LDS block 16-byte aligned)
+  w.cand_cap = (k * k + 3) / 4 * 4;
+  w.xi_stride = skm == 1 ? ldh + 3 * ld : skm == 2 ? ld8 / 4 + 3 * ld : skm == 3 ? pca_r + ld : (d + 3) / 4 * 4;
+  int pow2 = 128;
+  while (pow2 < need) pow2 *= 2;
+  w.hs_size = pow2;
+  w.waves = kWavesPerBlock;
+  if (w.bytes() <= kLdsBytes) return w;
+  w.hs_size = (need + 63) / 64 * 64;
+  for (int wv = kWavesPerBlock; wv >= 1; wv /= 2) {
+    w.waves = wv;
+    if (w.bytes() <= kLdsBytes) return w;
+  }
+  w.waves = 0;
+  return w;
+}
+
 template <int METRIC, bool FLIP, int SKM>
 void launch_w(const PropagateArgs& a, hipStream_t s) {
-  // at most 1 + k + k*k ids enter the set: at least twice that many slots
-  int hs_size = 128;
-  while (hs_size < 2 * (a.k * a.k + a.k + 1)) hs_size *= 2;
-  // (a multiple of 4 keeps every wave's LDS block 16-byte aligned)
-  const int cand_cap = (a.k * a.k + 3) / 4 * 4;
-  const int xi_stride = SKM == 1   ? a.ldh + 3 * a.ld
-                        : SKM == 2 ? a.ld8 / 4 + 3 * a.ld
-                        : SKM == 3 ? a.pca_r + a.ld
-                                   : (a.d + 3) / 4 * 4;
-  const size_t per_wave = (static_cast<size_t>(xi_stride) + hs_size + cand_cap + 4) * 4;
-  const size_t shmem = per_wave * kWavesPerBlock;
-  if (shmem > 160 * 1024) throw HipError("propagation: k or d too large for LDS");
-  const int64_t blocks = xcd_grid((a.n + kWavesPerBlock - 1) / kWavesPerBlock);
+  const WaveLds w = plan_lds(a.k, a.d, a.ld, a.ldh, a.ld8, a.pca_r, SKM);
+  if (w.waves == 0) throw HipError("propagation: k or d too large for LDS");
+  const int hs_size = w.hs_size, cand_cap = w.cand_cap, xi_stride = w.xi_stride;
+  const size_t shmem = w.bytes();
+  const int64_t blocks = xcd_grid((a.n + w.waves - 1) / w.waves);
   const SketchBound sb = sketch_bound(a.d);
-  hipLaunchKernelGGL((propagate_kernel<METRIC, FLIP, SKM>), dim3(static_cast<unsigned>(blocks)), dim3(256), shmem,
+  hipLaunchKernelGGL((propagate_kernel<METRIC, FLIP, SKM>), dim3(static_cast<unsigned>(blocks)), dim3(64 * w.waves), shmem,
                      s, a.x, a.d, a.ld, a.xh, a.ldh, a.eps, sb, a.q8, a.ld8, a.s8, a.a8, a.e8, a.pca, a.pca_r, a.pca_e,
                      a.pca_sigma, a.norms, a.order, a.n, a.k, hs_size, cand_cap,
                      xi_stride, a.ids_in, a.vals_in, a.ids_out, a.vals_out, a.ncand,
@@ -396,23 +420,30 @@ void launch_t(const PropagateArgs& a, hipStream_t s) {
     launch_w<METRIC, FLIP, 0>(a, s);
     return;
   }
+  // a screen whose wider per-wave layout does not fit even one wave's LDS
+  // gives way to the exact pass (the same graph)
+  auto fits = [&](int skm) { return plan_lds(a.k, a.d, a.ld, a.ldh, a.ld8, a.pca_r, skm).waves > 0; };
   if constexpr (METRIC == kEuclidean) {
-    if (a.pca != nullptr) {
+    if (a.pca != nullptr && fits(3)) {
       launch_w<METRIC, FLIP, 3>(a, s);
       return;
     }
-    if (a.q8 != nullptr) {
+    if (a.q8 != nullptr && fits(2)) {
       launch_w<METRIC, FLIP, 2>(a, s);
       return;
     }
   }
-  if (a.xh != nullptr)
+  if (a.xh != nullptr && fits(1))
     launch_w<METRIC, FLIP, 1>(a, s);
   else
     launch_w<METRIC, FLIP, 0>(a, s);
 }
 
 }  // namespace
+
+bool propagation_fits(int k, int d) {
+  return plan_lds(k, d, (d + 3) / 4 * 4, 0, 0, 0, 0).waves > 0;
+}
 
 void launch_propagate(const PropagateArgs& a, hipStream_t s) {
   if (a.k > 64) throw HipError("propagation supports k <= 64");

@@ -10,6 +10,11 @@ are checked through properties that do not depend on size:
     values are bit-equal to the oracle's FP32 distance of the pair;
   * the normalized points equal the oracle's normalization (C3: the D > 256
     quirk at full size);
+  * for COSINE in parity mode, where every selection stores float(1 - key)
+    and carried entries are re-keyed by their stored values (.cu:806, :1007,
+    :1911), each sampled stored value lies on the pair's orbit
+    delta, f(delta), f(f(delta)), ... with f(v) = float(1 - (double)v) and
+    delta the oracle's FP32 distance of the pair;
   * recall@k against the FP64 exact k-NN (C5: the configuration's recall
     target of 0.95 is met by the plain RP forest at T=6, L=256, P=2).
 """
@@ -63,8 +68,57 @@ def check_values(oracle, xn, ids, vals, metric, rows):
             assert np.float32(vals[i, j]).view(np.uint32) == ref.view(np.uint32), (i, c)
 
 
+def flip_orbit(delta, steps=24):
+    """delta and its images under v -> float(1 - (double)v)."""
+    out = {np.float32(delta).view(np.uint32).item()}
+    v = np.float32(delta)
+    for _ in range(steps):
+        v = np.float32(1.0 - np.float64(v))
+        out.add(v.view(np.uint32).item())
+    return out
+
+
+def check_parity_cosine(oracle, xn, ids, vals, k, rows):
+    n = len(ids)
+    assert (ids >= 0).all(), "a stub survived although every start range holds k+1 points"
+    assert not (ids == np.arange(n)[:, None]).any()
+    s = np.sort(ids, axis=1)
+    assert not (s[:, 1:] == s[:, :-1]).any(), "duplicate ids in a row"
+    flips = 0
+    for i in rows:
+        for j in range(k):
+            c = ids[i, j]
+            delta = oracle.distance(xn[i], xn[c], 2)
+            v = np.float32(vals[i, j]).view(np.uint32).item()
+            assert v in flip_orbit(delta), (i, c, vals[i, j], delta)
+            flips += v != delta.view(np.uint32).item()
+    print(f"[gpu] COSINE parity: {flips} of {len(rows) * k} sampled values are flipped distances", flush=True)
+    assert flips > 0
+
+
 def same(a, b):
     return (a[0] == b[0]).all() and (a[1].view(np.uint32) == b[1].view(np.uint32)).all()
+
+
+@pytest.fixture(scope="module")
+def c3_points(datasets):
+    return datasets.generate("C3")
+
+
+def test_c2_full_size(aknn, oracle, datasets):
+    """C2 (1M x 128, U{0..255}): plain and screened forms store the same
+    graph, rows are valid, the normalization is the oracle's, and sampled
+    values are the oracle's FP32 distances."""
+    cfg = datasets.CONFIGS["C2"]
+    x = datasets.generate("C2")
+    g, out = run_both(aknn, x, cfg, 0)
+    assert same(out["plain"], out["chosen"])
+    ids, vals = out["chosen"]
+    check_rows(ids, vals, cfg["k"])
+    xn = g.normalized()
+    assert (xn.view(np.uint32) == oracle.normalize(x).view(np.uint32)).all()
+    check_values(oracle, xn, ids, vals, 0, np.random.default_rng(12).choice(len(x), 200, replace=False))
+    g.close()
 
 
 def test_c5_full_size(aknn, oracle, datasets):
@@ -81,9 +135,9 @@ def test_c5_full_size(aknn, oracle, datasets):
     assert r >= 0.95
 
 
-def test_c3_taxicab_full_size(aknn, oracle, datasets):
+def test_c3_taxicab_full_size(aknn, oracle, datasets, c3_points):
     cfg = datasets.CONFIGS["C3"]
-    x = datasets.generate("C3")
+    x = c3_points
     g, out = run_both(aknn, x, cfg, 1)
     assert same(out["plain"], out["chosen"])
     ids, vals = out["chosen"]
@@ -94,21 +148,35 @@ def test_c3_taxicab_full_size(aknn, oracle, datasets):
     check_values(oracle, xn, ids, vals, 1, np.random.default_rng(3).choice(len(x), 100, replace=False))
 
 
-def test_c4_cosine_full_size(aknn, datasets):
+def test_c3_cosine_full_size(aknn, oracle, datasets, c3_points):
+    """C3's COSINE run (1M x 784, parity mode): both forms store the same
+    graph; rows hold k distinct ids other than the point; sampled stored
+    values are the oracle's distances or their flips."""
+    cfg = datasets.CONFIGS["C3"]
+    x = c3_points
+    g, out = run_both(aknn, x, cfg, 2)
+    assert same(out["plain"], out["chosen"])
+    ids, vals = out["chosen"]
+    xn = g.normalized()
+    assert (xn.view(np.uint32) == oracle.normalize(x).view(np.uint32)).all()
+    check_parity_cosine(oracle, xn, ids, vals, cfg["k"], np.random.default_rng(13).choice(len(x), 60, replace=False))
+    g.close()
+
+
+def test_c4_cosine_full_size(aknn, oracle, datasets):
+    """C4 (2M x 96, COSINE parity, k = 30): both forms store the same graph;
+    sampled stored values are the oracle's distances or their flips."""
     cfg = datasets.CONFIGS["C4"]
     x = datasets.generate("C4")
     g, out = run_both(aknn, x, cfg, 2)
     assert same(out["plain"], out["chosen"])
     ids, vals = out["chosen"]
-    # COSINE parity: stored values alternate between keys and 1 - key, so
-    # rows are sorted by the stored value only as the last selection left them
-    real = ids >= 0
-    assert not (ids == np.arange(len(ids))[:, None]).any()
-    s = np.sort(np.where(real, ids, -np.arange(1, cfg["k"] + 1)[None, :]), axis=1)
-    assert not (s[:, 1:] == s[:, :-1]).any()
+    xn = g.normalized()
+    check_parity_cosine(oracle, xn, ids, vals, cfg["k"], np.random.default_rng(14).choice(len(x), 100, replace=False))
     r = g.recall(np.random.default_rng(4).choice(len(x), 1000, replace=False))
     print(f"[gpu] C4 recall@30 {r:.4f}", flush=True)
     assert r > 0.0
+    g.close()
 
 
 def test_c5_learned_form_full_size(aknn, oracle, datasets):

@@ -268,11 +268,34 @@ def test_leaf_mfma_changes_nothing(aknn, oracle, datasets, n, d, k, trees, leaf,
 @pytest.mark.parametrize("leaf_mfma", [0, 1])
 def test_leaf_list_sizes(aknn, oracle, datasets, k, leaf_mfma):
     """Owner lists of 24 and 64 slots holding 23, 40 and 64 pairs: the row sits
-    in the top slots of the list with empty slots below it.  No propagation
-    round: its LDS-resident lists stop short of k=64."""
+    in the top slots of the list with empty slots below it (no propagation
+    round: the leaf kernels alone)."""
     x = datasets.gmm_c1(3000, 40, clusters=5, seed=50 + k)[0]
     g = gpu_run(aknn, x, k, 3, 80, 0, 0, leaf_mfma=leaf_mfma)
     check_graph(oracle, g, x, k, 3, 80, 0, 0)
+
+
+@pytest.mark.parametrize("k,metric,opts", [(48, 0, {}), (48, 0, {"sketch": 1}), (64, 0, {"sketch": 1, "leaf_mfma": 1}),
+                                           (64, 1, {}), (64, 2, {}), (64, 2, {"cosine_fixed": 1, "sketch": 1})])
+def test_large_k_propagation(aknn, oracle, datasets, k, metric, opts):
+    """Propagation rounds at k = 48 and 64 (the reference propagates any k,
+    .cu:2768, :2797-2805): each wave's hash set and candidate list
+    (2(k^2+k+1) and k^2 slots) no longer fit four waves per block, so the
+    launch takes fewer waves per block."""
+    x = datasets.gmm_c1(2500, 24, clusters=5, seed=60 + k + metric)[0]
+    g = gpu_run(aknn, x, k, 3, 80, 2, metric, **opts)
+    check_graph(oracle, g, x, k, 3, 80, 2, metric, cosine_fixed=opts.get("cosine_fixed", 0))
+
+
+@pytest.mark.parametrize("metric,opts", [(0, {"sketch": 1, "leaf_mfma": 1}), (1, {"sketch": 1}),
+                                         (2, {"sketch": 1, "cosine_fixed": 1})])
+def test_wide_rows_screened_propagation(aknn, oracle, datasets, metric, opts):
+    """D = 3000 with the FP16 sketch: one wave's x_i, sketch row and two
+    staged rows take 48 KB of LDS, so the screened rounds run two waves per
+    block."""
+    x = datasets.gmm_c1(1200, 3000, clusters=4, seed=71 + metric)[0]
+    g = gpu_run(aknn, x, 20, 2, 60, 2, metric, **opts)
+    check_graph(oracle, g, x, 20, 2, 60, 2, metric, cosine_fixed=opts.get("cosine_fixed", 0))
 
 
 @pytest.mark.parametrize("n,d,k,trees,leaf,seed", [(4000, 40, 8, 4, 24, 45), (2500, 96, 30, 3, 100, 46),
