@@ -233,6 +233,8 @@ int aknn_set_option(aknn_handle* h, const char* name, int64_t value) {
       h->io_threads = static_cast<int>(value);
     } else if (n == "pca_screen") {
       h->params.pca_screen = value != 0;
+    } else if (n == "cosine_table") {
+      h->params.cosine_table = value != 0;
     } else if (n == "graph_rounds") {
       h->params.graph_rounds = value != 0;
     } else if (n == "router_trees") {
@@ -337,6 +339,13 @@ int aknn_get_candidate_counts(aknn_handle* h, const char* stage, double* examine
     if (!stage) return fail(h, AKNN_ERR_ARGUMENT, "null stage");
     const std::string n(stage);
     const aknn::KernelStats* s = nullptr;
+    if (n == "cosine_table") {
+      // the COSINE parity initial pass of the last build: 1 when its keys came
+      // from the per-tree key table
+      if (examined) *examined = h->engine->cosine_table_used() ? 1.0 : 0.0;
+      if (rescored) *rescored = 0.0;
+      return AKNN_OK;
+    }
     if (n == "pca") {
       // the principal-subspace screen of the last build: its width (0: unused)
       // and the share of the sample's second moment in the top 16 directions

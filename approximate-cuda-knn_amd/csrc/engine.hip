@@ -73,12 +73,17 @@ void build_work_items(std::vector<Leaf>& leaves, std::vector<Leaf>& items) {
 // Shapes a stage cannot run, refused before any work: a propagation round
 // keeps one wave's hash set, candidate list and x_i in LDS; the COSINE parity
 // pass keeps x_i in 64 KB of LDS.
-static void check_shapes(const Params& p, int d) {
+static void check_shapes(const Params& p, int64_t n, int d) {
   if (p.props > 0 && !propagation_fits(p.k, d))
     throw std::invalid_argument("propagation: k = " + std::to_string(p.k) + " with D = " + std::to_string(d) +
                                 " needs more than 160 KB of LDS for one wave ((2(k^2+k+1) + k^2 + D) * 4 bytes)");
-  if (p.metric == kCosine && !p.cosine_fixed && (d + 3) / 4 * 4 * 4 > 64 * 1024)
-    throw std::invalid_argument("COSINE (parity mode) supports D <= 16384");
+  if (p.metric == kCosine && !p.cosine_fixed) {
+    // x_i, the summary bitmap and the selection pool in LDS
+    const int64_t words = ((n + 31) / 32 + 31) / 32 * 32;
+    const int64_t cap = std::min<int64_t>(static_cast<int64_t>(p.k) * p.k + p.k, n) - p.k;
+    if (cosine_parity_lds(d, words / 32, p.k, cap, false) > 160 * 1024 - 4096)
+      throw std::invalid_argument("COSINE (parity mode): D, N or k too large for LDS (x_i 4D, N/256 and 8(k^2+k) bytes)");
+  }
 }
 
 Engine::Engine(int device) : device_(device) {
@@ -169,6 +174,9 @@ void Engine::release() {
   changed_[0] = changed_[1] = nullptr;
   fb_ = ForestBuffers{};
   cps_ = CosineParityScratch{};
+  koff_ = kbsum_ = nullptr;
+  ktab_ = nullptr;
+  ktab_cap_ = 0;
   dirs_ = proj_ = nullptr;
   pca_p_ = pca_mom_ = pca_proj_ = pca_e_ = nullptr;
   pca_ = PcaBasis{};
@@ -217,7 +225,7 @@ void Engine::set_data(const float* x_rowmajor, int64_t n, int d, const Params& p
   if (p.leaf < 0 || p.props < 0) throw std::invalid_argument("leaf size and propagations must be >= 0");
   if (p.metric < kEuclidean || p.metric > kCosine) throw std::invalid_argument("unknown distance");
   if (p.split < 0 || p.split > 1) throw std::invalid_argument("split must be 0 (sign) or 1 (median)");
-  check_shapes(p, d);
+  check_shapes(p, n, d);
   AKNN_HIP_TRY(hipSetDevice(device_));
   AKNN_HIP_TRY(hipStreamSynchronize(stream_));
   release();
@@ -432,43 +440,103 @@ void Engine::initial_knn() {
     }
     for (int t = 0; t < p_.trees; ++t)
       launch_leaf_ranges(perm_ + t * n_, leaves_ + t * n_, nleaves_[t], rs_ + t * n_, rc_ + t * n_, stream_);
+    // the leaf lists of all trees in one round trip
+    std::vector<int64_t> leaf_off(p_.trees + 1, 0);
+    for (int t = 0; t < p_.trees; ++t) leaf_off[t + 1] = leaf_off[t] + nleaves_[t];
+    Leaf* lh = static_cast<Leaf*>(pinned(0, sizeof(Leaf) * static_cast<size_t>(std::max<int64_t>(leaf_off[p_.trees], 1))));
+    for (int t = 0; t < p_.trees; ++t)
+      AKNN_HIP_TRY(hipMemcpyAsync(lh + leaf_off[t], leaves_ + t * n_, sizeof(Leaf) * nleaves_[t], hipMemcpyDeviceToHost,
+                                  stream_));
+    // offsets of every (tree, point) range in the key table: the exclusive
+    // prefix sum of the range sizes, tree after tree
+    const int64_t tn = static_cast<int64_t>(n_) * p_.trees;
+    if (!koff_) {
+      koff_ = alloc<int64_t>(tn + 1);
+      kbsum_ = alloc<int64_t>((tn + 1023) / 1024 + 1);
+    }
+    launch_count_scan(rc_, tn, koff_, kbsum_, koff_ + tn, stream_);
+    int64_t ktotal = 0;
+    AKNN_HIP_TRY(hipMemcpyAsync(&ktotal, koff_ + tn, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
+    AKNN_HIP_TRY(hipStreamSynchronize(stream_));
     // a point's union holds at most the sum over trees of the largest start
-    // range: that bounds each group's candidate scratch, and the scratch
-    // budget (4 GB) sets how many groups run at once (256..2048)
+    // range: that bounds each group's candidate scratch
     int64_t bound = 0;
-    {
-      std::vector<Leaf> host;
-      for (int t = 0; t < p_.trees; ++t) {
-        host.resize(nleaves_[t]);
-        AKNN_HIP_TRY(hipMemcpy(host.data(), leaves_ + t * n_, sizeof(Leaf) * nleaves_[t], hipMemcpyDeviceToHost));
-        int32_t mx = 0;
-        for (const Leaf& l : host) mx = std::max(mx, l.rcount);
-        bound += mx;
-      }
+    for (int t = 0; t < p_.trees; ++t) {
+      int32_t mx = 0;
+      for (int64_t l = leaf_off[t]; l < leaf_off[t + 1]; ++l) mx = std::max(mx, lh[l].rcount);
+      bound += mx;
     }
     const int64_t need = std::min<int64_t>(n_, std::max<int64_t>(bound, 1));
-    if (cps_.bits && need > cps_.max_cand) cps_ = CosineParityScratch{};  // grow (old buffers freed at release)
+    // The key table (4 bytes per range member) and each group's key-by-id
+    // array (4n bytes) are used when they fit in 70% of the free device
+    // memory.  Each group works through its points one after another, so
+    // 2048 of them (8 per CU) keep the device busy: their scratch takes up to
+    // 24 GB with the table, 4 GB without.
+    // bitmap words per group padded to whole 32-word lines (the enumeration
+    // reads a line per summary bit); summary bits live in LDS
+    const int64_t words = ((n_ + 31) / 32 + 31) / 32 * 32, summary = words / 32;
+    auto groups_for = [&](bool table) {
+      const int64_t per_group = 4 * (words + summary) + 8 * need + (table ? 4 * n_ : 0);
+      const int64_t fit = (int64_t{table ? 24 : 4} << 30) / std::max<int64_t>(per_group, 1);
+      return static_cast<int>(std::max<int64_t>(8, std::min<int64_t>({2048, fit / 8 * 8, (n_ + 7) / 8 * 8})));
+    };
+    bool table = p_.cosine_table != 0;
+    if (table) {
+      size_t free_b = 0, total_b = 0;
+      AKNN_HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+      const int64_t have_tab = ktab_cap_ >= ktotal ? 0 : ktotal * 4;
+      const int64_t have_arr = cps_.keyarr && cps_.max_cand >= need ? 0 : int64_t{groups_for(true)} * n_ * 4;
+      table = static_cast<double>(have_tab + have_arr) <= 0.7 * static_cast<double>(free_b);
+    }
+    if (cps_.bits && (need > cps_.max_cand || (table && !cps_.keyarr))) cps_ = CosineParityScratch{};  // grow (old buffers freed at release)
     if (!cps_.bits) {
-      const int64_t words = (n_ + 31) / 32, summary = (words + 31) / 32;
-      const int64_t per_group = 4 * (words + summary) + 8 * need;
-      const int64_t fit = (int64_t{4} << 30) / std::max<int64_t>(per_group, 1);
-      cps_.groups = static_cast<int>(std::max<int64_t>(8, std::min<int64_t>({2048, fit / 8 * 8, (n_ + 7) / 8 * 8})));
+      cps_.groups = groups_for(table);
       cps_.words = words;
       cps_.summary_words = summary;
       cps_.max_cand = need;
       cps_.bits = alloc<uint32_t>(static_cast<size_t>(cps_.groups) * cps_.words);
-      cps_.summary = alloc<uint32_t>(static_cast<size_t>(cps_.groups) * cps_.summary_words);
+      cps_.summary = nullptr;
       cps_.cand = alloc<int32_t>(static_cast<size_t>(cps_.groups) * cps_.max_cand);
       cps_.key = alloc<float>(static_cast<size_t>(cps_.groups) * cps_.max_cand);
+      cps_.keyarr = table ? alloc<float>(static_cast<size_t>(cps_.groups) * n_) : nullptr;
       AKNN_HIP_TRY(hipMemsetAsync(cps_.bits, 0, sizeof(uint32_t) * cps_.groups * cps_.words, stream_));
-      AKNN_HIP_TRY(hipMemsetAsync(cps_.summary, 0, sizeof(uint32_t) * cps_.groups * cps_.summary_words, stream_));
+    }
+    table = table && cps_.keyarr != nullptr;
+    if (table && ktab_cap_ < ktotal) {
+      ktab_ = alloc<float>(static_cast<size_t>(ktotal));
+      ktab_cap_ = ktotal;
     }
     const int64_t dim_idx = std::min<int64_t>(static_cast<int64_t>(p_.k) * p_.k + p_.k, n_);
     AKNN_HIP_TRY(hipEventRecord(launch_ev(0), stream_));
+    if (table) {
+      // every tree's keys, per work item (the leaf k-NN's items: big leaves
+      // cut into 64-owner items, runs of small ones grouped)
+      std::vector<int64_t> item_off(p_.trees + 1, 0);
+      for (int t = 0; t < p_.trees; ++t) item_off[t + 1] = item_off[t] + nleaves_[t] + n_ / kItemOwners + 1;
+      const size_t ni = static_cast<size_t>(item_off[p_.trees]);
+      if (ni > items_cap_) {
+        items_ = alloc<Leaf>(ni);
+        items_cap_ = ni;
+      }
+      Leaf* ih = static_cast<Leaf*>(pinned(1, sizeof(Leaf) * ni));
+      std::vector<Leaf> host, its;
+      for (int t = 0; t < p_.trees; ++t) {
+        host.assign(lh + leaf_off[t], lh + leaf_off[t + 1]);
+        build_work_items(host, its);
+        const int nit = static_cast<int>(its.size());
+        if (nit > item_off[t + 1] - item_off[t]) throw HipError("leaf work items exceed their bound");
+        std::copy(its.begin(), its.end(), ih + item_off[t]);
+        AKNN_HIP_TRY(hipMemcpyAsync(items_ + item_off[t], ih + item_off[t], sizeof(Leaf) * nit, hipMemcpyHostToDevice,
+                                    stream_));
+        launch_cosine_keys(x_, d_, ld_, norms_, perm_ + t * n_, items_ + item_off[t], nit, kItemOwners, rs_ + t * n_,
+                           rc_ + t * n_, koff_ + t * n_, ktab_, stream_);
+      }
+    }
     launch_cosine_parity_initial(x_, d_, ld_, norms_, n_, p_.k, p_.trees, rs_, rc_, perm_, dim_idx - p_.k, cps_, ids,
-                                 vals, stream_);
+                                 vals, table ? ktab_ : nullptr, table ? koff_ : nullptr, stream_);
     AKNN_HIP_TRY(hipEventRecord(launch_ev(1), stream_));
     stats_leaf_knn.launches = 1;
+    cosine_table_used_ = table;
     return;
   }
   // Work items per tree (host, from the leaf lists): leaves in position order,
@@ -547,7 +615,7 @@ void Engine::initial_knn() {
     AKNN_HIP_TRY(hipEventRecord(launch_ev(2 * t), stream_));
     // the first tree meets stub rows (nothing to screen against): plain kernel
     if (mfma && t > 0)
-      launch_leaf_mfma(kmetric(), x_, d_, ld_, xh_, ldh_, eps_, nh_, norms_, perm_ + t * n_, it, nit, kItemOwners,
+      launch_leaf_mfma(kmetric(), x_, d_, ld_, xh_, ldh_, eps_, eps_ + n_, nh_, norms_, perm_ + t * n_, it, nit, kItemOwners,
                        p_.k, ids, vals, rs_ + t * n_, rc_ + t * n_, leaf_rescored_, leaf_rescored_ + 2 * kSlotWords,
                        stream_);
     else
@@ -749,7 +817,7 @@ bool Engine::propagate_graph(int base) {
 }
 
 void Engine::calculate() {
-  check_shapes(p_, d_);  // options may have changed the path since set_data
+  check_shapes(p_, n_, d_);  // options may have changed the path since set_data
   AKNN_HIP_TRY(hipSetDevice(device_));
   cur_ = 0;
   draws_.reset(p_.seed);

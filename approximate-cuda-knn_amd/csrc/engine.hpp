@@ -34,6 +34,7 @@ struct Params {
   int scale_all = 0;      // normalization scales every dimension, not only the first 256
   int graph_rounds = 1;   // once the round mode is fixed, replay the propagation rounds as one captured hipGraph
   int pca_screen = 0;     // EUCLIDEAN propagation screens with a principal-subspace bound when the data allow (same results)
+  int cosine_table = 1;   // COSINE parity initial pass reads its keys from a per-tree key table when it fits (same results)
 };
 
 // Per-stage times of the last calculate(), milliseconds.
@@ -82,6 +83,7 @@ class Engine {
     p_.scale_all = p.scale_all;
     p_.graph_rounds = p.graph_rounds;
     p_.pca_screen = p.pca_screen;
+    p_.cosine_table = p.cosine_table;
     drop_round_graph();
   }
   // Router bins of the points (host array [n], values < 2^bin_bits): the
@@ -118,6 +120,7 @@ class Engine {
   bool rounds_from_graph() const { return rounds_from_graph_; }
   // basis width of the last build's principal-subspace screen (0: not used)
   int pca_rank() const { return pca_.r; }
+  bool cosine_table_used() const { return cosine_table_used_; }
   double pca_retained16() const { return pca_.retained16; }
   int tree_iterations(int tree) const { return tree < static_cast<int>(iters_.size()) ? iters_[tree] : -1; }
 
@@ -203,6 +206,13 @@ class Engine {
   void* pinned(int slot, size_t bytes);
   int32_t* rc_ = nullptr;
   CosineParityScratch cps_{};
+  // COSINE parity key table: offsets [trees][n] (+ total), block sums of
+  // their scan, the keys, and whether the last build used it
+  int64_t* koff_ = nullptr;
+  int64_t* kbsum_ = nullptr;
+  float* ktab_ = nullptr;
+  int64_t ktab_cap_ = 0;
+  bool cosine_table_used_ = false;
   int64_t active_sum_ = 0;
   std::vector<uint8_t> round_screened_, tree_screened_;  // which rounds / trees used the screens
   int prop_mode_ = -1;  // propagation rounds learned on this data: -1 unknown, 0 screened, 1 exact

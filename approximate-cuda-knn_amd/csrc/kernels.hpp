@@ -170,10 +170,11 @@ void launch_sketch8(const float* x, int64_t n, int d, int ld, int8_t* q8, int ld
 // Initial leaf k-NN for one tree, EUCLIDEAN or COSINE with carried distances,
 // with the candidate pairs screened by half-precision MFMA dot products of the
 // sketch and proven lower bounds; the same graph as launch_leaf_knn.
-// norms: the COSINE squared-norm chains.  rescored, screened: optional device
-// counters of pairs whose exact distance was computed and of pairs screened.
+// norms: the COSINE squared-norm chains; eps_max: device address of the
+// largest sketch radius.  rescored, screened: optional device counters of
+// pairs whose exact distance was computed and of pairs screened.
 void launch_leaf_mfma(int metric, const float* x, int d, int ld, const _Float16* xh, int ldh, const float* eps,
-                      const float* nh, const float* norms, const int32_t* perm, const Leaf* leaves, int nleaves,
+                      const float* eps_max, const float* nh, const float* norms, const int32_t* perm, const Leaf* leaves, int nleaves,
                       int max_leaf, int k, int32_t* ids, float* vals, const int32_t* rsp, const int32_t* rcp,
                       unsigned long long* rescored, unsigned long long* screened, hipStream_t s);
 
@@ -196,11 +197,28 @@ struct CosineParityScratch {
   uint32_t* summary;  // [groups][summary_words]
   int32_t* cand;      // [groups][max_cand]
   float* key;         // [groups][max_cand]
+  float* keyarr;      // [groups][n] keys by candidate id (with a key table; else null)
   int64_t words, summary_words, max_cand;
   int groups;
 };
+// ktab/koff (nullable): the per-tree key table of launch_cosine_keys and its
+// offsets; without them the keys are computed from the rows per point.
 void launch_cosine_parity_initial(const float* x, int d, int ld, const float* norms, int64_t n, int k, int trees,
                                   const int32_t* rs, const int32_t* rc, const int32_t* perm, int64_t cap,
-                                  const CosineParityScratch& sc, int32_t* ids, float* vals, hipStream_t s);
+                                  const CosineParityScratch& sc, int32_t* ids, float* vals, const float* ktab,
+                                  const int64_t* koff, hipStream_t s);
+// Dynamic LDS bytes of the COSINE parity pass: x_i (without the key table),
+// the summary bitmap (n / 1024 bits) and the selection pool (k + cap pairs).
+size_t cosine_parity_lds(int d, int64_t summary_words, int k, int64_t cap, bool table);
+// One tree's COSINE keys (parity mode) of every owner of the work items
+// against its own start-node range: key(i, perm[r]) at keys[off[i] + r -
+// rs[i]]; max_owners: the largest item size.
+void launch_cosine_keys(const float* x, int d, int ld, const float* norms, const int32_t* perm, const Leaf* items,
+                        int nitems, int max_owners, const int32_t* rs, const int32_t* rc, const int64_t* off,
+                        float* keys, hipStream_t s);
+// offsets[i] = counts[0] + ... + counts[i-1] (int64); block_sums needs
+// ceil(n / 1024) entries; *total receives the sum.
+void launch_count_scan(const int32_t* counts, int64_t n, int64_t* offsets, int64_t* block_sums, int64_t* total,
+                       hipStream_t s);
 
 }  // namespace aknn

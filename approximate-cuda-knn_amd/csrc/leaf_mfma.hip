@@ -63,12 +63,24 @@ __device__ __forceinline__ float owner_threshold(float thk, float eo, double kap
   }
 }
 
+// Square of the EUCLIDEAN sketch-distance threshold of an owner against a
+// candidate of sketch radius ce, rounded up: (T + ce)^2 with margins for the
+// FP32 rounding of the sum and the square.
+__device__ __forceinline__ float e_threshold2(float t0, float ce) {
+  const float t = (t0 + ce) * 1.0000002384185791f;              // (1 + 4u)
+  return __builtin_fmaf(t, t, 1e-30f) * 1.0000004768371582f;    // (1 + 8u)
+}
+
+// bits [0, x) of a 64-bit lane mask, x in [0, 64]
+__device__ __forceinline__ uint64_t lane_mask_lt(int x) { return x >= 64 ? ~0ull : (1ull << x) - 1; }
+
 struct MfmaLeafArgs {
   const float* x;
   int d, ld;
   const _Float16* xh;
   int ldh;
   const float* eps;
+  const float* eps_max;  // the largest sketch radius (infinite if a row has none)
   const float* nh;
   const float* norms;  // COSINE: the exact squared-norm chains
   const int32_t* perm;
@@ -106,10 +118,10 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
   __shared__ __attribute__((aligned(16))) float o_e[kTile];  // C: sketch radius
   __shared__ __attribute__((aligned(16))) float o_s[kTile];  // C: sqrtf(aa), as the exact key uses
   __shared__ float c_n[kTile], c_e[kTile], c_s[kTile];
-  __shared__ uint64_t kp[kTile];  // re-scored pairs, packed for the owners' top-k
+  __shared__ uint64_t kp[kTile];   // re-scored pairs that can enter, packed for the owners' top-k
+  __shared__ uint64_t o_k[kTile];  // each owner's current k-th pair
   __shared__ __attribute__((aligned(16))) int32_t o_lo[kTile];  // owner's start-node positions [lo, hi)
   __shared__ __attribute__((aligned(16))) int32_t o_hi[kTile];
-  __shared__ uint64_t survm[kTile];
   __shared__ uint16_t plist[kTile * kTile];
 
   const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
@@ -141,7 +153,21 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
     o_s[lane] = sqrtf(aa);
     o_e[lane] = my_eps;
   }
-  o_t[lane] = owner_threshold<METRIC>(pair_key(pk[KMAX - 1]), my_eps, a.kap);
+  // EUCLIDEAN: with every candidate's sketch radius bounded by the largest
+  // (eps_max), the pair threshold depends on the owner only, so each owner
+  // holds its squared threshold; an infinite largest radius (a row the
+  // sketch cannot hold) leaves the candidates' own radii in the test
+  const float ce_max = *a.eps_max;
+  const bool per_cand = !(ce_max < __builtin_inff());
+  auto owner_t = [&](uint64_t kth) {
+    const float t = owner_threshold<METRIC>(pair_key(kth), my_eps, a.kap);
+    if constexpr (METRIC == kEuclidean) return per_cand ? t : e_threshold2(t, ce_max);
+    return t;
+  };
+  o_t[lane] = owner_t(pk[KMAX - 1]);
+  if constexpr (METRIC == kEuclidean) o_n[lane] *= a.c1m;
+  const int32_t my_lo = o_lo[lane], my_hi = o_hi[lane];
+  const int32_t my_pos = L.start + o0 + lane;  // the owner's own position: its self pair
   __syncthreads();
   const int32_t oa = oid[col], ob = oid[32 + col];
   const _Float16* arow0s = a.xh + static_cast<int64_t>(oa >= 0 ? oa : 0) * a.ldh;
@@ -156,27 +182,35 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
     const int32_t my_c = lane < ncand ? a.perm[L.rstart + cb + lane] : -1;
     cid[lane] = my_c;
     if constexpr (METRIC == kEuclidean) {
-      c_n[lane] = my_c >= 0 ? a.nh[my_c] : 0.0f;
+      c_n[lane] = my_c >= 0 ? a.nh[my_c] * a.c1m : 0.0f;
     } else {
       const float bb = my_c >= 0 ? a.norms[my_c] : 0.0f;
       c_n[lane] = norm_upper(bb, a.kap, tiny);
       c_s[lane] = sqrtf(bb);
     }
     c_e[lane] = my_c >= 0 ? a.eps[my_c] : 0.0f;
-    survm[lane] = 0;
     __syncthreads();
 
     // 1-2. the owner tiles ot = 0, 1 in turn, each against both candidate
     //      tiles: dot products of the sketches on the matrix cores, four K
     //      steps of fragments loaded before their MFMAs (one A fragment feeds
-    //      both candidate tiles), then the screen of the two quadrants.  This
-    //      lane holds candidate column j = 32*ct + col against owner rows
-    //      i = 32*ot + 8*g + 4*half + q (register 4*g + q).  Loads are never
-    //      behind a branch: rows of absent owners or candidates read row 0
-    //      (their sums are never screened in), and the K tail past ldh is
-    //      zeroed.
+    //      both candidate tiles), then the screen.  This lane holds candidate
+    //      column j = 32*ct + col against owner rows i = 32*ot + 8*g +
+    //      4*half + q (register 4*g + q).  Loads are never behind a branch:
+    //      rows of absent owners or candidates read row 0 (their pairs are
+    //      masked out below), and the K tail past ldh is zeroed.  One ballot
+    //      per (g, q, ct) gives the 32 candidates' verdicts for owner
+    //      32*ot + 8*g + q (low half) and owner ... + 4 (high half); each
+    //      owner lane keeps its half as its mask word for tile ct.
     const _Float16* bsafe0 = a.xh + static_cast<int64_t>(cid[col] >= 0 ? cid[col] : 0) * a.ldh;
     const _Float16* bsafe1 = a.xh + static_cast<int64_t>(cid[32 + col] >= 0 ? cid[32 + col] : 0) * a.ldh;
+    const float cn0 = c_n[col], cn1 = c_n[32 + col];
+    const float ce0 = c_e[col], ce1 = c_e[32 + col];
+    // this owner lane's survivor bits for candidate tiles 0 and 1; owner
+    // lane 32*ot + 8*g + 4*h + q takes half h of the ballots at (ot, g, q)
+    uint32_t mw0 = 0, mw1 = 0;
+    const int lkey = lane & 0x3B;
+    const bool hbit = (lane & 4) != 0;
 #pragma unroll 1
     for (int ot = 0; ot < 2; ++ot) {
       const _Float16* asafe = ot == 0 ? arow0s : arow1s;
@@ -214,42 +248,32 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
         }
       }
 #pragma unroll
-      for (int ct = 0; ct < 2; ++ct) {
-        const f32x16& acc = acc2[ct];
-        const int j = 32 * ct + col;
-        const int32_t cj = cid[j];
-        const float cn = c_n[j], ce = c_e[j];
-        const float cs = METRIC == kCosine ? c_s[j] : 0.0f;
-        const int32_t pos = L.rstart + cb + j;
+      for (int g = 0; g < 4; ++g) {
+        const int i0 = 32 * ot + 8 * g + 4 * half;
+        const float4 on4 = *reinterpret_cast<const float4*>(&o_n[i0]);
+        const float4 to4 = *reinterpret_cast<const float4*>(&o_t[i0]);
+        const float on[4] = {on4.x, on4.y, on4.z, on4.w};
+        const float to[4] = {to4.x, to4.y, to4.z, to4.w};
+        float oe[4] = {0.0f, 0.0f, 0.0f, 0.0f}, os[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if constexpr (METRIC == kCosine) {
+          const float4 oe4 = *reinterpret_cast<const float4*>(&o_e[i0]);
+          const float4 os4 = *reinterpret_cast<const float4*>(&o_s[i0]);
+          oe[0] = oe4.x; oe[1] = oe4.y; oe[2] = oe4.z; oe[3] = oe4.w;
+          os[0] = os4.x; os[1] = os4.y; os[2] = os4.z; os[3] = os4.w;
+        }
 #pragma unroll
-        for (int g = 0; g < 4; ++g) {
-          const int i0 = 32 * ot + 8 * g + 4 * half;
-          const float4 on4 = *reinterpret_cast<const float4*>(&o_n[i0]);
-          const float4 to4 = *reinterpret_cast<const float4*>(&o_t[i0]);
-          const int4 oi4 = *reinterpret_cast<const int4*>(&oid[i0]);
-          const int4 lo4 = *reinterpret_cast<const int4*>(&o_lo[i0]);
-          const int4 hi4 = *reinterpret_cast<const int4*>(&o_hi[i0]);
-          const int32_t olo[4] = {lo4.x, lo4.y, lo4.z, lo4.w};
-          const int32_t ohi[4] = {hi4.x, hi4.y, hi4.z, hi4.w};
-          const float on[4] = {on4.x, on4.y, on4.z, on4.w};
-          const float to[4] = {to4.x, to4.y, to4.z, to4.w};
-          const int32_t oi[4] = {oi4.x, oi4.y, oi4.z, oi4.w};
-          float oe[4] = {0.0f, 0.0f, 0.0f, 0.0f}, os[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-          if constexpr (METRIC == kCosine) {
-            const float4 oe4 = *reinterpret_cast<const float4*>(&o_e[i0]);
-            const float4 os4 = *reinterpret_cast<const float4*>(&o_s[i0]);
-            oe[0] = oe4.x; oe[1] = oe4.y; oe[2] = oe4.z; oe[3] = oe4.w;
-            os[0] = os4.x; os[1] = os4.y; os[2] = os4.z; os[3] = os4.w;
-          }
-          uint32_t hit = 0;
+        for (int ct = 0; ct < 2; ++ct) {
+          const f32x16& acc = acc2[ct];
+          const float cn = ct == 0 ? cn0 : cn1, ce = ct == 0 ? ce0 : ce1;
+          const float cs = METRIC == kCosine ? c_s[32 * ct + col] : 0.0f;
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             bool keep;
             if constexpr (METRIC == kEuclidean) {
-              const float s_lo = __builtin_fmaf(-2.0f, acc[4 * g + q], (on[q] + cn) * a.c1m);
-              const float t = (to[q] + ce) * 1.0000002384185791f;                    // (1 + 4u)
-              const float t2 = __builtin_fmaf(t, t, 1e-30f) * 1.0000004768371582f;   // (1 + 8u)
-              keep = !(s_lo > t2);
+              // on and cn carry the factor c1m: s_lo is a lower bound of the
+              // sketch squared distance, to the owner threshold's square
+              const float s_lo = __builtin_fmaf(-2.0f, acc[4 * g + q], on[q] + cn);
+              keep = !(s_lo > (per_cand ? e_threshold2(to[q], ce) : to[q]));
             } else {
               // reject when up(G) + rounding margin < T_o * den * (1 - 4u)
               const float G = acc[4 * g + q];
@@ -261,24 +285,32 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
               const float rhs = to[q] * den * 0.99999976158142090f - 1e-30f;  // (1 - 4u)
               keep = !(den > 0.0f && den <= 3.4e38f && lhs < rhs);
             }
-            hit |= static_cast<uint32_t>(oi[q] >= 0 && cj != oi[q] && pos >= olo[q] && pos < ohi[q] && keep) << q;
-          }
-          if (cj >= 0 && hit != 0) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-              if (hit & (1u << q)) atomicOr(reinterpret_cast<unsigned long long*>(&survm[i0 + q]), 1ull << j);
+            const uint64_t b = __ballot(keep);
+            const uint32_t w = hbit ? static_cast<uint32_t>(b >> 32) : static_cast<uint32_t>(b);
+            if (lkey == 32 * ot + 8 * g + q) {
+              if (ct == 0)
+                mw0 = w;
+              else
+                mw1 = w;
+            }
           }
         }
       }
     }
-    __syncthreads();
 
-    // 3. survivors: owner-major pair list, exact keys on all lanes, owners
-    //    insert.  A survivor already in the owner's row (found by an earlier
-    //    tree: close points share start nodes) is dropped here; its exact key
-    //    is already stored.
-    uint64_t m = survm[lane];
-    if (m != 0) {
+    // 3. survivors of this owner: its own start-node positions within the
+    //    tile (which also drops absent candidates), minus its self pair, minus
+    //    candidates already in its row (found by an earlier tree: close
+    //    points share start nodes; their exact keys are already stored).
+    const int32_t tbase = L.rstart + cb;
+    const int lo_c = my_lo - tbase > 0 ? my_lo - tbase : 0;
+    const int hi_c = my_hi - tbase < ncand ? my_hi - tbase : ncand;
+    uint64_t m = 0;
+    if (has_owner && hi_c > lo_c) {
+      m = (static_cast<uint64_t>(mw1) << 32) | mw0;
+      m &= lane_mask_lt(hi_c) & ~lane_mask_lt(lo_c);
+      const int sp = my_pos - tbase;
+      if (sp >= 0 && sp < kTile) m &= ~(1ull << sp);
       uint64_t mm = m;
       while (mm) {
         const int c = __ffsll(static_cast<long long>(mm)) - 1;
@@ -306,31 +338,43 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
       }
     }
     nres += total;
+    o_k[lane] = has_owner ? pk[KMAX - 1] : 0ull;
     __syncthreads();
+    // exact keys on all lanes; a pair that does not undercut its owner's
+    // current k-th pair cannot enter and is not handed over.  The pairs that
+    // can are compacted in list order (owner-major), so each owner finds its
+    // own from the ballot and its list segment [off, off + cnt).
     for (int base = 0; base < total; base += kTile) {
       const int p = base + lane;
-      float key = 0.0f;
-      int e = 0;
+      uint64_t v = ~0ull;
       if (p < total) {
-        e = plist[p];
+        const int e = plist[p];
         const float* xo = a.x + static_cast<int64_t>(oid[e >> 6]) * a.ld;
         const float* xc = a.x + static_cast<int64_t>(cid[e & 63]) * a.ld;
+        float key;
         if constexpr (METRIC == kEuclidean)
           key = finish_key<kEuclidean>(row_chain_global<kEuclidean, kRescoreUnroll>(xo, xc, a.d), 1.0f, 1.0f);
         else
           key = finish_key<kCosine>(row_chain_global<kCosine, kRescoreUnroll>(xo, xc, a.d), a.norms[oid[e >> 6]],
                                     a.norms[cid[e & 63]]);
+        v = pack_pair(key, order_id(cid[e & 63]));
+        if (!(v < o_k[e >> 6])) v = ~0ull;
       }
-      kp[lane] = pack_pair(key, order_id(cid[e & 63]));
+      const uint64_t ins = __ballot(v != ~0ull);
+      if (v != ~0ull) kp[__popcll(ins & lane_mask_lt(lane))] = v;
       __syncthreads();
       if (has_owner) {
-        const int lo = off > base ? off : base;
-        const int hi = off + cnt < base + kTile ? off + cnt : base + kTile;
-        for (int q = lo; q < hi; ++q) topk_offer<KMAX>(pk, kp[q - base]);
+        const int lo = (off > base ? off : base) - base;
+        const int hi = (off + cnt < base + kTile ? off + cnt : base + kTile) - base;
+        if (lo < hi) {
+          const int q0 = __popcll(ins & lane_mask_lt(lo)), q1 = __popcll(ins & lane_mask_lt(hi));
+          for (int q = q0; q < q1; ++q) topk_offer<KMAX>(pk, kp[q]);
+          o_k[lane] = pk[KMAX - 1];
+        }
       }
       __syncthreads();
     }
-    if (has_owner && cnt > 0) o_t[lane] = owner_threshold<METRIC>(pair_key(pk[KMAX - 1]), my_eps, a.kap);
+    if (has_owner && cnt > 0) o_t[lane] = owner_t(pk[KMAX - 1]);
   }
   // counters spread over slots one 128-byte line apart (summed by the host)
   const int64_t cslot = (blockIdx.x % kCountSlots) * kCountStride;
@@ -371,7 +415,7 @@ void launch_m(const MfmaLeafArgs& a, int k, hipStream_t s) {
 }  // namespace
 
 void launch_leaf_mfma(int metric, const float* x, int d, int ld, const _Float16* xh, int ldh, const float* eps,
-                      const float* nh, const float* norms, const int32_t* perm, const Leaf* leaves, int nleaves,
+                      const float* eps_max, const float* nh, const float* norms, const int32_t* perm, const Leaf* leaves, int nleaves,
                       int max_leaf, int k, int32_t* ids, float* vals, const int32_t* rsp, const int32_t* rcp,
                       unsigned long long* rescored, unsigned long long* screened, hipStream_t s) {
   if (nleaves <= 0) return;
@@ -385,6 +429,7 @@ void launch_leaf_mfma(int metric, const float* x, int d, int ld, const _Float16*
   a.xh = xh;
   a.ldh = ldh;
   a.eps = eps;
+  a.eps_max = eps_max;
   a.nh = nh;
   a.norms = norms;
   a.perm = perm;

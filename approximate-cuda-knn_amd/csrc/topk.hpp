@@ -62,14 +62,22 @@ __device__ __forceinline__ void topk_load(uint64_t (&pk)[KMAX], int k, bool vali
                                           const int32_t* __restrict__ ids) {
   const uint64_t none = pack_pair(__builtin_inff(), order_id(0x7fffffff));
   if (topk_row_vector<KMAX>(k, vals, ids)) {
+    // every piece is requested before any is used (the row pointer is
+    // readable for every lane), then invalid lanes are masked to `none`
+    float4 v[KMAX / 4];
+    int4 i[KMAX / 4];
 #pragma unroll
-    for (int p = 0; p < KMAX; p += 4) {
-      const float4 v = *reinterpret_cast<const float4*>(vals + p);
-      const int4 i = *reinterpret_cast<const int4*>(ids + p);
-      pk[p + 0] = valid ? pack_pair(v.x, order_id(i.x)) : none;
-      pk[p + 1] = valid ? pack_pair(v.y, order_id(i.y)) : none;
-      pk[p + 2] = valid ? pack_pair(v.z, order_id(i.z)) : none;
-      pk[p + 3] = valid ? pack_pair(v.w, order_id(i.w)) : none;
+    for (int p = 0; p < KMAX / 4; ++p) {
+      v[p] = reinterpret_cast<const float4*>(vals)[p];
+      i[p] = reinterpret_cast<const int4*>(ids)[p];
+    }
+    const uint64_t keep = valid ? ~0ull : 0ull;
+#pragma unroll
+    for (int p = 0; p < KMAX / 4; ++p) {
+      pk[4 * p + 0] = (pack_pair(v[p].x, order_id(i[p].x)) & keep) | (none & ~keep);
+      pk[4 * p + 1] = (pack_pair(v[p].y, order_id(i[p].y)) & keep) | (none & ~keep);
+      pk[4 * p + 2] = (pack_pair(v[p].z, order_id(i[p].z)) & keep) | (none & ~keep);
+      pk[4 * p + 3] = (pack_pair(v[p].w, order_id(i[p].w)) & keep) | (none & ~keep);
     }
     return;
   }

@@ -139,6 +139,13 @@ int aknn_deinitialize(aknn_handle* h);
  *   "io_threads"    host threads parsing the CSV in aknn_initialize (0, the
  *                   default: up to 16; 1: the sequential loader); the values
  *                   are the same for any count
+ *   "cosine_table"  1 (default): the COSINE parity initial pass computes each
+ *                   tree's keys (point, start-range member) once into a
+ *                   table, when it and its per-workgroup arrays fit in 70% of
+ *                   the free device memory (4 bytes per range member), and
+ *                   reads them in id order; 0: each point computes the keys
+ *                   of its candidates from their rows (the results are the
+ *                   same)
  *   "graph_rounds"  1 (default): once the propagation round mode is fixed (the
  *                   first calculate() probes it, or nothing is screened), the
  *                   rounds are captured into one hipGraph and replayed by later
@@ -188,8 +195,10 @@ int aknn_get_kernel_stats(aknn_handle* h, const char* name, int64_t* launches, d
  * points over all levels, and those the sketch left to the exact chain (0
  * without "sketch").  stage "pca": the principal-subspace screen's width
  * (16, 32, or 0 when unused) and the share of the sample's second moment its
- * top 16 directions hold.  No reference counterpart: the work accounting of the
- * kernel statistics above. */
+ * top 16 directions hold.  Stage "cosine_table": 1 when the COSINE parity
+ * initial pass read its keys from the key table (option cosine_table), else 0.
+ * No reference counterpart: the work accounting of the kernel statistics
+ * above. */
 int aknn_get_candidate_counts(aknn_handle* h, const char* stage, double* examined, double* rescored);
 
 /* Exact k-NN in FP64 of nq query points (indices into the data) against all

@@ -129,6 +129,23 @@ def test_cosine_bit_exact(aknn, oracle, datasets, cosine_fixed, props):
     check_graph(oracle, g, x, 6, 3, 24, props, 2, cosine_fixed=cosine_fixed)
 
 
+@pytest.mark.parametrize("n,d,k,trees,leaf,kind", [(2500, 20, 6, 3, 24, "gmm"), (1500, 300, 7, 3, 50, "sparse"),
+                                                   (3000, 40, 30, 5, 60, "gmm"), (800, 16, 12, 2, 0, "gmm")])
+def test_cosine_parity_key_table(aknn, oracle, datasets, n, d, k, trees, leaf, kind):
+    """COSINE parity initial pass from the per-tree key table (default) and
+    from per-point row streaming: the same graph, the oracle's (D = 300 takes
+    the 256-dimension chunked chains; leaf 0 gives one-point leaves whose start
+    ranges are their ancestors')."""
+    x = (datasets.gmm_c1(n, d, clusters=5, seed=80 + k)[0] if kind == "gmm"
+         else datasets.sparse_c3(n, d, clusters=6, seed=81))
+    a = gpu_run(aknn, x, k, trees, leaf, 1, 2)
+    assert a.candidate_counts("cosine_table")[0] == 1.0
+    b = gpu_run(aknn, x, k, trees, leaf, 1, 2, cosine_table=0)
+    assert b.candidate_counts("cosine_table")[0] == 0.0
+    _same_graph(a, b)
+    check_graph(oracle, a, x, k, trees, leaf, 1, 2)
+
+
 @pytest.mark.parametrize("metric", [0, 1, 2])
 def test_wide_rows_bit_exact(aknn, oracle, datasets, metric):
     """D=300 > 256: the normalization quirk and 256-dimension chunked sums."""
