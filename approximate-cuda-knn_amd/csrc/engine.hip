@@ -158,6 +158,7 @@ void Engine::release() {
   for (void* p : allocs_) (void)hipFree(p);
   allocs_.clear();
   x_ = x_raw_ = norms_ = mean_ = eps_ = nh_ = s8_ = e8_ = nullptr;
+  mean64_ = nullptr;
   q8_ = nullptr;
   a8_ = nullptr;
   leaf_rescored_ = nullptr;
@@ -843,7 +844,8 @@ void Engine::calculate() {
   fb_.amb_count = p_.sketch ? leaf_rescored_ + kSlotWords : nullptr;
   AKNN_HIP_TRY(hipMemcpyAsync(x_, x_raw_, sizeof(float) * n_ * ld_, hipMemcpyDeviceToDevice, stream_));
   AKNN_HIP_TRY(hipEventRecord(ev(0), stream_));
-  launch_normalize(x_, n_, d_, ld_, mean_, gmax_, p_.scale_all != 0, stream_);
+  if (p_.mean_fp64 && !mean64_) mean64_ = alloc<double>(static_cast<size_t>((n_ + 2047) / 2048) * d_);
+  launch_normalize(x_, n_, d_, ld_, mean_, gmax_, p_.scale_all != 0, p_.mean_fp64 ? mean64_ : nullptr, stream_);
   if (norms_) launch_norms(x_, n_, d_, ld_, norms_, stream_);
   if (sketch) {
     launch_sketch(x_, n_, d_, ld_, kmetric(), xh_, ldh_, eps_, nh_, stream_);

@@ -35,6 +35,7 @@ struct Params {
   int graph_rounds = 1;   // once the round mode is fixed, replay the propagation rounds as one captured hipGraph
   int pca_screen = 0;     // EUCLIDEAN propagation screens with a principal-subspace bound when the data allow (same results)
   int cosine_table = 1;   // COSINE parity initial pass reads its keys from a per-tree key table when it fits (same results)
+  int mean_fp64 = 0;      // normalization's mean as blocked FP64 sums (the reference: a sequential FP32 sum)
 };
 
 // Per-stage times of the last calculate(), milliseconds.
@@ -84,6 +85,8 @@ class Engine {
     p_.graph_rounds = p.graph_rounds;
     p_.pca_screen = p.pca_screen;
     p_.cosine_table = p.cosine_table;
+    if (p_.mean_fp64 != p.mean_fp64) prop_mode_ = -1;
+    p_.mean_fp64 = p.mean_fp64;
     drop_round_graph();
   }
   // Router bins of the points (host array [n], values < 2^bin_bits): the
@@ -181,6 +184,7 @@ class Engine {
   unsigned long long* leaf_rescored_ = nullptr;
   int ldh_ = 0;
   float* mean_ = nullptr;
+  double* mean64_ = nullptr;  // blocked FP64 partial sums of the mean (option mean_fp64), allocated on first use
   uint32_t* gmax_ = nullptr;
   int32_t* gids_[2] = {nullptr, nullptr};
   float* gvals_[2] = {nullptr, nullptr};

@@ -13,9 +13,11 @@ namespace aknn {
 // ---- normalization (euclideandistancematrixgpu.cu:19-85) -------------------
 // x: [n][ld] row-major, d <= ld.  mean: [d] scratch, gmax: [1] scratch.
 // scale_all: every dimension enters the max-abs and is scaled (the reference
-// does so for the first 256 only).
+// does so for the first 256 only).  mean64_part (nullable, ceil(n / 2048) * d
+// doubles): the mean as blocked FP64 sums instead of the reference's
+// sequential FP32 sum.
 void launch_normalize(float* x, int64_t n, int d, int ld, float* mean, uint32_t* gmax, bool scale_all,
-                      hipStream_t s);
+                      double* mean64_part, hipStream_t s);
 // Per-point COSINE squared norms (the chain of .cu:849-865 for one operand).
 void launch_norms(const float* x, int64_t n, int d, int ld, float* norms, hipStream_t s);
 

@@ -6,7 +6,8 @@
 // on the order, so this chain cannot be split).  The chain's speed is set by
 // how many rows are in flight, so fifteen waves stage rows through LDS for the
 // one wave that adds.  Centring, the max-abs reduction over the dimensions
-// below 256 and the scaling are plain parallel HBM passes.
+// below 256 and the scaling are plain parallel HBM passes.  Option
+// mean_fp64 replaces the sequential chain by blocked FP64 sums.
 #include "device_util.hpp"
 #include "kernels.hpp"
 
@@ -14,6 +15,7 @@ namespace aknn {
 
 namespace {
 
+constexpr int64_t kMean64Rows = 2048;            // rows per FP64 partial sum (mean_fp64)
 constexpr int kMeanLoaders = 15;                  // waves 1..15 load, wave 0 adds
 constexpr int kMeanDims = 16;                     // dimensions per workgroup (one chain per lane)
 constexpr int kMeanRowsPerLoad = 64 / kMeanDims;  // rows one 64-lane load covers (4)
@@ -142,6 +144,38 @@ __global__ __launch_bounds__(1024) void mean_kernel(const float* __restrict__ x,
 
 // dm: dimensions that enter the max-abs and get scaled (the reference:
 // min(D, 256), .cu:57-65, :76-84; option scale_all: D)
+// Mean in FP64 (option mean_fp64, not the reference's): block b sums rows
+// [b * kMean64Rows, (b+1) * kMean64Rows) of every dimension in row order in
+// FP64; the block sums are then added in block order and the mean rounded
+// once to FP32.  The order is fixed by n alone.
+__global__ __launch_bounds__(256) void mean64_partial_kernel(const float* __restrict__ x, int64_t n, int d, int ld,
+                                                            double* __restrict__ part) {
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kMean64Rows;
+  const int64_t r1 = r0 + kMean64Rows < n ? r0 + kMean64Rows : n;
+  for (int dd = threadIdx.x; dd < d; dd += 256) {
+    double sum = 0.0;
+    int64_t r = r0;
+    for (; r + 8 <= r1; r += 8) {
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = x[(r + u) * ld + dd];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) sum = sum + static_cast<double>(v[u]);
+    }
+    for (; r < r1; ++r) sum = sum + static_cast<double>(x[r * ld + dd]);
+    part[static_cast<int64_t>(blockIdx.x) * d + dd] = sum;
+  }
+}
+
+__global__ __launch_bounds__(256) void mean64_final_kernel(const double* __restrict__ part, int64_t blocks, int64_t n,
+                                                          int d, float* __restrict__ mean) {
+  const int dd = blockIdx.x * 256 + threadIdx.x;
+  if (dd >= d) return;
+  double sum = 0.0;
+  for (int64_t b = 0; b < blocks; ++b) sum = sum + part[b * d + dd];
+  mean[dd] = static_cast<float>(sum / static_cast<double>(n));
+}
+
 __global__ __launch_bounds__(256) void center_kernel(float* __restrict__ x, int64_t n, int d, int ld, int dm,
                                                      const float* __restrict__ mean, uint32_t* __restrict__ gmax) {
   __shared__ float red[256];
@@ -193,10 +227,17 @@ int grid_for(int64_t work, int64_t per_block, int cap) {
 }  // namespace
 
 void launch_normalize(float* x, int64_t n, int d, int ld, float* mean, uint32_t* gmax, bool scale_all,
-                      hipStream_t s) {
+                      double* mean64_part, hipStream_t s) {
   const int dm = scale_all ? d : (d < 256 ? d : 256);
   AKNN_HIP_TRY(hipMemsetAsync(gmax, 0, sizeof(uint32_t), s));
-  hipLaunchKernelGGL(mean_kernel, dim3((d + kMeanDims - 1) / kMeanDims), dim3(1024), 0, s, x, n, d, ld, mean);
+  if (mean64_part != nullptr) {
+    const int64_t blocks = (n + kMean64Rows - 1) / kMean64Rows;
+    hipLaunchKernelGGL(mean64_partial_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, x, n, d, ld,
+                       mean64_part);
+    hipLaunchKernelGGL(mean64_final_kernel, dim3((d + 255) / 256), dim3(256), 0, s, mean64_part, blocks, n, d, mean);
+  } else {
+    hipLaunchKernelGGL(mean_kernel, dim3((d + kMeanDims - 1) / kMeanDims), dim3(1024), 0, s, x, n, d, ld, mean);
+  }
   AKNN_HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(center_kernel, dim3(grid_for(n * ld, 256, 8192)), dim3(256), 0, s, x, n, d, ld, dm, mean, gmax);
   AKNN_HIP_TRY(hipGetLastError());

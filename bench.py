@@ -10,7 +10,8 @@ C5 (the default, 4M x 128, k=20, EUCLIDEAN) compares three forms at the
 shape the plain form needs for recall@20 >= 0.95:
   plain    the reference's sign-split forest, exact FP32 everywhere;
   median   median-split trees on seeded directions, FP16 / principal-subspace
-           screens with exact FP32 re-scoring (no router);
+           screens with exact FP32 re-scoring, the mean summed in FP64 over
+           point blocks instead of one sequential FP32 chain (no router);
   learned  the same with each tree's first levels routed by its own trained
            router (the chosen form: `value` is its throughput).
 Routers are trained before the timed region on a graph the method builds for
@@ -78,8 +79,8 @@ FORMS = {
 C5_SHAPE = {"trees": 6, "leaf": 256, "props": 2}
 C5_FORMS = {
     "plain": {"opts": {"sketch": 0, "leaf_mfma": 0, "split": 0, "pca_screen": 0, "router_trees": 0}},
-    "median": {"opts": {"sketch": 1, "leaf_mfma": 1, "split": 1, "pca_screen": 1, "router_trees": 0}},
-    "learned": {"opts": {"sketch": 1, "leaf_mfma": 1, "split": 1, "pca_screen": 1, "router_trees": 6},
+    "median": {"opts": {"sketch": 1, "leaf_mfma": 1, "split": 1, "pca_screen": 1, "router_trees": 0, "mean_fp64": 1}},
+    "learned": {"opts": {"sketch": 1, "leaf_mfma": 1, "split": 1, "pca_screen": 1, "router_trees": 6, "mean_fp64": 1},
                 "routers": 6},
 }
 ROUTER = {"n": 200_000, "sample_seed": 105, "bin_bits": 8, "hidden": 256, "epochs": 10, "batch": 1024, "lr": 2e-3,
@@ -125,7 +126,7 @@ def _entry(ms, units, bound, peak, unit, scale, **extra):
     return e
 
 
-def stage_rooflines(res, d, n, router_info):
+def stage_rooflines(res, d, n, router_info, options):
     """Per stage, the algorithmic units of its kernels over their measured time
     against the engine that bounds them: forest and propagation bytes against
     HBM, exact leaf chains against FP32, screened leaf launches (GEMM-form
@@ -133,8 +134,9 @@ def stage_rooflines(res, d, n, router_info):
     matrix cores."""
     k = res["kernels"]
     st = res["stages_ms"]
-    out = {"normalize": {"ms": st["normalize"], "bound": "latency",
-                         "note": "sequential per-dimension FP32 mean (one dependent add per row)"}}
+    out = {"normalize": {"ms": st["normalize"], "bound": "hbm" if options.get("mean_fp64") else "latency",
+                         "note": ("mean as FP64 sums over 2048-point blocks" if options.get("mean_fp64") else
+                                  "sequential per-dimension FP32 mean (one dependent add per row)")}}
     router_ms = 0.0
     if router_info:
         router_ms = sum(router_info["assign_ms"])
@@ -312,7 +314,7 @@ def main():
     cfg_name = args.config if args.n is None else f"{args.config}_n{n}"
     for name in names:
         ri = router_stats if (c5 and C5_FORMS[name].get("routers")) else None
-        results[name]["stages"] = stage_rooflines(results[name], cfg["d"], n, ri)
+        results[name]["stages"] = stage_rooflines(results[name], cfg["d"], n, ri, results[name]["options"])
         results[name]["roofline"] = roofline(results[name]["stages"], cfg_name, name)
         results[name]["config"] = shapes[name]
         if name in recalls:

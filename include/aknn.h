@@ -129,6 +129,11 @@ int aknn_deinitialize(aknn_handle* h);
  *                   over all dimensions (the reference: the first 256 only,
  *                   .cu:57-65, :76-84)
  *   "fixed"         sets cosine_fixed, taxicab_l1 and scale_all together
+ *   "mean_fp64"     1: normalization's per-dimension mean is summed in FP64
+ *                   over blocks of 2048 points in point order, the block sums
+ *                   added in block order and the mean rounded once to FP32
+ *                   (the reference: one sequential FP32 sum over all points,
+ *                   .cu:29-40, a chain of N dependent adds); 0 (default)
  *   "pca_screen"    1: EUCLIDEAN propagation screens each candidate with the
  *                   distance of its projection on the data's top 16 or 32
  *                   principal directions (recomputed per build from a sample;

@@ -86,7 +86,9 @@ int32_t orc_pick(int32_t r, int32_t count) {
 void orc_normalize(float* x, int64_t n, int d) { orc_normalize_ex(x, n, d, 0); }
 
 /* scale_all (fixed mode): every dimension enters the max-abs and is scaled. */
-void orc_normalize_ex(float* x, int64_t n, int d, int scale_all) {
+void orc_normalize_ex(float* x, int64_t n, int d, int scale_all) { orc_normalize_mode(x, n, d, scale_all, 0); }
+
+void orc_normalize_mode(float* x, int64_t n, int d, int scale_all, int mean_fp64) {
     int dd;
     int dm = scale_all ? d : (d < 256 ? d : 256);
     float g = 0.0f;
@@ -94,8 +96,20 @@ void orc_normalize_ex(float* x, int64_t n, int d, int scale_all) {
     for (dd = 0; dd < d; ++dd) {
         float m = 0.0f;
         int64_t i;
-        for (i = 0; i < n; ++i) m = m + x[i * d + dd];
-        m = m / (float)n;
+        if (mean_fp64) {
+            double tot = 0.0;
+            int64_t b;
+            for (b = 0; b < n; b += 2048) {
+                double part = 0.0;
+                int64_t e = b + 2048 < n ? b + 2048 : n;
+                for (i = b; i < e; ++i) part = part + (double)x[i * d + dd];
+                tot = tot + part;
+            }
+            m = (float)(tot / (double)n);
+        } else {
+            for (i = 0; i < n; ++i) m = m + x[i * d + dd];
+            m = m / (float)n;
+        }
         for (i = 0; i < n; ++i) x[i * d + dd] = x[i * d + dd] - m;
     }
     for (dd = 0; dd < dm; ++dd) {

@@ -59,6 +59,11 @@ void orc_normalize(float* x, int64_t n, int d);
 /* scale_all 1 (fixed mode): the max-abs and the division cover every
  * dimension instead of the first 256. */
 void orc_normalize_ex(float* x, int64_t n, int d, int scale_all);
+/* orc_normalize_ex with the mean taken (mean_fp64 != 0) as the library's
+ * option mean_fp64 takes it: FP64 sums over blocks of 2048 points in point
+ * order, block sums added in block order, the mean rounded once to FP32 (not
+ * the reference's arithmetic). */
+void orc_normalize_mode(float* x, int64_t n, int d, int scale_all, int mean_fp64);
 
 /* A.7: the FP32 key of a pair (EUCLIDEAN sqrt(L2^2), TAXICAB sqrt(L1),
  * COSINE float(1 - (double)cos)). */

@@ -65,6 +65,7 @@ def lib():
                                  ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_int, _i32p, _f32p, _f64p]
         L.orc_normalize_ex.argtypes = [_f32p, ctypes.c_int64, ctypes.c_int, ctypes.c_int]
+        L.orc_normalize_mode.argtypes = [_f32p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_mlp.argtypes = [_f32p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                               _f32p, _f32p, ctypes.c_float, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                               ctypes.c_void_p]
@@ -89,9 +90,9 @@ def rng(seed=1):
     return g
 
 
-def normalize(x, scale_all=0):
+def normalize(x, scale_all=0, mean_fp64=0):
     x = np.ascontiguousarray(x, dtype=np.float32).copy()
-    lib().orc_normalize_ex(x, x.shape[0], x.shape[1], scale_all)
+    lib().orc_normalize_mode(x, x.shape[0], x.shape[1], scale_all, mean_fp64)
     return x
 
 

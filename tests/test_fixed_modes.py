@@ -56,6 +56,43 @@ def test_scale_all_vs_numpy(oracle, shape):
         assert (bits(got) == bits(oracle.normalize(x))).all()
 
 
+@pytest.mark.parametrize("n,d", [(5000, 3), (4097, 5), (2048, 2), (700, 300)])
+def test_mean_fp64_vs_python(oracle, n, d):
+    """mean_fp64: FP64 sums over blocks of 2048 points in order, block sums in
+    order, rounded once (restated here in pure Python floats, which are FP64),
+    then the reference's centring and max-abs scaling."""
+    rng = np.random.default_rng(n + d)
+    x = (rng.normal(2, 3, (n, d)) * 10).astype(np.float32)
+    got = oracle.normalize(x, mean_fp64=1)
+    want = x.copy()
+    for j in range(d):
+        tot = 0.0
+        for b in range(0, n, 2048):
+            part = 0.0
+            for v in x[b:b + 2048, j].tolist():
+                part = part + v
+            tot = tot + part
+        want[:, j] = want[:, j] - np.float32(tot / n)
+    dm = min(d, 256)
+    want[:, :dm] = want[:, :dm] / np.abs(want[:, :dm]).max()
+    assert (bits(got) == bits(want)).all()
+
+
+def test_mean_fp64_integer_data_is_exact(oracle):
+    """Integer values: every FP64 partial sum is exact, so the mean is the
+    exact mean rounded once, whatever the blocking (a dropped block or a
+    wrong divisor changes it)."""
+    rng = np.random.default_rng(2)
+    n, d = 6000, 4
+    x = rng.integers(0, 200000, (n, d)).astype(np.float32)  # sums pass 2^24: the FP32 chain rounds
+    got = oracle.normalize(x, mean_fp64=1)
+    exact = np.float32(x.astype(np.int64).sum(0) / n)
+    want = x - exact
+    want = want / np.abs(want).max()
+    assert (bits(got) == bits(want)).all()
+    assert not (bits(got) == bits(oracle.normalize(x))).all()  # the sequential FP32 mean differs here
+
+
 def test_l1_pipeline_brute_force(oracle, datasets):
     """One tree with a leaf bound >= N: every point's candidates are all the
     others, so the initial graph is the brute-force top-k by (L1, id)."""
@@ -115,3 +152,15 @@ def test_gpu_fixed_mode_wide_rows(aknn, oracle, datasets, metric):
     assert np.abs(xn).max() == 1.0
     gi, gv = g.graph()
     assert (gi == ids).all() and (bits(gv) == bits(vals)).all()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,d", [(5000, 24), (70000, 128), (3000, 300)])
+def test_gpu_mean_fp64_bit_exact(aknn, oracle, datasets, n, d):
+    """The GPU's blocked FP64 mean and normalization equal the oracle's."""
+    x = datasets.gmm_c1(n, d, clusters=5, seed=90 + d)[0]
+    g = aknn.Graph()
+    g.initialize(x, 6, 2, 40, 0, 0, mean_fp64=1)
+    g.calculate()
+    assert (bits(g.normalized()) == bits(oracle.normalize(x, mean_fp64=1))).all()
+    g.close()
