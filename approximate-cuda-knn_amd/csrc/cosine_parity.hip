@@ -243,13 +243,16 @@ __global__ __launch_bounds__(1024) void count_scan_kernel(const int32_t* __restr
   if (i < n) out[i] = before + incl - x;
 }
 
-// First k of the pool[0, np) of packed (key, id) pairs (any order; equal
-// pairs are interchangeable), by a radix select on the 64-bit values, 8 bits
-// per round from the top, over all 256 threads: the threshold digit is found
-// from a histogram of the entries still tied with the prefix, until the
-// bucket holding the k-th entry is taken whole.  out[0, k) receives them.
-__device__ void pool_select(const uint64_t* pool, int np, int k, uint64_t* out, uint32_t* hist, int* sel) {
+// First k of the entries a[0, na) and b[0, nb) of packed (key, id) pairs
+// (any order; equal pairs are interchangeable), by a radix select on the
+// 64-bit values, 8 bits per round from the top, over all 256 threads: the
+// threshold digit is found from a histogram of the entries still tied with
+// the prefix, until the bucket holding the k-th entry is taken whole.
+// out[0, k) receives them.
+__device__ void pool_select(const uint64_t* a, int na, const uint64_t* b, int nb, int k, uint64_t* out,
+                            uint32_t* hist, int* sel) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int np = na + nb;
   uint64_t prefix = 0, mask = 0;
   int need = k;
   bool whole = false;
@@ -257,7 +260,7 @@ __device__ void pool_select(const uint64_t* pool, int np, int k, uint64_t* out, 
     hist[tid] = 0;
     __syncthreads();
     for (int e = tid; e < np; e += 256) {
-      const uint64_t v = pool[e];
+      const uint64_t v = e < na ? a[e] : b[e - na];
       if ((v & mask) == prefix) atomicAdd(&hist[(v >> shift) & 255], 1u);
     }
     __syncthreads();
@@ -273,14 +276,14 @@ __device__ void pool_select(const uint64_t* pool, int np, int k, uint64_t* out, 
       if (excl < need && need <= incl) {
         const int hb[4] = {static_cast<int>(h0), static_cast<int>(h1), static_cast<int>(h2), static_cast<int>(h3)};
         int c = excl;
-        for (int b = 0; b < 4; ++b) {
-          if (c + hb[b] >= need) {
-            sel[0] = 4 * lane + b;
+        for (int q = 0; q < 4; ++q) {
+          if (c + hb[q] >= need) {
+            sel[0] = 4 * lane + q;
             sel[1] = c;
-            sel[2] = hb[b];
+            sel[2] = hb[q];
             break;
           }
-          c += hb[b];
+          c += hb[q];
         }
       }
     }
@@ -300,7 +303,7 @@ __device__ void pool_select(const uint64_t* pool, int np, int k, uint64_t* out, 
   __syncthreads();
   const uint64_t top = prefix | ~mask;
   for (int e = tid; e < np; e += 256) {
-    const uint64_t v = pool[e];
+    const uint64_t v = e < na ? a[e] : b[e - na];
     bool take = whole ? v <= top : v < prefix;
     if (!whole && v == prefix) take = atomicAdd(&sel[4], 1) < need;
     if (take) out[atomicAdd(&sel[3], 1)] = v;
@@ -308,8 +311,159 @@ __device__ void pool_select(const uint64_t* pool, int np, int k, uint64_t* out, 
   __syncthreads();
 }
 
+// Orders the LDS accesses of the lanes of one wave.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Lane j's 64-bit value, broadcast (j uniform); the halves are read as
+// unsigned 32-bit values so that the low half does not sign-extend.
+__device__ __forceinline__ uint64_t lane_u64(uint64_t v, int j) {
+  const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v >> 32), j));
+  const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(v)), j));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// Bitonic sort of one 64-bit value per lane over the wave, ascending.
+__device__ __forceinline__ uint64_t wave_sort_u64(uint64_t v) {
+  const int lane = threadIdx.x & 63;
+  for (int size = 2; size <= 64; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      const uint32_t ohi = __shfl_xor(static_cast<uint32_t>(v >> 32), stride);
+      const uint32_t olo = __shfl_xor(static_cast<uint32_t>(v), stride);
+      const uint64_t o = (static_cast<uint64_t>(ohi) << 32) | olo;
+      const bool up = (lane & size) == 0, lower = (lane & stride) == 0;
+      const bool take = lower == up ? o < v : !(o < v);
+      v = take ? o : v;
+    }
+  }
+  return v;
+}
+
+// Count of the entries of the ascending array a[0, len) (len <= 64, LDS)
+// below v.
+__device__ __forceinline__ int count_below(const uint64_t* a, int len, uint64_t v) {
+  int c = 0;
+#pragma unroll
+  for (int step = 32; step >= 1; step >>= 1)
+    if (c + step <= len && a[c + step - 1] < v) c += step;
+  return c;
+}
+
+// First min(k, n) of the distinct packed pairs src[0, n) (LDS) by one wave,
+// in any order, into out: a radix select from the highest bit on which the
+// entries differ (the bits above it are common to all), up to 8 bits per
+// round, over a per-wave histogram wh[256].  Returns the count taken.
+__device__ int wave_topk(const uint64_t* src, int n, int k, uint64_t* out, uint32_t* wh) {
+  const int lane = threadIdx.x & 63;
+  if (n <= k) {
+    for (int e = lane; e < n; e += 64) out[e] = src[e];
+    return n;
+  }
+  uint32_t alo = ~0u, ahi = ~0u, olo = 0u, ohi = 0u;
+  for (int e = lane; e < n; e += 64) {
+    const uint64_t v = src[e];
+    alo &= static_cast<uint32_t>(v);
+    ahi &= static_cast<uint32_t>(v >> 32);
+    olo |= static_cast<uint32_t>(v);
+    ohi |= static_cast<uint32_t>(v >> 32);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    alo &= __shfl_xor(alo, o);
+    ahi &= __shfl_xor(ahi, o);
+    olo |= __shfl_xor(olo, o);
+    ohi |= __shfl_xor(ohi, o);
+  }
+  const uint64_t all_and = (static_cast<uint64_t>(ahi) << 32) | alo;
+  const uint64_t dif = ((static_cast<uint64_t>(ohi) << 32) | olo) ^ all_and;  // nonzero: n > k >= 1 distinct
+  int hi_bit = 63 - __clzll(static_cast<long long>(dif));
+  uint64_t mask = hi_bit >= 63 ? 0ull : ~0ull << (hi_bit + 1);
+  uint64_t prefix = all_and & mask;
+  int need = k;
+  for (;;) {
+    const int width = hi_bit + 1 < 8 ? hi_bit + 1 : 8;
+    const int shift = hi_bit + 1 - width;
+    const uint32_t dm = (1u << width) - 1u;
+    *reinterpret_cast<uint4*>(wh + 4 * lane) = make_uint4(0u, 0u, 0u, 0u);
+    wave_sync();
+    for (int e = lane; e < n; e += 64) {
+      const uint64_t v = src[e];
+      if ((v & mask) == prefix) atomicAdd(&wh[(v >> shift) & dm], 1u);
+    }
+    wave_sync();
+    const uint4 h = *reinterpret_cast<const uint4*>(wh + 4 * lane);
+    const int sum = static_cast<int>(h.x + h.y + h.z + h.w);
+    int incl = sum;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int t = __shfl_up(incl, o);
+      if (lane >= o) incl += t;
+    }
+    const int excl = incl - sum;
+    const bool hit = excl < need && need <= incl;
+    int dd = 0, before = 0, cnt = 0;
+    if (hit) {
+      const int hb[4] = {static_cast<int>(h.x), static_cast<int>(h.y), static_cast<int>(h.z), static_cast<int>(h.w)};
+      int c = excl;
+      for (int q = 0; q < 4; ++q) {
+        if (c + hb[q] >= need) {
+          dd = 4 * lane + q;
+          before = c;
+          cnt = hb[q];
+          break;
+        }
+        c += hb[q];
+      }
+    }
+    const int hl = __ffsll(static_cast<long long>(__ballot(hit))) - 1;
+    dd = __shfl(dd, hl);
+    before = __shfl(before, hl);
+    cnt = __shfl(cnt, hl);
+    prefix |= static_cast<uint64_t>(dd) << shift;
+    mask |= static_cast<uint64_t>(dm) << shift;
+    need -= before;
+    hi_bit = shift - 1;
+    // distinct entries: once every bit is fixed the bucket holds one entry
+    if (cnt == need || shift == 0) break;
+    wave_sync();
+  }
+  const uint64_t top = prefix | ~mask;
+  int got = 0;
+  for (int e0 = 0; e0 < n; e0 += 64) {
+    const int e = e0 + lane;
+    const uint64_t v = e < n ? src[e] : ~0ull;
+    const bool take = e < n && v <= top;
+    const uint64_t b = __ballot(take);
+    if (take) out[got + __popcll(b & ((1ull << lane) - 1ull))] = v;
+    got += __popcll(b);
+  }
+  return got;
+}
+
+// LDS union of a point's start-node ranges (key table form): a point whose
+// ranges hold at most kUnionCap members in all (duplicates counted) takes its
+// (id, key) members into registers, kUnionSlots per thread, and ranks them in
+// id order through a bitmap window of kWinIds ids in LDS slid over [0, n):
+// per window the members' bits are set, the set bits counted per word and
+// scanned, and each member writes itself at its rank (duplicates write the
+// same entry to the same slot).  The distinct members land in ascending id
+// order as packed (key, id) pairs, with no global scratch.  Ranges of up to
+// kUnionTrees trees are held in LDS for the members' tree lookup.
+constexpr int kUnionCap = 8192;
+constexpr int kUnionSlots = kUnionCap / 256;
+constexpr int kWinWords = 16384;  // 64 KB of LDS
+constexpr int kWinIds = kWinWords * 32;
+constexpr int kWordsPerThread = kWinWords / 256;
+constexpr int kGroupsPerThread = kWordsPerThread / 8;  // groups of 8 words (256 ids)
+static_assert(kGroupsPerThread == 8, "group prefixes are published as one 16-byte store per thread");
+constexpr int kUnionTrees = 64;
+constexpr int kGatherBatch = 8;
+constexpr int kRankBatch = 4;
+// chunk lists (each chunk's own first k) held for the merge
+constexpr int kLocalCap = 1024;
+
 template <bool TABLE>
-__global__ __launch_bounds__(256) void cosine_parity_kernel(const float* __restrict__ x, int d, int ld,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void cosine_parity_kernel(const float* __restrict__ x, int d, int ld,
                                                             const float* __restrict__ norms, int64_t n, int k,
                                                             int trees, const int32_t* __restrict__ rs,
                                                             const int32_t* __restrict__ rc,
@@ -319,140 +473,404 @@ __global__ __launch_bounds__(256) void cosine_parity_kernel(const float* __restr
                                                             const int32_t* __restrict__ order,
                                                             const float* __restrict__ ktab,
                                                             const int64_t* __restrict__ koff) {
-  // LDS: x_i (without the key table), the summary bitmap (one bit per 32-id
-  // word of the global bitmap), the selection pool and the carried list
+  // LDS, two layouts over one region: the global-bitmap path keeps x_i
+  // (without the key table), the summary bitmap (one bit per 32-id word of
+  // the global bitmap) and the chunk; the LDS union (key table) keeps the
+  // distinct members and the bitmap window
   extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
   const int xi_words = TABLE ? 0 : (d + 3) / 4 * 4;
   const int sum_words = static_cast<int>((sc.summary_words + 3) / 4 * 4);
   float* xi = reinterpret_cast<float*>(lds);
   uint32_t* summ = reinterpret_cast<uint32_t*>(lds) + xi_words;
-  uint64_t* pool = reinterpret_cast<uint64_t*>(summ + sum_words);  // [k + cap]
+  uint64_t* pool = reinterpret_cast<uint64_t*>(summ + sum_words);  // [cap]
+  uint32_t* win = reinterpret_cast<uint32_t*>(lds);  // [kWinWords] bitmap window, then the chunk
+  uint64_t* uo = TABLE ? sc.uo + static_cast<int64_t>(blockIdx.x) * kUnionCap : nullptr;  // distinct members
   __shared__ uint64_t car[64], outl[64];
   __shared__ uint32_t hist[256];
   __shared__ int sh[4], sel[5];
+  __shared__ int tr_c[kUnionTrees], tr_s[kUnionTrees], tr_p[kUnionTrees + 1];
+  __shared__ int64_t tr_o[kUnionTrees];
+  __shared__ __attribute__((aligned(16))) uint16_t gpre[TABLE ? kWinWords / 8 : 8];  // per-group window prefixes
+  __shared__ uint64_t locl[TABLE ? kLocalCap : 1];  // each chunk's own first k (LDS union path)
   uint32_t* bits = sc.bits + static_cast<int64_t>(blockIdx.x) * sc.words;
   int32_t* cand = sc.cand + static_cast<int64_t>(blockIdx.x) * sc.max_cand;
   float* ckey = sc.key + static_cast<int64_t>(blockIdx.x) * sc.max_cand;
   float* kid = TABLE ? sc.keyarr + static_cast<int64_t>(blockIdx.x) * n : nullptr;  // key by candidate id
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int w = tid; w < sum_words; w += 256) summ[w] = 0;
+  const bool lds_union = TABLE && trees <= kUnionTrees;
+  // the zeroed parts of each layout: the summary bitmap (cleared as it is
+  // enumerated) and the bitmap window (cleared window by window); the other
+  // layout's use of the region dirties them
+  bool summ_clean = false, win_clean = false;
+
+  // chunk loop (.cu:224-239, :1007): the pool is the carried list (keyed by
+  // its stored values) and the next chunk of candidates in id order (keyed
+  // by their distances); its first k by (key, id) are stored as 1 - key and
+  // sorted by (stored, id)
+  auto select_chunk = [&](const uint64_t* chunk, int nc) {
+    pool_select(car, k, chunk, nc, k, outl, hist, sel);
+    if (wave == 0) {
+      float key = __builtin_inff();
+      int32_t id = INT_MAX;
+      if (lane < k) {
+        const uint64_t v = outl[lane];
+        key = static_cast<float>(1.0 - static_cast<double>(pair_key(v)));
+        id = pair_id(v);
+      }
+      wave_sort(key, id);
+      if (lane < k) car[lane] = pack_pair(key, order_id(id));
+    }
+    __syncthreads();
+  };
 
   const int64_t g = xcd_block(blockIdx.x, gridDim.x);
   const int64_t per = (n + gridDim.x - 1) / gridDim.x;
   // logical group g takes the stretch [g*per, (g+1)*per) of the leaf order
   for (int64_t pidx = g * per; pidx < n && pidx < (g + 1) * per; ++pidx) {
     const int64_t i = order != nullptr ? order[pidx] : pidx;
-    if constexpr (!TABLE)
-      for (int dd = tid; dd < d; dd += 256) xi[dd] = x[i * ld + dd];
-    __syncthreads();
-    // union of the start-node ranges (.cu:160-191): bits in the global
-    // bitmap (L2 atomics), summary bits in LDS; with the key table each
-    // member's key goes to its id's slot (a member of several ranges gets
-    // the same key from each)
-    for (int t = 0; t < trees; ++t) {
-      const int32_t s0 = rs[static_cast<int64_t>(t) * n + i], c0 = rc[static_cast<int64_t>(t) * n + i];
-      const int32_t* pt = perm + static_cast<int64_t>(t) * n + s0;
-      const float* kt = TABLE ? ktab + koff[static_cast<int64_t>(t) * n + i] : nullptr;
-      // four members per thread per pass, their loads issued together
-      for (int p0 = 0; p0 < c0; p0 += 1024) {
-        int32_t jv[4];
-        float kv[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int p = p0 + 256 * u + tid;
-          jv[u] = p < c0 ? pt[p] : -1;
-          if constexpr (TABLE) kv[u] = p < c0 ? kt[p] : 0.0f;
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int32_t j = jv[u];
-          if (j < 0) continue;
-          if constexpr (TABLE) kid[j] = kv[u];
-          __hip_atomic_fetch_or(&bits[j >> 5], 1u << (j & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          atomicOr(&summ[j >> 10], 1u << ((j >> 5) & 31));
-        }
-      }
-    }
-    // every access to the bitmap is a workgroup-scope atomic and the key
-    // slots are read after a workgroup barrier: coherent within the
-    // workgroup, with no cache maintenance beyond the CU
-    __syncthreads();
-    // enumerate in ascending id order, without i itself (.cu:190-191); the
-    // words of each summary word's set bits are read together and cleared
-    int m = 0;
-    for (int sb = 0; sb < sum_words; sb += 256) {
-      const int sw = sb + tid;
-      const uint32_t s = sw < sum_words ? summ[sw] : 0u;
-      uint32_t bw[32];
-      int cnt = 0;
-      if (s != 0) {
-        // only the words whose summary bit is set, all requested before use
-        uint32_t* line = bits + static_cast<int64_t>(sw) * 32;
-#pragma unroll
-        for (int b = 0; b < 32; ++b)
-          bw[b] = ((s >> b) & 1u) ? __hip_atomic_load(&line[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) : 0u;
-        const int64_t iw = i >> 5;
-        if ((iw >> 5) == sw) bw[iw & 31] &= ~(1u << (i & 31));
-#pragma unroll
-        for (int b = 0; b < 32; ++b) cnt += __popc(bw[b]);
-      }
-      int tot;
-      int off = m + block_excl_scan256(cnt, sh, &tot);
-      if (s != 0) {
-        uint32_t* wline = bits + static_cast<int64_t>(sw) * 32;
-#pragma unroll
-        for (int b = 0; b < 32; ++b)
-          if ((s >> b) & 1u) __hip_atomic_store(&wline[b], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        summ[sw] = 0;
-#pragma unroll
-        for (int b = 0; b < 32; ++b)
-          for (uint32_t r = bw[b]; r; r &= r - 1)
-            cand[off++] = static_cast<int32_t>((static_cast<int64_t>(sw) * 32 + b) * 32 + __ffs(r) - 1);
-      }
-      m += tot;
-    }
-    if constexpr (!TABLE) {
-      __syncthreads();
-      const float ni = norms[i];
-      for (int c = tid; c < m; c += 256) {
-        const int32_t j = cand[c];
-        const float t = row_chain<kCosine>(xi, x + static_cast<int64_t>(j) * ld, d);
-        ckey[c] = finish_key<kCosine>(t, ni, norms[j]);
-      }
-    }
-    __syncthreads();
-    // chunk loop (.cu:224-239, :1007): the pool is the carried list (keyed by
-    // its stored values) and the next cap candidates in id order (keyed by
-    // their distances); its first k by (key, id) are stored as 1 - key and
-    // sorted by (stored, id)
     if (tid < 64) car[tid] = pack_pair(kStubDist, order_id(kStubId));
-    __syncthreads();
-    for (int64_t c0 = 0; c0 < m; c0 += cap) {
-      const int nc = static_cast<int>(c0 + cap < m ? cap : m - c0);
-      const int np = k + nc;
-      for (int e = tid; e < np; e += 256) {
-        if (e < k) {
-          pool[e] = car[e];
-        } else {
-          const int32_t j = cand[c0 + e - k];
-          const float key = TABLE ? kid[j] : ckey[c0 + e - k];
+    int total = INT_MAX;
+    if (lds_union) {
+      if (tid < trees) {
+        const int64_t o = static_cast<int64_t>(tid) * n + i;
+        tr_c[tid] = rc[o];
+        tr_s[tid] = rs[o];
+        tr_o[tid] = koff[o];
+      }
+      __syncthreads();
+      if (wave == 0) {
+        const int c = lane < trees ? tr_c[lane] : 0;
+        int incl = c;
+        for (int o = 1; o < 64; o <<= 1) {
+          const int t = __shfl_up(incl, o);
+          if (lane >= o) incl += t;
+        }
+        tr_p[lane + 1] = incl;
+        if (lane == 0) tr_p[0] = 0;
+      }
+      __syncthreads();
+      total = tr_p[trees];
+    }
+    if (lds_union && total <= kUnionCap) {
+      // members into registers: member e is position e - tr_p[t] of tree t's
+      // range, t the last tree with tr_p[t] <= e; the point itself is left out
+      // (.cu:190-191)
+      // (in batches of kGatherBatch slots: 2 * kGatherBatch loads in flight)
+      const int nslots = (total + 255) / 256;
+      int32_t jv[kUnionSlots];
+      float kv[kUnionSlots];
+#pragma unroll
+      for (int b0 = 0; b0 < kUnionSlots; b0 += kGatherBatch) {
+#pragma unroll
+        for (int u = b0; u < b0 + kGatherBatch; ++u) {
+          jv[u] = -1;
+          kv[u] = 0.0f;
+          const int e = 256 * u + tid;
+          if (b0 < nslots && e < total) {
+            int t = 0;
+#pragma unroll
+            for (int st = kUnionTrees / 2; st >= 1; st >>= 1)
+              if (t + st <= trees && tr_p[t + st] <= e) t += st;
+            const int p = e - tr_p[t];
+            jv[u] = perm[static_cast<int64_t>(t) * n + tr_s[t] + p];
+            kv[u] = ktab[tr_o[t] + p];
+          }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+#pragma unroll
+      for (int u = 0; u < kUnionSlots; ++u)
+        if (jv[u] == i) jv[u] = -1;
+      if (!win_clean)
+        for (int w = tid; w < kWinWords / 4; w += 256) reinterpret_cast<uint4*>(win)[w] = make_uint4(0u, 0u, 0u, 0u);
+      win_clean = true;
+      summ_clean = false;
+      __syncthreads();
+      // ids are below 2^31: the window offset of an absent member (-1)
+      // wraps far past the window
+      int m = 0;
+      for (int32_t w0 = 0; w0 < n; w0 += kWinIds) {
+        // the members are opaque to the compiler at each window (empty asm,
+        // no instruction), so nothing derived from them is kept live across
+        // windows: two registers per slot
+#pragma unroll
+        for (int u = 0; u < kUnionSlots; ++u) asm volatile("" : "+v"(jv[u]), "+v"(kv[u]));
+#pragma unroll
+        for (int b0 = 0; b0 < kUnionSlots; b0 += kGatherBatch)
+          if (b0 < nslots) {
+#pragma unroll
+            for (int u = b0; u < b0 + kGatherBatch; ++u) {
+              const uint32_t r = static_cast<uint32_t>(jv[u] - w0);
+              if (r < kWinIds) atomicOr(&win[r >> 5], 1u << (r & 31));
+            }
+          }
+        __syncthreads();
+        // this thread's words [kWordsPerThread * tid, ...): set bits counted
+        // per group of 8 words, scanned over the block, and each group's
+        // window-relative prefix published (below 2^16: at most kUnionCap
+        // members)
+        const uint4* wt = reinterpret_cast<const uint4*>(win) + kWordsPerThread / 4 * tid;
+        int gc[kGroupsPerThread];
+        int cnt = 0;
+#pragma unroll
+        for (int q = 0; q < kGroupsPerThread; ++q) {
+          const uint4 a4 = wt[2 * q], b4 = wt[2 * q + 1];
+          gc[q] = __popc(a4.x) + __popc(a4.y) + __popc(a4.z) + __popc(a4.w) + __popc(b4.x) + __popc(b4.y) +
+                  __popc(b4.z) + __popc(b4.w);
+          cnt += gc[q];
+        }
+        int incl = cnt;
+        for (int o = 1; o < 64; o <<= 1) {
+          const int t = __shfl_up(incl, o);
+          if (lane >= o) incl += t;
+        }
+        if (lane == 63) sh[wave] = incl;
+        __syncthreads();
+        int before = 0;
+        for (int w = 0; w < wave; ++w) before += sh[w];
+        const int wtot = sh[0] + sh[1] + sh[2] + sh[3];
+        {
+          int c = before + incl - cnt;
+          uint32_t pk[kGroupsPerThread / 2];
+#pragma unroll
+          for (int q = 0; q < kGroupsPerThread / 2; ++q) {
+            const int c0 = c;
+            c += gc[2 * q];
+            pk[q] = static_cast<uint32_t>(c0) | (static_cast<uint32_t>(c) << 16);
+            c += gc[2 * q + 1];
+          }
+          reinterpret_cast<uint4*>(gpre)[tid] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
+        }
+        __syncthreads();
+        // ranks: the group prefix and the group's eight words of every member
+        // requested before any is used (an absent member reads group 0 and
+        // writes nothing); the members go to the workgroup's global array
+#pragma unroll
+        for (int b0 = 0; b0 < kUnionSlots; b0 += kRankBatch)
+          if (b0 < nslots) {
+            uint4 wa[kRankBatch], wb[kRankBatch];
+            uint32_t pv[kRankBatch];
+#pragma unroll
+            for (int u = 0; u < kRankBatch; ++u) {
+              const uint32_t r = static_cast<uint32_t>(jv[b0 + u] - w0);
+              const uint32_t gi = r < kWinIds ? r >> 8 : 0u;
+              wa[u] = reinterpret_cast<const uint4*>(win)[2 * gi];
+              wb[u] = reinterpret_cast<const uint4*>(win)[2 * gi + 1];
+              pv[u] = gpre[gi];
+            }
+#pragma unroll
+            for (int u = 0; u < kRankBatch; ++u) {
+              const uint32_t r = static_cast<uint32_t>(jv[b0 + u] - w0);
+              if (r < kWinIds) {
+                const uint32_t q = (r >> 5) & 7u;
+                const uint32_t w8[8] = {wa[u].x, wa[u].y, wa[u].z, wa[u].w, wb[u].x, wb[u].y, wb[u].z, wb[u].w};
+                int rank = m + static_cast<int>(pv[u]);
+#pragma unroll
+                for (uint32_t t = 0; t < 8; ++t) {
+                  const uint32_t msk = t < q ? ~0u : (t == q ? (1u << (r & 31)) - 1u : 0u);
+                  rank += __popc(w8[t] & msk);
+                }
+                uo[rank] = pack_pair(kv[b0 + u], order_id(jv[b0 + u]));
+              }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+          }
+        __syncthreads();
+        {
+          uint4* wc = reinterpret_cast<uint4*>(win) + kWordsPerThread / 4 * tid;
+#pragma unroll
+          for (int q = 0; q < kWordsPerThread / 4; ++q) wc[q] = make_uint4(0u, 0u, 0u, 0u);
+        }
+        m += wtot;
+        __syncthreads();
+      }
+      // the members (global writes ordered before these reads by the
+      // workgroup barriers) into LDS over the window: at most kUnionCap
+      uint64_t* cp = reinterpret_cast<uint64_t*>(lds);
+      for (int e = tid; e < m; e += 256) cp[e] = uo[e];
+      if (m > 0) win_clean = false;
+      __syncthreads();
+      const int nch = static_cast<int>((m + cap - 1) / cap);
+      if (nch * k <= kLocalCap && k <= 64) {
+        // The chunk loop's selections, split: the first k of (carried
+        // list + chunk) are the first k of (carried list + the chunk's own
+        // first k), so each chunk's own first k is taken first, four chunks
+        // at a time (one wave each), and wave 0 then merges them into the
+        // carried list in chunk order, flipping and sorting as before.
+        uint32_t* wh = reinterpret_cast<uint32_t*>(gpre) + 256 * wave;
+        for (int c = wave; c < nch; c += 4) {
+          const int c0 = c * static_cast<int>(cap);
+          const int nc = m - c0 < cap ? m - c0 : static_cast<int>(cap);
+          const int got = wave_topk(cp + c0, nc, k, locl + c * k, wh);
+          wave_sync();
+          // the chunk's list in ascending packed order
+          const uint64_t v = wave_sort_u64(lane < got ? locl[c * k + lane] : ~0ull);
+          if (lane < got) locl[c * k + lane] = v;
+          wave_sync();
+        }
+        __syncthreads();
+        if (wave == 0) {
+          // the carried list stays in ascending packed order while no stored
+          // value is NaN (the (stored, id) order is then the packed order);
+          // the ranks then come from binary searches, else from counts
+          bool packed_sorted = true;
+          for (int c = 0; c < nch; ++c) {
+            const int c0 = c * static_cast<int>(cap);
+            const int nc = m - c0 < cap ? m - c0 : static_cast<int>(cap);
+            const int nl = nc < k ? nc : k;  // the chunk's list length
+            const uint64_t* lc = locl + c * k;
+            // rank of each entry among (carried list + chunk list) in packed
+            // order; equal carried entries (stubs) by position
+            const uint64_t cv = lane < k ? car[lane] : ~0ull;
+            const uint64_t lv = lane < nl ? lc[lane] : ~0ull;
+            int rcar = 0, rl = 0;
+            if (packed_sorted) {
+              rcar = lane + count_below(lc, nl, cv);
+              rl = lane + count_below(car, k, lv);
+            } else {
+              // lane j's entries broadcast through scalar registers
+#pragma unroll 4
+              for (int j = 0; j < 64; ++j) {
+                if (j < k) {
+                  const uint64_t u = lane_u64(cv, j);
+                  rcar += (u < cv || (u == cv && j < lane)) ? 1 : 0;
+                  rl += u < lv ? 1 : 0;
+                }
+                if (j < nl) {
+                  const uint64_t u = lane_u64(lv, j);
+                  rcar += u < cv ? 1 : 0;
+                  rl += u < lv ? 1 : 0;
+                }
+              }
+            }
+            wave_sync();
+            if (lane < k && rcar < k) outl[rcar] = cv;
+            if (lane < nl && rl < k) outl[rl] = lv;
+            wave_sync();
+            float key = __builtin_inff();
+            int32_t id = INT_MAX;
+            if (lane < k) {
+              const uint64_t v = outl[lane];
+              key = static_cast<float>(1.0 - static_cast<double>(pair_key(v)));
+              id = pair_id(v);
+            }
+            // 1 - key reverses the order: with the first k ascending and no
+            // two stored values equal (nor NaN), lane l goes to k - 1 - l;
+            // otherwise the (stored, id) sort
+            const float nkey = __shfl_down(key, 1);
+            const bool nan = key != key;
+            const bool tie = lane + 1 < k && nkey == key;
+            if (packed_sorted && __ballot(lane < k && (nan || tie)) == 0) {
+              if (lane < k) car[k - 1 - lane] = pack_pair(key, order_id(id));
+            } else {
+              wave_sort(key, id);
+              if (lane < k) car[lane] = pack_pair(key, order_id(id));
+              if (__ballot(lane < k && nan) != 0) packed_sorted = false;
+            }
+            wave_sync();
+          }
+        }
+        __syncthreads();
+      } else {
+        for (int c0 = 0; c0 < m; c0 += static_cast<int>(cap)) {
+          const int nc = static_cast<int>(c0 + cap < m ? cap : m - c0);
+          select_chunk(cp + c0, nc);
+        }
+      }
+    } else {
+      if (!summ_clean)
+        for (int w = tid; w < sum_words; w += 256) summ[w] = 0;
+      summ_clean = true;
+      win_clean = false;
+      if constexpr (!TABLE)
+        for (int dd = tid; dd < d; dd += 256) xi[dd] = x[i * ld + dd];
+      __syncthreads();
+      // union of the start-node ranges (.cu:160-191): bits in the global
+      // bitmap (L2 atomics), summary bits in LDS; with the key table each
+      // member's key goes to its id's slot (a member of several ranges gets
+      // the same key from each)
+      for (int t = 0; t < trees; ++t) {
+        const int32_t s0 = rs[static_cast<int64_t>(t) * n + i], c0 = rc[static_cast<int64_t>(t) * n + i];
+        const int32_t* pt = perm + static_cast<int64_t>(t) * n + s0;
+        const float* kt = TABLE ? ktab + koff[static_cast<int64_t>(t) * n + i] : nullptr;
+        // four members per thread per pass, their loads issued together
+        for (int p0 = 0; p0 < c0; p0 += 1024) {
+          int32_t jv[4];
+          float kv[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int p = p0 + 256 * u + tid;
+            jv[u] = p < c0 ? pt[p] : -1;
+            if constexpr (TABLE) kv[u] = p < c0 ? kt[p] : 0.0f;
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int32_t j = jv[u];
+            if (j < 0) continue;
+            if constexpr (TABLE) kid[j] = kv[u];
+            __hip_atomic_fetch_or(&bits[j >> 5], 1u << (j & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            atomicOr(&summ[j >> 10], 1u << ((j >> 5) & 31));
+          }
+        }
+      }
+      // every access to the bitmap is a workgroup-scope atomic and the key
+      // slots are read after a workgroup barrier: coherent within the
+      // workgroup, with no cache maintenance beyond the CU
+      __syncthreads();
+      // enumerate in ascending id order, without i itself (.cu:190-191); the
+      // words of each summary word's set bits are read together and cleared
+      int m = 0;
+      for (int sb = 0; sb < sum_words; sb += 256) {
+        const int sw = sb + tid;
+        const uint32_t s = sw < sum_words ? summ[sw] : 0u;
+        uint32_t bw[32];
+        int cnt = 0;
+        if (s != 0) {
+          // only the words whose summary bit is set, all requested before use
+          uint32_t* line = bits + static_cast<int64_t>(sw) * 32;
+#pragma unroll
+          for (int b = 0; b < 32; ++b)
+            bw[b] = ((s >> b) & 1u) ? __hip_atomic_load(&line[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) : 0u;
+          const int64_t iw = i >> 5;
+          if ((iw >> 5) == sw) bw[iw & 31] &= ~(1u << (i & 31));
+#pragma unroll
+          for (int b = 0; b < 32; ++b) cnt += __popc(bw[b]);
+        }
+        int tot;
+        int off = m + block_excl_scan256(cnt, sh, &tot);
+        if (s != 0) {
+          uint32_t* wline = bits + static_cast<int64_t>(sw) * 32;
+#pragma unroll
+          for (int b = 0; b < 32; ++b)
+            if ((s >> b) & 1u) __hip_atomic_store(&wline[b], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          summ[sw] = 0;
+#pragma unroll
+          for (int b = 0; b < 32; ++b)
+            for (uint32_t r = bw[b]; r; r &= r - 1)
+              cand[off++] = static_cast<int32_t>((static_cast<int64_t>(sw) * 32 + b) * 32 + __ffs(r) - 1);
+        }
+        m += tot;
+      }
+      if constexpr (!TABLE) {
+        __syncthreads();
+        const float ni = norms[i];
+        for (int c = tid; c < m; c += 256) {
+          const int32_t j = cand[c];
+          const float t = row_chain<kCosine>(xi, x + static_cast<int64_t>(j) * ld, d);
+          ckey[c] = finish_key<kCosine>(t, ni, norms[j]);
+        }
+      }
+      __syncthreads();
+      for (int64_t c0 = 0; c0 < m; c0 += cap) {
+        const int nc = static_cast<int>(c0 + cap < m ? cap : m - c0);
+        for (int e = tid; e < nc; e += 256) {
+          const int32_t j = cand[c0 + e];
+          const float key = TABLE ? kid[j] : ckey[c0 + e];
           pool[e] = pack_pair(key, order_id(j));
         }
+        __syncthreads();
+        select_chunk(pool, nc);
       }
-      __syncthreads();
-      pool_select(pool, np, k, outl, hist, sel);
-      if (wave == 0) {
-        float key = __builtin_inff();
-        int32_t id = INT_MAX;
-        if (lane < k) {
-          const uint64_t v = outl[lane];
-          key = static_cast<float>(1.0 - static_cast<double>(pair_key(v)));
-          id = pair_id(v);
-        }
-        wave_sort(key, id);
-        if (lane < k) car[lane] = pack_pair(key, order_id(id));
-      }
-      __syncthreads();
     }
     if (tid < k) {
       const uint64_t v = car[tid];
@@ -466,9 +884,13 @@ __global__ __launch_bounds__(256) void cosine_parity_kernel(const float* __restr
 }  // namespace
 
 size_t cosine_parity_lds(int d, int64_t summary_words, int k, int64_t cap, bool table) {
+  (void)k;
   const size_t xi = table ? 0 : static_cast<size_t>((d + 3) / 4 * 4) * 4;
   const size_t summ = static_cast<size_t>((summary_words + 3) / 4 * 4) * 4;
-  return xi + summ + static_cast<size_t>(k + cap) * 8;
+  const size_t bitmap_path = xi + summ + static_cast<size_t>(cap) * 8;
+  // the LDS union's layout (distinct members, bitmap window) shares the region
+  const size_t union_path = table ? size_t{kWinWords} * 4 : 0;
+  return bitmap_path > union_path ? bitmap_path : union_path;
 }
 
 void launch_cosine_parity_initial(const float* x, int d, int ld, const float* norms, int64_t n, int k, int trees,
@@ -478,7 +900,8 @@ void launch_cosine_parity_initial(const float* x, int d, int ld, const float* no
   if (k > 64) throw HipError("COSINE parity path supports k <= 64");
   if (sc.groups % 8 != 0) throw HipError("COSINE parity path: groups must be a multiple of 8");
   const bool table = ktab != nullptr;
-  if (table && (koff == nullptr || sc.keyarr == nullptr)) throw HipError("COSINE parity path: key table without offsets");
+  if (table && (koff == nullptr || sc.keyarr == nullptr || sc.uo == nullptr))
+    throw HipError("COSINE parity path: key table without offsets or scratch");
   const size_t shmem = cosine_parity_lds(d, sc.summary_words, k, cap, table);
   if (shmem > 160 * 1024 - 4096) throw HipError("COSINE parity path: d, n or k too large for LDS");
   if (table)

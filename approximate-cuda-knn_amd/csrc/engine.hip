@@ -477,7 +477,7 @@ void Engine::initial_knn() {
     // reads a line per summary bit); summary bits live in LDS
     const int64_t words = ((n_ + 31) / 32 + 31) / 32 * 32, summary = words / 32;
     auto groups_for = [&](bool table) {
-      const int64_t per_group = 4 * (words + summary) + 8 * need + (table ? 4 * n_ : 0);
+      const int64_t per_group = 4 * (words + summary) + 8 * need + (table ? 4 * n_ + 8 * 8192 : 0);
       const int64_t fit = (int64_t{table ? 24 : 4} << 30) / std::max<int64_t>(per_group, 1);
       return static_cast<int>(std::max<int64_t>(8, std::min<int64_t>({2048, fit / 8 * 8, (n_ + 7) / 8 * 8})));
     };
@@ -500,6 +500,7 @@ void Engine::initial_knn() {
       cps_.cand = alloc<int32_t>(static_cast<size_t>(cps_.groups) * cps_.max_cand);
       cps_.key = alloc<float>(static_cast<size_t>(cps_.groups) * cps_.max_cand);
       cps_.keyarr = table ? alloc<float>(static_cast<size_t>(cps_.groups) * n_) : nullptr;
+      cps_.uo = table ? alloc<uint64_t>(static_cast<size_t>(cps_.groups) * 8192) : nullptr;
       AKNN_HIP_TRY(hipMemsetAsync(cps_.bits, 0, sizeof(uint32_t) * cps_.groups * cps_.words, stream_));
     }
     table = table && cps_.keyarr != nullptr;

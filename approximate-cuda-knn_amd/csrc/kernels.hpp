@@ -200,6 +200,7 @@ struct CosineParityScratch {
   int32_t* cand;      // [groups][max_cand]
   float* key;         // [groups][max_cand]
   float* keyarr;      // [groups][n] keys by candidate id (with a key table; else null)
+  uint64_t* uo;       // [groups][8192] a point's distinct members in id order (with a key table; else null)
   int64_t words, summary_words, max_cand;
   int groups;
 };
