@@ -39,6 +39,7 @@
 #include "device_util.hpp"
 #include "kernels.hpp"
 #include "topk.hpp"
+#include "wave_ops.hpp"
 
 namespace aknn {
 
@@ -311,135 +312,6 @@ __device__ void pool_select(const uint64_t* a, int na, const uint64_t* b, int nb
   __syncthreads();
 }
 
-// Orders the LDS accesses of the lanes of one wave.
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-}
-
-// Lane j's 64-bit value, broadcast (j uniform); the halves are read as
-// unsigned 32-bit values so that the low half does not sign-extend.
-__device__ __forceinline__ uint64_t lane_u64(uint64_t v, int j) {
-  const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v >> 32), j));
-  const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(v)), j));
-  return (static_cast<uint64_t>(hi) << 32) | lo;
-}
-
-// Bitonic sort of one 64-bit value per lane over the wave, ascending.
-__device__ __forceinline__ uint64_t wave_sort_u64(uint64_t v) {
-  const int lane = threadIdx.x & 63;
-  for (int size = 2; size <= 64; size <<= 1) {
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      const uint32_t ohi = __shfl_xor(static_cast<uint32_t>(v >> 32), stride);
-      const uint32_t olo = __shfl_xor(static_cast<uint32_t>(v), stride);
-      const uint64_t o = (static_cast<uint64_t>(ohi) << 32) | olo;
-      const bool up = (lane & size) == 0, lower = (lane & stride) == 0;
-      const bool take = lower == up ? o < v : !(o < v);
-      v = take ? o : v;
-    }
-  }
-  return v;
-}
-
-// Count of the entries of the ascending array a[0, len) (len <= 64, LDS)
-// below v.
-__device__ __forceinline__ int count_below(const uint64_t* a, int len, uint64_t v) {
-  int c = 0;
-#pragma unroll
-  for (int step = 32; step >= 1; step >>= 1)
-    if (c + step <= len && a[c + step - 1] < v) c += step;
-  return c;
-}
-
-// First min(k, n) of the distinct packed pairs src[0, n) (LDS) by one wave,
-// in any order, into out: a radix select from the highest bit on which the
-// entries differ (the bits above it are common to all), up to 8 bits per
-// round, over a per-wave histogram wh[256].  Returns the count taken.
-__device__ int wave_topk(const uint64_t* src, int n, int k, uint64_t* out, uint32_t* wh) {
-  const int lane = threadIdx.x & 63;
-  if (n <= k) {
-    for (int e = lane; e < n; e += 64) out[e] = src[e];
-    return n;
-  }
-  uint32_t alo = ~0u, ahi = ~0u, olo = 0u, ohi = 0u;
-  for (int e = lane; e < n; e += 64) {
-    const uint64_t v = src[e];
-    alo &= static_cast<uint32_t>(v);
-    ahi &= static_cast<uint32_t>(v >> 32);
-    olo |= static_cast<uint32_t>(v);
-    ohi |= static_cast<uint32_t>(v >> 32);
-  }
-  for (int o = 32; o > 0; o >>= 1) {
-    alo &= __shfl_xor(alo, o);
-    ahi &= __shfl_xor(ahi, o);
-    olo |= __shfl_xor(olo, o);
-    ohi |= __shfl_xor(ohi, o);
-  }
-  const uint64_t all_and = (static_cast<uint64_t>(ahi) << 32) | alo;
-  const uint64_t dif = ((static_cast<uint64_t>(ohi) << 32) | olo) ^ all_and;  // nonzero: n > k >= 1 distinct
-  int hi_bit = 63 - __clzll(static_cast<long long>(dif));
-  uint64_t mask = hi_bit >= 63 ? 0ull : ~0ull << (hi_bit + 1);
-  uint64_t prefix = all_and & mask;
-  int need = k;
-  for (;;) {
-    const int width = hi_bit + 1 < 8 ? hi_bit + 1 : 8;
-    const int shift = hi_bit + 1 - width;
-    const uint32_t dm = (1u << width) - 1u;
-    *reinterpret_cast<uint4*>(wh + 4 * lane) = make_uint4(0u, 0u, 0u, 0u);
-    wave_sync();
-    for (int e = lane; e < n; e += 64) {
-      const uint64_t v = src[e];
-      if ((v & mask) == prefix) atomicAdd(&wh[(v >> shift) & dm], 1u);
-    }
-    wave_sync();
-    const uint4 h = *reinterpret_cast<const uint4*>(wh + 4 * lane);
-    const int sum = static_cast<int>(h.x + h.y + h.z + h.w);
-    int incl = sum;
-    for (int o = 1; o < 64; o <<= 1) {
-      const int t = __shfl_up(incl, o);
-      if (lane >= o) incl += t;
-    }
-    const int excl = incl - sum;
-    const bool hit = excl < need && need <= incl;
-    int dd = 0, before = 0, cnt = 0;
-    if (hit) {
-      const int hb[4] = {static_cast<int>(h.x), static_cast<int>(h.y), static_cast<int>(h.z), static_cast<int>(h.w)};
-      int c = excl;
-      for (int q = 0; q < 4; ++q) {
-        if (c + hb[q] >= need) {
-          dd = 4 * lane + q;
-          before = c;
-          cnt = hb[q];
-          break;
-        }
-        c += hb[q];
-      }
-    }
-    const int hl = __ffsll(static_cast<long long>(__ballot(hit))) - 1;
-    dd = __shfl(dd, hl);
-    before = __shfl(before, hl);
-    cnt = __shfl(cnt, hl);
-    prefix |= static_cast<uint64_t>(dd) << shift;
-    mask |= static_cast<uint64_t>(dm) << shift;
-    need -= before;
-    hi_bit = shift - 1;
-    // distinct entries: once every bit is fixed the bucket holds one entry
-    if (cnt == need || shift == 0) break;
-    wave_sync();
-  }
-  const uint64_t top = prefix | ~mask;
-  int got = 0;
-  for (int e0 = 0; e0 < n; e0 += 64) {
-    const int e = e0 + lane;
-    const uint64_t v = e < n ? src[e] : ~0ull;
-    const bool take = e < n && v <= top;
-    const uint64_t b = __ballot(take);
-    if (take) out[got + __popcll(b & ((1ull << lane) - 1ull))] = v;
-    got += __popcll(b);
-  }
-  return got;
-}
-
 // LDS union of a point's start-node ranges (key table form): a point whose
 // ranges hold at most kUnionCap members in all (duplicates counted) takes its
 // (id, key) members into registers, kUnionSlots per thread, and ranks them in
@@ -457,7 +329,7 @@ constexpr int kWordsPerThread = kWinWords / 256;
 constexpr int kGroupsPerThread = kWordsPerThread / 8;  // groups of 8 words (256 ids)
 static_assert(kGroupsPerThread == 8, "group prefixes are published as one 16-byte store per thread");
 constexpr int kUnionTrees = 64;
-constexpr int kGatherBatch = 8;
+constexpr int kGatherBatch = 16;
 constexpr int kRankBatch = 4;
 // chunk lists (each chunk's own first k) held for the merge
 constexpr int kLocalCap = 1024;
@@ -539,12 +411,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
       }
       __syncthreads();
       if (wave == 0) {
-        const int c = lane < trees ? tr_c[lane] : 0;
-        int incl = c;
-        for (int o = 1; o < 64; o <<= 1) {
-          const int t = __shfl_up(incl, o);
-          if (lane >= o) incl += t;
-        }
+        const int incl = wave_incl_add(lane < trees ? tr_c[lane] : 0);
         tr_p[lane + 1] = incl;
         if (lane == 0) tr_p[0] = 0;
       }
@@ -619,11 +486,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
                   __popc(b4.z) + __popc(b4.w);
           cnt += gc[q];
         }
-        int incl = cnt;
-        for (int o = 1; o < 64; o <<= 1) {
-          const int t = __shfl_up(incl, o);
-          if (lane >= o) incl += t;
-        }
+        const int incl = wave_incl_add(cnt);
         if (lane == 63) sh[wave] = incl;
         __syncthreads();
         int before = 0;
@@ -687,7 +550,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
       // the members (global writes ordered before these reads by the
       // workgroup barriers) into LDS over the window: at most kUnionCap
       uint64_t* cp = reinterpret_cast<uint64_t*>(lds);
-      for (int e = tid; e < m; e += 256) cp[e] = uo[e];
+      for (int e0 = 0; e0 < m; e0 += 256 * 8) {
+        uint64_t t8[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int e = e0 + 256 * u + tid;
+          t8[u] = e < m ? uo[e] : 0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int e = e0 + 256 * u + tid;
+          if (e < m) cp[e] = t8[u];
+        }
+      }
       if (m > 0) win_clean = false;
       __syncthreads();
       const int nch = static_cast<int>((m + cap - 1) / cap);

@@ -27,6 +27,8 @@
 
 #include "device_util.hpp"
 #include "kernels.hpp"
+#include "topk.hpp"
+#include "wave_ops.hpp"
 
 namespace aknn {
 
@@ -34,6 +36,7 @@ namespace {
 
 constexpr int kWavesPerBlock = 4;  // waves per block when their LDS fits (fewer otherwise)
 constexpr size_t kLdsBytes = 160 * 1024;
+
 
 
 // Open-addressing set of ids in LDS (-1 = empty), linear probing.  The table
@@ -83,6 +86,12 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
   int32_t* hs = reinterpret_cast<int32_t*>(xrow + xi_stride);
   int32_t* cand = hs + hs_size;
   int32_t* cnt = cand + cand_cap;
+  // COSINE parity, once the candidates' keys are packed: a histogram and
+  // two lists of k packed pairs over the candidate list (its capacity is at
+  // least 256 + 4k words then)
+  uint32_t* fhist = reinterpret_cast<uint32_t*>(cand);
+  uint64_t* ftop = reinterpret_cast<uint64_t*>(cand + 256);
+  uint64_t* fcar = ftop + k;
 
   // Points are taken in `order` (the first tree's leaf order): waves running at
   // the same time then own nearby points whose candidate rows overlap, and
@@ -298,7 +307,74 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
   }
 
   bool changed = false;
-  if (nnew > 0) {
+  if constexpr (FLIP) {
+    // COSINE parity (carried entries keyed by their flipped stored values,
+    // so nearly every candidate undercuts the k-th): the keys of all
+    // candidates first (one candidate per lane streaming its row), kept as
+    // packed pairs over the finished hash set; then the first k of (row +
+    // candidates), which are the first k of (row + the candidates' own
+    // first k): a radix select of the candidates and a merge by ranks.  A
+    // NaN key (no strict order) takes the insertion path, in candidate
+    // order as before.
+    if (nnew > 0) {
+      uint64_t* kv = reinterpret_cast<uint64_t*>(hs);  // [m] (hs has >= 2 m words)
+      bool nan = lane < k && key != key;
+      for (int base = 0; base < m; base += 64) {
+        const int ci = base + lane;
+        const int32_t c = ci < m ? cand[ci] : -1;
+        if (c >= 0) {
+          const float dk = finish_key<METRIC>(row_chain<METRIC>(xi, x + static_cast<int64_t>(c) * ld, d), ni,
+                                              METRIC == kCosine ? norms[c] : 1.0f);
+          nan |= dk != dk;
+          kv[ci] = pack_pair(dk, order_id(c));
+        }
+      }
+      wave_sync();
+      if (__ballot(nan) == 0) {
+        const int got = wave_topk(kv, m, k, ftop, fhist);
+        wave_sync();
+        const uint64_t tv = wave_sort_u64(lane < got ? ftop[lane] : ~0ull);
+        const uint64_t cv = lane < k ? pack_pair(key, order_id(id)) : ~0ull;
+        if (lane < got) ftop[lane] = tv;
+        if (lane < k) fcar[lane] = cv;
+        wave_sync();
+        // ranks in (row + candidates' first k); equal row entries (stubs)
+        // by position, the row being sorted
+        const int rc_ = lane + count_below(ftop, got, cv);
+        const int rt = lane + count_below(fcar, k, tv);
+        if (lane < k && rc_ < k) kv[rc_] = cv;
+        if (lane < got && rt < k) kv[rt] = tv;
+        wave_sync();
+        if (lane < k) {
+          const uint64_t v = kv[lane];
+          key = pair_key(v);
+          id = pair_id(v);
+        }
+      } else {
+        for (int base = 0; base < m; base += 64) {
+          const int ci = base + lane;
+          const uint64_t v = ci < m ? kv[ci] : 0ull;
+          const int32_t c = ci < m ? pair_id(v) : -1;
+          const float dk = ci < m ? pair_key(v) : __builtin_inff();
+          uint64_t surv = __ballot(c >= 0 && key_less(dk, c, thk, thi));
+          while (surv) {
+            const int src = __ffsll(static_cast<long long>(surv)) - 1;
+            surv &= surv - 1;
+            const float nk = __shfl(dk, src);
+            const int32_t nid = __shfl(c, src);
+            if (key_less(nk, nid, thk, thi)) {
+              wave_list_insert(key, id, k, nk, nid);
+              thk = __shfl(key, k - 1);
+              thi = __shfl(id, k - 1);
+            }
+          }
+        }
+      }
+      if (lane < k) key = static_cast<float>(1.0 - static_cast<double>(key));
+      wave_sort(key, id);
+      changed = true;
+    }
+  } else if (nnew > 0) {
     // exact pass: in the plain form, and after the principal-subspace screen
     // (a handful of survivors: one round trip for all of them), one candidate
     // per lane streams its row; after the sketch screens the survivors' rows
@@ -348,11 +424,6 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
         }
       }
     }
-    if constexpr (FLIP) {
-      if (lane < k) key = static_cast<float>(1.0 - static_cast<double>(key));
-      wave_sort(key, id);
-      changed = true;
-    }
   }
   if (changed_out != nullptr && lane == 0) changed_out[i] = changed ? 1 : 0;
   if (lane < k) {
@@ -372,11 +443,13 @@ struct WaveLds {
 // At most 1 + k + k*k ids enter the set, which has at least twice as many
 // slots: a power of two when four waves fit with it (the measured layout),
 // else the smallest multiple of 64.
-WaveLds plan_lds(int k, int d, int ld, int ldh, int ld8, int pca_r, int skm) {
+WaveLds plan_lds(int k, int d, int ld, int ldh, int ld8, int pca_r, int skm, bool flip) {
   WaveLds w;
   const int need = 2 * (k * k + k + 1);
   // (multiples of 4 keep every wave's LDS block 16-byte aligned)
   w.cand_cap = (k * k + 3) / 4 * 4;
+  // COSINE parity: the selection's histogram and lists reuse the list
+  if (flip && w.cand_cap < 256 + 4 * k) w.cand_cap = (256 + 4 * k + 3) / 4 * 4;
   w.xi_stride = skm == 1 ? ldh + 3 * ld : skm == 2 ? ld8 / 4 + 3 * ld : skm == 3 ? pca_r + ld : (d + 3) / 4 * 4;
   int pow2 = 128;
   while (pow2 < need) pow2 *= 2;
@@ -394,7 +467,7 @@ WaveLds plan_lds(int k, int d, int ld, int ldh, int ld8, int pca_r, int skm) {
 
 template <int METRIC, bool FLIP, int SKM>
 void launch_w(const PropagateArgs& a, hipStream_t s) {
-  const WaveLds w = plan_lds(a.k, a.d, a.ld, a.ldh, a.ld8, a.pca_r, SKM);
+  const WaveLds w = plan_lds(a.k, a.d, a.ld, a.ldh, a.ld8, a.pca_r, SKM, FLIP);
   if (w.waves == 0) throw HipError("propagation: k or d too large for LDS");
   const int hs_size = w.hs_size, cand_cap = w.cand_cap, xi_stride = w.xi_stride;
   const size_t shmem = w.bytes();
@@ -422,7 +495,7 @@ void launch_t(const PropagateArgs& a, hipStream_t s) {
   }
   // a screen whose wider per-wave layout does not fit even one wave's LDS
   // gives way to the exact pass (the same graph)
-  auto fits = [&](int skm) { return plan_lds(a.k, a.d, a.ld, a.ldh, a.ld8, a.pca_r, skm).waves > 0; };
+  auto fits = [&](int skm) { return plan_lds(a.k, a.d, a.ld, a.ldh, a.ld8, a.pca_r, skm, FLIP).waves > 0; };
   if constexpr (METRIC == kEuclidean) {
     if (a.pca != nullptr && fits(3)) {
       launch_w<METRIC, FLIP, 3>(a, s);
@@ -442,7 +515,7 @@ void launch_t(const PropagateArgs& a, hipStream_t s) {
 }  // namespace
 
 bool propagation_fits(int k, int d) {
-  return plan_lds(k, d, (d + 3) / 4 * 4, 0, 0, 0, 0).waves > 0;
+  return plan_lds(k, d, (d + 3) / 4 * 4, 0, 0, 0, 0, true).waves > 0;
 }
 
 void launch_propagate(const PropagateArgs& a, hipStream_t s) {
