@@ -38,6 +38,11 @@ constexpr int kTile = 64;
 // 16-byte steps of the re-score chain unrolled together: 2 * 8 row loads in
 // flight per lane (the MFMA phase's registers are free by then)
 constexpr int kRescoreUnroll = 8;
+// staged re-score: dimensions per LDS stage, batches of 64 pairs advanced
+// together, and the fewest pairs of a tile that take it
+constexpr int kStageDims = 16;
+constexpr int kStageBatches = 4;
+constexpr int kStageMin = 24;
 
 // Owner threshold.  EUCLIDEAN: the smallest T such that a pair whose sketch
 // distance exceeds T + eps_candidate has an exact key above thk
@@ -123,6 +128,7 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
   __shared__ __attribute__((aligned(16))) int32_t o_lo[kTile];  // owner's start-node positions [lo, hi)
   __shared__ __attribute__((aligned(16))) int32_t o_hi[kTile];
   __shared__ uint16_t plist[kTile * kTile];
+  __shared__ __attribute__((aligned(16))) float stg[2 * kTile * kStageDims];  // staged owner and candidate rows
 
   const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
   const int64_t leaf_idx = lb / a.chunks_per_leaf;
@@ -340,26 +346,13 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
     nres += total;
     o_k[lane] = has_owner ? pk[KMAX - 1] : 0ull;
     __syncthreads();
-    // exact keys on all lanes; a pair that does not undercut its owner's
-    // current k-th pair cannot enter and is not handed over.  The pairs that
-    // can are compacted in list order (owner-major), so each owner finds its
-    // own from the ballot and its list segment [off, off + cnt).
-    for (int base = 0; base < total; base += kTile) {
-      const int p = base + lane;
-      uint64_t v = ~0ull;
-      if (p < total) {
-        const int e = plist[p];
-        const float* xo = a.x + static_cast<int64_t>(oid[e >> 6]) * a.ld;
-        const float* xc = a.x + static_cast<int64_t>(cid[e & 63]) * a.ld;
-        float key;
-        if constexpr (METRIC == kEuclidean)
-          key = finish_key<kEuclidean>(row_chain_global<kEuclidean, kRescoreUnroll>(xo, xc, a.d), 1.0f, 1.0f);
-        else
-          key = finish_key<kCosine>(row_chain_global<kCosine, kRescoreUnroll>(xo, xc, a.d), a.norms[oid[e >> 6]],
-                                    a.norms[cid[e & 63]]);
-        v = pack_pair(key, order_id(cid[e & 63]));
-        if (!(v < o_k[e >> 6])) v = ~0ull;
-      }
+    // exact keys; a pair that does not undercut its owner's current k-th
+    // pair cannot enter and is not handed over.  The pairs that can are
+    // compacted in list order (owner-major), so each owner finds its own
+    // from the ballot and its list segment [off, off + cnt).  None of them
+    // is in its owner's row: the survivor list left out the row's ids, and
+    // the tiles of a range hold distinct candidates.
+    auto hand_over = [&](int base, uint64_t v) {
       const uint64_t ins = __ballot(v != ~0ull);
       if (v != ~0ull) kp[__popcll(ins & lane_mask_lt(lane))] = v;
       __syncthreads();
@@ -368,11 +361,126 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
         const int hi = (off + cnt < base + kTile ? off + cnt : base + kTile) - base;
         if (lo < hi) {
           const int q0 = __popcll(ins & lane_mask_lt(lo)), q1 = __popcll(ins & lane_mask_lt(hi));
-          for (int q = q0; q < q1; ++q) topk_offer<KMAX>(pk, kp[q]);
+#pragma unroll 4
+          for (int q = q0; q < q1; ++q) topk_insert<KMAX>(pk, kp[q]);
           o_k[lane] = pk[KMAX - 1];
         }
       }
       __syncthreads();
+    };
+    if (total >= kStageMin) {
+      // The tile's 64 owner and 64 candidate rows are staged through LDS
+      // kStageDims dimensions at a time with coalesced loads (four lanes per
+      // row), and up to kStageBatches batches of 64 pairs advance their
+      // chains from LDS: each chain is the exact chain's steps in dimension
+      // order, with its 256-dimension partial sums added in the same order.
+      // Lane l stages the quarter l & 3 of rows (l >> 2) + 16 j: rows below
+      // 64 are owners, the rest candidates (absent ones stage zeros).
+      const float* src[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int r = (lane >> 2) + 16 * j;
+        const int32_t rid = r < kTile ? oid[r] : cid[r - kTile];
+        src[j] = rid >= 0 ? a.x + static_cast<int64_t>(rid) * a.ld : nullptr;
+      }
+      const int q4 = 4 * (lane & 3);
+      for (int bg = 0; bg < total; bg += kTile * kStageBatches) {
+        float pp[kStageBatches], tt[kStageBatches];
+        int ep[kStageBatches];
+#pragma unroll
+        for (int b = 0; b < kStageBatches; ++b) {
+          pp[b] = 0.0f;
+          tt[b] = 0.0f;
+          const int pi = bg + kTile * b + lane;
+          ep[b] = pi < total ? static_cast<int>(plist[pi]) : -1;
+        }
+        // the next stage's rows are requested before this stage's chains run
+        float4 v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          v[j] = src[j] != nullptr && q4 < a.d ? *reinterpret_cast<const float4*>(src[j] + q4)
+                                               : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int c0 = 0; c0 < a.d; c0 += kStageDims) {
+          __syncthreads();  // the previous stage is consumed
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            const int r = (lane >> 2) + 16 * j;
+            *reinterpret_cast<float4*>(stg + r * kStageDims + q4) = v[j];
+          }
+          __syncthreads();
+          const int cn = c0 + kStageDims;
+#pragma unroll
+          for (int j = 0; j < 8; ++j)
+            v[j] = src[j] != nullptr && cn + q4 < a.d ? *reinterpret_cast<const float4*>(src[j] + cn + q4)
+                                                      : make_float4(0.f, 0.f, 0.f, 0.f);
+          const int nd = a.d - c0 < kStageDims ? a.d - c0 : kStageDims;
+          const bool cut = c0 > 0 && (c0 & 255) == 0;  // a new 256-dimension partial sum
+#pragma unroll
+          for (int b = 0; b < kStageBatches; ++b) {
+            if (ep[b] >= 0) {
+              if (cut) {
+                tt[b] = tt[b] + pp[b];
+                pp[b] = 0.0f;
+              }
+              const float* so = stg + (ep[b] >> 6) * kStageDims;
+              const float* sc = stg + (kTile + (ep[b] & 63)) * kStageDims;
+              float p = pp[b];
+              if (nd == kStageDims) {
+#pragma unroll
+                for (int g = 0; g < kStageDims / 4; ++g) {
+                  const float4 av = *reinterpret_cast<const float4*>(so + 4 * g);
+                  const float4 bv = *reinterpret_cast<const float4*>(sc + 4 * g);
+                  p = chain_step<METRIC>(av.x, bv.x, p);
+                  p = chain_step<METRIC>(av.y, bv.y, p);
+                  p = chain_step<METRIC>(av.z, bv.z, p);
+                  p = chain_step<METRIC>(av.w, bv.w, p);
+                }
+              } else {
+                for (int dd = 0; dd < nd; ++dd) p = chain_step<METRIC>(so[dd], sc[dd], p);
+              }
+              pp[b] = p;
+            }
+          }
+        }
+#pragma unroll
+        for (int b = 0; b < kStageBatches; ++b) {
+          const int base = bg + kTile * b;
+          if (base >= total) break;
+          uint64_t v = ~0ull;
+          if (ep[b] >= 0) {
+            const int e = ep[b];
+            const float t = tt[b] + pp[b];
+            float key;
+            if constexpr (METRIC == kEuclidean)
+              key = finish_key<kEuclidean>(t, 1.0f, 1.0f);
+            else
+              key = finish_key<kCosine>(t, a.norms[oid[e >> 6]], a.norms[cid[e & 63]]);
+            v = pack_pair(key, order_id(cid[e & 63]));
+            if (!(v < o_k[e >> 6])) v = ~0ull;
+          }
+          hand_over(base, v);
+        }
+      }
+    } else {
+      // few pairs: one lane per pair walks both rows in global memory
+      for (int base = 0; base < total; base += kTile) {
+        const int p = base + lane;
+        uint64_t v = ~0ull;
+        if (p < total) {
+          const int e = plist[p];
+          const float* xo = a.x + static_cast<int64_t>(oid[e >> 6]) * a.ld;
+          const float* xc = a.x + static_cast<int64_t>(cid[e & 63]) * a.ld;
+          float key;
+          if constexpr (METRIC == kEuclidean)
+            key = finish_key<kEuclidean>(row_chain_global<kEuclidean, kRescoreUnroll>(xo, xc, a.d), 1.0f, 1.0f);
+          else
+            key = finish_key<kCosine>(row_chain_global<kCosine, kRescoreUnroll>(xo, xc, a.d), a.norms[oid[e >> 6]],
+                                      a.norms[cid[e & 63]]);
+          v = pack_pair(key, order_id(cid[e & 63]));
+          if (!(v < o_k[e >> 6])) v = ~0ull;
+        }
+        hand_over(base, v);
+      }
     }
     if (has_owner && cnt > 0) o_t[lane] = owner_t(pk[KMAX - 1]);
   }

@@ -124,6 +124,20 @@ __device__ __forceinline__ bool topk_has(const uint64_t (&pk)[KMAX], uint64_t v)
   return has;
 }
 
+// Inserts v when it undercuts the k-th pair; the caller guarantees that v's
+// id is not in the list.
+template <int KMAX>
+__device__ __forceinline__ void topk_insert(uint64_t (&pk)[KMAX], uint64_t v) {
+  if (v < pk[KMAX - 1]) {
+    bool lt[KMAX];
+#pragma unroll
+    for (int p = 0; p < KMAX; ++p) lt[p] = v < pk[p];
+#pragma unroll
+    for (int p = KMAX - 1; p > 0; --p) pk[p] = lt[p - 1] ? pk[p - 1] : (lt[p] ? v : pk[p]);
+    pk[0] = lt[0] ? v : pk[0];
+  }
+}
+
 // Inserts v when it undercuts the k-th pair and its id is not in the list
 // yet (a candidate an earlier tree already stored carries the same exact
 // key, so it must not enter twice); the k-th pair drops out.
