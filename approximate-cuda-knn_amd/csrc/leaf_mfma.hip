@@ -394,13 +394,20 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
           const int pi = bg + kTile * b + lane;
           ep[b] = pi < total ? static_cast<int>(plist[pi]) : -1;
         }
-        // the next stage's rows are requested before this stage's chains run
-        float4 v[8];
+        // with lists of up to 24 the next stage's rows are requested before
+        // this stage's chains run (longer lists keep the registers for two
+        // waves per SIMD and load each stage in turn)
+        constexpr bool kAhead = KMAX <= 24;
+        auto stage_load = [&](float4 (&v)[8], int c) {
 #pragma unroll
-        for (int j = 0; j < 8; ++j)
-          v[j] = src[j] != nullptr && q4 < a.d ? *reinterpret_cast<const float4*>(src[j] + q4)
-                                               : make_float4(0.f, 0.f, 0.f, 0.f);
+          for (int j = 0; j < 8; ++j)
+            v[j] = src[j] != nullptr && c + q4 < a.d ? *reinterpret_cast<const float4*>(src[j] + c + q4)
+                                                     : make_float4(0.f, 0.f, 0.f, 0.f);
+        };
+        float4 v[8];
+        if constexpr (kAhead) stage_load(v, 0);
         for (int c0 = 0; c0 < a.d; c0 += kStageDims) {
+          if constexpr (!kAhead) stage_load(v, c0);
           __syncthreads();  // the previous stage is consumed
 #pragma unroll
           for (int j = 0; j < 8; ++j) {
@@ -408,11 +415,7 @@ __global__ __launch_bounds__(64) void leaf_mfma_kernel(MfmaLeafArgs a) {
             *reinterpret_cast<float4*>(stg + r * kStageDims + q4) = v[j];
           }
           __syncthreads();
-          const int cn = c0 + kStageDims;
-#pragma unroll
-          for (int j = 0; j < 8; ++j)
-            v[j] = src[j] != nullptr && cn + q4 < a.d ? *reinterpret_cast<const float4*>(src[j] + cn + q4)
-                                                      : make_float4(0.f, 0.f, 0.f, 0.f);
+          if constexpr (kAhead) stage_load(v, c0 + kStageDims);
           const int nd = a.d - c0 < kStageDims ? a.d - c0 : kStageDims;
           const bool cut = c0 > 0 && (c0 & 255) == 0;  // a new 256-dimension partial sum
 #pragma unroll
