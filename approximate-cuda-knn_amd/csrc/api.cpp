@@ -233,6 +233,8 @@ int aknn_set_option(aknn_handle* h, const char* name, int64_t value) {
       h->io_threads = static_cast<int>(value);
     } else if (n == "pca_screen") {
       h->params.pca_screen = value != 0;
+    } else if (n == "pca_fp16") {
+      h->params.pca_fp16 = value != 0;
     } else if (n == "mean_fp64") {
       h->params.mean_fp64 = value != 0;
     } else if (n == "cosine_table") {

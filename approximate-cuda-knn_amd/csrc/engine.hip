@@ -662,7 +662,10 @@ static PropagateArgs round_args(const Params& p, int kmetric, bool flip, bool sc
     a.xh = nullptr;
     a.eps = nullptr;
     a.q8 = nullptr;
-    a.pca = pca_proj;
+    if (p.pca_fp16)
+      a.pca16 = reinterpret_cast<const _Float16*>(pca_proj);  // the buffer holds FP16 rows then
+    else
+      a.pca = pca_proj;
     a.pca_r = pb.r;
     a.pca_e = pca_e;
     a.pca_sigma = pb.sigma;
@@ -868,7 +871,11 @@ void Engine::calculate() {
     if (pca_.r > 0) {
       AKNN_HIP_TRY(hipMemcpyAsync(pca_p_, pca_.p.data(), sizeof(float) * pca_.p.size(), hipMemcpyHostToDevice,
                                   stream_));
-      launch_pca_project(x_, n_, d_, ld_, pca_p_, pca_.r, pca_proj_, pca_e_, stream_);
+      if (p_.pca_fp16)
+        launch_pca_project(x_, n_, d_, ld_, pca_p_, pca_.r, nullptr, pca_e_, stream_,
+                           reinterpret_cast<_Float16*>(pca_proj_));
+      else
+        launch_pca_project(x_, n_, d_, ld_, pca_p_, pca_.r, pca_proj_, pca_e_, stream_);
       launch_max_nonneg(pca_e_, n_, pca_e_ + n_, stream_);
     }
   }

@@ -34,6 +34,7 @@ struct Params {
   int scale_all = 0;      // normalization scales every dimension, not only the first 256
   int graph_rounds = 1;   // once the round mode is fixed, replay the propagation rounds as one captured hipGraph
   int pca_screen = 0;     // EUCLIDEAN propagation screens with a principal-subspace bound when the data allow (same results)
+  int pca_fp16 = 1;       // the screen's projections are kept in FP16, their rounding inside the radii (same results)
   int cosine_table = 1;   // COSINE parity initial pass reads its keys from a per-tree key table when it fits (same results)
   int mean_fp64 = 0;      // normalization's mean as blocked FP64 sums (the reference: a sequential FP32 sum)
 };
@@ -84,6 +85,8 @@ class Engine {
     p_.scale_all = p.scale_all;
     p_.graph_rounds = p.graph_rounds;
     p_.pca_screen = p.pca_screen;
+    if (p_.pca_fp16 != p.pca_fp16) prop_mode_ = -1;
+    p_.pca_fp16 = p.pca_fp16;
     p_.cosine_table = p.cosine_table;
     if (p_.mean_fp64 != p.mean_fp64) prop_mode_ = -1;
     p_.mean_fp64 = p.mean_fp64;

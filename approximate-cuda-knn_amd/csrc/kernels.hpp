@@ -119,6 +119,7 @@ struct PropagateArgs {
   const float* norms = nullptr;
   // principal-subspace screen (EUCLIDEAN; used instead of the sketches when set)
   const float* pca = nullptr;
+  const _Float16* pca16 = nullptr;  // FP16 projections (taken instead of pca when set)
   int pca_r = 0;
   const float* pca_e = nullptr;
   double pca_sigma = 1.0;
@@ -150,9 +151,12 @@ struct PcaBasis {
 };
 // moments_dev: device scratch of 33*d*d floats.  Synchronises the stream.
 PcaBasis compute_pca_basis(const float* x, int64_t n, int d, int ld, float* moments_dev, hipStream_t s);
-// proj [n][r] = fl(p x_i), rad[n] >= ||proj_i - p x_i||.
+// proj [n][r] = fl(p x_i), rad[n] >= ||proj_i - p x_i||.  With proj16
+// (proj then null): proj16 [n][r] = the FP16 roundings of fl(p x_i) and
+// rad[n] >= ||proj16_i - p x_i|| (the rounding's exact norm added; infinite
+// when a coordinate is beyond FP16's range).
 void launch_pca_project(const float* x, int64_t n, int d, int ld, const float* p, int r, float* proj, float* rad,
-                        hipStream_t s);
+                        hipStream_t s, _Float16* proj16 = nullptr);
 // *out = max of n non-negative floats (+inf allowed).  The screens' radius
 // arrays keep this at index n: one upper bound for every candidate, read
 // once per wave instead of one scattered 4-byte load per candidate.

@@ -67,7 +67,8 @@ __global__ __launch_bounds__(256) void moments_reduce_kernel(const float* __rest
 template <int R>
 __global__ __launch_bounds__(256) void pca_project_kernel(const float* __restrict__ x, int64_t n, int d, int ld,
                                                           const float* __restrict__ p, float* __restrict__ proj,
-                                                          float* __restrict__ rad, double g, double tiny) {
+                                                          _Float16* __restrict__ proj16, float* __restrict__ rad,
+                                                          double g, double tiny) {
   __shared__ float ps[R * 256];
   const int64_t i = blockIdx.x * 256ll + threadIdx.x;
   const float* xr = x + (i < n ? i : 0) * ld;
@@ -118,14 +119,26 @@ __global__ __launch_bounds__(256) void pca_project_kernel(const float* __restric
     }
   }
   if (i < n) {
-    double e2 = 0.0;
+    double e2 = 0.0, h2 = 0.0;
+    bool over = false;
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-      proj[i * R + j] = acc[j];
+      if (proj16 != nullptr) {
+        // FP16 rounding of the coordinate, its error kept exactly (FP64
+        // holds the difference of an FP32 and an FP16 value)
+        const _Float16 h = static_cast<_Float16>(acc[j]);
+        const double hd = static_cast<double>(static_cast<float>(h)) - static_cast<double>(acc[j]);
+        over |= !(__builtin_fabsf(static_cast<float>(h)) <= 65504.0f);
+        h2 += hd * hd;
+        proj16[i * R + j] = h;
+      } else {
+        proj[i * R + j] = acc[j];
+      }
       const double ej = g * static_cast<double>(ab[j]) * (1.0 + g) + tiny;
       e2 += ej * ej;
     }
-    const double e = sqrt(e2) * (1.0 + 1e-12);
+    // ||h - P x|| <= ||h - p|| + ||p - P x||
+    const double e = over ? __builtin_inf() : (sqrt(e2) + sqrt(h2)) * (1.0 + 1e-12);
     float f = static_cast<float>(e);
     if (static_cast<double>(f) < e) f = __uint_as_float(__float_as_uint(f) + 1u);
     rad[i] = f;
@@ -274,14 +287,16 @@ PcaBasis compute_pca_basis(const float* x, int64_t n, int d, int ld, float* mome
 }
 
 void launch_pca_project(const float* x, int64_t n, int d, int ld, const float* p, int r, float* proj, float* rad,
-                        hipStream_t s) {
+                        hipStream_t s, _Float16* proj16) {
   const double g = 2.0 * (d + 8) / 16777216.0;
   const double tiny = (d + 16) * 1.401298464324817e-45;
   const unsigned grid = static_cast<unsigned>((n + 255) / 256);
   if (r == 16)
-    hipLaunchKernelGGL(pca_project_kernel<16>, dim3(grid), dim3(256), 0, s, x, n, d, ld, p, proj, rad, g, tiny);
+    hipLaunchKernelGGL(pca_project_kernel<16>, dim3(grid), dim3(256), 0, s, x, n, d, ld, p, proj, proj16, rad, g,
+                       tiny);
   else if (r == 32)
-    hipLaunchKernelGGL(pca_project_kernel<32>, dim3(grid), dim3(256), 0, s, x, n, d, ld, p, proj, rad, g, tiny);
+    hipLaunchKernelGGL(pca_project_kernel<32>, dim3(grid), dim3(256), 0, s, x, n, d, ld, p, proj, proj16, rad, g,
+                       tiny);
   else
     throw HipError("pca: r must be 16 or 32");
   AKNN_HIP_TRY(hipGetLastError());

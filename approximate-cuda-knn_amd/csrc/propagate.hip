@@ -53,7 +53,8 @@ __device__ __forceinline__ bool hash_insert(int32_t* hs, uint32_t size, int32_t 
 }
 
 // SKM: 0 exact pass only, 1 FP16 sketch screen, 2 int8 sketch screen,
-// 3 principal-subspace screen (pca.hip).
+// 3 principal-subspace screen (pca.hip) on FP32 projections, 4 the same on
+// FP16 projections (their rounding is inside each point's radius).
 template <int METRIC, bool FLIP, int SKM>
 __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict__ x, int d, int ld,
                                                         const _Float16* __restrict__ xh, int ldh,
@@ -61,7 +62,8 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
                                                         const int8_t* __restrict__ q8, int ld8,
                                                         const float* __restrict__ s8, const int32_t* __restrict__ a8,
                                                         const float* __restrict__ e8,
-                                                        const float* __restrict__ pca, int pca_r,
+                                                        const float* __restrict__ pca,
+                                                        const _Float16* __restrict__ pca16, int pca_r,
                                                         const float* __restrict__ pca_e, double pca_sigma,
                                                         const float* __restrict__ norms,
                                                         const int32_t* __restrict__ order, int64_t n, int k,
@@ -81,7 +83,7 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
   const size_t per_wave_words = static_cast<size_t>(xi_stride) + hs_size + cand_cap + 4;
   float* xrow = reinterpret_cast<float*>(smem) + per_wave_words * wave;  // 16-byte aligned
   constexpr bool SK = SKM > 0;
-  float* xf = xrow + (SKM == 1 ? ldh : (SKM == 2 ? ld8 / 4 : (SKM == 3 ? pca_r : 0)));
+  float* xf = xrow + (SKM == 1 ? ldh : (SKM == 2 ? ld8 / 4 : (SKM >= 3 ? pca_r : 0)));
   float* stg = xf + ld;
   int32_t* hs = reinterpret_cast<int32_t*>(xrow + xi_stride);
   int32_t* cand = hs + hs_size;
@@ -129,8 +131,9 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
     for (int dd = lane; dd < ld8 / 4; dd += 64)
       reinterpret_cast<int32_t*>(xrow)[dd] = reinterpret_cast<const int32_t*>(q8 + i * ld8)[dd];
     for (int dd = lane; dd < ld; dd += 64) xf[dd] = x[i * ld + dd];
-  } else if constexpr (SKM == 3) {
-    for (int dd = lane; dd < pca_r; dd += 64) xrow[dd] = pca[i * pca_r + dd];
+  } else if constexpr (SKM >= 3) {
+    for (int dd = lane; dd < pca_r; dd += 64)
+      xrow[dd] = SKM == 4 ? static_cast<float>(pca16[i * pca_r + dd]) : pca[i * pca_r + dd];
     for (int dd = lane; dd < ld; dd += 64) xf[dd] = x[i * ld + dd];
   } else {
     for (int dd = lane; dd < d; dd += 64) xrow[dd] = x[i * ld + dd];
@@ -187,17 +190,17 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
     if (m > 0) {
       // screen: survivors are written back to cand[] in order (slot
       // w + rank <= base + lane was already read by this batch)
-      const float ei = SKM == 2 ? e8[i] : (SKM == 3 ? pca_e[i] : eps[i]);
+      const float ei = SKM == 2 ? e8[i] : (SKM >= 3 ? pca_e[i] : eps[i]);
       // every candidate's radius is bounded by the largest (kept at index n):
       // one load per wave instead of a scattered 4-byte load per candidate.
       // An infinite maximum (a row the sketch cannot hold) would switch the
       // screen off for all; then each candidate's own radius is read.
-      const float ec_max = SKM == 3 ? pca_e[n] : (SKM == 1 ? eps[n] : 0.0f);
+      const float ec_max = SKM >= 3 ? pca_e[n] : (SKM == 1 ? eps[n] : 0.0f);
       const bool per_cand = !(ec_max < __builtin_inff());
       const double si = SKM == 2 ? static_cast<double>(s8[i]) : 0.0;
       const double ti = SKM == 2 ? si * si * static_cast<double>(a8[i]) : 0.0;
       int w = 0;
-      if constexpr (SKM == 3) {
+      if constexpr (SKM >= 3) {
         // two batches of candidates per memory round trip: their ids and
         // projections are loaded before either is tested (compaction writes
         // slots at or below ones already read)
@@ -205,13 +208,32 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
         for (int base = 0; base < m; base += 128) {
           int32_t cc[2];
           float4 pv[2][8];
+          uint4 ph[2][4];
 #pragma unroll
           for (int u = 0; u < 2; ++u) {
             const int ci = base + 64 * u + lane;
             cc[u] = ci < m ? cand[ci] : -1;
-            const float4* pc = reinterpret_cast<const float4*>(pca + static_cast<int64_t>(cc[u] >= 0 ? cc[u] : 0) * pca_r);
+            const int64_t row = static_cast<int64_t>(cc[u] >= 0 ? cc[u] : 0) * pca_r;
+            if constexpr (SKM == 4) {
+              const uint4* pc = reinterpret_cast<const uint4*>(pca16 + row);
 #pragma unroll
-            for (int g = 0; g < 8; ++g) pv[u][g] = g < pca_r / 4 ? pc[g] : make_float4(0.f, 0.f, 0.f, 0.f);
+              for (int g = 0; g < 4; ++g) ph[u][g] = g < pca_r / 8 ? pc[g] : make_uint4(0u, 0u, 0u, 0u);
+            } else {
+              const float4* pc = reinterpret_cast<const float4*>(pca + row);
+#pragma unroll
+              for (int g = 0; g < 8; ++g) pv[u][g] = g < pca_r / 4 ? pc[g] : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+          }
+          if constexpr (SKM == 4) {
+            // the FP16 values as FP32 (exact), four per group as in the FP32 form
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+#pragma unroll
+              for (int g = 0; g < 8; ++g) {
+                const uint4 h = ph[u][g >> 1];
+                const uint32_t w0 = (g & 1) ? h.z : h.x, w1 = (g & 1) ? h.w : h.y;
+                pv[u][g] = make_float4(half_lo(w0), half_hi(w0), half_lo(w1), half_hi(w1));
+              }
           }
           const float4* pq = reinterpret_cast<const float4*>(xs);
 #pragma unroll
@@ -380,7 +402,7 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
     // per lane streams its row; after the sketch screens the survivors' rows
     // are first staged in LDS two at a time by the whole wave, then chained
     // from LDS
-    constexpr bool STAGE = SK && SKM != 3;
+    constexpr bool STAGE = SK && SKM < 3;
     const int step = STAGE ? 2 : 64;
     for (int base = 0; base < m; base += step) {
       const int ci = base + lane;
@@ -450,7 +472,7 @@ WaveLds plan_lds(int k, int d, int ld, int ldh, int ld8, int pca_r, int skm, boo
   w.cand_cap = (k * k + 3) / 4 * 4;
   // COSINE parity: the selection's histogram and lists reuse the list
   if (flip && w.cand_cap < 256 + 4 * k) w.cand_cap = (256 + 4 * k + 3) / 4 * 4;
-  w.xi_stride = skm == 1 ? ldh + 3 * ld : skm == 2 ? ld8 / 4 + 3 * ld : skm == 3 ? pca_r + ld : (d + 3) / 4 * 4;
+  w.xi_stride = skm == 1 ? ldh + 3 * ld : skm == 2 ? ld8 / 4 + 3 * ld : skm >= 3 ? pca_r + ld : (d + 3) / 4 * 4;
   int pow2 = 128;
   while (pow2 < need) pow2 *= 2;
   w.hs_size = pow2;
@@ -474,7 +496,8 @@ void launch_w(const PropagateArgs& a, hipStream_t s) {
   const int64_t blocks = xcd_grid((a.n + w.waves - 1) / w.waves);
   const SketchBound sb = sketch_bound(a.d);
   hipLaunchKernelGGL((propagate_kernel<METRIC, FLIP, SKM>), dim3(static_cast<unsigned>(blocks)), dim3(64 * w.waves), shmem,
-                     s, a.x, a.d, a.ld, a.xh, a.ldh, a.eps, sb, a.q8, a.ld8, a.s8, a.a8, a.e8, a.pca, a.pca_r, a.pca_e,
+                     s, a.x, a.d, a.ld, a.xh, a.ldh, a.eps, sb, a.q8, a.ld8, a.s8, a.a8, a.e8, a.pca, a.pca16, a.pca_r,
+                     a.pca_e,
                      a.pca_sigma, a.norms, a.order, a.n, a.k, hs_size, cand_cap,
                      xi_stride, a.ids_in, a.vals_in, a.ids_out, a.vals_out, a.ncand,
                      FLIP ? nullptr : a.hist.ids_prev, a.hist.changed_prev, a.hist.changed_out);
@@ -497,6 +520,10 @@ void launch_t(const PropagateArgs& a, hipStream_t s) {
   // gives way to the exact pass (the same graph)
   auto fits = [&](int skm) { return plan_lds(a.k, a.d, a.ld, a.ldh, a.ld8, a.pca_r, skm, FLIP).waves > 0; };
   if constexpr (METRIC == kEuclidean) {
+    if (a.pca16 != nullptr && fits(4)) {
+      launch_w<METRIC, FLIP, 4>(a, s);
+      return;
+    }
     if (a.pca != nullptr && fits(3)) {
       launch_w<METRIC, FLIP, 3>(a, s);
       return;
@@ -525,7 +552,7 @@ void launch_propagate(const PropagateArgs& a, hipStream_t s) {
     throw HipError("propagation: malformed int8 sketch");
   if (a.xh != nullptr && (a.eps == nullptr || a.ldh % 8 != 0 || a.ldh < a.d))
     throw HipError("propagation: malformed sketch");
-  if (a.pca != nullptr && (a.pca_e == nullptr || (a.pca_r != 16 && a.pca_r != 32) || !(a.pca_sigma > 0.0)))
+  if ((a.pca != nullptr || a.pca16 != nullptr) && (a.pca_e == nullptr || (a.pca_r != 16 && a.pca_r != 32) || !(a.pca_sigma > 0.0)))
     throw HipError("propagation: malformed principal-subspace screen");
   switch (a.metric) {
     case kEuclidean:

@@ -141,6 +141,11 @@ int aknn_deinitialize(aknn_handle* h);
  *                   [16, 256]), a proven lower bound of the exact key; the
  *                   exact chain runs where the bound does not exclude a
  *                   candidate (the results are the same); 0 (default): off
+ *   "pca_fp16"      1 (default): the screen's projections are stored in FP16
+ *                   (32 or 64 bytes read per candidate) and each point's
+ *                   radius includes its projection's FP16 rounding, so the
+ *                   bound stays proven; 0: FP32 projections (the results are
+ *                   the same)
  *   "io_threads"    host threads parsing the CSV in aknn_initialize (0, the
  *                   default: up to 16; 1: the sequential loader); the values
  *                   are the same for any count
