@@ -43,10 +43,9 @@ struct Bucket {
   int32_t rstart;
   int32_t rcount;
   int32_t rbefore;   // RIGHT elements before astart (filled per level)
-  int32_t child[2];  // next-level bucket index of the LEFT/RIGHT child, -1 if a leaf
-  int32_t cast[2];   // next-level astart of the LEFT/RIGHT child
-  int32_t cfs[2];    // final-array start of the LEFT/RIGHT child
-  int32_t pad[2];
+  int32_t nright;    // RIGHT elements of the bucket (filled per level)
+  int32_t tree;      // tree of the bucket within a batch of trees built together (0 for one tree)
+  int32_t pad[6];
 };
 static_assert(sizeof(Bucket) == 64, "Bucket is one 64-byte record");
 
@@ -57,7 +56,7 @@ struct ForestCtl {
   int32_t done;        // set by the first level that has no bucket to split
   int32_t iterations;  // reference loop iterations (levels with a split + 1)
   int32_t nleaves;     // leaves written so far
-  int32_t pad;
+  int32_t total_right; // RIGHT elements of the current level
   int64_t active_sum;  // sum over levels of active elements (for byte accounting)
 };
 

@@ -179,6 +179,10 @@ void Engine::release() {
   ktab_ = nullptr;
   ktab_cap_ = 0;
   dirs_ = proj_ = nullptr;
+  fbm_ = ForestBuffers{};
+  fbm_trees_ = 0;
+  proj_nd_ = 0;
+  tree_ctl_ = nullptr;
   pca_p_ = pca_mom_ = pca_proj_ = pca_e_ = nullptr;
   pca_ = PcaBasis{};
   nd_ = 0;
@@ -268,6 +272,8 @@ void Engine::set_data(const float* x_rowmajor, int64_t n, int d, const Params& p
   fb_.wpre = alloc<int32_t>(blocks * 16);
   fb_.blk_r = alloc<int32_t>(blocks);
   fb_.blk_roff = alloc<int32_t>(blocks);
+  fb_.bchild = alloc<int32_t>(fb_.max_buckets);
+  fb_.bact = alloc<int32_t>(fb_.max_buckets);
   fb_.ctl = alloc<ForestCtl>(1);
   ncand_ = alloc<unsigned long long>(static_cast<size_t>(kCountWords) * std::max(p.props, 1));
   changed_[0] = alloc<uint8_t>(n);
@@ -343,10 +349,7 @@ void Engine::build_forest_median() {
   const int nd = median_depth(n_, leaf_eff_);
   const int64_t key = (static_cast<int64_t>(p_.seed) << 20) ^ (static_cast<int64_t>(p_.trees) << 8) ^ nd;
   if (!dirs_ || nd != nd_ || key != dirs_key_) {
-    if (!dirs_ || nd > nd_) {
-      dirs_ = alloc<float>(static_cast<size_t>(p_.trees) * nd * ld_);
-      proj_ = alloc<float>(static_cast<size_t>(nd) * n_);
-    }
+    if (!dirs_ || nd > nd_) dirs_ = alloc<float>(static_cast<size_t>(p_.trees) * nd * ld_);
     nd_ = nd;
     std::vector<float> all(static_cast<size_t>(p_.trees) * nd * ld_), one;
     for (int t = 0; t < p_.trees; ++t) {
@@ -375,48 +378,64 @@ void Engine::build_forest_median() {
     nbins_sets = nr;
   }
   const int routed_trees = bins_set_ ? std::min(p_.router_trees, p_.trees) : 0;
-  for (int t = 0; t < p_.trees; ++t) {
-    int32_t* perm = perm_ + static_cast<int64_t>(t) * n_;
-    Leaf* leaves = leaves_ + static_cast<int64_t>(t) * n_;
-    const bool routed = t < routed_trees;
-    const int bb = routed ? bin_bits_ : 0;
+  // Trees are independent here (no shared draw stream), so a batch of them is
+  // built level by level together: one active list, one bucket list, three
+  // passes per level for all of them, instead of per tree.
+  const int bt = median_batch_trees(nd);
+  MedianRules rules;
+  rules.routed_trees = routed_trees;
+  rules.bin_bits = bin_bits_;
+  rules.nsets = std::max(nbins_sets, 1);
+  rules.bins = bins_;
+  rules.proj = proj_;
+  rules.nd = nd;
+  rules.n = n_;
+  std::vector<int32_t> tl_host(2 * static_cast<size_t>(bt));
+  for (int t0 = 0; t0 < p_.trees; t0 += bt) {
+    const int nt = std::min(bt, p_.trees - t0);
+    rules.tree0 = t0;
+    int32_t* perm = perm_ + static_cast<int64_t>(t0) * n_;
+    Leaf* leaves = leaves_ + static_cast<int64_t>(t0) * n_;
     // projections are made 16 directions at a time, the first time a level
     // needs one: routed trees use only the directions below their depth past
-    // the bin levels.  The first batch runs over all rows in order; a later
-    // one only over the points still active then (the few of deep buckets).
-    const float* tdirs = dirs_ + static_cast<size_t>(t) * nd * ld_;
-    int projected = 0;
-    launch_forest_init(fb_, perm, leaves, n_, 0, 0, leaf_eff_, p_.k + 1, stream_);
+    // the bin levels.  A tree's first batch runs over all rows in order; a
+    // later one only over the points still active then (the few of deep
+    // buckets, of every tree of the batch).
+    std::vector<int> projected(nt, 0);
+    launch_forest_init_batch(fbm_, nt, perm, leaves, tree_ctl_, tree_ctl_ + bt, n_, leaf_eff_, stream_);
     int level = 0;
     for (;;) {
       for (int b = 0; b < p_.level_batch; ++b, ++level) {
-        if (level < bb) {
-          const int32_t* tb = bins_ + static_cast<int64_t>(t % std::max(nbins_sets, 1)) * n_;
-          launch_forest_level_rule(fb_, 0, nullptr, tb, bb - 1 - level, perm, leaves, level, leaf_eff_, p_.k + 1,
-                                   stream_);
-        } else if (level - bb < nd) {
-          if (level - bb >= projected) {
-            const int cnt = std::min(kProjectStep, nd - projected);
-            launch_project(x_, n_, d_, ld_, tdirs + static_cast<size_t>(projected) * ld_, cnt,
-                           proj_ + static_cast<int64_t>(projected) * n_, projected > 0 ? &fb_ : nullptr, level,
-                           stream_);
-            projected += cnt;
-            proj_dirs_ += cnt;
-            ++proj_launches_;
-          }
-          launch_forest_level_rule(fb_, 1, proj_ + static_cast<int64_t>(level - bb) * n_, nullptr, 0, perm, leaves,
-                                   level, leaf_eff_, p_.k + 1, stream_);
-        } else {
-          launch_forest_level_rule(fb_, 2, nullptr, nullptr, 0, perm, leaves, level, leaf_eff_, p_.k + 1, stream_);
+        bool any_median = false;
+        for (int tl = 0; tl < nt; ++tl) {
+          const int bb = t0 + tl < routed_trees ? bin_bits_ : 0;
+          const int dir = level - bb;
+          if (dir < 0 || dir >= nd) continue;
+          any_median = true;
+          if (dir < projected[tl]) continue;
+          const int cnt = std::min(kProjectStep, nd - projected[tl]);
+          launch_project(x_, n_, d_, ld_, dirs_ + (static_cast<size_t>(t0 + tl) * nd + projected[tl]) * ld_, cnt,
+                         proj_ + (static_cast<int64_t>(tl) * nd + projected[tl]) * n_,
+                         projected[tl] > 0 ? &fbm_ : nullptr, level, stream_);
+          projected[tl] += cnt;
+          proj_dirs_ += cnt;
+          ++proj_launches_;
         }
+        launch_forest_level_batch(fbm_, rules, any_median, level, perm, leaves, tree_ctl_, tree_ctl_ + bt, leaf_eff_,
+                                  p_.k + 1, stream_);
       }
-      AKNN_HIP_TRY(hipMemcpyAsync(ctl_host_, fb_.ctl, sizeof(ForestCtl), hipMemcpyDeviceToHost, stream_));
+      AKNN_HIP_TRY(hipMemcpyAsync(ctl_host_, fbm_.ctl, sizeof(ForestCtl), hipMemcpyDeviceToHost, stream_));
       AKNN_HIP_TRY(hipStreamSynchronize(stream_));
       if (ctl_host_->done) break;
-      if (level > n_ + bb + 2) throw HipError("forest level loop did not terminate");
+      if (level > n_ + nd + 32) throw HipError("forest level loop did not terminate");
     }
-    iters_[t] = ctl_host_->iterations;
-    nleaves_[t] = ctl_host_->nleaves;
+    AKNN_HIP_TRY(hipMemcpyAsync(tl_host.data(), tree_ctl_, sizeof(int32_t) * 2 * bt, hipMemcpyDeviceToHost, stream_));
+    AKNN_HIP_TRY(hipStreamSynchronize(stream_));
+    for (int tl = 0; tl < nt; ++tl) {
+      nleaves_[t0 + tl] = tl_host[tl];
+      // levels that split a bucket, plus the last one that found none
+      iters_[t0 + tl] = tl_host[bt + tl] + 2;
+    }
     active_sum_ += ctl_host_->active_sum;
   }
   // algorithmic bytes: one pass over the rows and the projection writes per
@@ -425,6 +444,42 @@ void Engine::build_forest_median() {
   stats_forest.units = static_cast<double>(proj_launches_) * n_ * 4.0 * d_ +
                        static_cast<double>(proj_dirs_) * n_ * 4.0 + static_cast<double>(active_sum_) * 20.0 +
                        8.0 * n_ * p_.trees;
+}
+
+// Trees of a median-split batch: as many as the batch buffers (16 bytes of
+// active lists and 4 * nd of projections per point and tree, plus buckets)
+// can hold in a quarter of the free device memory, at most 64 and at most
+// 2^31 - 1 active points.  Allocated on the first build of a data set.
+int Engine::median_batch_trees(int nd) {
+  if (fbm_.ctl != nullptr && nd <= proj_nd_) return fbm_trees_;
+  const int64_t per_tree = n_ * (16 + 4 * static_cast<int64_t>(nd) + 1) + 128 * (n_ / (leaf_eff_ + 1) + 2);
+  size_t free_b = 0, total_b = 0;
+  AKNN_HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+  int64_t bt = std::min<int64_t>({p_.trees, 64, 0x7fffffffLL / n_,
+                                  static_cast<int64_t>(free_b / 4) / std::max<int64_t>(per_tree, 1)});
+  bt = std::max<int64_t>(bt, 1);
+  const int64_t cap = bt * n_;
+  const int64_t blocks = (cap + 1023) / 1024;
+  fbm_ = ForestBuffers{};
+  fbm_.n = cap;
+  fbm_.max_buckets = bt * (n_ / (leaf_eff_ + 1) + 2);
+  for (int b = 0; b < 2; ++b) {
+    fbm_.act_ids[b] = alloc<int32_t>(cap);
+    fbm_.act_bkt[b] = alloc<int32_t>(cap);
+    fbm_.bkt[b] = alloc<Bucket>(fbm_.max_buckets);
+  }
+  fbm_.rmask = alloc<uint64_t>(blocks * 16);
+  fbm_.wpre = alloc<int32_t>(blocks * 16);
+  fbm_.blk_r = alloc<int32_t>(blocks);
+  fbm_.blk_roff = alloc<int32_t>(blocks);
+  fbm_.bchild = alloc<int32_t>(fbm_.max_buckets);
+  fbm_.bact = alloc<int32_t>(fbm_.max_buckets);
+  fbm_.ctl = alloc<ForestCtl>(1);
+  tree_ctl_ = alloc<int32_t>(2 * bt);
+  proj_ = alloc<float>(static_cast<size_t>(bt) * nd * n_);
+  proj_nd_ = nd;
+  fbm_trees_ = static_cast<int>(bt);
+  return fbm_trees_;
 }
 
 void Engine::initial_knn() {

@@ -225,9 +225,16 @@ class Engine {
   int prop_mode_ = -1;  // propagation rounds learned on this data: -1 unknown, 0 screened, 1 exact
   DrawStream draws_;
   // median-split forest: directions [trees][nd][ld] (cached per seed),
-  // projections of one tree [nd][n], router bins [n]
+  // projections of a batch of trees [batch][nd][n], router bins [n]; the
+  // batch's level-loop buffers (fbm_, for fbm_trees_ trees at once) and
+  // per-tree leaf counts and last split levels (tree_ctl_ [2][fbm_trees_])
   float* dirs_ = nullptr;
   float* proj_ = nullptr;
+  ForestBuffers fbm_{};
+  int fbm_trees_ = 0;
+  int proj_nd_ = 0;
+  int32_t* tree_ctl_ = nullptr;
+  int median_batch_trees(int nd);
   int nd_ = 0;
   int64_t dirs_key_ = -1;
   int64_t proj_dirs_ = 0;      // directions projected in the last median forest

@@ -1,4 +1,4 @@
-// Random-projection forest, one level of one tree per three launches.
+// Random-projection forest, level by level.
 //
 // Semantics follow buildUpTheTrees / makePartitionOfLeaf0..9
 // (euclideandistancematrixgpu.cu:2130-2686): a bucket with more than `leaf`
@@ -14,12 +14,17 @@
 //   side:    one lane per active point (a quad of lanes with the sketch
 //            screen): projection chain -> one ballot bit, per-1024-point
 //            block counts and per-wave prefixes;
-//   level:   one workgroup: scan of the block counts, the RIGHT count of
-//            every bucket from popcounts, child records, pivots, compaction;
+//   scans:   the block counts' offsets (one workgroup), every bucket's RIGHT
+//            count from popcounts (one lane per bucket), the offsets of the
+//            buckets' children and of their points (one workgroup);
 //   scatter: one lane per active point: stable rank from popcounts, write to
-//            the next level's compact arrays or to the final leaf array.
+//            the next level's compact arrays or to the final leaf array; the
+//            lane of a bucket's first point writes its child records
+//            (pivots) and leaves.
 // The partition is stable (the reference's atomic order is one legal order;
-// this fixes the one where each side keeps the bucket's order).
+// this fixes the one where each side keeps the bucket's order).  Median-split
+// trees (no shared draw stream) are built several at a time through the same
+// passes: one active list and one bucket list for the batch.
 #include "device_util.hpp"
 #include "kernels.hpp"
 
@@ -93,7 +98,6 @@ __global__ __launch_bounds__(256) void forest_init_kernel(ForestBuffers fb, int3
       b.q2 = q2;
       b.rstart = 0;
       b.rcount = static_cast<int32_t>(n);
-      b.child[0] = b.child[1] = -1;
       fb.bkt[0][0] = b;
     } else {
       leaves[0] = Leaf{0, static_cast<int32_t>(n), 0, static_cast<int32_t>(n)};
@@ -332,17 +336,211 @@ __device__ __forceinline__ int32_t rights_before(int64_t pos, int nact, int32_t 
   return blk_roff[pos >> 10] + wpre[pos >> 6] + __popcll(w & below);
 }
 
-__global__ __launch_bounds__(1024) void forest_level_kernel(Bucket* __restrict__ bcur, Bucket* __restrict__ bnext,
+// Where finished buckets go.  One tree: `leaves`, counted by ctl->nleaves.
+// A batch of trees (median split): tree t's leaves at leaves + t * stride,
+// counted by tree_nleaves[t], and tree_last[t] = the last level at which
+// tree t still has a bucket to split.
+struct LeafSink {
+  Leaf* leaves;
+  int64_t stride;
+  int32_t* tree_nleaves;
+  int32_t* tree_last;
+};
+
+// Exclusive scan of the side pass's per-block RIGHT counts (one workgroup:
+// each thread sums a contiguous run of blocks, loads eight at a time, then
+// one block-wide scan of the run sums); the level's RIGHT total goes to the
+// control block.
+__global__ __launch_bounds__(1024) void forest_block_scan_kernel(const int32_t* __restrict__ blk_r,
+                                                                 int32_t* __restrict__ blk_roff, ForestCtl* ctl,
+                                                                 int parity) {
+  __shared__ int sh[16];
+  const int nact = ctl->nact[parity];
+  const int nblk = (nact + kSideBlock - 1) / kSideBlock;
+  const int per = (nblk + 1023) / 1024;
+  const int b0 = threadIdx.x * per < nblk ? threadIdx.x * per : nblk;
+  const int b1 = b0 + per < nblk ? b0 + per : nblk;
+  int v = 0;
+  for (int i = b0; i < b1; i += 8) {
+    int t[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t[u] = i + u < b1 ? blk_r[i + u] : 0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v += t[u];
+  }
+  int tot;
+  int run = block_scan(v, sh, &tot) - v;
+  for (int i = b0; i < b1; i += 8) {
+    int t[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t[u] = i + u < b1 ? blk_r[i + u] : 0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (i + u < b1) blk_roff[i + u] = run;
+      run += t[u];
+    }
+  }
+  if (threadIdx.x == 0) ctl->total_right = tot;
+}
+
+// Per bucket of the level (one lane each): its RIGHT count from the ballot
+// words and the block offsets, stored in the record with the RIGHT elements
+// before it; the children that still split and their active elements; the
+// sides that do not split written as leaves.  Leaf slots are taken with one
+// atomic per wave and tree (a bucket list is in tree order, so a wave's
+// buckets nearly always belong to one tree).
+__global__ __launch_bounds__(256) void forest_bucket_size_kernel(Bucket* __restrict__ bcur,
+                                                                 const int32_t* __restrict__ blk_roff,
+                                                                 const int32_t* __restrict__ wpre,
+                                                                 const uint64_t* __restrict__ rmask, ForestCtl* ctl,
+                                                                 int parity, int level, int leaf_eff, int kp1,
+                                                                 int32_t* __restrict__ bchild,
+                                                                 int32_t* __restrict__ bact, LeafSink sink) {
+  const int nb = ctl->nbkt[parity];
+  const int nact = ctl->nact[parity];
+  const int32_t total = ctl->total_right;
+  const int lane = threadIdx.x & 63;
+  for (int base = blockIdx.x * 256; base < nb; base += gridDim.x * 256) {
+    const int bi = base + threadIdx.x;
+    const bool valid = bi < nb;
+    Bucket b{};
+    int32_t sz[2] = {0, 0};
+    if (valid) {
+      b = bcur[bi];
+      const int32_t rbef = rights_before(b.astart, nact, total, blk_roff, wpre, rmask);
+      const int32_t rend = rights_before(static_cast<int64_t>(b.astart) + b.size, nact, total, blk_roff, wpre, rmask);
+      sz[1] = rend - rbef;
+      sz[0] = b.size - sz[1];
+      bcur[bi].rbefore = rbef;
+      bcur[bi].nright = sz[1];
+      bchild[bi] = static_cast<int>(sz[0] > leaf_eff) + static_cast<int>(sz[1] > leaf_eff);
+      bact[bi] = (sz[0] > leaf_eff ? sz[0] : 0) + (sz[1] > leaf_eff ? sz[1] : 0);
+    }
+    // an empty side (possible only under the bin rule of routed trees) is
+    // not a leaf
+    const bool leaf0 = sz[0] > 0 && sz[0] <= leaf_eff, leaf1 = sz[1] > 0 && sz[1] <= leaf_eff;
+    const int nl = static_cast<int>(leaf0) + static_cast<int>(leaf1);
+    const bool splits = valid && (sz[0] > leaf_eff || sz[1] > leaf_eff);
+    if (sink.tree_last != nullptr) {
+      // this level's children make level + 1 a level of the tree: one store
+      // per run of lanes of the same tree
+      const int tr = splits ? b.tree : -1;
+      const int nxt = __shfl_down(tr, 1);
+      if (splits && (lane == 63 || nxt != tr)) sink.tree_last[b.tree] = level + 1;
+    }
+    uint64_t pending = __ballot(valid && nl > 0);
+    while (pending) {
+      const int leader = __ffsll(static_cast<long long>(pending)) - 1;
+      const int lt = __shfl(b.tree, leader);
+      const bool mine = valid && nl > 0 && b.tree == lt;
+      const uint64_t grp = __ballot(mine);
+      int pre = mine ? nl : 0;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(pre, o);
+        if (lane >= o) pre += t;
+      }
+      const int tot = __shfl(pre, 63);
+      pre -= mine ? nl : 0;
+      int slot0 = 0;
+      if (lane == leader) {
+        int32_t* cnt = sink.tree_nleaves != nullptr ? sink.tree_nleaves + lt : &ctl->nleaves;
+        slot0 = atomicAdd(cnt, tot);
+      }
+      slot0 = __shfl(slot0, leader);
+      if (mine) {
+        int slot = slot0 + pre;
+        for (int c = 0; c < 2; ++c) {
+          if (!(c ? leaf1 : leaf0)) continue;
+          const int32_t size = sz[c];
+          const int32_t fst = b.fstart + (c ? sz[0] : 0);
+          const bool own_range = size >= kp1;
+          sink.leaves[b.tree * sink.stride + slot] =
+              Leaf{fst, size, own_range ? fst : b.rstart, own_range ? size : b.rcount};
+          ++slot;
+        }
+      }
+      pending &= ~grp;
+    }
+  }
+}
+
+// Exclusive scans, in bucket order, of the children and of their active
+// elements (one workgroup, contiguous runs per thread): each bucket's first
+// child index and first active offset, in place.  The next level's bucket
+// and element counts go to the control block.  A level with no bucket is the
+// reference's last loop iteration: nothing splits, the pair is drawn anyway.
+__global__ __launch_bounds__(1024) void forest_bucket_scan_kernel(int32_t* __restrict__ bchild,
+                                                                  int32_t* __restrict__ bact, ForestCtl* ctl,
+                                                                  int parity, int level) {
+  __shared__ int sh[16];
+  const int nb = ctl->nbkt[parity];
+  if (nb == 0) {
+    if (threadIdx.x == 0) {
+      if (ctl->done == 0) {
+        ctl->done = 1;
+        ctl->iterations = level + 1;
+      }
+      ctl->nbkt[parity ^ 1] = 0;
+      ctl->nact[parity ^ 1] = 0;
+    }
+    return;
+  }
+  const int per = (nb + 1023) / 1024;
+  const int c0 = threadIdx.x * per < nb ? threadIdx.x * per : nb;
+  const int c1 = c0 + per < nb ? c0 + per : nb;
+  int vc = 0, va = 0;
+  for (int i = c0; i < c1; i += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (i + u < c1) {
+        vc += bchild[i + u];
+        va += bact[i + u];
+      }
+    }
+  }
+  int tot_c, tot_a;
+  int rc = block_scan(vc, sh, &tot_c) - vc;
+  int ra = block_scan(va, sh, &tot_a) - va;
+  for (int i = c0; i < c1; i += 8) {
+    int tc[8], ta[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      tc[u] = i + u < c1 ? bchild[i + u] : 0;
+      ta[u] = i + u < c1 ? bact[i + u] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (i + u < c1) {
+        bchild[i + u] = rc;
+        bact[i + u] = ra;
+      }
+      rc += tc[u];
+      ra += ta[u];
+    }
+  }
+  if (threadIdx.x == 0) {
+    ctl->nbkt[parity ^ 1] = tot_c;
+    ctl->nact[parity ^ 1] = tot_a;
+    ctl->active_sum += ctl->nact[parity];
+  }
+}
+
+// The three passes above in one workgroup, for one tree (its bucket lists
+// are short enough that a launch costs more than the work): block offsets,
+// then buckets 1024 at a time (sizes, children, active elements and leaves,
+// each counted by a block scan, leaf slots in bucket order).
+__global__ __launch_bounds__(1024) void forest_level_kernel(Bucket* __restrict__ bcur,
                                                             const int32_t* __restrict__ blk_r,
                                                             int32_t* __restrict__ blk_roff,
                                                             const uint64_t* __restrict__ rmask,
                                                             const int32_t* __restrict__ wpre, ForestCtl* ctl,
-                                                            int parity, int level, int32_t v1, int32_t v2,
-                                                            int leaf_eff, int kp1, Leaf* __restrict__ leaves) {
+                                                            int parity, int level, int leaf_eff, int kp1,
+                                                            int32_t* __restrict__ bchild, int32_t* __restrict__ bact,
+                                                            Leaf* __restrict__ leaves) {
   __shared__ int sh[16];
   const int nb = ctl->nbkt[parity];
   if (nb == 0) {
-    // The reference's last loop iteration: nothing splits, the pair is drawn anyway.
     if (threadIdx.x == 0) {
       if (ctl->done == 0) {
         ctl->done = 1;
@@ -355,113 +553,129 @@ __global__ __launch_bounds__(1024) void forest_level_kernel(Bucket* __restrict__
   }
   const int nact = ctl->nact[parity];
   const int nblk = (nact + kSideBlock - 1) / kSideBlock;
-  int run = 0;
-  for (int base = 0; base < nblk; base += 1024) {
-    const int i = base + threadIdx.x;
-    const int v = i < nblk ? blk_r[i] : 0;
+  int32_t total_r;
+  {
+    const int per = (nblk + 1023) / 1024;
+    const int b0 = threadIdx.x * per < nblk ? threadIdx.x * per : nblk;
+    const int b1 = b0 + per < nblk ? b0 + per : nblk;
+    int v = 0;
+    for (int i = b0; i < b1; ++i) v += blk_r[i];
     int tot;
-    const int incl = block_scan(v, sh, &tot);
-    if (i < nblk) blk_roff[i] = run + incl - v;
-    run += tot;
+    int run = block_scan(v, sh, &tot) - v;
+    for (int i = b0; i < b1; ++i) {
+      const int c = blk_r[i];
+      blk_roff[i] = run;
+      run += c;
+    }
+    total_r = tot;
   }
-  const int32_t total_r = run;
   __syncthreads();
-
-  int run_b = 0, run_a = 0;
+  int run_c = 0, run_a = 0, run_l = ctl->nleaves;
   for (int base = 0; base < nb; base += 1024) {
     const int bi = base + threadIdx.x;
     Bucket b{};
-    int32_t sz[2] = {0, 0}, rbef = 0;
-    bool act[2] = {false, false};
+    int32_t sz[2] = {0, 0};
     if (bi < nb) {
       b = bcur[bi];
-      rbef = rights_before(b.astart, nact, total_r, blk_roff, wpre, rmask);
+      const int32_t rbef = rights_before(b.astart, nact, total_r, blk_roff, wpre, rmask);
       const int32_t rend = rights_before(static_cast<int64_t>(b.astart) + b.size, nact, total_r, blk_roff, wpre, rmask);
       sz[1] = rend - rbef;
       sz[0] = b.size - sz[1];
-      act[0] = sz[0] > leaf_eff;
-      act[1] = sz[1] > leaf_eff;
+      bcur[bi].rbefore = rbef;
+      bcur[bi].nright = sz[1];
     }
-    const int nchild = static_cast<int>(act[0]) + static_cast<int>(act[1]);
-    const int asz = (act[0] ? sz[0] : 0) + (act[1] ? sz[1] : 0);
-    int tot_c, tot_a;
-    const int ic = block_scan(nchild, sh, &tot_c);
-    const int ia = block_scan(asz, sh, &tot_a);
+    const bool leaf0 = sz[0] > 0 && sz[0] <= leaf_eff, leaf1 = sz[1] > 0 && sz[1] <= leaf_eff;
+    const int nch = static_cast<int>(sz[0] > leaf_eff) + static_cast<int>(sz[1] > leaf_eff);
+    const int nl = static_cast<int>(leaf0) + static_cast<int>(leaf1);
+    const int na = (sz[0] > leaf_eff ? sz[0] : 0) + (sz[1] > leaf_eff ? sz[1] : 0);
+    // children and leaves (at most 2048 each per pass) share one scan
+    int tot_cl, tot_a;
+    const int icl = block_scan(nch | (nl << 16), sh, &tot_cl) - (nch | (nl << 16));
+    const int ia = block_scan(na, sh, &tot_a) - na;
     if (bi < nb) {
-      int cidx = run_b + ic - nchild;
-      int cast = run_a + ia - asz;
-      b.rbefore = rbef;
+      bchild[bi] = run_c + (icl & 0xffff);
+      bact[bi] = run_a + ia;
+      int slot = run_l + (icl >> 16);
       for (int c = 0; c < 2; ++c) {
-        const int32_t size = sz[c];
+        if (!(c ? leaf1 : leaf0)) continue;
         const int32_t fst = b.fstart + (c ? sz[0] : 0);
-        const bool own_range = size >= kp1;
-        const int32_t rstart = own_range ? fst : b.rstart;
-        const int32_t rcount = own_range ? size : b.rcount;
-        b.cfs[c] = fst;
-        if (act[c]) {
-          // Leaf7 pivots (.cu:2317-2347); size >= 2 here
-          int32_t q1 = v1 % size, q2 = v2 % size;
-          if (q1 == q2) q2 = (q1 + 1) % size;
-          Bucket nbk{};
-          nbk.astart = cast;
-          nbk.size = size;
-          nbk.fstart = fst;
-          nbk.q1 = q1;
-          nbk.q2 = q2;
-          nbk.rstart = rstart;
-          nbk.rcount = rcount;
-          nbk.child[0] = nbk.child[1] = -1;
-          bnext[cidx] = nbk;
-          b.child[c] = cidx;
-          b.cast[c] = cast;
-          ++cidx;
-          cast += size;
-        } else {
-          // an empty side (possible only under the bin rule of routed trees)
-          // is not a leaf
-          if (size > 0) {
-            const int slot = atomicAdd(&ctl->nleaves, 1);
-            leaves[slot] = Leaf{fst, size, rstart, rcount};
-          }
-          b.child[c] = -1;
-          b.cast[c] = 0;
-        }
+        const bool own_range = sz[c] >= kp1;
+        leaves[slot++] = Leaf{fst, sz[c], own_range ? fst : b.rstart, own_range ? sz[c] : b.rcount};
       }
-      bcur[bi] = b;
     }
-    run_b += tot_c;
+    run_c += tot_cl & 0xffff;
+    run_l += tot_cl >> 16;
     run_a += tot_a;
   }
   if (threadIdx.x == 0) {
-    ctl->nbkt[parity ^ 1] = run_b;
+    ctl->nbkt[parity ^ 1] = run_c;
     ctl->nact[parity ^ 1] = run_a;
+    ctl->nleaves = run_l;
     ctl->active_sum += nact;
   }
 }
 
+// One lane per active element: its stable rank on its side from the ballot
+// words, then a move to the child's place in the next level's list, or to
+// the tree's leaf array when that side does not split.  The lane of each
+// bucket's first element writes the bucket's children (Leaf7 pivots for the
+// sign split, .cu:2317-2347).
 __global__ __launch_bounds__(256) void forest_scatter_kernel(const int32_t* __restrict__ act_ids,
                                                              const int32_t* __restrict__ act_bkt,
                                                              const Bucket* __restrict__ bkt,
+                                                             const int32_t* __restrict__ bchild,
+                                                             const int32_t* __restrict__ bact,
                                                              const uint64_t* __restrict__ rmask,
                                                              const int32_t* __restrict__ wpre,
-                                                             const int32_t* __restrict__ blk_roff,
-                                                             const ForestCtl* ctl, int parity,
-                                                             int32_t* __restrict__ act_ids_n,
-                                                             int32_t* __restrict__ act_bkt_n, int32_t* __restrict__ perm) {
+                                                             const int32_t* __restrict__ blk_roff, ForestCtl* ctl,
+                                                             int parity, int level, int32_t v1, int32_t v2,
+                                                             int leaf_eff, int kp1, int32_t* __restrict__ act_ids_n,
+                                                             int32_t* __restrict__ act_bkt_n,
+                                                             Bucket* __restrict__ bnext, int32_t* __restrict__ perm,
+                                                             int64_t perm_stride) {
   const int nact = ctl->nact[parity];
   for (int64_t m = blockIdx.x * 256ll + threadIdx.x; m < nact; m += gridDim.x * 256ll) {
     const int32_t id = act_ids[m];
-    const Bucket& b = bkt[act_bkt[m]];
+    const int32_t bi = act_bkt[m];
+    const Bucket& b = bkt[bi];
+    const int32_t sz[2] = {b.size - b.nright, b.nright};
+    const bool act[2] = {sz[0] > leaf_eff, sz[1] > leaf_eff};
     const uint64_t w = rmask[m >> 6];
     const int bit = static_cast<int>((w >> (m & 63)) & 1ull);
     const int32_t rb = blk_roff[m >> 10] + wpre[m >> 6] + __popcll(w & ((1ull << (m & 63)) - 1ull));
     const int32_t rank = bit ? rb - b.rbefore : (static_cast<int32_t>(m) - b.astart) - (rb - b.rbefore);
-    const int32_t child = b.child[bit];
-    if (child >= 0) {
-      act_ids_n[b.cast[bit] + rank] = id;
-      act_bkt_n[b.cast[bit] + rank] = child;
+    const int32_t cidx0 = bchild[bi], cast0 = bact[bi];
+    if (act[bit]) {
+      const int32_t off = bit && act[0];
+      const int32_t cast = cast0 + (off ? sz[0] : 0);
+      act_ids_n[cast + rank] = id;
+      act_bkt_n[cast + rank] = cidx0 + off;
     } else {
-      perm[b.cfs[bit] + rank] = id;
+      perm[b.tree * perm_stride + b.fstart + (bit ? sz[0] : 0) + rank] = id;
+    }
+    if (m == b.astart) {
+      int32_t cidx = cidx0, cast = cast0;
+      for (int c = 0; c < 2; ++c) {
+        if (!act[c]) continue;
+        const int32_t size = sz[c];
+        const int32_t fst = b.fstart + (c ? sz[0] : 0);
+        const bool own_range = size >= kp1;
+        // size >= 2 here
+        int32_t q1 = v1 % size, q2 = v2 % size;
+        if (q1 == q2) q2 = (q1 + 1) % size;
+        Bucket nbk{};
+        nbk.astart = cast;
+        nbk.size = size;
+        nbk.fstart = fst;
+        nbk.q1 = q1;
+        nbk.q2 = q2;
+        nbk.rstart = own_range ? fst : b.rstart;
+        nbk.rcount = own_range ? size : b.rcount;
+        nbk.tree = b.tree;
+        bnext[cidx] = nbk;
+        ++cidx;
+        cast += size;
+      }
     }
   }
 }
@@ -493,15 +707,14 @@ constexpr int kProjXs = kProjChunk + 4;  // LDS row stride (floats): 16-byte rea
 // ctl->nact[parity]): only they, or their subsets at deeper levels, read
 // these directions).  One lane per point, 256 points per
 // block: each step stages the block's rows for 32 dimensions in LDS with
-// coalesced 16-byte loads (8 lanes per 128-byte row piece), and the
-// directions' 32 dimensions beside them (read as broadcasts); each lane then
-// advances its kProjGroup chains from its own staged row.
+// coalesced 16-byte loads (8 lanes per 128-byte row piece); each lane then
+// advances its kProjGroup chains from its own staged row, the directions'
+// values (the same for every lane) coming through scalar loads.
 __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ x, int64_t n, int d, int ld,
                                                       const float* __restrict__ dirs, int nd,
                                                       float* __restrict__ proj, const int32_t* __restrict__ act_ids,
                                                       const ForestCtl* ctl, int parity) {
   __shared__ __attribute__((aligned(16))) float xs[256 * kProjXs];
-  __shared__ __attribute__((aligned(16))) float rs[kProjGroup][kProjChunk];
   const int t = threadIdx.x;
   const int64_t i0 = blockIdx.x * 256ll;
   const int64_t rows = act_ids != nullptr ? ctl->nact[parity] : n;
@@ -525,17 +738,16 @@ __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ 
       const float4 v = c < d ? *reinterpret_cast<const float4*>(x + row * ld + c) : make_float4(0.f, 0.f, 0.f, 0.f);
       *reinterpret_cast<float4*>(xs + r * kProjXs + 4 * q) = v;
     }
-    for (int e = t; e < kProjGroup * kProjChunk; e += 256) {
-      const int g = e / kProjChunk, c = e % kProjChunk;
-      rs[g][c] = (g < nd && c < cw) ? dirs[static_cast<int64_t>(g) * ld + c0 + c] : 0.0f;
-    }
     __syncthreads();
+    // the directions' values are the same for every lane: scalar loads from
+    // the direction rows (rows past nd read row nd - 1; their chains are not
+    // stored)
     int c = 0;
     for (; c + 4 <= cw; c += 4) {
       const float4 v = *reinterpret_cast<const float4*>(xr + c);
 #pragma unroll
       for (int g = 0; g < kProjGroup; ++g) {
-        const float4 r = *reinterpret_cast<const float4*>(&rs[g][c]);
+        const float4 r = *reinterpret_cast<const float4*>(dirs + static_cast<int64_t>(g < nd ? g : nd - 1) * ld + c0 + c);
         acc[g] = __builtin_fmaf(v.x, r.x, acc[g]);
         acc[g] = __builtin_fmaf(v.y, r.y, acc[g]);
         acc[g] = __builtin_fmaf(v.z, r.z, acc[g]);
@@ -545,7 +757,8 @@ __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ 
     for (; c < cw; ++c) {
       const float v = xr[c];
 #pragma unroll
-      for (int g = 0; g < kProjGroup; ++g) acc[g] = __builtin_fmaf(v, rs[g][c], acc[g]);
+      for (int g = 0; g < kProjGroup; ++g)
+        acc[g] = __builtin_fmaf(v, dirs[static_cast<int64_t>(g < nd ? g : nd - 1) * ld + c0 + c], acc[g]);
     }
   }
   if (i0 + t < rows) {
@@ -558,17 +771,45 @@ __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ 
 
 constexpr int kMedianSamples = 511;
 
-// Pivot of every active bucket: m = min(size, 511) samples at relative
-// positions q * size / m, ordered by (ord(p), rel) with a bitonic sort in LDS;
-// the sample at index m / 2 is stored as (q1 = rel*, q2 = ord(p*)).
-__global__ __launch_bounds__(256) void forest_median_kernel(const float* __restrict__ pj,
+// Which rule splits the buckets of batch tree tl at a level (arg: the bin
+// bit, or the direction whose projections decide).
+__device__ __forceinline__ int rule_of(const MedianRules& r, int tl, int level, int& arg) {
+  const int bb = r.tree0 + tl < r.routed_trees ? r.bin_bits : 0;
+  if (level < bb) {
+    arg = bb - 1 - level;
+    return 0;
+  }
+  if (level - bb < r.nd) {
+    arg = level - bb;
+    return 1;
+  }
+  arg = 0;
+  return 2;
+}
+
+__device__ __forceinline__ const float* rule_proj(const MedianRules& r, int tl, int dir) {
+  return r.proj + (static_cast<int64_t>(tl) * r.nd + dir) * r.n;
+}
+
+__device__ __forceinline__ const int32_t* rule_bins(const MedianRules& r, int tl) {
+  return r.bins + static_cast<int64_t>((r.tree0 + tl) % r.nsets) * r.n;
+}
+
+// Pivot of every active bucket under the projection rule: m = min(size, 511)
+// samples at relative positions q * size / m, ordered by (ord(p), rel) with a
+// bitonic sort in LDS; the sample at index m / 2 is stored as (q1 = rel*,
+// q2 = ord(p*)).  Buckets of trees at a bin or position level are skipped.
+__global__ __launch_bounds__(256) void forest_median_kernel(MedianRules rules, int level,
                                                             const int32_t* __restrict__ act_ids,
                                                             Bucket* __restrict__ bkt, const ForestCtl* ctl,
                                                             int parity) {
   __shared__ uint64_t key[512];
   const int nb = ctl->nbkt[parity];
   for (int bi = blockIdx.x; bi < nb; bi += gridDim.x) {
-    const int32_t astart = bkt[bi].astart, size = bkt[bi].size;
+    const int32_t astart = bkt[bi].astart, size = bkt[bi].size, tl = bkt[bi].tree;
+    int dir;
+    if (rule_of(rules, tl, level, dir) != 1) continue;
+    const float* pj = rule_proj(rules, tl, dir);
     const int m = size < kMedianSamples ? size : kMedianSamples;
     for (int q = threadIdx.x; q < 512; q += 256) {
       uint64_t kv = ~0ull;
@@ -602,11 +843,10 @@ __global__ __launch_bounds__(256) void forest_median_kernel(const float* __restr
   }
 }
 
-// Side pass of a median-split level (RULE 0 bins, 1 projection, 2 position),
-// with the same ballot words and prefixes as forest_side_kernel.
-template <int RULE>
-__global__ __launch_bounds__(256) void forest_side_rule_kernel(const int32_t* __restrict__ bins, int shift,
-                                                               const float* __restrict__ pj,
+// Side pass of a median-split level over the active points of every tree of
+// the batch, each under its tree's rule (bins, projection or position), with
+// the same ballot words and prefixes as forest_side_kernel.
+__global__ __launch_bounds__(256) void forest_side_rule_kernel(MedianRules rules, int level,
                                                                const int32_t* __restrict__ act_ids,
                                                                const int32_t* __restrict__ act_bkt,
                                                                const Bucket* __restrict__ bkt, const ForestCtl* ctl,
@@ -622,17 +862,17 @@ __global__ __launch_bounds__(256) void forest_side_rule_kernel(const int32_t* __
       bool right = false;
       if (m < nact) {
         const int32_t id = act_ids[m];
-        if constexpr (RULE == 0) {
-          right = ((bins[id] >> shift) & 1) != 0;
+        const Bucket& b = bkt[act_bkt[m]];
+        int arg;
+        const int rule = rule_of(rules, b.tree, level, arg);
+        const int32_t rel = static_cast<int32_t>(m) - b.astart;
+        if (rule == 0) {
+          right = ((rule_bins(rules, b.tree)[id] >> arg) & 1) != 0;
+        } else if (rule == 1) {
+          const uint32_t u = ord_u32(rule_proj(rules, b.tree, arg)[id]), pu = static_cast<uint32_t>(b.q2);
+          right = u > pu || (u == pu && rel >= b.q1);
         } else {
-          const Bucket& b = bkt[act_bkt[m]];
-          const int32_t rel = static_cast<int32_t>(m) - b.astart;
-          if constexpr (RULE == 1) {
-            const uint32_t u = ord_u32(pj[id]), pu = static_cast<uint32_t>(b.q2);
-            right = u > pu || (u == pu && rel >= b.q1);
-          } else {
-            right = rel >= b.size / 2;
-          }
+          right = rel >= b.size / 2;
         }
       }
       const uint64_t bal = __ballot(right);
@@ -655,6 +895,48 @@ __global__ __launch_bounds__(256) void forest_side_rule_kernel(const int32_t* __
   }
 }
 
+// Roots of a batch of median-split trees: tree tl's points at [tl * n, (tl +
+// 1) * n) of the active list, bucket tl.  When the root does not split every
+// tree is one leaf of all points in order.
+__global__ __launch_bounds__(256) void forest_init_batch_kernel(ForestBuffers fb, int ntrees, int32_t* __restrict__ perm,
+                                                                LeafSink sink, int64_t n, int leaf_eff) {
+  const bool root_splits = n > leaf_eff;
+  const int64_t tot = n * ntrees;
+  for (int64_t m = blockIdx.x * 256ll + threadIdx.x; m < tot; m += gridDim.x * 256ll) {
+    const int32_t tl = static_cast<int32_t>(m / n);
+    const int32_t id = static_cast<int32_t>(m - tl * n);
+    fb.act_ids[0][m] = id;
+    fb.act_bkt[0][m] = tl;
+    if (!root_splits) perm[m] = id;
+  }
+  if (blockIdx.x == 0) {
+    for (int tl = threadIdx.x; tl < ntrees; tl += 256) {
+      if (root_splits) {
+        Bucket b{};
+        b.astart = static_cast<int32_t>(tl * n);
+        b.size = static_cast<int32_t>(n);
+        b.fstart = 0;
+        b.rstart = 0;
+        b.rcount = static_cast<int32_t>(n);
+        b.tree = tl;
+        fb.bkt[0][tl] = b;
+        sink.tree_nleaves[tl] = 0;
+        sink.tree_last[tl] = 0;
+      } else {
+        sink.leaves[tl * sink.stride] = Leaf{0, static_cast<int32_t>(n), 0, static_cast<int32_t>(n)};
+        sink.tree_nleaves[tl] = 1;
+        sink.tree_last[tl] = -1;
+      }
+    }
+    if (threadIdx.x == 0) {
+      ForestCtl c{};
+      c.nact[0] = root_splits ? static_cast<int32_t>(tot) : 0;
+      c.nbkt[0] = root_splits ? ntrees : 0;
+      *fb.ctl = c;
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void leaf_ranges_kernel(const int32_t* __restrict__ perm,
                                                           const Leaf* __restrict__ leaves, int nleaves,
                                                           int32_t* __restrict__ rs, int32_t* __restrict__ rc) {
@@ -669,6 +951,33 @@ __global__ __launch_bounds__(256) void leaf_ranges_kernel(const int32_t* __restr
 }
 
 int clamp_grid(int64_t g, int cap) { return static_cast<int>(g < 1 ? 1 : (g > cap ? cap : g)); }
+
+// The passes of a level after its side pass: block offsets, bucket sizes and
+// leaves, bucket offsets (one workgroup for a single tree, three passes for a
+// batch of trees), then the scatter that also writes the next level's
+// buckets.
+void level_tail(const ForestBuffers& fb, int par, int level, int32_t v1, int32_t v2, int leaf_eff, int kp1,
+                int32_t* perm, LeafSink sink, hipStream_t s) {
+  if (sink.tree_nleaves == nullptr) {
+    hipLaunchKernelGGL(forest_level_kernel, dim3(1), dim3(1024), 0, s, fb.bkt[par], fb.blk_r, fb.blk_roff, fb.rmask,
+                       fb.wpre, fb.ctl, par, level, leaf_eff, kp1, fb.bchild, fb.bact, sink.leaves);
+    AKNN_HIP_TRY(hipGetLastError());
+  } else {
+    hipLaunchKernelGGL(forest_block_scan_kernel, dim3(1), dim3(1024), 0, s, fb.blk_r, fb.blk_roff, fb.ctl, par);
+    AKNN_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(forest_bucket_size_kernel, dim3(clamp_grid((fb.max_buckets + 255) / 256, 4096)), dim3(256), 0,
+                       s, fb.bkt[par], fb.blk_roff, fb.wpre, fb.rmask, fb.ctl, par, level, leaf_eff, kp1, fb.bchild,
+                       fb.bact, sink);
+    AKNN_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(forest_bucket_scan_kernel, dim3(1), dim3(1024), 0, s, fb.bchild, fb.bact, fb.ctl, par, level);
+    AKNN_HIP_TRY(hipGetLastError());
+  }
+  hipLaunchKernelGGL(forest_scatter_kernel, dim3(clamp_grid((fb.n + 255) / 256, 8192)), dim3(256), 0, s,
+                     fb.act_ids[par], fb.act_bkt[par], fb.bkt[par], fb.bchild, fb.bact, fb.rmask, fb.wpre, fb.blk_roff,
+                     fb.ctl, par, level, v1, v2, leaf_eff, kp1, fb.act_ids[par ^ 1], fb.act_bkt[par ^ 1],
+                     fb.bkt[par ^ 1], perm, sink.stride);
+  AKNN_HIP_TRY(hipGetLastError());
+}
 
 }  // namespace
 
@@ -712,13 +1021,7 @@ void launch_forest_level(const ForestBuffers& fb, const float* x, int d, int ld,
                          fb.bkt[par], fb.ctl, par, fb.rmask, fb.wpre, fb.blk_r);
   }
   AKNN_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(forest_level_kernel, dim3(1), dim3(1024), 0, s, fb.bkt[par], fb.bkt[par ^ 1], fb.blk_r,
-                     fb.blk_roff, fb.rmask, fb.wpre, fb.ctl, par, level, v1, v2, leaf_eff, kp1, leaves);
-  AKNN_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(forest_scatter_kernel, dim3(clamp_grid((fb.n + 255) / 256, 8192)), dim3(256), 0, s,
-                     fb.act_ids[par], fb.act_bkt[par], fb.bkt[par], fb.rmask, fb.wpre, fb.blk_roff, fb.ctl, par,
-                     fb.act_ids[par ^ 1], fb.act_bkt[par ^ 1], perm);
-  AKNN_HIP_TRY(hipGetLastError());
+  level_tail(fb, par, level, v1, v2, leaf_eff, kp1, perm, LeafSink{leaves, 0, nullptr, nullptr}, s);
 }
 
 void launch_project(const float* x, int64_t n, int d, int ld, const float* dirs, int nd, float* proj,
@@ -727,39 +1030,40 @@ void launch_project(const float* x, int64_t n, int d, int ld, const float* dirs,
   if (ld % 4 != 0 || ld < d) throw HipError("project: malformed row width");
   if (nd > kProjGroup) throw HipError("project: at most 16 directions per launch");
   const int par = level & 1;
-  hipLaunchKernelGGL(project_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, x, n, d, ld, dirs,
+  // over an active list the grid covers its capacity (a batch of trees holds
+  // up to fb->n entries); the kernel stops at the list's length
+  const int64_t rows = fb ? fb->n : n;
+  hipLaunchKernelGGL(project_kernel, dim3(static_cast<unsigned>((rows + 255) / 256)), dim3(256), 0, s, x, n, d, ld, dirs,
                      nd, proj, fb ? fb->act_ids[par] : nullptr, fb ? fb->ctl : nullptr, par);
   AKNN_HIP_TRY(hipGetLastError());
 }
 
-void launch_forest_level_rule(const ForestBuffers& fb, int rule, const float* pj, const int32_t* bins, int shift,
-                              int32_t* perm, Leaf* leaves, int level, int leaf_eff, int kp1, hipStream_t s) {
+void launch_forest_init_batch(const ForestBuffers& fb, int ntrees, int32_t* perm, Leaf* leaves, int32_t* tree_nleaves,
+                              int32_t* tree_last, int64_t n, int leaf_eff, hipStream_t s) {
+  if (ntrees < 1 || n * ntrees > fb.n || n * ntrees > 0x7fffffffLL) throw HipError("forest: batch exceeds its buffers");
+  hipLaunchKernelGGL(forest_init_batch_kernel, dim3(clamp_grid((n * ntrees + 255) / 256, 8192)), dim3(256), 0, s, fb,
+                     ntrees, perm, LeafSink{leaves, n, tree_nleaves, tree_last}, n, leaf_eff);
+  AKNN_HIP_TRY(hipGetLastError());
+}
+
+void launch_forest_level_batch(const ForestBuffers& fb, const MedianRules& rules, bool any_median, int level,
+                               int32_t* perm, Leaf* leaves, int32_t* tree_nleaves, int32_t* tree_last, int leaf_eff,
+                               int kp1, hipStream_t s) {
+  if (rules.nsets < 1 || (rules.routed_trees > rules.tree0 && rules.bins == nullptr))
+    throw HipError("forest: bin rule without bins");
+  if (rules.nd > 0 && rules.proj == nullptr) throw HipError("forest: projection rule without projections");
   const int par = level & 1;
-  const dim3 sg(clamp_grid((fb.n + kSideBlock - 1) / kSideBlock, 2048));
-  if (rule == 0) {
-    if (bins == nullptr || shift < 0 || shift > 30) throw HipError("forest: bin rule without bins");
-    hipLaunchKernelGGL(forest_side_rule_kernel<0>, sg, dim3(256), 0, s, bins, shift, pj, fb.act_ids[par],
-                       fb.act_bkt[par], fb.bkt[par], fb.ctl, par, fb.rmask, fb.wpre, fb.blk_r);
-  } else if (rule == 1) {
-    if (pj == nullptr) throw HipError("forest: projection rule without projections");
-    hipLaunchKernelGGL(forest_median_kernel, dim3(clamp_grid(fb.max_buckets, 4096)), dim3(256), 0, s, pj,
+  if (any_median) {
+    hipLaunchKernelGGL(forest_median_kernel, dim3(clamp_grid(fb.max_buckets, 8192)), dim3(256), 0, s, rules, level,
                        fb.act_ids[par], fb.bkt[par], fb.ctl, par);
     AKNN_HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(forest_side_rule_kernel<1>, sg, dim3(256), 0, s, bins, shift, pj, fb.act_ids[par],
-                       fb.act_bkt[par], fb.bkt[par], fb.ctl, par, fb.rmask, fb.wpre, fb.blk_r);
-  } else {
-    hipLaunchKernelGGL(forest_side_rule_kernel<2>, sg, dim3(256), 0, s, bins, shift, pj, fb.act_ids[par],
-                       fb.act_bkt[par], fb.bkt[par], fb.ctl, par, fb.rmask, fb.wpre, fb.blk_r);
   }
+  const dim3 sg(clamp_grid((fb.n + kSideBlock - 1) / kSideBlock, 4096));
+  hipLaunchKernelGGL(forest_side_rule_kernel, sg, dim3(256), 0, s, rules, level, fb.act_ids[par], fb.act_bkt[par],
+                     fb.bkt[par], fb.ctl, par, fb.rmask, fb.wpre, fb.blk_r);
   AKNN_HIP_TRY(hipGetLastError());
-  // the level kernel's pivot draw (v1, v2) is unused by these rules
-  hipLaunchKernelGGL(forest_level_kernel, dim3(1), dim3(1024), 0, s, fb.bkt[par], fb.bkt[par ^ 1], fb.blk_r,
-                     fb.blk_roff, fb.rmask, fb.wpre, fb.ctl, par, level, 0, 0, leaf_eff, kp1, leaves);
-  AKNN_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(forest_scatter_kernel, dim3(clamp_grid((fb.n + 255) / 256, 8192)), dim3(256), 0, s,
-                     fb.act_ids[par], fb.act_bkt[par], fb.bkt[par], fb.rmask, fb.wpre, fb.blk_roff, fb.ctl, par,
-                     fb.act_ids[par ^ 1], fb.act_bkt[par ^ 1], perm);
-  AKNN_HIP_TRY(hipGetLastError());
+  // the pivot draw (v1, v2) is unused by these rules
+  level_tail(fb, par, level, 0, 0, leaf_eff, kp1, perm, LeafSink{leaves, rules.n, tree_nleaves, tree_last}, s);
 }
 
 void launch_leaf_ranges(const int32_t* perm, const Leaf* leaves, int nleaves, int32_t* rs, int32_t* rc,

@@ -30,6 +30,10 @@ struct ForestBuffers {
   int32_t* wpre;
   int32_t* blk_r;
   int32_t* blk_roff;
+  // per bucket of a level: children, then (after the scan) the index of the
+  // first child; active elements of the children, then their first offset
+  int32_t* bchild;
+  int32_t* bact;
   ForestCtl* ctl;
   int64_t n;
   int64_t max_buckets;
@@ -50,14 +54,36 @@ void launch_forest_level(const ForestBuffers& fb, const float* x, int d, int ld,
 // chain in dimension order), j < nd <= kProjectStep, dirs [nd][ld], one pass
 // over all rows in order (fb null) or over the rows of the points active at
 // `level` (fb's list for its parity).
-// launch_forest_level_rule:
-// one level under rule 0 (bit `shift` of bins[id]), 1 (sampled median of the
-// level's projections pj[n]) or 2 (position); the scatter is the same.
 constexpr int kProjectStep = 16;
 void launch_project(const float* x, int64_t n, int d, int ld, const float* dirs, int nd, float* proj,
                     const ForestBuffers* fb, int level, hipStream_t s);
-void launch_forest_level_rule(const ForestBuffers& fb, int rule, const float* pj, const int32_t* bins, int shift,
-                              int32_t* perm, Leaf* leaves, int level, int leaf_eff, int kp1, hipStream_t s);
+// A batch of median-split trees is built level by level together: one
+// active list, one bucket list (each bucket records its batch tree) and the
+// same three passes per level for every tree.  Batch tree tl (global index
+// tree0 + tl) splits on bit bb - 1 - level of its router bins for its first
+// bb levels (bb = bin_bits when tree0 + tl < routed_trees, else 0), then on
+// the sampled median of its projections on direction level - bb (proj [batch
+// trees][nd][n]), then by position.
+struct MedianRules {
+  int tree0 = 0;
+  int routed_trees = 0;
+  int bin_bits = 0;
+  int nsets = 1;                  // bin arrays: tree t reads bins + (t % nsets) * n
+  const int32_t* bins = nullptr;  // [nsets][n]
+  const float* proj = nullptr;
+  int nd = 0;
+  int64_t n = 0;
+};
+// fb sized for ntrees * n active points; perm, leaves: [ntrees][n];
+// tree_nleaves, tree_last: [ntrees] (leaves written, last level that still
+// split a bucket of the tree, -1 for none).
+void launch_forest_init_batch(const ForestBuffers& fb, int ntrees, int32_t* perm, Leaf* leaves, int32_t* tree_nleaves,
+                              int32_t* tree_last, int64_t n, int leaf_eff, hipStream_t s);
+// any_median: some tree of the batch is at a projection level (its buckets
+// need their sampled pivots).
+void launch_forest_level_batch(const ForestBuffers& fb, const MedianRules& rules, bool any_median, int level,
+                               int32_t* perm, Leaf* leaves, int32_t* tree_nleaves, int32_t* tree_last, int leaf_eff,
+                               int kp1, hipStream_t s);
 // Per-point start ranges from a leaf list (for export and the COSINE parity path).
 void launch_leaf_ranges(const int32_t* perm, const Leaf* leaves, int nleaves, int32_t* rs, int32_t* rc,
                         hipStream_t s);
