@@ -26,7 +26,7 @@ EXPORTS = (
     "aknn_router_gradients", "aknn_router_logits", "aknn_router_param_count", "aknn_router_get_params",
     "aknn_router_set_params", "aknn_router_save", "aknn_router_load", "aknn_router_stats",
     "aknn_router_last_error", "aknn_set_router", "aknn_read_points", "aknn_router_bins", "aknn_add_router",
-    "aknn_recall_noalloc",
+    "aknn_recall_noalloc", "aknn_get_leaf_form",
 )
 
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -69,6 +69,8 @@ def load():
                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     L.aknn_get_candidate_counts.argtypes = [H, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                             ctypes.POINTER(ctypes.c_double)]
+    L.aknn_get_leaf_form.argtypes = [H, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(ctypes.c_double)]
     _i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
     L.aknn_exact_knn.argtypes = [H, _i64p, ctypes.c_int64, ctypes.c_int, _i32p, _f64p]
     L.aknn_recall.argtypes = [H, _i64p, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)]
@@ -201,6 +203,17 @@ class Graph:
         rs = ctypes.c_double()
         self._check(self._L.aknn_get_candidate_counts(self._h, stage.encode(), ctypes.byref(ex), ctypes.byref(rs)))
         return ex.value, rs.value
+
+    def leaf_form(self):
+        """Leaf k-NN launch form of the last calculate() ("all_trees" or
+        "per_tree") and the two forms' times (ms) measured by the call that
+        chose it (option leaf_batch 2)."""
+        f = ctypes.c_int32()
+        a = ctypes.c_double()
+        b = ctypes.c_double()
+        self._check(self._L.aknn_get_leaf_form(self._h, ctypes.byref(f), ctypes.byref(a), ctypes.byref(b)))
+        return {"form": "all_trees" if f.value else "per_tree", "tuning_ms": {"per_tree": a.value,
+                                                                            "all_trees": b.value}}
 
     def exact_knn(self, queries, kq):
         """FP64 exact k-NN of the query indices: (ids [nq][kq], dist [nq][kq])."""

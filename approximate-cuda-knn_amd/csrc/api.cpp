@@ -241,6 +241,9 @@ int aknn_set_option(aknn_handle* h, const char* name, int64_t value) {
       h->params.cosine_table = value != 0;
     } else if (n == "graph_rounds") {
       h->params.graph_rounds = value != 0;
+    } else if (n == "leaf_batch") {
+      if (value < 0 || value > 2) return fail(h, AKNN_ERR_ARGUMENT, "leaf_batch is 0, 1 or 2");
+      h->params.leaf_batch = static_cast<int>(value);
     } else if (n == "router_trees") {
       if (value < 0 || value > 1 << 20) return fail(h, AKNN_ERR_ARGUMENT, "router_trees must be >= 0");
       h->params.router_trees = static_cast<int>(value);
@@ -312,7 +315,7 @@ int aknn_get_kernel_stats(aknn_handle* h, const char* name, int64_t* launches, d
       if (units) *units = gr ? h->engine->stats_propagate.units : 0.0;
       return AKNN_OK;
     } else if (n == "leaf_mfma") {
-      // the screened (matrix-core) launches of the leaf k-NN: GEMM-form flops
+      // the screened (matrix-core) trees of the leaf k-NN: GEMM-form flops
       const aknn::KernelStats& l = h->engine->stats_leaf_knn;
       const int ldh = (h->engine->d() + 7) / 8 * 8;
       if (launches) *launches = l.screened_launches;
@@ -320,11 +323,11 @@ int aknn_get_kernel_stats(aknn_handle* h, const char* name, int64_t* launches, d
       if (units) *units = 2.0 * ldh * l.screened;
       return AKNN_OK;
     } else if (n == "leaf_exact") {
-      // the leaf k-NN launches that ran the exact FP32 chain on every pair
+      // the leaf k-NN trees that ran the exact FP32 chain on every pair
       // (all trees without the screen): 3D flops per pair
       const aknn::KernelStats& l = h->engine->stats_leaf_knn;
       const double per_pair = 3.0 * h->engine->d();
-      if (launches) *launches = l.launches - l.screened_launches;
+      if (launches) *launches = l.segments - l.screened_launches;
       if (total_ms) *total_ms = l.total_ms - l.screened_ms;
       if (units) *units = l.units - per_pair * l.screened;
       return AKNN_OK;
@@ -333,6 +336,15 @@ int aknn_get_kernel_stats(aknn_handle* h, const char* name, int64_t* launches, d
     if (launches) *launches = s->launches;
     if (total_ms) *total_ms = s->total_ms;
     if (units) *units = s->units;
+    return AKNN_OK;
+  });
+}
+
+int aknn_get_leaf_form(aknn_handle* h, int32_t* form, double* per_tree_ms, double* all_trees_ms) {
+  return guarded(h, [&] {
+    if (!h->engine || !h->calculated) return fail(h, AKNN_ERR_STATE, "nothing calculated");
+    const int f = h->engine->leaf_form(per_tree_ms, all_trees_ms);
+    if (form) *form = f;
     return AKNN_OK;
   });
 }

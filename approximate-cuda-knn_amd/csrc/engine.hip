@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cstring>
 #include <stdexcept>
+#include <thread>
 
 namespace aknn {
 
@@ -171,6 +172,8 @@ void Engine::release() {
   rs_ = rc_ = nullptr;
   items_ = nullptr;
   items_cap_ = 0;
+  leaf_st_ = LeafAllState{};
+  leaf_all_used_ = false;
   ncand_ = nullptr;
   changed_[0] = changed_[1] = nullptr;
   fb_ = ForestBuffers{};
@@ -236,6 +239,7 @@ void Engine::set_data(const float* x_rowmajor, int64_t n, int d, const Params& p
   release();
   p_ = p;
   prop_mode_ = -1;
+  leaf_form_ = -1;
   n_ = n;
   d_ = d;
   ld_ = (d + 3) / 4 * 4;
@@ -484,7 +488,9 @@ int Engine::median_batch_trees(int nd) {
 
 void Engine::initial_knn() {
   stats_leaf_knn = KernelStats{};
+  stats_leaf_knn.segments = p_.trees;
   tree_screened_.assign(p_.trees, 0);
+  leaf_all_used_ = false;
   int32_t* ids = gids_[cur_];
   float* vals = gvals_[cur_];
   launch_stub_init(ids, vals, n_ * p_.k, stream_);
@@ -635,6 +641,36 @@ void Engine::initial_knn() {
     AKNN_HIP_TRY(hipMemsetAsync(leaf_rescored_, 0, sizeof(unsigned long long) * kSlotWords, stream_));
     AKNN_HIP_TRY(hipMemsetAsync(leaf_rescored_ + 2 * kSlotWords, 0, sizeof(unsigned long long) * kSlotWords, stream_));
   }
+  // leaf_batch 2: the launch form learned on this data set.  Unknown: the
+  // all-trees launch runs first (a refused one leaves the stub rows as they
+  // are), then the same rows again from stubs one launch per tree, and the
+  // faster form is kept.
+  const int form = p_.leaf_batch == 2 ? leaf_form_ : p_.leaf_batch;
+  if (form == 1 && initial_knn_all(lh, leaf_off, mfma)) return;
+  bool tune = form < 0;
+  float all_ms = 0;
+  if (tune) {
+    if (initial_knn_all(lh, leaf_off, mfma)) {
+      // its item staging is reused below and its error flag read: all done
+      AKNN_HIP_TRY(hipStreamSynchronize(stream_));
+      leaf_all_stats(mfma);
+      AKNN_HIP_TRY(hipEventElapsedTime(&all_ms, ev(7), launch_ev(1)));
+      leaf_all_used_ = false;
+      stats_leaf_knn = KernelStats{};
+      stats_leaf_knn.segments = p_.trees;
+      tree_screened_.assign(p_.trees, 0);
+      launch_stub_init(ids, vals, n_ * p_.k, stream_);
+      if (mfma) {
+        AKNN_HIP_TRY(hipMemsetAsync(leaf_rescored_, 0, sizeof(unsigned long long) * kSlotWords, stream_));
+        AKNN_HIP_TRY(hipMemsetAsync(leaf_rescored_ + 2 * kSlotWords, 0, sizeof(unsigned long long) * kSlotWords,
+                                    stream_));
+      }
+      AKNN_HIP_TRY(hipEventRecord(ev(4), stream_));
+    } else {
+      tune = false;
+      leaf_form_ = 0;
+    }
+  }
   // leaf-screen counters (re-scored, -, screened slots) copied after each tree
   const size_t cw = 3 * static_cast<size_t>(kSlotWords);
   unsigned long long* ch = static_cast<unsigned long long*>(pinned(2, sizeof(unsigned long long) * cw * p_.trees));
@@ -684,6 +720,103 @@ void Engine::initial_knn() {
                                   hipMemcpyDeviceToHost, stream_));
   }
   stats_leaf_knn.launches = p_.trees;
+  if (tune) {
+    AKNN_HIP_TRY(hipEventRecord(ev(5), stream_));
+    AKNN_HIP_TRY(hipEventSynchronize(ev(5)));
+    float per_tree = 0;
+    AKNN_HIP_TRY(hipEventElapsedTime(&per_tree, ev(4), ev(5)));
+    leaf_tune_ms_[0] = per_tree;
+    leaf_tune_ms_[1] = all_ms;
+    leaf_form_ = all_ms < per_tree ? 1 : 0;
+  }
+}
+
+// Every tree's leaf k-NN in one persistent launch (launch_leaf_all).  The
+// work items of all trees are built first, on host threads in parallel, and
+// go up in one copy with their offsets; the launch's tree modes, clock stamps
+// and error flag come back behind it.  False when the device cannot hold the
+// launch's grid (nothing launched).
+bool Engine::initial_knn_all(const Leaf* lh, const std::vector<int64_t>& leaf_off, bool mfma) {
+  const int T = p_.trees;
+  // the tuning compares from here: the host's item building holds the device
+  AKNN_HIP_TRY(hipEventRecord(ev(7), stream_));
+  std::vector<std::vector<Leaf>>& items = items_host_;
+  items.assign(T, {});
+  {
+    const int nth = std::min(T, 16);
+    std::vector<std::thread> th;
+    for (int w = 0; w < nth; ++w)
+      th.emplace_back([&, w] {
+        std::vector<Leaf> host;
+        for (int t = w; t < T; t += nth) {
+          host.assign(lh + leaf_off[t], lh + leaf_off[t + 1]);
+          build_work_items(host, items[t]);
+        }
+      });
+    for (std::thread& x : th) x.join();
+  }
+  std::vector<int64_t> off(T + 1, 0);
+  for (int t = 0; t < T; ++t) {
+    if (static_cast<int64_t>(items[t].size()) > nleaves_[t] + n_ / kItemOwners + 1)
+      throw HipError("leaf work items exceed their bound");
+    off[t + 1] = off[t] + static_cast<int64_t>(items[t].size());
+  }
+  if (static_cast<size_t>(off[T]) > items_cap_) throw HipError("leaf work items exceed their buffer");
+  Leaf* ih = static_cast<Leaf*>(pinned(1, sizeof(Leaf) * static_cast<size_t>(std::max<int64_t>(off[T], 1))));
+  for (int t = 0; t < T; ++t) std::copy(items[t].begin(), items[t].end(), ih + off[t]);
+  // slot 3: modes [T] (padded to 8 bytes) | stamps [T + 1] | error
+  const size_t b_mode = sizeof(int32_t) * T, b_st = sizeof(uint64_t) * (T + 1);
+  const size_t b_mode8 = (b_mode + 7) / 8 * 8;
+  char* h3 = static_cast<char*>(pinned(3, b_mode8 + b_st + sizeof(int)));
+  if (!leaf_st_.queue) {
+    leaf_st_.targs = alloc<char>(leaf_all_targs_bytes(T));
+    leaf_st_.queue = alloc<unsigned>(8 * static_cast<size_t>(T));
+    leaf_st_.bar = alloc<unsigned>(2);
+    leaf_st_.mode = alloc<int32_t>(T);
+    leaf_st_.stamps = alloc<unsigned long long>(T + 1);
+    leaf_st_.prev = alloc<unsigned long long>(2);
+    leaf_st_.error = alloc<int>(1);
+    AKNN_HIP_TRY(hipDeviceGetAttribute(&wall_khz_, hipDeviceAttributeWallClockRate, device_));
+  }
+  AKNN_HIP_TRY(hipMemcpyAsync(items_, ih, sizeof(Leaf) * off[T], hipMemcpyHostToDevice, stream_));
+  AKNN_HIP_TRY(hipEventRecord(launch_ev(0), stream_));
+  const bool ok = launch_leaf_all(kmetric(), mfma, x_, d_, ld_, xh_, ldh_, eps_, eps_ ? eps_ + n_ : nullptr, nh_,
+                                  norms_, perm_, items_, off.data(), T, n_, p_.k, gids_[cur_], gvals_[cur_], rs_,
+                                  rc_, mfma ? leaf_rescored_ : nullptr, mfma ? leaf_rescored_ + 2 * kSlotWords : nullptr,
+                                  leaf_st_, device_, stream_);
+  if (!ok) return false;
+  AKNN_HIP_TRY(hipEventRecord(launch_ev(1), stream_));
+  char* hm = h3;
+  char* hs = hm + b_mode8;
+  AKNN_HIP_TRY(hipMemcpyAsync(hm, leaf_st_.mode, b_mode, hipMemcpyDeviceToHost, stream_));
+  AKNN_HIP_TRY(hipMemcpyAsync(hs, leaf_st_.stamps, b_st, hipMemcpyDeviceToHost, stream_));
+  AKNN_HIP_TRY(hipMemcpyAsync(hs + b_st, leaf_st_.error, sizeof(int), hipMemcpyDeviceToHost, stream_));
+  stats_leaf_knn.launches = 1;
+  leaf_all_used_ = true;
+  return true;
+}
+
+// After the stream has finished: the all-trees launch's tree modes and
+// per-tree times (device clock stamps at its barriers).
+void Engine::leaf_all_stats(bool mfma) {
+  const int T = p_.trees;
+  const size_t b_mode8 = (sizeof(int32_t) * T + 7) / 8 * 8, b_st = sizeof(uint64_t) * (T + 1);
+  const char* h3 = static_cast<const char*>(pinned_[3].p);
+  const int32_t* mode = reinterpret_cast<const int32_t*>(h3);
+  const char* hs = h3 + b_mode8;
+  uint64_t st[2];
+  int err = 0;
+  std::memcpy(&err, hs + b_st, sizeof(int));
+  if (err) throw HipError("leaf k-NN: a grid barrier wait ran past its limit");
+  for (int t = 0; t < T; ++t) {
+    std::memcpy(st, hs + sizeof(uint64_t) * t, sizeof(st));
+    const double ms = static_cast<double>(st[1] - st[0]) / std::max(wall_khz_, 1);
+    tree_screened_[t] = mfma && mode[t] ? 1 : 0;
+    if (tree_screened_[t]) {
+      stats_leaf_knn.screened_ms += ms;
+      stats_leaf_knn.screened_launches += 1;
+    }
+  }
 }
 
 // Arguments of propagation round r, which reads graph buffer `cur` and
@@ -957,11 +1090,12 @@ void Engine::calculate() {
     float ms = 0;
     AKNN_HIP_TRY(hipEventElapsedTime(&ms, launch_ev(2 * l), launch_ev(2 * l + 1)));
     stats_leaf_knn.total_ms += ms;
-    if (mfma && tree_screened_[l]) {
+    if (!leaf_all_used_ && mfma && tree_screened_[l]) {
       stats_leaf_knn.screened_ms += ms;
       stats_leaf_knn.screened_launches += 1;
     }
   }
+  if (leaf_all_used_) leaf_all_stats(mfma);
   std::vector<unsigned long long> nc(2 * static_cast<size_t>(std::max(p_.props, 1)));
   for (int r = 0; r < p_.props; ++r) round_counts(r, &nc[2 * r]);
   stats_propagate.examined = stats_propagate.rescored = 0;

@@ -37,6 +37,8 @@ struct Params {
   int pca_fp16 = 1;       // the screen's projections are kept in FP16, their rounding inside the radii (same results)
   int cosine_table = 1;   // COSINE parity initial pass reads its keys from a per-tree key table when it fits (same results)
   int mean_fp64 = 0;      // normalization's mean as blocked FP64 sums (the reference: a sequential FP32 sum)
+  int leaf_batch = 2;     // initial leaf k-NN launches: 0 one per tree, 1 all trees in one persistent launch,
+                          // 2 the faster of the two, timed on the first calculate() of a data set (same results)
 };
 
 // Per-stage times of the last calculate(), milliseconds.
@@ -53,8 +55,9 @@ struct KernelStats {
   double rescored = 0;  // propagation: candidates whose exact distance was computed;
                         // leaf k-NN: pairs of screened trees whose exact distance was computed
   double screened = 0;  // leaf k-NN: pairs screened on the matrix cores
-  int64_t screened_launches = 0;
+  int64_t screened_launches = 0;  // leaf k-NN: trees that ran screened
   double screened_ms = 0;
+  int64_t segments = 0;  // leaf k-NN: trees (one launch may run all of them)
 };
 
 class Engine {
@@ -90,6 +93,8 @@ class Engine {
     p_.cosine_table = p.cosine_table;
     if (p_.mean_fp64 != p.mean_fp64) prop_mode_ = -1;
     p_.mean_fp64 = p.mean_fp64;
+    p_.leaf_batch = p.leaf_batch;
+    leaf_form_ = -1;  // the leaf launch form is learned again under the new options
     drop_round_graph();
   }
   // Router bins of the points (host array [n], values < 2^bin_bits): the
@@ -128,6 +133,13 @@ class Engine {
   int pca_rank() const { return pca_.r; }
   bool cosine_table_used() const { return cosine_table_used_; }
   double pca_retained16() const { return pca_.retained16; }
+  // leaf launch form of the last calculate() (0 per tree, 1 all trees) and
+  // the times the learning call measured for each (0 until it ran)
+  int leaf_form(double* per_tree_ms, double* all_trees_ms) const {
+    if (per_tree_ms) *per_tree_ms = leaf_tune_ms_[0];
+    if (all_trees_ms) *all_trees_ms = leaf_tune_ms_[1];
+    return leaf_all_used_ ? 1 : 0;
+  }
   int tree_iterations(int tree) const { return tree < static_cast<int>(iters_.size()) ? iters_[tree] : -1; }
 
   const StageTimes& times() const { return times_; }
@@ -204,14 +216,24 @@ class Engine {
   size_t items_cap_ = 0;   // capacity of items_ (entries)
   std::vector<std::vector<Leaf>> items_host_;
   // pinned host staging, grown on demand: 0 leaf lists, 1 work items,
-  // 2 leaf-screen counters after each tree
+  // 2 leaf-screen counters after each tree, 3 the all-trees launch's tree
+  // modes, clock stamps and error flag
   struct Pinned {
     void* p = nullptr;
     size_t cap = 0;
   };
-  Pinned pinned_[3];
+  Pinned pinned_[4];
   void* pinned(int slot, size_t bytes);
   int32_t* rc_ = nullptr;
+  // the all-trees leaf launch's state (per-tree arguments, queues, barrier,
+  // modes, stamps, counters, error), allocated on first use
+  LeafAllState leaf_st_{};
+  bool leaf_all_used_ = false;
+  int leaf_form_ = -1;  // leaf launch form learned on this data set: -1 unknown, 0 per tree, 1 all trees
+  double leaf_tune_ms_[2] = {0, 0};  // the learning call's times of the two forms
+  int wall_khz_ = 0;
+  bool initial_knn_all(const Leaf* lh, const std::vector<int64_t>& leaf_off, bool mfma);
+  void leaf_all_stats(bool mfma);
   CosineParityScratch cps_{};
   // COSINE parity key table: offsets [trees][n] (+ total), block sums of
   // their scan, the keys, and whether the last build used it

@@ -210,6 +210,31 @@ void launch_leaf_mfma(int metric, const float* x, int d, int ld, const _Float16*
                       int max_leaf, int k, int32_t* ids, float* vals, const int32_t* rsp, const int32_t* rcp,
                       unsigned long long* rescored, unsigned long long* screened, hipStream_t s);
 
+// Every tree's initial leaf k-NN in one persistent cooperative launch (trees
+// in order, a grid barrier between them; each tree's work items from eight
+// per-XCD queues).  Tree 0 runs the exact body; with `screen` (EUCLIDEAN or
+// COSINE-fixed and the sketch given) later trees run the MFMA-screened body
+// until a screened tree re-scores more than 30% of its pairs.  Same graph as
+// launch_leaf_knn / launch_leaf_mfma tree by tree.  perm, rs, rc: [trees][n];
+// items: every tree's work items (device), tree after tree, at host offsets
+// item_off [trees + 1].  Returns false, launching nothing, when the runtime
+// refuses the cooperative grid.
+struct LeafAllState {
+  void* targs;                  // leaf_all_targs_bytes(trees): per-tree arguments
+  unsigned* queue;              // [trees][8] dequeue counters
+  unsigned* bar;                // [2] barrier arrivals and generation
+  int32_t* mode;                // [trees] out: 1 where the tree ran screened
+  unsigned long long* stamps;   // [trees + 1] out: wall clock at each tree's start and at the end
+  unsigned long long* prev;     // [2] counter totals after the last screened tree
+  int* error;                   // out: a barrier wait ran past its limit
+};
+size_t leaf_all_targs_bytes(int trees);
+bool launch_leaf_all(int metric, bool screen, const float* x, int d, int ld, const _Float16* xh, int ldh,
+                     const float* eps, const float* eps_max, const float* nh, const float* norms, const int32_t* perm,
+                     const Leaf* items, const int64_t* item_off, int trees, int64_t n, int k, int32_t* ids,
+                     float* vals, const int32_t* rs, const int32_t* rc, unsigned long long* rescored,
+                     unsigned long long* screened, const LeafAllState& st, int device, hipStream_t s);
+
 // ---- FP64 exact k-NN and recall (the measurement reference) ------------------
 // sq: FP64 squared norms (COSINE only, launch_sq64).  ids/dist: [nq][kq],
 // sorted by (distance, id), the query excluded.

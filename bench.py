@@ -218,6 +218,7 @@ def run_form(g, name, args, n, sync):
     stats["leaf_mfma"]["pairs_screened"] = scr
     stats["leaf_mfma"]["pairs_rescored"] = res
     return {
+        "leaf_launch": g.leaf_form(),
         "value": n / wall,
         "ms_per_step": wall * 1e3,
         "build_s": float(np.mean([st["total"] for st in stage])) / 1e3,
@@ -340,13 +341,14 @@ def main():
         "stages_ms": r["stages_ms"],
         "stages": r["stages"],
         "roofline": r["roofline"],
+        "leaf_launch": r["leaf_launch"],
     }
     if head in recalls:
         line["recall_at_k"] = recalls[head]
         line["recall_queries"] = int(len(q))
         line["recall_reference"] = "FP64 exact k-NN (aknn_exact_knn), self excluded, ties at the k-th distance count"
     line["forms"] = {nm: {kk: results[nm][kk] for kk in ("value", "ms_per_step", "build_s", "stages_ms", "stages",
-                                                           "roofline", "config", "options", "kernels")
+                                                           "roofline", "config", "options", "kernels", "leaf_launch")
                           if kk in results[nm]} for nm in names}
     for nm in names:
         if nm in recalls:

@@ -160,6 +160,16 @@ int aknn_deinitialize(aknn_handle* h);
  *                   first calculate() probes it, or nothing is screened), the
  *                   rounds are captured into one hipGraph and replayed by later
  *                   calculate() calls (same results); 0: launch round by round
+ *   "leaf_batch"    initial leaf k-NN launches (same results in every form).
+ *                   1: every tree in one persistent cooperative launch (trees
+ *                   in order with a grid barrier between them, each tree's
+ *                   work items from per-XCD queues, the screen switched per
+ *                   tree on the device); 0: one launch per tree (also taken
+ *                   when the runtime refuses the cooperative grid); 2
+ *                   (default): the faster of the two on this data set, timed
+ *                   on its first calculate() (both run there, the second from
+ *                   stub rows again) and kept until the data or an option
+ *                   changes
  *   "router_trees"  with split 1: trees 0..router_trees-1 take the router bins
  *                   (aknn_set_bins or an attached router) as their first
  *                   levels, one bit per level, before the median levels */
@@ -187,15 +197,25 @@ int aknn_get_forest(aknn_handle* h, int tree, int32_t* perm, int32_t* range_star
 int aknn_get_stage_times(aknn_handle* h, double* ms, int n);
 
 /* Kernel statistics of the last calculate(): name is "leaf_knn",
- * "propagate", "forest", "leaf_mfma" (the screened leaf k-NN launches
- * alone), "leaf_exact" (the leaf k-NN launches that ran the exact chain on
- * every pair) or "propagate_graph" (the rounds replayed from the captured
- * graph; zero launches when they were launched one by one).  launches,
- * summed launch time (HIP events on the launch stream), and algorithmic units
- * summed over the launches (flops for leaf_knn and leaf_exact: 3D per pair;
- * GEMM-form flops for leaf_mfma: 2 * padded D per pair; bytes for propagate
- * and forest). */
+ * "propagate", "forest", "leaf_mfma" (the screened trees of the leaf k-NN
+ * alone), "leaf_exact" (the leaf k-NN trees that ran the exact chain on every
+ * pair) or "propagate_graph" (the rounds replayed from the captured graph;
+ * zero launches when they were launched one by one).  launches (for leaf_mfma
+ * and leaf_exact: trees, each a launch or a segment of the all-trees launch),
+ * summed time (HIP events on the launch stream; a segment's from the device
+ * clock at the launch's grid barriers), and algorithmic units summed over
+ * them (flops for leaf_knn and leaf_exact: 3D per pair; GEMM-form flops for
+ * leaf_mfma: 2 * padded D per pair; bytes for propagate and forest). */
 int aknn_get_kernel_stats(aknn_handle* h, const char* name, int64_t* launches, double* total_ms, double* units);
+
+/* The leaf k-NN launch form of the last calculate(): *form 1 when every tree
+ * ran in one persistent launch, 0 one launch per tree; per_tree_ms and
+ * all_trees_ms (nullable) receive the two forms' times measured by the call
+ * that chose it (option leaf_batch 2; 0 before any choice).  No counterpart
+ * in the reference (its initial pass is one kernel over all trees,
+ * euclideandistancematrixgpu.cu:2067-2108): an accessor of this library's
+ * launch-form tuning, for callers measuring it. */
+int aknn_get_leaf_form(aknn_handle* h, int32_t* form, double* per_tree_ms, double* all_trees_ms);
 
 /* Candidate counts of the last calculate().  stage "propagate": new
  * candidates examined over all rounds, and those whose exact FP32 distance was
