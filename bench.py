@@ -171,7 +171,9 @@ def roofline(stages, config, form):
     """The top-level roofline: the single kernel with the largest summed launch
     time, its algorithmic units per launch over its mean launch duration (HIP
     events on the launch stream)."""
-    cands = [s for s in ("leaf_exact", "leaf_mfma", "propagation") if s in stages]
+    # a stage without algorithmic units (the COSINE parity initial pass, whose
+    # chunk loop is bound by L2 latency, not by a peak) is not a roofline
+    cands = [s for s in ("leaf_exact", "leaf_mfma", "propagation") if s in stages and stages[s]["units"] > 0]
     dom = max(cands, key=lambda s: stages[s]["ms"])
     s = stages[dom]
     bound = "hbm" if dom == "propagation" else "mfma"
