@@ -25,7 +25,7 @@
 // Key table (when it fits in device memory): the keys are not streamed per
 // point.  Every point's start-node range of every tree is contiguous in that
 // tree's leaf order, and the points of one leaf share it, so
-// cosine_keys_kernel computes each tree's keys (owner, range member) with the
+// range_keys_kernel computes each tree's keys (owner, range member) with the
 // exact leaf kernel's tiling (64 owners x 64 candidates, 8x8 pairs per lane,
 // 32-dimension chunks staged in LDS, each pair's chain in dimension order)
 // and writes them to a table at the owner's offset (the exclusive prefix sum
@@ -67,9 +67,10 @@ constexpr int kPad = 4;
 
 // Keys of one tree for the owners of each work item against their own
 // start-node ranges: key(i, perm[r]) at keys[off[i] + r - rs[i]] for r in
-// [rs[i], rs[i] + rc[i]).  One wave per (item, 64-owner chunk).
-template <bool BIGD>
-__global__ __launch_bounds__(64) void cosine_keys_kernel(const float* __restrict__ x, int d, int ld,
+// [rs[i], rs[i] + rc[i]).  One wave per (item, 64-owner chunk).  Any metric
+// (the COSINE parity pass, and the leaf merge of lists longer than 64).
+template <int METRIC, bool BIGD>
+__global__ __launch_bounds__(64) void range_keys_kernel(const float* __restrict__ x, int d, int ld,
                                                          const float* __restrict__ norms,
                                                          const int32_t* __restrict__ perm,
                                                          const Leaf* __restrict__ items, int nitems,
@@ -94,7 +95,7 @@ __global__ __launch_bounds__(64) void cosine_keys_kernel(const float* __restrict
   const int nown = L.size - o0 < kTile ? L.size - o0 : kTile;
   const bool has_owner = lane < nown;
   const int32_t my_id = has_owner ? perm[L.start + o0 + lane] : 0;
-  onorm[lane] = has_owner ? norms[my_id] : 1.0f;
+  onorm[lane] = METRIC == kCosine && has_owner ? norms[my_id] : 1.0f;
   olo[lane] = has_owner ? rsp[my_id] : 0;
   ohi[lane] = has_owner ? rsp[my_id] + rcp[my_id] : 0;
   obase[lane] = has_owner ? off[my_id] - rsp[my_id] : 0;
@@ -104,7 +105,7 @@ __global__ __launch_bounds__(64) void cosine_keys_kernel(const float* __restrict
     const int ncand = L.rcount - cb < kTile ? L.rcount - cb : kTile;
     __syncthreads();
     const int32_t my_c = lane < ncand ? perm[L.rstart + cb + lane] : 0;
-    cnorm[lane] = lane < ncand ? norms[my_c] : 1.0f;
+    cnorm[lane] = METRIC == kCosine && lane < ncand ? norms[my_c] : 1.0f;
     float acc[8][8];
     float tot[BIGD ? 8 : 1][BIGD ? 8 : 1];
 #pragma unroll
@@ -160,7 +161,7 @@ __global__ __launch_bounds__(64) void cosine_keys_kernel(const float* __restrict
 #pragma unroll
         for (int i = 0; i < 8; ++i)
 #pragma unroll
-          for (int j = 0; j < 8; ++j) acc[i][j] = chain_step<kCosine>(a[i], b[j], acc[i][j]);
+          for (int j = 0; j < 8; ++j) acc[i][j] = chain_step<METRIC>(a[i], b[j], acc[i][j]);
       }
     }
     __syncthreads();
@@ -169,7 +170,7 @@ __global__ __launch_bounds__(64) void cosine_keys_kernel(const float* __restrict
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const float t = BIGD ? tot[i][j] + acc[i][j] : 0.0f + acc[i][j];
-        Dt[8 * r + i][8 * c + j] = finish_key<kCosine>(t, onorm[8 * r + i], cnorm[8 * c + j]);
+        Dt[8 * r + i][8 * c + j] = finish_key<METRIC>(t, onorm[8 * r + i], cnorm[8 * c + j]);
       }
     __syncthreads();
     // row by row: lane j stores owner o's key for candidate position
@@ -788,20 +789,43 @@ void launch_cosine_parity_initial(const float* x, int d, int ld, const float* no
   AKNN_HIP_TRY(hipGetLastError());
 }
 
-void launch_cosine_keys(const float* x, int d, int ld, const float* norms, const int32_t* perm, const Leaf* items,
-                        int nitems, int max_owners, const int32_t* rs, const int32_t* rc, const int64_t* off,
-                        float* keys, hipStream_t s) {
-  if (nitems <= 0) return;
-  if (ld % 4 != 0 || ld < d) throw HipError("COSINE keys: row stride must be a multiple of 4 covering d");
-  const int chunks = (max_owners + kTile - 1) / kTile;
+template <int METRIC>
+void launch_keys_m(const float* x, int d, int ld, const float* norms, const int32_t* perm, const Leaf* items,
+                   int nitems, int chunks, const int32_t* rs, const int32_t* rc, const int64_t* off, float* keys,
+                   hipStream_t s) {
   const int64_t grid = xcd_grid(static_cast<int64_t>(nitems) * chunks);
   if (d > 256)
-    hipLaunchKernelGGL(cosine_keys_kernel<true>, dim3(static_cast<unsigned>(grid)), dim3(kTile), 0, s, x, d, ld,
-                       norms, perm, items, nitems, chunks, rs, rc, off, keys);
+    hipLaunchKernelGGL((range_keys_kernel<METRIC, true>), dim3(static_cast<unsigned>(grid)), dim3(kTile), 0, s, x, d,
+                       ld, norms, perm, items, nitems, chunks, rs, rc, off, keys);
   else
-    hipLaunchKernelGGL(cosine_keys_kernel<false>, dim3(static_cast<unsigned>(grid)), dim3(kTile), 0, s, x, d, ld,
-                       norms, perm, items, nitems, chunks, rs, rc, off, keys);
+    hipLaunchKernelGGL((range_keys_kernel<METRIC, false>), dim3(static_cast<unsigned>(grid)), dim3(kTile), 0, s, x, d,
+                       ld, norms, perm, items, nitems, chunks, rs, rc, off, keys);
   AKNN_HIP_TRY(hipGetLastError());
+}
+
+void launch_range_keys(int metric, const float* x, int d, int ld, const float* norms, const int32_t* perm,
+                       const Leaf* items, int nitems, int max_owners, const int32_t* rs, const int32_t* rc,
+                       const int64_t* off, float* keys, hipStream_t s) {
+  if (nitems <= 0) return;
+  if (ld % 4 != 0 || ld < d) throw HipError("range keys: row stride must be a multiple of 4 covering d");
+  if (metric == kCosine && norms == nullptr) throw HipError("range keys: COSINE needs the norms");
+  const int chunks = (max_owners + kTile - 1) / kTile;
+  switch (metric) {
+    case kEuclidean:
+      launch_keys_m<kEuclidean>(x, d, ld, norms, perm, items, nitems, chunks, rs, rc, off, keys, s);
+      break;
+    case kTaxicab:
+      launch_keys_m<kTaxicab>(x, d, ld, norms, perm, items, nitems, chunks, rs, rc, off, keys, s);
+      break;
+    case kTaxicabL1:
+      launch_keys_m<kTaxicabL1>(x, d, ld, norms, perm, items, nitems, chunks, rs, rc, off, keys, s);
+      break;
+    case kCosine:
+      launch_keys_m<kCosine>(x, d, ld, norms, perm, items, nitems, chunks, rs, rc, off, keys, s);
+      break;
+    default:
+      throw HipError("unknown distance");
+  }
 }
 
 void launch_count_scan(const int32_t* counts, int64_t n, int64_t* offsets, int64_t* block_sums, int64_t* total,

@@ -75,6 +75,14 @@ void build_work_items(std::vector<Leaf>& leaves, std::vector<Leaf>& items) {
 // keeps one wave's hash set, candidate list and x_i in LDS; the COSINE parity
 // pass keeps x_i in 64 KB of LDS.
 static void check_shapes(const Params& p, int64_t n, int d) {
+  if (p.k > 64) {
+    // lists longer than 64 take the sort-merge kernels (big_k.hip), which
+    // need the stored value to be the key
+    if (p.metric == kCosine && !p.cosine_fixed)
+      throw std::invalid_argument("k > 64 with COSINE needs option cosine_fixed (carried distances): the reference's "
+                                  "flip of stored values is supported for k <= 64");
+    return;
+  }
   if (p.props > 0 && !propagation_fits(p.k, d))
     throw std::invalid_argument("propagation: k = " + std::to_string(p.k) + " with D = " + std::to_string(d) +
                                 " needs more than 160 KB of LDS for one wave ((2(k^2+k+1) + k^2 + D) * 4 bytes)");
@@ -225,7 +233,7 @@ void Engine::set_bins_device(const int32_t* bins, int bin_bits) {
 
 void Engine::set_data(const float* x_rowmajor, int64_t n, int d, const Params& p) {
   if (n < 1 || d < 1) throw std::invalid_argument("N and D must be positive");
-  if (p.k < 1 || p.k > 64) throw std::invalid_argument("k must be in [1, 64]");
+  if (p.k < 1 || p.k > kBigKMax) throw std::invalid_argument("k must be in [1, " + std::to_string(kBigKMax) + "]");
   if (n < static_cast<int64_t>(p.k) + 1)
     throw std::invalid_argument("N must be at least k+1 (the reference walks past the root otherwise)");
   if (n > 0x7fffffffLL) throw std::invalid_argument("N must fit in int32");
@@ -591,7 +599,7 @@ void Engine::initial_knn() {
         std::copy(its.begin(), its.end(), ih + item_off[t]);
         AKNN_HIP_TRY(hipMemcpyAsync(items_ + item_off[t], ih + item_off[t], sizeof(Leaf) * nit, hipMemcpyHostToDevice,
                                     stream_));
-        launch_cosine_keys(x_, d_, ld_, norms_, perm_ + t * n_, items_ + item_off[t], nit, kItemOwners, rs_ + t * n_,
+        launch_range_keys(kCosine, x_, d_, ld_, norms_, perm_ + t * n_, items_ + item_off[t], nit, kItemOwners, rs_ + t * n_,
                            rc_ + t * n_, koff_ + t * n_, ktab_, stream_);
       }
     }
@@ -600,6 +608,10 @@ void Engine::initial_knn() {
     AKNN_HIP_TRY(hipEventRecord(launch_ev(1), stream_));
     stats_leaf_knn.launches = 1;
     cosine_table_used_ = table;
+    return;
+  }
+  if (p_.k > 64) {
+    initial_knn_big();
     return;
   }
   // Work items per tree (host, from the leaf lists): leaves in position order,
@@ -729,6 +741,78 @@ void Engine::initial_knn() {
     leaf_tune_ms_[1] = all_ms;
     leaf_form_ = all_ms < per_tree ? 1 : 0;
   }
+}
+
+// Lists longer than 64 (big_k.hip), tree by tree: each tree's keys of every
+// point against its start range into a table (the leaf kernels' tiling over
+// the work items), then every point's row merged with its table row.
+void Engine::initial_knn_big() {
+  if (!rs_) {
+    rs_ = alloc<int32_t>(static_cast<size_t>(n_) * p_.trees);
+    rc_ = alloc<int32_t>(static_cast<size_t>(n_) * p_.trees);
+  }
+  const int64_t tn = static_cast<int64_t>(n_) * p_.trees;
+  if (!koff_) {
+    koff_ = alloc<int64_t>(tn + 1);
+    kbsum_ = alloc<int64_t>((tn + 1023) / 1024 + 1);
+  }
+  std::vector<int64_t> leaf_off(p_.trees + 1, 0);
+  for (int t = 0; t < p_.trees; ++t) leaf_off[t + 1] = leaf_off[t] + nleaves_[t];
+  Leaf* lh = static_cast<Leaf*>(pinned(0, sizeof(Leaf) * static_cast<size_t>(std::max<int64_t>(leaf_off[p_.trees], 1))));
+  for (int t = 0; t < p_.trees; ++t)
+    AKNN_HIP_TRY(hipMemcpyAsync(lh + leaf_off[t], leaves_ + t * n_, sizeof(Leaf) * nleaves_[t], hipMemcpyDeviceToHost,
+                                stream_));
+  AKNN_HIP_TRY(hipStreamSynchronize(stream_));
+  const size_t cap_items = static_cast<size_t>(*std::max_element(nleaves_.begin(), nleaves_.end())) + n_ / kItemOwners + 1;
+  if (cap_items > items_cap_) {
+    items_ = alloc<Leaf>(cap_items);
+    items_cap_ = cap_items;
+  }
+  Leaf* ih = static_cast<Leaf*>(pinned(1, sizeof(Leaf) * cap_items));
+  int32_t* ids = gids_[cur_];
+  float* vals = gvals_[cur_];
+  std::vector<Leaf> host, its;
+  for (int t = 0; t < p_.trees; ++t) {
+    int32_t* rs = rs_ + t * n_;
+    int32_t* rc = rc_ + t * n_;
+    launch_leaf_ranges(perm_ + t * n_, leaves_ + t * n_, nleaves_[t], rs, rc, stream_);
+    launch_count_scan(rc, n_, koff_, kbsum_, koff_ + n_, stream_);
+    int64_t total = 0;
+    AKNN_HIP_TRY(hipMemcpyAsync(&total, koff_ + n_, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
+    host.assign(lh + leaf_off[t], lh + leaf_off[t + 1]);
+    build_work_items(host, its);
+    // the previous tree's items and table are in use until the stream gets here
+    AKNN_HIP_TRY(hipStreamSynchronize(stream_));
+    if (total > ktab_cap_) {
+      ktab_ = alloc<float>(static_cast<size_t>(total));
+      ktab_cap_ = total;
+    }
+    std::copy(its.begin(), its.end(), ih);
+    AKNN_HIP_TRY(hipMemcpyAsync(items_, ih, sizeof(Leaf) * its.size(), hipMemcpyHostToDevice, stream_));
+    AKNN_HIP_TRY(hipEventRecord(launch_ev(2 * t), stream_));
+    launch_range_keys(kmetric(), x_, d_, ld_, norms_, perm_ + t * n_, items_, static_cast<int>(its.size()),
+                      kItemOwners, rs, rc, koff_, ktab_, stream_);
+    launch_big_leaf_merge(ktab_, koff_, perm_ + t * n_, rs, rc, n_, p_.k, ids, vals, stream_);
+    AKNN_HIP_TRY(hipEventRecord(launch_ev(2 * t + 1), stream_));
+  }
+  stats_leaf_knn.launches = p_.trees;
+}
+
+// Propagation rounds for lists longer than 64 (big_k.hip): every round
+// exact, every neighbour expanded.
+void Engine::propagate_big() {
+  const int base = 2 * p_.trees;
+  round_screened_.assign(std::max(p_.props, 0), 0);
+  rounds_from_graph_ = false;
+  AKNN_HIP_TRY(hipMemsetAsync(ncand_, 0, sizeof(unsigned long long) * kCountWords * std::max(p_.props, 1), stream_));
+  for (int r = 0; r < p_.props; ++r) {
+    AKNN_HIP_TRY(hipEventRecord(launch_ev(base + 2 * r), stream_));
+    launch_big_propagate(kmetric(), x_, d_, ld_, norms_, n_, p_.k, gids_[cur_], gvals_[cur_], gids_[cur_ ^ 1],
+                         gvals_[cur_ ^ 1], stream_);
+    AKNN_HIP_TRY(hipEventRecord(launch_ev(base + 2 * r + 1), stream_));
+    cur_ ^= 1;
+  }
+  stats_propagate.launches = p_.props;
 }
 
 // Every tree's leaf k-NN in one persistent launch (launch_leaf_all).  The
@@ -902,6 +986,10 @@ void Engine::round_counts(int r, unsigned long long out[2]) {
 
 void Engine::propagate() {
   stats_propagate = KernelStats{};
+  if (p_.k > 64) {
+    propagate_big();
+    return;
+  }
   const bool flip = p_.metric == kCosine && !p_.cosine_fixed;
   const int base = 2 * p_.trees;
   AKNN_HIP_TRY(hipMemsetAsync(ncand_, 0, sizeof(unsigned long long) * kCountWords * std::max(p_.props, 1), stream_));

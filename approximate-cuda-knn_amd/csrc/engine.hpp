@@ -233,6 +233,8 @@ class Engine {
   double leaf_tune_ms_[2] = {0, 0};  // the learning call's times of the two forms
   int wall_khz_ = 0;
   bool initial_knn_all(const Leaf* lh, const std::vector<int64_t>& leaf_off, bool mfma);
+  void initial_knn_big();
+  void propagate_big();
   void leaf_all_stats(bool mfma);
   CosineParityScratch cps_{};
   // COSINE parity key table: offsets [trees][n] (+ total), block sums of

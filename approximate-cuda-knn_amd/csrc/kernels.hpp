@@ -235,6 +235,20 @@ bool launch_leaf_all(int metric, bool screen, const float* x, int d, int ld, con
                      float* vals, const int32_t* rs, const int32_t* rc, unsigned long long* rescored,
                      unsigned long long* screened, const LeafAllState& st, int device, hipStream_t s);
 
+// ---- lists longer than 64 (big_k.hip) -----------------------------------------
+// k in (64, kBigKMax]: lists in LDS, merged by sorting (list + a chunk of
+// candidates) and keeping the first k distinct pairs.
+constexpr int kBigKMax = 256;
+// Every point's row merged with one tree's keys of its start range (the key
+// table of launch_range_keys at offsets off[i]).
+void launch_big_leaf_merge(const float* keys, const int64_t* off, const int32_t* perm, const int32_t* rs,
+                           const int32_t* rc, int64_t n, int k, int32_t* ids, float* vals, hipStream_t s);
+// One exact propagation round (EUCLIDEAN, TAXICAB, COSINE with carried
+// distances: metrics whose stored value is the key).
+void launch_big_propagate(int metric, const float* x, int d, int ld, const float* norms, int64_t n, int k,
+                          const int32_t* ids_in, const float* vals_in, int32_t* ids_out, float* vals_out,
+                          hipStream_t s);
+
 // ---- FP64 exact k-NN and recall (the measurement reference) ------------------
 // sq: FP64 squared norms (COSINE only, launch_sq64).  ids/dist: [nq][kq],
 // sorted by (distance, id), the query excluded.
@@ -259,7 +273,7 @@ struct CosineParityScratch {
   int64_t words, summary_words, max_cand;
   int groups;
 };
-// ktab/koff (nullable): the per-tree key table of launch_cosine_keys and its
+// ktab/koff (nullable): the per-tree key table of launch_range_keys and its
 // offsets; without them the keys are computed from the rows per point.
 void launch_cosine_parity_initial(const float* x, int d, int ld, const float* norms, int64_t n, int k, int trees,
                                   const int32_t* rs, const int32_t* rc, const int32_t* perm, int64_t cap,
@@ -268,12 +282,13 @@ void launch_cosine_parity_initial(const float* x, int d, int ld, const float* no
 // Dynamic LDS bytes of the COSINE parity pass: x_i (without the key table),
 // the summary bitmap (n / 1024 bits) and the selection pool (k + cap pairs).
 size_t cosine_parity_lds(int d, int64_t summary_words, int k, int64_t cap, bool table);
-// One tree's COSINE keys (parity mode) of every owner of the work items
-// against its own start-node range: key(i, perm[r]) at keys[off[i] + r -
-// rs[i]]; max_owners: the largest item size.
-void launch_cosine_keys(const float* x, int d, int ld, const float* norms, const int32_t* perm, const Leaf* items,
-                        int nitems, int max_owners, const int32_t* rs, const int32_t* rc, const int64_t* off,
-                        float* keys, hipStream_t s);
+// One tree's keys of every owner of the work items against its own
+// start-node range (any metric: the COSINE parity pass, and the leaf merge of
+// lists longer than 64): key(i, perm[r]) at keys[off[i] + r - rs[i]];
+// max_owners: the largest item size.
+void launch_range_keys(int metric, const float* x, int d, int ld, const float* norms, const int32_t* perm,
+                       const Leaf* items, int nitems, int max_owners, const int32_t* rs, const int32_t* rc,
+                       const int64_t* off, float* keys, hipStream_t s);
 // offsets[i] = counts[0] + ... + counts[i-1] (int64); block_sums needs
 // ceil(n / 1024) entries; *total receives the sum.
 void launch_count_scan(const int32_t* counts, int64_t n, int64_t* offsets, int64_t* block_sums, int64_t* total,
