@@ -224,6 +224,44 @@ __device__ __forceinline__ double sketch_lower_bound(float p, float ex, float ey
   return lb == lb ? lb : -__builtin_inf();
 }
 
+// An upper bound of the exact COSINE key 1 - fl(tot / den) from the same
+// terms: tot >= p - (kap (A+ex)(B+ey) + ex B + (A+ex) ey + kap A B), the
+// error of the sketch's dot product being bounded on both sides alike.
+// Non-finite or degenerate inputs give +inf.
+__device__ __forceinline__ double sketch_upper_bound_cosine(float p, float ex, float ey, float aa, float bb,
+                                                            const SketchBound& sb) {
+  const float den = sqrtf(aa) * sqrtf(bb);
+  if (!(den > 0.0f) || !(den <= 3.4e38f)) return __builtin_inf();
+  const double a = sqrt((static_cast<double>(aa) + sb.tiny) * (1.0 + 2.0 * sb.kap));
+  const double b = sqrt((static_cast<double>(bb) + sb.tiny) * (1.0 + 2.0 * sb.kap));
+  const double xe = ex, ye = ey;
+  const double lo = static_cast<double>(p) - sb.kap * (a + xe) * (b + ye) - xe * b - (a + xe) * ye -
+                    sb.kap * a * b - 2.0 * sb.tiny;
+  double q = lo / static_cast<double>(den);
+  q -= fabs(q) * 2.384185791015625e-07 + 1e-40;  // the FP32 quotient's rounding, 2^-22 relative
+  const double ub = (1.0 - q) + 9.5367431640625e-07;  // the FP32 store of 1 - quotient, 2^-20 absolute
+  return ub == ub ? ub : __builtin_inf();
+}
+
+// The float next to f below (above); +inf steps to the largest finite float.
+__device__ __forceinline__ float float_below(float f) {
+  const uint32_t b = __float_as_uint(f);
+  if (f > 0.0f) return __uint_as_float(b - 1u);
+  if (f == 0.0f) return -1.401298464e-45f;
+  return __uint_as_float(b + 1u);
+}
+__device__ __forceinline__ float float_above(float f) { return -float_below(-f); }
+
+// The largest float not above v (the smallest not below v).
+__device__ __forceinline__ float float_down(double v) {
+  const float f = static_cast<float>(v);
+  return static_cast<double>(f) > v ? float_below(f) : f;
+}
+__device__ __forceinline__ float float_up(double v) {
+  const float f = static_cast<float>(v);
+  return static_cast<double>(f) < v ? float_above(f) : f;
+}
+
 // Row-width class of a kernel instantiation: rows of exactly 4*NV floats in
 // registers (NV = 8, 16, 24, 32), or 0 for any width, streamed.
 inline int row_width_class(int d) {

@@ -158,7 +158,7 @@ class Engine {
   // flipped stored values give no distance threshold)
   // metric code of the distance kernels
   int kmetric() const { return p_.metric == kTaxicab && p_.taxicab_l1 ? kTaxicabL1 : p_.metric; }
-  bool propagation_screened() const { return p_.sketch && !(p_.metric == kCosine && !p_.cosine_fixed); }
+  bool propagation_screened() const { return p_.sketch != 0; }
   bool leaf_mfma_active() const {
     return p_.leaf_mfma && p_.trees > 1 && (p_.metric == kEuclidean || (p_.metric == kCosine && p_.cosine_fixed));
   }

@@ -186,7 +186,7 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
     if (ncand != nullptr && lane == 0)
       atomicAdd(ncand + (blockIdx.x % kCountSlots) * kCountStride, static_cast<unsigned long long>(m));
   }
-  if constexpr (SK) {
+  if constexpr (SK && !FLIP) {
     if (m > 0) {
       // screen: survivors are written back to cand[] in order (slot
       // w + rank <= base + lane was already read by this batch)
@@ -341,14 +341,100 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
     if (nnew > 0) {
       uint64_t* kv = reinterpret_cast<uint64_t*>(hs);  // [m] (hs has >= 2 m words)
       bool nan = lane < k && key != key;
-      for (int base = 0; base < m; base += 64) {
-        const int ci = base + lane;
-        const int32_t c = ci < m ? cand[ci] : -1;
-        if (c >= 0) {
-          const float dk = finish_key<METRIC>(row_chain<METRIC>(xi, x + static_cast<int64_t>(c) * ld, d), ni,
-                                              METRIC == kCosine ? norms[c] : 1.0f);
-          nan |= dk != dk;
-          kv[ci] = pack_pair(dk, order_id(c));
+      bool exact_all = true;
+      if constexpr (SKM == 1) {
+        // The screen of the candidates' own first k: each candidate's sketch
+        // bounds lb <= key <= ub; T, the k-th smallest (ub, id), bounds the
+        // k-th smallest exact key of the candidates from above, so a
+        // candidate whose lb is above T is not among their first k and only
+        // the others are keyed exactly.  Degenerate bounds, NaN carried
+        // values or at most k candidates: every candidate is keyed.
+        float* lbv = reinterpret_cast<float*>(cand);  // cand[ci]'s slot once its id is in kv
+        const float ei = eps[i];
+        const float ec_max = eps[n];
+        const bool per_cand = !(ec_max < __builtin_inff());
+        bool degen = false;
+        for (int base = 0; base < m; base += 64) {
+          const int ci = base + lane;
+          const int32_t c = ci < m ? cand[ci] : -1;
+          if (c >= 0) {
+            const float p = sketch_chain<METRIC>(xs, xh + static_cast<int64_t>(c) * ldh, ldh);
+            const float ec = per_cand ? eps[c] : ec_max;
+            const double lb = sketch_lower_bound<METRIC>(p, ei, ec, ni, norms[c], sb);
+            const double ub = sketch_upper_bound_cosine(p, ei, ec, ni, norms[c], sb);
+            degen |= !(lb > -__builtin_inf()) || !(ub < __builtin_inf());
+            kv[ci] = pack_pair(float_up(ub), order_id(c));
+            lbv[ci] = float_down(lb);
+          }
+        }
+        wave_sync();
+        if (__ballot(degen || nan) == 0 && m > k) {
+          uint32_t* shist = reinterpret_cast<uint32_t*>(cand + ((m + 3) & ~3));
+          uint64_t* stop = reinterpret_cast<uint64_t*>(shist + 256);
+          const int got = wave_topk(kv, m, k, stop, shist);
+          wave_sync();
+          uint64_t tv = lane < got ? stop[lane] : 0ull;
+#pragma unroll
+          for (int o = 32; o >= 1; o >>= 1) {
+            const uint32_t ohi = __shfl_xor(static_cast<uint32_t>(tv >> 32), o);
+            const uint32_t olo = __shfl_xor(static_cast<uint32_t>(tv), o);
+            const uint64_t ov = (static_cast<uint64_t>(ohi) << 32) | olo;
+            tv = ov > tv ? ov : tv;
+          }
+          const float thr = pair_key(tv);
+          // survivors keyed exactly, compacted in order into kv (slot w +
+          // rank <= base + lane was read by this batch)
+          int w = 0;
+          for (int base = 0; base < m; base += 64) {
+            const int ci = base + lane;
+            const uint64_t v = ci < m ? kv[ci] : 0ull;
+            const bool pass = ci < m && !(lbv[ci] > thr);
+            uint64_t ev = 0ull;
+            if (pass) {
+              const int32_t c = pair_id(v);
+              const float dk = finish_key<METRIC>(row_chain<METRIC>(xi, x + static_cast<int64_t>(c) * ld, d), ni,
+                                                  norms[c]);
+              nan |= dk != dk;
+              ev = pack_pair(dk, order_id(c));
+            }
+            const uint64_t bal = __ballot(pass);
+            wave_sync();
+            if (pass) kv[w + __popcll(bal & ((1ull << lane) - 1))] = ev;
+            w += __popcll(bal);
+          }
+          wave_sync();
+          m = w;
+          exact_all = false;
+          if (ncand != nullptr && lane == 0)
+            atomicAdd(ncand + (kCountSlots + blockIdx.x % kCountSlots) * kCountStride,
+                      static_cast<unsigned long long>(m));
+        } else {
+          for (int base = 0; base < m; base += 64) {
+            const int ci = base + lane;
+            if (ci < m) {
+              const int32_t c = pair_id(kv[ci]);
+              const float dk = finish_key<METRIC>(row_chain<METRIC>(xi, x + static_cast<int64_t>(c) * ld, d), ni,
+                                                  norms[c]);
+              nan |= dk != dk;
+              kv[ci] = pack_pair(dk, order_id(c));
+            }
+          }
+          exact_all = false;
+          if (ncand != nullptr && lane == 0)
+            atomicAdd(ncand + (kCountSlots + blockIdx.x % kCountSlots) * kCountStride,
+                      static_cast<unsigned long long>(m));
+        }
+      }
+      if (exact_all) {
+        for (int base = 0; base < m; base += 64) {
+          const int ci = base + lane;
+          const int32_t c = ci < m ? cand[ci] : -1;
+          if (c >= 0) {
+            const float dk = finish_key<METRIC>(row_chain<METRIC>(xi, x + static_cast<int64_t>(c) * ld, d), ni,
+                                                METRIC == kCosine ? norms[c] : 1.0f);
+            nan |= dk != dk;
+            kv[ci] = pack_pair(dk, order_id(c));
+          }
         }
       }
       wave_sync();
@@ -470,8 +556,10 @@ WaveLds plan_lds(int k, int d, int ld, int ldh, int ld8, int pca_r, int skm, boo
   const int need = 2 * (k * k + k + 1);
   // (multiples of 4 keep every wave's LDS block 16-byte aligned)
   w.cand_cap = (k * k + 3) / 4 * 4;
-  // COSINE parity: the selection's histogram and lists reuse the list
+  // COSINE parity: the selection's histogram and lists reuse the list; with
+  // the sketch screen they follow the candidates' lower bounds
   if (flip && w.cand_cap < 256 + 4 * k) w.cand_cap = (256 + 4 * k + 3) / 4 * 4;
+  if (flip && skm == 1) w.cand_cap = (k * k + 3) / 4 * 4 + 256 + 4 * k;
   w.xi_stride = skm == 1 ? ldh + 3 * ld : skm == 2 ? ld8 / 4 + 3 * ld : skm >= 3 ? pca_r + ld : (d + 3) / 4 * 4;
   int pow2 = 128;
   while (pow2 < need) pow2 *= 2;
@@ -510,15 +598,18 @@ void launch_w(const PropagateArgs& a, hipStream_t s) {
 template <int METRIC, bool FLIP>
 void launch_t(const PropagateArgs& a, hipStream_t s) {
   // COSINE parity: carried entries are keyed by their flipped stored values,
-  // so the k-th entry is no distance threshold and the screen would keep
-  // nearly every candidate; the exact path with its set runs instead
+  // so the k-th entry is no distance threshold; the sketch screen there
+  // bounds the candidates' own first k instead
+  auto fits = [&](int skm) { return plan_lds(a.k, a.d, a.ld, a.ldh, a.ld8, a.pca_r, skm, FLIP).waves > 0; };
   if constexpr (FLIP) {
-    launch_w<METRIC, FLIP, 0>(a, s);
+    if (a.xh != nullptr && fits(1))
+      launch_w<METRIC, FLIP, 1>(a, s);
+    else
+      launch_w<METRIC, FLIP, 0>(a, s);
     return;
   }
   // a screen whose wider per-wave layout does not fit even one wave's LDS
   // gives way to the exact pass (the same graph)
-  auto fits = [&](int skm) { return plan_lds(a.k, a.d, a.ld, a.ldh, a.ld8, a.pca_r, skm, FLIP).waves > 0; };
   if constexpr (METRIC == kEuclidean) {
     if (a.pca16 != nullptr && fits(4)) {
       launch_w<METRIC, FLIP, 4>(a, s);

@@ -229,11 +229,25 @@ def test_sketch_changes_nothing(aknn, oracle, datasets, metric, fixed, d):
     ex0, rs0 = b.candidate_counts()
     print(f"[gpu] examined {ex:.0f} rescored {rs:.0f} (plain {rs0:.0f})", flush=True)
     assert ex0 == rs0
-    if metric == 2 and not fixed:
-        assert (ex, rs) == (ex0, rs0)  # COSINE parity keeps the exact path
-    else:
-        assert ex == ex0 and 0 < rs < ex
+    # COSINE parity screens its candidates' own first k (the flip makes the
+    # row's k-th value no threshold); the others screen against the row
+    assert ex == ex0 and 0 < rs < ex
     check_graph(oracle, a, x, 8, 2, 24, 4, metric, cosine_fixed=fixed)
+
+
+@pytest.mark.parametrize("n,d,k,props", [(5000, 96, 30, 3), (3000, 300, 20, 2), (2000, 24, 12, 4)])
+def test_cosine_parity_screen(aknn, oracle, datasets, n, d, k, props):
+    """COSINE parity rounds with the sketch screen: the candidates' own first
+    k bounded by the k-th smallest sketch upper bound; the oracle's graph, and
+    that of the unscreened rounds, with fewer candidates keyed exactly."""
+    x = datasets.sphere_c4(n, d, seed=7) if d == 96 else datasets.gmm_c1(n, d, clusters=8, seed=7)[0]
+    print(f"[gpu] COSINE parity screen n={n} d={d} k={k} rounds={props}", flush=True)
+    a = gpu_run(aknn, x, k, 3, 2 * k, props, 2, sketch=1)
+    b = gpu_run(aknn, x, k, 3, 2 * k, props, 2, sketch=0)
+    _same_graph(a, b)
+    ex, rs = a.candidate_counts()
+    assert 0 < rs < ex
+    check_graph(oracle, a, x, k, 3, 2 * k, props, 2)
 
 
 def _adversarial(n=3000, d=300, seed=5):
