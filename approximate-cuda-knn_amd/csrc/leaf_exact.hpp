@@ -55,7 +55,7 @@ __device__ __forceinline__ void item_sync() {
   if constexpr (kGroupSync)
     __syncthreads();
   else
-    item_sync<kGroupSync>();
+    wave_sync();
 }
 
 // One (leaf, 64-owner chunk) or work item lb, by one wave.
