@@ -122,6 +122,47 @@ __device__ __forceinline__ float row_chain_regs(const float* a, const float* __r
   return 0.0f + p;
 }
 
+// The FP16 sketch of a normalized point: xh[i] is x[i] rounded to FP16
+// (round to nearest even) with values below the smallest normal FP16 (2^-14)
+// flushed to zero, so no FP16 subnormal is ever stored; rows are padded with
+// zeros to ldh (a multiple of 8 halves, 16 B).  eps[i] is an upper bound of
+// ||x[i] - xh[i]|| in the norm the distance needs (L1 for TAXICAB, L2
+// otherwise): every difference x - xh is exact in FP32, squares and sums are
+// taken in FP64 and the result is rounded up to FP32.  nh[i] is ||xh[i]||^2
+// rounded to nearest FP32 from its exact FP64 sum (the squares of halves are
+// exact in FP64).  A row that does not fit FP16 (|x| >= 65520, or NaN) gets
+// eps = +inf: every bound built from it lets the exact path decide.
+//
+// sketch_element: one element's half, its error term added to acc and its
+// square to sq; bad set when the element does not fit.
+__device__ __forceinline__ _Float16 sketch_element(float v, bool l1, double& acc, double& sq, bool& bad) {
+  _Float16 h = static_cast<_Float16>(v);
+  float hv = static_cast<float>(h);
+  if (!(__builtin_fabsf(hv) <= 65504.0f)) {  // overflow to inf, or NaN input
+    bad = true;
+    h = static_cast<_Float16>(0.0f);
+    hv = 0.0f;
+  } else if (__builtin_fabsf(hv) < 6.103515625e-05f) {  // FP16 subnormal range: store zero
+    h = static_cast<_Float16>(0.0f);
+    hv = 0.0f;
+  }
+  const double e = static_cast<double>(v) - static_cast<double>(hv);  // exact
+  acc += l1 ? __builtin_fabs(e) : e * e;
+  sq += static_cast<double>(hv) * static_cast<double>(hv);
+  return h;
+}
+
+// The radius from a row's summed error terms: D exact terms summed in FP64
+// (relative error below D * 2^-53), widened by 1e-12, the root taken for L2,
+// rounded up to FP32; +inf for a row that does not fit.
+__device__ __forceinline__ float sketch_radius(double acc, bool l1, bool bad) {
+  double r = acc * (1.0 + 1e-12);
+  if (!l1) r = __builtin_sqrt(r) * (1.0 + 1e-15);
+  float f = static_cast<float>(r);
+  if (static_cast<double>(f) < r) f = __uint_as_float(__float_as_uint(f) + 1u);  // r >= 0: next float up
+  return bad || !(f == f) ? __builtin_inff() : f;
+}
+
 // Chain of a FP16 sketch row b (global, ldh halves, 16-byte aligned, zero
 // padded) against a query sketch a held as floats (zero padded to ldh): one
 // unchunked chain in FP32, each half converted exactly.  Feeds

@@ -419,6 +419,16 @@ void Engine::build_forest_median() {
     for (;;) {
       for (int b = 0; b < p_.level_batch; ++b, ++level) {
         bool any_median = false;
+        // trees whose first directions this level needs share passes over
+        // all rows (up to kProjSetMax trees a pass)
+        ProjSet set{};
+        int groups = 0;
+        auto flush = [&]() {
+          if (groups == 0) return;
+          launch_project_set(x_, n_, d_, ld_, set, groups, stream_);
+          ++proj_launches_;
+          groups = 0;
+        };
         for (int tl = 0; tl < nt; ++tl) {
           const int bb = t0 + tl < routed_trees ? bin_bits_ : 0;
           const int dir = level - bb;
@@ -426,13 +436,21 @@ void Engine::build_forest_median() {
           any_median = true;
           if (dir < projected[tl]) continue;
           const int cnt = std::min(kProjectStep, nd - projected[tl]);
-          launch_project(x_, n_, d_, ld_, dirs_ + (static_cast<size_t>(t0 + tl) * nd + projected[tl]) * ld_, cnt,
-                         proj_ + (static_cast<int64_t>(tl) * nd + projected[tl]) * n_,
-                         projected[tl] > 0 ? &fbm_ : nullptr, level, stream_);
+          const float* dirs = dirs_ + (static_cast<size_t>(t0 + tl) * nd + projected[tl]) * ld_;
+          float* proj = proj_ + (static_cast<int64_t>(tl) * nd + projected[tl]) * n_;
+          if (projected[tl] == 0) {
+            set.dirs[groups] = dirs;
+            set.proj[groups] = proj;
+            set.nd[groups] = cnt;
+            if (++groups == kProjSetMax) flush();
+          } else {
+            launch_project(x_, n_, d_, ld_, dirs, cnt, proj, &fbm_, level, stream_);
+            ++proj_launches_;
+          }
           projected[tl] += cnt;
           proj_dirs_ += cnt;
-          ++proj_launches_;
         }
+        flush();
         launch_forest_level_batch(fbm_, rules, any_median, level, perm, leaves, tree_ctl_, tree_ctl_ + bt, leaf_eff_,
                                   p_.k + 1, stream_);
       }
@@ -1122,13 +1140,20 @@ void Engine::calculate() {
   fb_.eps = p_.sketch ? eps_ : nullptr;
   fb_.nh = p_.sketch ? nh_ : nullptr;
   fb_.amb_count = p_.sketch ? leaf_rescored_ + kSlotWords : nullptr;
-  AKNN_HIP_TRY(hipMemcpyAsync(x_, x_raw_, sizeof(float) * n_ * ld_, hipMemcpyDeviceToDevice, stream_));
   AKNN_HIP_TRY(hipEventRecord(ev(0), stream_));
   if (p_.mean_fp64 && !mean64_) mean64_ = alloc<double>(static_cast<size_t>((n_ + 2047) / 2048) * d_);
-  launch_normalize(x_, n_, d_, ld_, mean_, gmax_, p_.scale_all != 0, p_.mean_fp64 ? mean64_ : nullptr, stream_);
+  // the uploaded points stay as they are: normalization writes x_ from them
+  // (and the FP16 sketch in the same row pass)
+  SketchOut sko;
+  sko.l1 = kmetric() == kTaxicab || kmetric() == kTaxicabL1;
+  sko.xh = xh_;
+  sko.ldh = ldh_;
+  sko.eps = eps_;
+  sko.nh = nh_;
+  launch_normalize(x_raw_, x_, n_, d_, ld_, mean_, gmax_, p_.scale_all != 0, p_.mean_fp64 ? mean64_ : nullptr,
+                   sketch ? &sko : nullptr, stream_);
   if (norms_) launch_norms(x_, n_, d_, ld_, norms_, stream_);
   if (sketch) {
-    launch_sketch(x_, n_, d_, ld_, kmetric(), xh_, ldh_, eps_, nh_, stream_);
     launch_max_nonneg(eps_, n_, eps_ + n_, stream_);
     launch_max_nonneg(nh_, n_, nh_ + n_, stream_);
   }

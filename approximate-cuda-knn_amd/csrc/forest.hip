@@ -701,27 +701,32 @@ constexpr int kProjGroup = kProjectStep;  // directions per launch (one pass ove
 constexpr int kProjChunk = 32;  // dimensions staged in LDS per step
 constexpr int kProjXs = kProjChunk + 4;  // LDS row stride (floats): 16-byte reads of 8 lanes hit distinct banks
 
-// P[j * n + i] = x_i . r_j as one FMA chain in dimension order (FP32, the
-// oracle's chain), for j < nd <= kProjGroup and every point i (act_ids null,
-// rows in order), or every point of a level's active list (act_ids[0 ..
-// ctl->nact[parity]): only they, or their subsets at deeper levels, read
-// these directions).  One lane per point, 256 points per
-// block: each step stages the block's rows for 32 dimensions in LDS with
-// coalesced 16-byte loads (8 lanes per 128-byte row piece); each lane then
-// advances its kProjGroup chains from its own staged row, the directions'
-// values (the same for every lane) coming through scalar loads.
+// For each of G direction groups q: P_q[j * n + i] = x_i . r_qj as one FMA
+// chain in dimension order (FP32, the oracle's chain), for j < nd_q <=
+// kProjGroup and every point i (act_ids null, rows in order), or every point
+// of a level's active list (act_ids[0 .. ctl->nact[parity]): only they, or
+// their subsets at deeper levels, read these directions).  One lane per
+// point, 256 points per block: each step stages the block's rows for 32
+// dimensions in LDS with coalesced 16-byte loads (8 lanes per 128-byte row
+// piece), and the directions' values for those dimensions ([group][dim]
+// [direction], read by every lane at once); each lane then advances its
+// G * kProjGroup chains from its own staged row.  Several trees' first
+// directions thus share one pass over the rows.
+template <int G>
 __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ x, int64_t n, int d, int ld,
-                                                      const float* __restrict__ dirs, int nd,
-                                                      float* __restrict__ proj, const int32_t* __restrict__ act_ids,
+                                                      ProjSet ps, const int32_t* __restrict__ act_ids,
                                                       const ForestCtl* ctl, int parity) {
   __shared__ __attribute__((aligned(16))) float xs[256 * kProjXs];
+  __shared__ __attribute__((aligned(16))) float dv[G][kProjChunk][kProjGroup];
   const int t = threadIdx.x;
   const int64_t i0 = blockIdx.x * 256ll;
   const int64_t rows = act_ids != nullptr ? ctl->nact[parity] : n;
   if (i0 >= rows) return;  // the grid covers n; the list is shorter at deep levels
-  float acc[kProjGroup];
+  float acc[G][kProjGroup];
 #pragma unroll
-  for (int g = 0; g < kProjGroup; ++g) acc[g] = 0.0f;
+  for (int q = 0; q < G; ++q)
+#pragma unroll
+    for (int g = 0; g < kProjGroup; ++g) acc[q][g] = 0.0f;
   const float* xr = xs + t * kProjXs;
   for (int c0 = 0; c0 < d; c0 += kProjChunk) {
     const int cw = d - c0 < kProjChunk ? d - c0 : kProjChunk;
@@ -738,34 +743,36 @@ __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ 
       const float4 v = c < d ? *reinterpret_cast<const float4*>(x + row * ld + c) : make_float4(0.f, 0.f, 0.f, 0.f);
       *reinterpret_cast<float4*>(xs + r * kProjXs + 4 * q) = v;
     }
-    __syncthreads();
-    // the directions' values are the same for every lane: scalar loads from
-    // the direction rows (rows past nd read row nd - 1; their chains are not
-    // stored)
-    int c = 0;
-    for (; c + 4 <= cw; c += 4) {
-      const float4 v = *reinterpret_cast<const float4*>(xr + c);
-#pragma unroll
-      for (int g = 0; g < kProjGroup; ++g) {
-        const float4 r = *reinterpret_cast<const float4*>(dirs + static_cast<int64_t>(g < nd ? g : nd - 1) * ld + c0 + c);
-        acc[g] = __builtin_fmaf(v.x, r.x, acc[g]);
-        acc[g] = __builtin_fmaf(v.y, r.y, acc[g]);
-        acc[g] = __builtin_fmaf(v.z, r.z, acc[g]);
-        acc[g] = __builtin_fmaf(v.w, r.w, acc[g]);
-      }
+    // direction values of this chunk (directions past nd read the last one:
+    // their chains are not stored; dimensions past d are not read)
+    for (int e = t; e < G * kProjChunk * kProjGroup; e += 256) {
+      const int q = e / (kProjChunk * kProjGroup), r = e % (kProjChunk * kProjGroup);
+      const int g = r / kProjChunk, c = r % kProjChunk;
+      const int nq = ps.nd[q];
+      dv[q][c][g] = c < cw ? ps.dirs[q][static_cast<int64_t>(g < nq ? g : nq - 1) * ld + c0 + c] : 0.0f;
     }
-    for (; c < cw; ++c) {
+    __syncthreads();
+    for (int c = 0; c < cw; ++c) {
       const float v = xr[c];
 #pragma unroll
-      for (int g = 0; g < kProjGroup; ++g)
-        acc[g] = __builtin_fmaf(v, dirs[static_cast<int64_t>(g < nd ? g : nd - 1) * ld + c0 + c], acc[g]);
+      for (int q = 0; q < G; ++q)
+#pragma unroll
+        for (int g4 = 0; g4 < kProjGroup / 4; ++g4) {
+          const float4 r = *reinterpret_cast<const float4*>(&dv[q][c][4 * g4]);
+          acc[q][4 * g4 + 0] = __builtin_fmaf(v, r.x, acc[q][4 * g4 + 0]);
+          acc[q][4 * g4 + 1] = __builtin_fmaf(v, r.y, acc[q][4 * g4 + 1]);
+          acc[q][4 * g4 + 2] = __builtin_fmaf(v, r.z, acc[q][4 * g4 + 2]);
+          acc[q][4 * g4 + 3] = __builtin_fmaf(v, r.w, acc[q][4 * g4 + 3]);
+        }
     }
   }
   if (i0 + t < rows) {
     const int64_t i = act_ids != nullptr ? act_ids[i0 + t] : i0 + t;
 #pragma unroll
-    for (int g = 0; g < kProjGroup; ++g)
-      if (g < nd) proj[static_cast<int64_t>(g) * n + i] = acc[g];
+    for (int q = 0; q < G; ++q)
+#pragma unroll
+      for (int g = 0; g < kProjGroup; ++g)
+        if (g < ps.nd[q]) ps.proj[q][static_cast<int64_t>(g) * n + i] = acc[q][g];
   }
 }
 
@@ -1030,11 +1037,33 @@ void launch_project(const float* x, int64_t n, int d, int ld, const float* dirs,
   if (ld % 4 != 0 || ld < d) throw HipError("project: malformed row width");
   if (nd > kProjGroup) throw HipError("project: at most 16 directions per launch");
   const int par = level & 1;
+  ProjSet ps{};
+  ps.dirs[0] = dirs;
+  ps.proj[0] = proj;
+  ps.nd[0] = nd;
   // over an active list the grid covers its capacity (a batch of trees holds
   // up to fb->n entries); the kernel stops at the list's length
   const int64_t rows = fb ? fb->n : n;
-  hipLaunchKernelGGL(project_kernel, dim3(static_cast<unsigned>((rows + 255) / 256)), dim3(256), 0, s, x, n, d, ld, dirs,
-                     nd, proj, fb ? fb->act_ids[par] : nullptr, fb ? fb->ctl : nullptr, par);
+  hipLaunchKernelGGL(project_kernel<1>, dim3(static_cast<unsigned>((rows + 255) / 256)), dim3(256), 0, s, x, n, d, ld,
+                     ps, fb ? fb->act_ids[par] : nullptr, fb ? fb->ctl : nullptr, par);
+  AKNN_HIP_TRY(hipGetLastError());
+}
+
+void launch_project_set(const float* x, int64_t n, int d, int ld, const ProjSet& ps, int groups, hipStream_t s) {
+  if (groups <= 0 || n <= 0) return;
+  if (ld % 4 != 0 || ld < d) throw HipError("project: malformed row width");
+  for (int q = 0; q < groups; ++q)
+    if (ps.nd[q] <= 0 || ps.nd[q] > kProjGroup) throw HipError("project: 1 to 16 directions per group");
+  const dim3 g(static_cast<unsigned>((n + 255) / 256)), b(256);
+  switch (groups) {
+    case 1: hipLaunchKernelGGL(project_kernel<1>, g, b, 0, s, x, n, d, ld, ps, nullptr, nullptr, 0); break;
+    case 2: hipLaunchKernelGGL(project_kernel<2>, g, b, 0, s, x, n, d, ld, ps, nullptr, nullptr, 0); break;
+    case 3: hipLaunchKernelGGL(project_kernel<3>, g, b, 0, s, x, n, d, ld, ps, nullptr, nullptr, 0); break;
+    case 4: hipLaunchKernelGGL(project_kernel<4>, g, b, 0, s, x, n, d, ld, ps, nullptr, nullptr, 0); break;
+    case 5: hipLaunchKernelGGL(project_kernel<5>, g, b, 0, s, x, n, d, ld, ps, nullptr, nullptr, 0); break;
+    case 6: hipLaunchKernelGGL(project_kernel<6>, g, b, 0, s, x, n, d, ld, ps, nullptr, nullptr, 0); break;
+    default: throw HipError("project: at most 6 direction groups per pass");
+  }
   AKNN_HIP_TRY(hipGetLastError());
 }
 

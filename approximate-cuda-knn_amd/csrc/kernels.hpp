@@ -16,8 +16,23 @@ namespace aknn {
 // does so for the first 256 only).  mean64_part (nullable, ceil(n / 2048) * d
 // doubles): the mean as blocked FP64 sums instead of the reference's
 // sequential FP32 sum.
-void launch_normalize(float* x, int64_t n, int d, int ld, float* mean, uint32_t* gmax, bool scale_all,
-                      double* mean64_part, hipStream_t s);
+//
+// The FP16 sketch outputs of launch_normalize's row pass (device_util.hpp,
+// sketch_element): xh [n][ldh] halves (ldh % 8 == 0, ldh >= d); eps[n]: upper
+// bounds of the rounding error's norm (l1: L1 for TAXICAB, else L2); nh[n]:
+// ||xh||^2.
+struct SketchOut {
+  bool l1 = false;
+  _Float16* xh = nullptr;
+  int ldh = 0;
+  float* eps = nullptr;
+  float* nh = nullptr;
+};
+// src: the uploaded points (read only); x: the normalized points (every
+// element of every row written, the padding copied).  sk (nullable): the
+// FP16 sketch of the normalized rows, formed in the same pass.
+void launch_normalize(const float* src, float* x, int64_t n, int d, int ld, float* mean, uint32_t* gmax,
+                      bool scale_all, double* mean64_part, const SketchOut* sk, hipStream_t s);
 // Per-point COSINE squared norms (the chain of .cu:849-865 for one operand).
 void launch_norms(const float* x, int64_t n, int d, int ld, float* norms, hipStream_t s);
 
@@ -57,6 +72,16 @@ void launch_forest_level(const ForestBuffers& fb, const float* x, int d, int ld,
 constexpr int kProjectStep = 16;
 void launch_project(const float* x, int64_t n, int d, int ld, const float* dirs, int nd, float* proj,
                     const ForestBuffers* fb, int level, hipStream_t s);
+// Up to kProjSetMax groups of directions (several trees' first directions)
+// projected in one pass over all rows in order: group q writes
+// proj[q][j * n + i] = x_i . dirs[q][j] for j < nd[q] <= kProjectStep.
+constexpr int kProjSetMax = 6;
+struct ProjSet {
+  const float* dirs[kProjSetMax];
+  float* proj[kProjSetMax];
+  int nd[kProjSetMax];
+};
+void launch_project_set(const float* x, int64_t n, int d, int ld, const ProjSet& ps, int groups, hipStream_t s);
 // A batch of median-split trees is built level by level together: one
 // active list, one bucket list (each bucket records its batch tree) and the
 // same three passes per level for every tree.  Batch tree tl (global index
@@ -188,12 +213,8 @@ void launch_pca_project(const float* x, int64_t n, int d, int ld, const float* p
 // once per wave instead of one scattered 4-byte load per candidate.
 void launch_max_nonneg(const float* v, int64_t n, float* out, hipStream_t s);
 
-// ---- FP16 sketch of the normalized points -----------------------------------
-// xh: [n][ldh] halves (ldh % 8 == 0, ldh >= d); eps[n]: upper bounds of the
-// rounding error's norm (L1 for TAXICAB, L2 otherwise); nh[n]: ||xh||^2.
-void launch_sketch(const float* x, int64_t n, int d, int ld, int metric, _Float16* xh, int ldh, float* eps,
-                   float* nh, hipStream_t s);
-
+// ---- sketches of the normalized points ---------------------------------------
+// (the FP16 sketch: launch_normalize's SketchOut)
 // Int8 sketch: q8 [n][ld8] (ld8 % 16 == 0, ld8 >= d), per-row scale s8,
 // a8 = sum q^2, e8 = upper bound of ||x - s8 q8||_2.
 void launch_sketch8(const float* x, int64_t n, int d, int ld, int8_t* q8, int ld8, float* s8, int32_t* a8, float* e8,

@@ -5,9 +5,13 @@
 // dimension walks all points in order (the rounding of a sequential sum depends
 // on the order, so this chain cannot be split).  The chain's speed is set by
 // how many rows are in flight, so fifteen waves stage rows through LDS for the
-// one wave that adds.  Centring, the max-abs reduction over the dimensions
-// below 256 and the scaling are plain parallel HBM passes.  Option
-// mean_fp64 replaces the sequential chain by blocked FP64 sums.
+// one wave that adds.  Option mean_fp64 replaces the sequential chain by
+// blocked FP64 sums.  Step 2 is two passes over the uploaded points, which
+// stay as they are: the max-abs of the centred values over the dimensions
+// below 256 (read only), then one pass per row that centres, scales and
+// writes the normalized row, and forms its FP16 sketch from the values it
+// wrote (the same arithmetic, element for element, as centring, scaling and
+// sketching in separate passes).
 #include "device_util.hpp"
 #include "kernels.hpp"
 
@@ -167,29 +171,41 @@ __global__ __launch_bounds__(256) void mean64_partial_kernel(const float* __rest
   }
 }
 
+// One workgroup per dimension: the block sums are read into LDS 2048 at a
+// time by all threads, and one thread adds them in block order (a dependent
+// FP64 add per block instead of a dependent global load).
+constexpr int kMean64Stage = 2048;
 __global__ __launch_bounds__(256) void mean64_final_kernel(const double* __restrict__ part, int64_t blocks, int64_t n,
                                                           int d, float* __restrict__ mean) {
-  const int dd = blockIdx.x * 256 + threadIdx.x;
-  if (dd >= d) return;
+  __shared__ double buf[kMean64Stage];
+  const int dd = blockIdx.x;
   double sum = 0.0;
-  for (int64_t b = 0; b < blocks; ++b) sum = sum + part[b * d + dd];
-  mean[dd] = static_cast<float>(sum / static_cast<double>(n));
+  for (int64_t b0 = 0; b0 < blocks; b0 += kMean64Stage) {
+    const int cnt = blocks - b0 < kMean64Stage ? static_cast<int>(blocks - b0) : kMean64Stage;
+    for (int j = threadIdx.x; j < cnt; j += 256) buf[j] = part[(b0 + j) * d + dd];
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int j = 0; j < cnt; ++j) sum = sum + buf[j];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) mean[dd] = static_cast<float>(sum / static_cast<double>(n));
 }
 
-__global__ __launch_bounds__(256) void center_kernel(float* __restrict__ x, int64_t n, int d, int ld, int dm,
+constexpr int kRowWaves = 4;  // rows (one per wave) of a 256-thread block
+
+// Max-abs of the centred values x - mean over dimensions below dm, one wave
+// per row (grid-stride), into gmax (non-negative floats order like their bits).
+__global__ __launch_bounds__(256) void maxabs_kernel(const float* __restrict__ x, int64_t n, int ld, int dm,
                                                      const float* __restrict__ mean, uint32_t* __restrict__ gmax) {
   __shared__ float red[256];
+  const int lane = threadIdx.x & 63;
   float local = 0.0f;
-  const int64_t total = n * ld;
-  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < total; e += gridDim.x * 256ll) {
-    const int dd = static_cast<int>(e % ld);
-    if (dd < d) {
-      const float v = x[e] - mean[dd];
-      x[e] = v;
-      if (dd < dm) {
-        const float a = __builtin_fabsf(v);
-        local = a > local ? a : local;
-      }
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kRowWaves + (threadIdx.x >> 6); i < n;
+       i += static_cast<int64_t>(gridDim.x) * kRowWaves) {
+    const float* xr = x + i * ld;
+    for (int dd = lane; dd < dm; dd += 64) {
+      const float a = __builtin_fabsf(xr[dd] - mean[dd]);
+      local = a > local ? a : local;
     }
   }
   red[threadIdx.x] = local;
@@ -202,13 +218,49 @@ __global__ __launch_bounds__(256) void center_kernel(float* __restrict__ x, int6
   if (threadIdx.x == 0) atomicMax(gmax, __float_as_uint(red[0]));
 }
 
-__global__ __launch_bounds__(256) void scale_kernel(float* __restrict__ x, int64_t n, int d, int ld, int dm,
-                                                    const uint32_t* __restrict__ gmax) {
+// Row i of the normalized matrix, one wave per row: (src - mean) for
+// dimensions below d, divided by the max-abs below dm; the row's padding
+// copied.  SK: its FP16 sketch from the values written (sketch_element:
+// rounding, flushes, exact error terms), eps[i] and nh[i] from the wave's
+// FP64 sums (butterfly order).
+template <bool SK>
+__global__ __launch_bounds__(256) void apply_kernel(const float* __restrict__ src, float* __restrict__ x, int64_t n,
+                                                    int d, int ld, int dm, const float* __restrict__ mean,
+                                                    const uint32_t* __restrict__ gmax, int l1, _Float16* __restrict__ xh,
+                                                    int ldh, float* __restrict__ eps, float* __restrict__ nh) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kRowWaves + (threadIdx.x >> 6);
+  if (i >= n) return;
   const float g = __uint_as_float(*gmax);
-  const int64_t total = n * ld;
-  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < total; e += gridDim.x * 256ll) {
-    const int dd = static_cast<int>(e % ld);
-    if (dd < dm) x[e] = x[e] / g;
+  const float* sr = src + i * ld;
+  float* xr = x + i * ld;
+  double acc = 0.0, sq = 0.0;
+  bool bad = false;
+  const int span = SK && ldh > ld ? ldh : ld;
+  for (int dd = lane; dd < span; dd += 64) {
+    float v = 0.0f;
+    if (dd < ld) {
+      v = sr[dd];
+      if (dd < d) {
+        v = v - mean[dd];
+        if (dd < dm) v = v / g;
+      }
+      xr[dd] = v;
+    }
+    if constexpr (SK) {
+      if (dd < ldh) xh[i * ldh + dd] = sketch_element(dd < d ? v : 0.0f, l1 != 0, acc, sq, bad);
+    }
+  }
+  if constexpr (SK) {
+    for (int off = 32; off > 0; off >>= 1) {
+      acc += __shfl_xor(acc, off);
+      sq += __shfl_xor(sq, off);
+    }
+    bad = __any(bad);
+    if (lane == 0) {
+      eps[i] = sketch_radius(acc, l1 != 0, bad);
+      nh[i] = static_cast<float>(sq);
+    }
   }
 }
 
@@ -226,22 +278,30 @@ int grid_for(int64_t work, int64_t per_block, int cap) {
 
 }  // namespace
 
-void launch_normalize(float* x, int64_t n, int d, int ld, float* mean, uint32_t* gmax, bool scale_all,
-                      double* mean64_part, hipStream_t s) {
+void launch_normalize(const float* src, float* x, int64_t n, int d, int ld, float* mean, uint32_t* gmax,
+                      bool scale_all, double* mean64_part, const SketchOut* sk, hipStream_t s) {
   const int dm = scale_all ? d : (d < 256 ? d : 256);
   AKNN_HIP_TRY(hipMemsetAsync(gmax, 0, sizeof(uint32_t), s));
   if (mean64_part != nullptr) {
     const int64_t blocks = (n + kMean64Rows - 1) / kMean64Rows;
-    hipLaunchKernelGGL(mean64_partial_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, x, n, d, ld,
+    hipLaunchKernelGGL(mean64_partial_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, src, n, d, ld,
                        mean64_part);
-    hipLaunchKernelGGL(mean64_final_kernel, dim3((d + 255) / 256), dim3(256), 0, s, mean64_part, blocks, n, d, mean);
+    hipLaunchKernelGGL(mean64_final_kernel, dim3(d), dim3(256), 0, s, mean64_part, blocks, n, d, mean);
   } else {
-    hipLaunchKernelGGL(mean_kernel, dim3((d + kMeanDims - 1) / kMeanDims), dim3(1024), 0, s, x, n, d, ld, mean);
+    hipLaunchKernelGGL(mean_kernel, dim3((d + kMeanDims - 1) / kMeanDims), dim3(1024), 0, s, src, n, d, ld, mean);
   }
   AKNN_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(center_kernel, dim3(grid_for(n * ld, 256, 8192)), dim3(256), 0, s, x, n, d, ld, dm, mean, gmax);
+  hipLaunchKernelGGL(maxabs_kernel, dim3(grid_for(n, kRowWaves, 8192)), dim3(256), 0, s, src, n, ld, dm, mean, gmax);
   AKNN_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(scale_kernel, dim3(grid_for(n * ld, 256, 8192)), dim3(256), 0, s, x, n, d, ld, dm, gmax);
+  const dim3 rows(static_cast<unsigned>((n + kRowWaves - 1) / kRowWaves));
+  if (sk != nullptr) {
+    if (sk->ldh % 8 != 0 || sk->ldh < d) throw HipError("sketch: row stride must be a multiple of 8 covering d");
+    hipLaunchKernelGGL(apply_kernel<true>, rows, dim3(256), 0, s, src, x, n, d, ld, dm, mean, gmax, sk->l1 ? 1 : 0,
+                       sk->xh, sk->ldh, sk->eps, sk->nh);
+  } else {
+    hipLaunchKernelGGL(apply_kernel<false>, rows, dim3(256), 0, s, src, x, n, d, ld, dm, mean, gmax, 0, nullptr, 0,
+                       nullptr, nullptr);
+  }
   AKNN_HIP_TRY(hipGetLastError());
 }
 
