@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cstring>
 #include <stdexcept>
+#include <exception>
 #include <thread>
 
 namespace aknn {
@@ -107,6 +108,7 @@ Engine::Engine(int device) : device_(device) {
     events_.push_back(e);
   }
   AKNN_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&ctl_host_), sizeof(ForestCtl), hipHostMallocDefault));
+  AKNN_HIP_TRY(hipEventCreate(&pca_ev_));
 }
 
 Engine::~Engine() {
@@ -116,6 +118,7 @@ Engine::~Engine() {
   release();
   for (auto e : events_) (void)hipEventDestroy(e);
   for (auto e : launch_events_) (void)hipEventDestroy(e);
+  if (pca_ev_) (void)hipEventDestroy(pca_ev_);
   if (ctl_host_) (void)hipHostFree(ctl_host_);
   for (Pinned& b : pinned_)
     if (b.p) (void)hipHostFree(b.p);
@@ -1159,16 +1162,41 @@ void Engine::calculate() {
   }
   if (sketch8) launch_sketch8(x_, n_, d_, ld_, q8_, ld8_, s8_, a8_, e8_, stream_);
   pca_ = PcaBasis{};
+  // This build's principal subspace: the sample moments on the device, then
+  // the eigen-solve on a host thread while the device builds the forest and
+  // the leaf lists (only the propagation rounds read the basis); every
+  // point's projection and radius are made just before the rounds.
+  std::thread pca_thread;
+  PcaBasis pca_next;
+  std::exception_ptr pca_error;
   if (p_.pca_screen && p_.metric == kEuclidean && p_.props > 0 && d_ >= 16 && d_ <= 256) {
-    // this build's principal subspace (a stream sync and a host eigen-solve
-    // inside the bracket), then every point's projection and radius
     if (!pca_mom_) {
       pca_mom_ = alloc<float>(static_cast<size_t>(33) * d_ * d_);
       pca_p_ = alloc<float>(static_cast<size_t>(32) * d_);
       pca_proj_ = alloc<float>(static_cast<size_t>(n_) * 32);
       pca_e_ = alloc<float>(n_ + 1);  // [n]: the largest radius
     }
-    pca_ = compute_pca_basis(x_, n_, d_, ld_, pca_mom_, stream_);
+    if (launch_pca_moments(x_, n_, d_, ld_, pca_mom_, stream_)) {
+      float* mh = static_cast<float*>(pinned(4, sizeof(float) * static_cast<size_t>(d_) * d_));
+      AKNN_HIP_TRY(hipMemcpyAsync(mh, pca_mom_, sizeof(float) * static_cast<size_t>(d_) * d_, hipMemcpyDeviceToHost,
+                                  stream_));
+      AKNN_HIP_TRY(hipEventRecord(pca_ev_, stream_));
+      pca_thread = std::thread([&, mh] {
+        try {
+          AKNN_HIP_TRY(hipSetDevice(device_));
+          AKNN_HIP_TRY(hipEventSynchronize(pca_ev_));
+          pca_next = pca_basis_from_moments(mh, d_);
+        } catch (...) {
+          pca_error = std::current_exception();
+        }
+      });
+    }
+  }
+  auto pca_finish = [&] {
+    if (!pca_thread.joinable()) return;
+    pca_thread.join();
+    if (pca_error) std::rethrow_exception(pca_error);
+    pca_ = std::move(pca_next);
     if (pca_.r > 0) {
       AKNN_HIP_TRY(hipMemcpyAsync(pca_p_, pca_.p.data(), sizeof(float) * pca_.p.size(), hipMemcpyHostToDevice,
                                   stream_));
@@ -1179,15 +1207,21 @@ void Engine::calculate() {
         launch_pca_project(x_, n_, d_, ld_, pca_p_, pca_.r, pca_proj_, pca_e_, stream_);
       launch_max_nonneg(pca_e_, n_, pca_e_ + n_, stream_);
     }
-  }
+  };
   AKNN_HIP_TRY(hipEventRecord(ev(1), stream_));
-  if (p_.split == 1)
-    build_forest_median();
-  else
-    build_forest();
-  AKNN_HIP_TRY(hipEventRecord(ev(2), stream_));
-  initial_knn();
-  AKNN_HIP_TRY(hipEventRecord(ev(3), stream_));
+  try {
+    if (p_.split == 1)
+      build_forest_median();
+    else
+      build_forest();
+    AKNN_HIP_TRY(hipEventRecord(ev(2), stream_));
+    initial_knn();
+    AKNN_HIP_TRY(hipEventRecord(ev(3), stream_));
+  } catch (...) {
+    if (pca_thread.joinable()) pca_thread.join();
+    throw;
+  }
+  pca_finish();
   propagate();
   AKNN_HIP_TRY(hipEventRecord(ev(6), stream_));
   AKNN_HIP_TRY(hipEventSynchronize(ev(6)));

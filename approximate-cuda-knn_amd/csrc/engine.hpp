@@ -222,7 +222,7 @@ class Engine {
     void* p = nullptr;
     size_t cap = 0;
   };
-  Pinned pinned_[4];
+  Pinned pinned_[5];
   void* pinned(int slot, size_t bytes);
   int32_t* rc_ = nullptr;
   // the all-trees leaf launch's state (per-tree arguments, queues, barrier,
@@ -275,6 +275,7 @@ class Engine {
   float* pca_proj_ = nullptr;
   float* pca_e_ = nullptr;
   PcaBasis pca_;
+  hipEvent_t pca_ev_ = nullptr;  // the sample moments have reached the host
   hipGraphExec_t round_exec_ = nullptr;
   int round_graph_mode_ = -1;
   double round_graph_sigma_ = 0.0;

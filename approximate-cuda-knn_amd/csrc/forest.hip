@@ -217,11 +217,60 @@ __device__ __forceinline__ int sketch_side_quad(const float* __restrict__ x, int
   return 0;
 }
 
+// The same decision with the pivot difference w = x_p2 - x_p1 (the exact FP32
+// differences) and its squared norm ww staged in LDS for the bucket: one
+// point's quad then reads only its sketch row.
+__device__ __forceinline__ int sketch_side_quad_w(int ld, const _Float16* __restrict__ xh, int ldh, float eo, float no,
+                                                  int32_t id, const float* w, float ww, const SketchBound& sb, int q) {
+  const _Float16* hc = xh + static_cast<int64_t>(id) * ldh;
+  float sp = 0.0f;
+  const int pieces = ld / 8;
+#pragma unroll 4
+  for (int pc = q; pc < pieces; pc += 4) {
+    const int dd = 8 * pc;
+    const uint4 h = *reinterpret_cast<const uint4*>(hc + dd);
+    const float4 w0 = *reinterpret_cast<const float4*>(w + dd);
+    const float4 w1 = *reinterpret_cast<const float4*>(w + dd + 4);
+    const float wv[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+    const float hv[8] = {half_lo(h.x), half_hi(h.x), half_lo(h.y), half_hi(h.y),
+                         half_lo(h.z), half_hi(h.z), half_lo(h.w), half_hi(h.w)};
+#pragma unroll
+    for (int j = 0; j < 8; ++j) sp = __builtin_fmaf(hv[j], wv[j], sp);
+  }
+  if (ld % 8 != 0 && q == pieces % 4) {  // ld % 8 == 4: one group of four, taken by the next lane in turn
+    const int dd = 8 * pieces;
+    const uint2 h = *reinterpret_cast<const uint2*>(hc + dd);
+    const float4 w0 = *reinterpret_cast<const float4*>(w + dd);
+    const float wv[4] = {w0.x, w0.y, w0.z, w0.w};
+    const float hv[4] = {half_lo(h.x), half_hi(h.x), half_lo(h.y), half_hi(h.y)};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) sp = __builtin_fmaf(hv[j], wv[j], sp);
+  }
+  sp += __shfl_xor(sp, 1);
+  sp += __shfl_xor(sp, 2);
+  const double wn = sqrt(static_cast<double>(ww)) * (1.0 + sb.kap);
+  const double xn = sqrt(static_cast<double>(no)) * (1.0 + sb.kap);
+  const double e = eo;
+  const double bound = wn * (e + sb.kap * (xn + e)) + 2.0 * sb.tiny;
+  if (static_cast<double>(sp) > bound) return 1;
+  if (static_cast<double>(sp) < -bound) return -1;
+  return 0;
+}
+
+// Buckets of a 1024-point block whose pivot data is staged in LDS (an active
+// bucket holds more than L points, so a block spans few), and the widest row
+// staged.
+constexpr int kSideStage = 8;
+constexpr int kSideStageLd = 512;
+
 // Side kernel with the sketch: one pass decides most points from their half
 // rows, a quad of lanes per point so the rows are read coalesced; the
-// decisions become the block's ballot words through LDS.  The undecided ones
-// of a 1024-point block are queued in LDS and take the exact chain together,
-// so a straggler does not hold a whole wave.
+// decisions become the block's ballot words through LDS.  The first
+// kSideStage buckets of a block have their pivot difference, its norm and
+// their records staged in LDS, so a point reads only its own sketch row; the
+// points of further buckets read the pivot rows themselves.  The undecided
+// ones of a 1024-point block are queued in LDS and take the exact chain
+// together, so a straggler does not hold a whole wave.
 __global__ __launch_bounds__(256) void forest_side_sketch_kernel(
     const float* __restrict__ x, int d, int ld, const _Float16* __restrict__ xh, int ldh,
     const float* __restrict__ eps, const float* __restrict__ nh, int64_t n, SketchBound sb,
@@ -234,6 +283,11 @@ __global__ __launch_bounds__(256) void forest_side_sketch_kernel(
   __shared__ int16_t amb[kSideBlock];
   __shared__ int8_t dec[kSideBlock];  // 0 LEFT, 1 RIGHT, 2 undecided
   __shared__ int namb;
+  __shared__ __attribute__((aligned(16))) float wst[kSideStage][kSideStageLd];
+  __shared__ float wwst[kSideStage];
+  __shared__ int32_t bst[kSideStage][3];  // astart, q1, q2
+  __shared__ int32_t pst[kSideStage][2];  // pivot ids
+  __shared__ int32_t b_first_s, nst_s;
   const int nact = ctl->nact[parity];
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -245,7 +299,40 @@ __global__ __launch_bounds__(256) void forest_side_sketch_kernel(
   const float emax = eps[n], nmax = nh[n];
   const bool per_point = !(emax < __builtin_inff());
   for (int64_t blk = blockIdx.x; blk * kSideBlock < nact; blk += gridDim.x) {
-    if (threadIdx.x == 0) namb = 0;
+    if (threadIdx.x == 0) {
+      namb = 0;
+      // the active list runs bucket by bucket: the block's buckets are a
+      // contiguous range of bucket indices
+      const int64_t m0 = blk * kSideBlock;
+      const int64_t m1 = (m0 + kSideBlock < nact ? m0 + kSideBlock : nact) - 1;
+      const int32_t bf = act_bkt[m0], bl = act_bkt[m1];
+      b_first_s = bf;
+      nst_s = ld <= kSideStageLd ? (bl - bf + 1 < kSideStage ? bl - bf + 1 : kSideStage) : 0;
+    }
+    __syncthreads();
+    const int32_t b_first = b_first_s;
+    const int nst = nst_s;
+    if (threadIdx.x < nst) {
+      const Bucket& b = bkt[b_first + threadIdx.x];
+      bst[threadIdx.x][0] = b.astart;
+      bst[threadIdx.x][1] = b.q1;
+      bst[threadIdx.x][2] = b.q2;
+      pst[threadIdx.x][0] = act_ids[b.astart + b.q1];
+      pst[threadIdx.x][1] = act_ids[b.astart + b.q2];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < nst * ld; e += 256) {
+      const int j = e / ld, dd = e - j * ld;
+      wst[j][dd] = x[static_cast<int64_t>(pst[j][1]) * ld + dd] - x[static_cast<int64_t>(pst[j][0]) * ld + dd];
+    }
+    __syncthreads();
+    for (int j = wave; j < nst; j += 4) {
+      // any order of the FP32 sum stays within the bound's rounding allowance
+      float acc = 0.0f;
+      for (int dd = lane; dd < ld; dd += 64) acc = __builtin_fmaf(wst[j][dd], wst[j][dd], acc);
+      for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+      if (lane == 0) wwst[j] = acc;
+    }
     __syncthreads();
     // one point per quad at a time (64 quads, 16 points each)
     for (int t = 0; t < kSideBlock / 64; ++t) {
@@ -253,15 +340,29 @@ __global__ __launch_bounds__(256) void forest_side_sketch_kernel(
       const int64_t m = blk * kSideBlock + ml;
       int r = 0;
       if (m < nact) {
-        const Bucket& b = bkt[act_bkt[m]];
-        const int32_t rel = static_cast<int32_t>(m) - b.astart;
-        if (rel == b.q2) {
-          r = 1;  // Leaf2 writes RIGHT for p2 last (.cu:2178-2179)
-        } else if (rel != b.q1) {
-          const int32_t id = act_ids[m];
-          const int sd = sketch_side_quad(x, ld, xh, ldh, per_point ? eps[id] : emax, per_point ? nh[id] : nmax, id,
-                                          act_ids[b.astart + b.q1], act_ids[b.astart + b.q2], sb, q);
-          r = sd > 0 ? 1 : (sd == 0 ? 2 : 0);
+        const int32_t bi = act_bkt[m];
+        const int j = bi - b_first;
+        if (j >= 0 && j < nst) {
+          const int32_t rel = static_cast<int32_t>(m) - bst[j][0];
+          if (rel == bst[j][2]) {
+            r = 1;  // Leaf2 writes RIGHT for p2 last (.cu:2178-2179)
+          } else if (rel != bst[j][1]) {
+            const int32_t id = act_ids[m];
+            const int sd = sketch_side_quad_w(ld, xh, ldh, per_point ? eps[id] : emax, per_point ? nh[id] : nmax, id,
+                                              wst[j], wwst[j], sb, q);
+            r = sd > 0 ? 1 : (sd == 0 ? 2 : 0);
+          }
+        } else {
+          const Bucket& b = bkt[bi];
+          const int32_t rel = static_cast<int32_t>(m) - b.astart;
+          if (rel == b.q2) {
+            r = 1;
+          } else if (rel != b.q1) {
+            const int32_t id = act_ids[m];
+            const int sd = sketch_side_quad(x, ld, xh, ldh, per_point ? eps[id] : emax, per_point ? nh[id] : nmax, id,
+                                            act_ids[b.astart + b.q1], act_ids[b.astart + b.q2], sb, q);
+            r = sd > 0 ? 1 : (sd == 0 ? 2 : 0);
+          }
         }
       }
       if (q == 0) {

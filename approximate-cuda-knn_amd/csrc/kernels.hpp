@@ -200,8 +200,12 @@ struct PcaBasis {
   double sigma = 1.0;
   double retained16 = 0.0;  // share of the sample's second moment in the top 16 directions
 };
-// moments_dev: device scratch of 33*d*d floats.  Synchronises the stream.
-PcaBasis compute_pca_basis(const float* x, int64_t n, int d, int ld, float* moments_dev, hipStream_t s);
+// The sample's second moments into moments_dev[0, d*d) (device scratch of
+// 33*d*d floats); false (nothing launched) when D is outside [16, 256] or
+// n < 64.  pca_basis_from_moments: the basis from those moments on the host
+// (subspace iteration and Rayleigh-Ritz in FP64; no device call).
+bool launch_pca_moments(const float* x, int64_t n, int d, int ld, float* moments_dev, hipStream_t s);
+PcaBasis pca_basis_from_moments(const float* moments, int d);
 // proj [n][r] = fl(p x_i), rad[n] >= ||proj_i - p x_i||.  With proj16
 // (proj then null): proj16 [n][r] = the FP16 roundings of fl(p x_i) and
 // rad[n] >= ||proj16_i - p x_i|| (the rounding's exact norm added; infinite

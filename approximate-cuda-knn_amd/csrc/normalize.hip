@@ -191,21 +191,28 @@ __global__ __launch_bounds__(256) void mean64_final_kernel(const double* __restr
   if (threadIdx.x == 0) mean[dd] = static_cast<float>(sum / static_cast<double>(n));
 }
 
-constexpr int kRowWaves = 4;  // rows (one per wave) of a 256-thread block
-
-// Max-abs of the centred values x - mean over dimensions below dm, one wave
-// per row (grid-stride), into gmax (non-negative floats order like their bits).
+// Max-abs of the centred values x - mean over dimensions below dm, 16 lanes
+// per row reading 16-byte pieces (16 rows per block, grid-stride), into gmax
+// (non-negative floats order like their bits).
 __global__ __launch_bounds__(256) void maxabs_kernel(const float* __restrict__ x, int64_t n, int ld, int dm,
                                                      const float* __restrict__ mean, uint32_t* __restrict__ gmax) {
   __shared__ float red[256];
-  const int lane = threadIdx.x & 63;
+  const int sub = threadIdx.x & 15;
   float local = 0.0f;
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kRowWaves + (threadIdx.x >> 6); i < n;
-       i += static_cast<int64_t>(gridDim.x) * kRowWaves) {
+  const int pieces = (dm + 3) / 4;  // ld % 4 == 0 and dm <= ld: a piece that starts below dm stays in the row
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * 16 + (threadIdx.x >> 4); i < n;
+       i += static_cast<int64_t>(gridDim.x) * 16) {
     const float* xr = x + i * ld;
-    for (int dd = lane; dd < dm; dd += 64) {
-      const float a = __builtin_fabsf(xr[dd] - mean[dd]);
-      local = a > local ? a : local;
+    for (int c = sub; c < pieces; c += 16) {
+      const float4 v = *reinterpret_cast<const float4*>(xr + 4 * c);
+      const float vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        if (4 * c + e < dm) {
+          const float a = __builtin_fabsf(vv[e] - mean[4 * c + e]);
+          local = a > local ? a : local;
+        }
+      }
     }
   }
   red[threadIdx.x] = local;
@@ -218,46 +225,66 @@ __global__ __launch_bounds__(256) void maxabs_kernel(const float* __restrict__ x
   if (threadIdx.x == 0) atomicMax(gmax, __float_as_uint(red[0]));
 }
 
-// Row i of the normalized matrix, one wave per row: (src - mean) for
-// dimensions below d, divided by the max-abs below dm; the row's padding
-// copied.  SK: its FP16 sketch from the values written (sketch_element:
-// rounding, flushes, exact error terms), eps[i] and nh[i] from the wave's
-// FP64 sums (butterfly order).
-template <bool SK>
+// Row i of the normalized matrix, LPR lanes per row (64 / LPR rows per wave),
+// 16-byte pieces: (src - mean) for dimensions below d, divided by the max-abs
+// below dm; the row's padding copied.  SK: its FP16 sketch from the values
+// written (sketch_element: rounding, flushes, exact error terms), eps[i] and
+// nh[i] from the row's lanes' FP64 sums.
+template <bool SK, int LPR>
 __global__ __launch_bounds__(256) void apply_kernel(const float* __restrict__ src, float* __restrict__ x, int64_t n,
                                                     int d, int ld, int dm, const float* __restrict__ mean,
                                                     const uint32_t* __restrict__ gmax, int l1, _Float16* __restrict__ xh,
                                                     int ldh, float* __restrict__ eps, float* __restrict__ nh) {
-  const int lane = threadIdx.x & 63;
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * kRowWaves + (threadIdx.x >> 6);
-  if (i >= n) return;
+  constexpr int kRows = 256 / LPR;  // rows per block
+  const int sub = threadIdx.x % LPR;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kRows + threadIdx.x / LPR;
+  const bool live = i < n;  // every lane stays for the row reductions
   const float g = __uint_as_float(*gmax);
-  const float* sr = src + i * ld;
+  const float* sr = src + (live ? i : 0) * ld;
   float* xr = x + i * ld;
   double acc = 0.0, sq = 0.0;
   bool bad = false;
-  const int span = SK && ldh > ld ? ldh : ld;
-  for (int dd = lane; dd < span; dd += 64) {
-    float v = 0.0f;
-    if (dd < ld) {
-      v = sr[dd];
-      if (dd < d) {
-        v = v - mean[dd];
-        if (dd < dm) v = v / g;
+  const int span = (SK && ldh > ld ? ldh : ld) / 4;  // 16-byte pieces (ld and ldh are multiples of 4)
+  for (int c = sub; c < span; c += LPR) {
+    const int d0 = 4 * c;
+    float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (d0 < ld) {
+      const float4 s4 = *reinterpret_cast<const float4*>(sr + d0);
+      v[0] = s4.x;
+      v[1] = s4.y;
+      v[2] = s4.z;
+      v[3] = s4.w;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        if (d0 + e < d) {
+          v[e] = v[e] - mean[d0 + e];
+          if (d0 + e < dm) v[e] = v[e] / g;
+        }
       }
-      xr[dd] = v;
+      if (live) *reinterpret_cast<float4*>(xr + d0) = make_float4(v[0], v[1], v[2], v[3]);
     }
     if constexpr (SK) {
-      if (dd < ldh) xh[i * ldh + dd] = sketch_element(dd < d ? v : 0.0f, l1 != 0, acc, sq, bad);
+      if (d0 < ldh) {
+        _Float16 h[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) h[e] = sketch_element(d0 + e < d ? v[e] : 0.0f, l1 != 0, acc, sq, bad);
+        typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+        const half4 hv = {h[0], h[1], h[2], h[3]};
+        if (live) *reinterpret_cast<half4*>(xh + i * ldh + d0) = hv;
+      }
     }
   }
   if constexpr (SK) {
-    for (int off = 32; off > 0; off >>= 1) {
+    for (int off = LPR / 2; off > 0; off >>= 1) {
       acc += __shfl_xor(acc, off);
       sq += __shfl_xor(sq, off);
     }
-    bad = __any(bad);
-    if (lane == 0) {
+    // the row's flag: a lane mask of the wave, LPR bits per row
+    const uint64_t bb = __ballot(bad);
+    const int r0 = (threadIdx.x & 63) / LPR * LPR;
+    const uint64_t rm = LPR == 64 ? ~0ull : ((1ull << LPR) - 1) << r0;
+    bad = (bb & rm) != 0;
+    if (live && sub == 0) {
       eps[i] = sketch_radius(acc, l1 != 0, bad);
       nh[i] = static_cast<float>(sq);
     }
@@ -291,16 +318,28 @@ void launch_normalize(const float* src, float* x, int64_t n, int d, int ld, floa
     hipLaunchKernelGGL(mean_kernel, dim3((d + kMeanDims - 1) / kMeanDims), dim3(1024), 0, s, src, n, d, ld, mean);
   }
   AKNN_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(maxabs_kernel, dim3(grid_for(n, kRowWaves, 8192)), dim3(256), 0, s, src, n, ld, dm, mean, gmax);
+  if (ld % 4 != 0) throw HipError("normalize: row stride must be a multiple of 4");
+  hipLaunchKernelGGL(maxabs_kernel, dim3(grid_for(n, 16, 8192)), dim3(256), 0, s, src, n, ld, dm, mean, gmax);
   AKNN_HIP_TRY(hipGetLastError());
-  const dim3 rows(static_cast<unsigned>((n + kRowWaves - 1) / kRowWaves));
+  // 16 lanes per row for rows of up to 64 pieces (16 rows per block), else 64
+  const bool narrow = ld <= 256 && (sk == nullptr || sk->ldh <= 256);
+  const int per_block = narrow ? 256 / 16 : 256 / 64;
+  const dim3 rows(static_cast<unsigned>((n + per_block - 1) / per_block));
   if (sk != nullptr) {
     if (sk->ldh % 8 != 0 || sk->ldh < d) throw HipError("sketch: row stride must be a multiple of 8 covering d");
-    hipLaunchKernelGGL(apply_kernel<true>, rows, dim3(256), 0, s, src, x, n, d, ld, dm, mean, gmax, sk->l1 ? 1 : 0,
-                       sk->xh, sk->ldh, sk->eps, sk->nh);
+    if (narrow)
+      hipLaunchKernelGGL((apply_kernel<true, 16>), rows, dim3(256), 0, s, src, x, n, d, ld, dm, mean, gmax,
+                         sk->l1 ? 1 : 0, sk->xh, sk->ldh, sk->eps, sk->nh);
+    else
+      hipLaunchKernelGGL((apply_kernel<true, 64>), rows, dim3(256), 0, s, src, x, n, d, ld, dm, mean, gmax,
+                         sk->l1 ? 1 : 0, sk->xh, sk->ldh, sk->eps, sk->nh);
   } else {
-    hipLaunchKernelGGL(apply_kernel<false>, rows, dim3(256), 0, s, src, x, n, d, ld, dm, mean, gmax, 0, nullptr, 0,
-                       nullptr, nullptr);
+    if (narrow)
+      hipLaunchKernelGGL((apply_kernel<false, 16>), rows, dim3(256), 0, s, src, x, n, d, ld, dm, mean, gmax, 0,
+                         nullptr, 0, nullptr, nullptr);
+    else
+      hipLaunchKernelGGL((apply_kernel<false, 64>), rows, dim3(256), 0, s, src, x, n, d, ld, dm, mean, gmax, 0,
+                         nullptr, 0, nullptr, nullptr);
   }
   AKNN_HIP_TRY(hipGetLastError());
 }

@@ -184,9 +184,8 @@ void jacobi_eigen(std::vector<double>& a, int q, std::vector<double>& w, std::ve
 
 }  // namespace
 
-PcaBasis compute_pca_basis(const float* x, int64_t n, int d, int ld, float* moments_dev, hipStream_t s) {
-  PcaBasis b;
-  if (d < 16 || d > 256 || n < 64) return b;
+bool launch_pca_moments(const float* x, int64_t n, int d, int ld, float* moments_dev, hipStream_t s) {
+  if (d < 16 || d > 256 || n < 64) return false;
   const int64_t m = std::min<int64_t>(n, 16384);
   const int64_t stride = n / m;
   const unsigned pg = static_cast<unsigned>((static_cast<int64_t>(d) * d + 255) / 256);
@@ -195,10 +194,12 @@ PcaBasis compute_pca_basis(const float* x, int64_t n, int d, int ld, float* mome
   AKNN_HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(moments_reduce_kernel, dim3(pg), dim3(256), 0, s, part, d, moments_dev);
   AKNN_HIP_TRY(hipGetLastError());
-  std::vector<float> mf(static_cast<size_t>(d) * d);
-  AKNN_HIP_TRY(hipMemcpyAsync(mf.data(), moments_dev, sizeof(float) * mf.size(), hipMemcpyDeviceToHost, s));
-  AKNN_HIP_TRY(hipStreamSynchronize(s));
-  std::vector<double> c(mf.begin(), mf.end());
+  return true;
+}
+
+PcaBasis pca_basis_from_moments(const float* moments, int d) {
+  PcaBasis b;
+  std::vector<double> c(moments, moments + static_cast<size_t>(d) * d);
   double trace = 0.0;
   for (int i = 0; i < d; ++i) trace += c[static_cast<size_t>(i) * d + i];
   if (!(trace > 0.0)) return b;
