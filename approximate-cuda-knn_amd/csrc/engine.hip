@@ -173,6 +173,7 @@ void Engine::release() {
   mean64_ = nullptr;
   q8_ = nullptr;
   a8_ = nullptr;
+  se8_ = nullptr;
   leaf_rescored_ = nullptr;
   xh_ = nullptr;
   gmax_ = nullptr;
@@ -1131,18 +1132,24 @@ void Engine::calculate() {
     // leaf re-scored, forest undecided, leaf screened: kSlotWords each
     leaf_rescored_ = alloc<unsigned long long>(3 * static_cast<size_t>(kSlotWords));
   }
-  const bool sketch8 = propagation_screened() && p_.sketch_int8 && p_.metric == kEuclidean && p_.props > 0;
+  const bool prop8 = propagation_screened() && p_.sketch_int8 && p_.metric == kEuclidean && p_.props > 0;
+  const bool forest8 = p_.sketch && p_.forest_int8 && p_.split == 0;  // the sign forest's side test
+  const bool sketch8 = prop8 || forest8;
   if (sketch8 && !q8_) {
     q8_ = alloc<int8_t>(static_cast<size_t>(n_) * ld8_);
     s8_ = alloc<float>(n_);
     a8_ = alloc<int32_t>(n_);
     e8_ = alloc<float>(n_);
   }
+  if (forest8 && !se8_) se8_ = alloc<float2>(n_);
   fb_.xh = p_.sketch ? xh_ : nullptr;
   fb_.ldh = ldh_;
   fb_.eps = p_.sketch ? eps_ : nullptr;
   fb_.nh = p_.sketch ? nh_ : nullptr;
   fb_.amb_count = p_.sketch ? leaf_rescored_ + kSlotWords : nullptr;
+  fb_.q8 = forest8 ? q8_ : nullptr;
+  fb_.ld8 = ld8_;
+  fb_.se8 = forest8 ? se8_ : nullptr;
   AKNN_HIP_TRY(hipEventRecord(ev(0), stream_));
   if (p_.mean_fp64 && !mean64_) mean64_ = alloc<double>(static_cast<size_t>((n_ + 2047) / 2048) * d_);
   // the uploaded points stay as they are: normalization writes x_ from them
@@ -1160,7 +1167,7 @@ void Engine::calculate() {
     launch_max_nonneg(eps_, n_, eps_ + n_, stream_);
     launch_max_nonneg(nh_, n_, nh_ + n_, stream_);
   }
-  if (sketch8) launch_sketch8(x_, n_, d_, ld_, q8_, ld8_, s8_, a8_, e8_, stream_);
+  if (sketch8) launch_sketch8(x_, n_, d_, ld_, q8_, ld8_, s8_, a8_, e8_, forest8 ? se8_ : nullptr, stream_);
   pca_ = PcaBasis{};
   // This build's principal subspace: the sample moments on the device, then
   // the eigen-solve on a host thread while the device builds the forest and

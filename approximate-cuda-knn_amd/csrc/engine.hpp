@@ -28,6 +28,7 @@ struct Params {
   int sketch = 0;         // propagation screens candidates with the FP16 sketch bound (same results)
   int leaf_mfma = 0;      // leaf k-NN of trees after the first screens pairs on the matrix cores (same results)
   int sketch_int8 = 0;    // with sketch, EUCLIDEAN propagation screens with the int8 sketch (same results)
+  int forest_int8 = 1;    // with sketch, the forest side test of staged buckets reads the int8 sketch (same results)
   int split = 0;          // forest rule: 0 the reference's sign split, 1 median split on seeded directions
   int router_trees = 0;   // median split: trees 0..router_trees-1 take the router bins as their first levels
   int taxicab_l1 = 0;     // TAXICAB keys and stores the L1 total instead of its square root
@@ -80,6 +81,7 @@ class Engine {
     p_.sketch = p.sketch;
     p_.leaf_mfma = p.leaf_mfma;
     p_.sketch_int8 = p.sketch_int8;
+    p_.forest_int8 = p.forest_int8;
     if (p_.split != p.split) prop_mode_ = -1;
     p_.split = p.split;
     p_.router_trees = p.router_trees;
@@ -195,6 +197,7 @@ class Engine {
   float* s8_ = nullptr;
   int32_t* a8_ = nullptr;
   float* e8_ = nullptr;
+  float2* se8_ = nullptr;  // (scale, radius) pairs of the int8 sketch for the forest side test [n]
   int ld8_ = 0;
   unsigned long long* leaf_rescored_ = nullptr;
   int ldh_ = 0;

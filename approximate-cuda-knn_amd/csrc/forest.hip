@@ -151,6 +151,29 @@ __global__ __launch_bounds__(256) void forest_side_kernel(const float* __restric
   }
 }
 
+// side_is_right with the staged w = x_p2 - x_p1 (the same FP32 differences
+// the chain forms): the same chain in dimension order, the point's row read
+// eight 16-byte pieces at a time before their FMAs.
+__device__ __forceinline__ bool side_is_right_w(const float* __restrict__ xc, const float* w, int d) {
+  float s = 0.0f;
+  int dd = 0;
+  for (; dd + 32 <= d; dd += 32) {
+    float4 c[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) c[u] = *reinterpret_cast<const float4*>(xc + dd + 4 * u);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const float4 wv = *reinterpret_cast<const float4*>(w + dd + 4 * u);
+      s = __builtin_fmaf(c[u].x, wv.x, s);
+      s = __builtin_fmaf(c[u].y, wv.y, s);
+      s = __builtin_fmaf(c[u].z, wv.z, s);
+      s = __builtin_fmaf(c[u].w, wv.w, s);
+    }
+  }
+  for (; dd < d; ++dd) s = __builtin_fmaf(xc[dd], w[dd], s);
+  return !(s < 0.0f);
+}
+
 // Side of a point decided from its FP16 sketch, or 0 when the sketch cannot
 // decide.  The projection of the sketch, s' = sum xh_d * w_d with the exact
 // w_d = x_p2,d - x_p1,d of the chain, differs from the exact FP32 chain s by
@@ -257,6 +280,78 @@ __device__ __forceinline__ int sketch_side_quad_w(int ld, const _Float16* __rest
   return 0;
 }
 
+// The same decision from the int8 sketch x = s q + r (||r|| <= e8): with the
+// staged w, t = q . w as an FP32 chain and s' = s t, the exact chain s
+// differs from s' by at most ||w|| (e8 + kap (||x|| + s ||q||)) + 2^-22 |s'|
+// (the exact chain's and t's rounding, ||q|| <= 127 sqrt(d), the product's
+// rounding), with ||x|| bounded by xn for every point.  One point's quad
+// reads its 16-byte pieces of the int8 row.  w is staged over ld8 values,
+// zero past ld.
+__device__ __forceinline__ int int8_side_quad_w(int d, const int8_t* __restrict__ q8, int ld8, int32_t id, float2 se,
+                                                double xn, const float* w, float ww, const SketchBound& sb, int q) {
+  const int8_t* qr = q8 + static_cast<int64_t>(id) * ld8;
+  float t = 0.0f;
+  const int pieces = ld8 / 16;
+  for (int pc = q; pc < pieces; pc += 4) {
+    const int dd = 16 * pc;
+    const int4 v = *reinterpret_cast<const int4*>(qr + dd);
+    const int32_t vw[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float4 wv = *reinterpret_cast<const float4*>(w + dd + 4 * u);
+      t = __builtin_fmaf(static_cast<float>(static_cast<int8_t>(vw[u] & 0xff)), wv.x, t);
+      t = __builtin_fmaf(static_cast<float>(static_cast<int8_t>((vw[u] >> 8) & 0xff)), wv.y, t);
+      t = __builtin_fmaf(static_cast<float>(static_cast<int8_t>((vw[u] >> 16) & 0xff)), wv.z, t);
+      t = __builtin_fmaf(static_cast<float>(static_cast<int8_t>((vw[u] >> 24) & 0xff)), wv.w, t);
+    }
+  }
+  t += __shfl_xor(t, 1);
+  t += __shfl_xor(t, 2);
+  const float sp = se.x * t;
+  const double wn = sqrt(static_cast<double>(ww)) * (1.0 + sb.kap);
+  const double qn = 127.0 * sqrt(static_cast<double>(d));
+  const double bound = wn * (static_cast<double>(se.y) + sb.kap * (xn + static_cast<double>(se.x) * qn)) +
+                       fabs(static_cast<double>(sp)) * 2.384185791015625e-07 + 2.0 * sb.tiny;
+  if (static_cast<double>(sp) > bound) return 1;
+  if (static_cast<double>(sp) < -bound) return -1;
+  return 0;
+}
+
+// One 16-byte piece (16 int8 values) of q against the staged w: t += q . w.
+__device__ __forceinline__ float int8_piece_dot(int4 v, const float* w, float t) {
+  const int32_t vw[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const float4 wv = *reinterpret_cast<const float4*>(w + 4 * u);
+    t = __builtin_fmaf(static_cast<float>(static_cast<int8_t>(vw[u] & 0xff)), wv.x, t);
+    t = __builtin_fmaf(static_cast<float>(static_cast<int8_t>((vw[u] >> 8) & 0xff)), wv.y, t);
+    t = __builtin_fmaf(static_cast<float>(static_cast<int8_t>((vw[u] >> 16) & 0xff)), wv.z, t);
+    t = __builtin_fmaf(static_cast<float>(static_cast<int8_t>((vw[u] >> 24) & 0xff)), wv.w, t);
+  }
+  return t;
+}
+
+// int8_side_quad_w with the lane's pieces q and q + 4 already loaded (rows
+// of at most 8 pieces); every lane of the quad takes part in the shuffles,
+// active or not.
+__device__ __forceinline__ int int8_side_pre(int d, int pieces, bool active, int4 qa, int4 qb, float2 se, double xn,
+                                             const float* w, float ww, const SketchBound& sb, int q) {
+  float t = 0.0f;
+  if (active && q < pieces) t = int8_piece_dot(qa, w + 16 * q, t);
+  if (active && q + 4 < pieces) t = int8_piece_dot(qb, w + 16 * (q + 4), t);
+  t += __shfl_xor(t, 1);
+  t += __shfl_xor(t, 2);
+  if (!active) return 0;
+  const float sp = se.x * t;
+  const double wn = sqrt(static_cast<double>(ww)) * (1.0 + sb.kap);
+  const double qn = 127.0 * sqrt(static_cast<double>(d));
+  const double bound = wn * (static_cast<double>(se.y) + sb.kap * (xn + static_cast<double>(se.x) * qn)) +
+                       fabs(static_cast<double>(sp)) * 2.384185791015625e-07 + 2.0 * sb.tiny;
+  if (static_cast<double>(sp) > bound) return 1;
+  if (static_cast<double>(sp) < -bound) return -1;
+  return 0;
+}
+
 // Buckets of a 1024-point block whose pivot data is staged in LDS (an active
 // bucket holds more than L points, so a block spans few), and the widest row
 // staged.
@@ -267,8 +362,9 @@ constexpr int kSideStageLd = 512;
 // rows, a quad of lanes per point so the rows are read coalesced; the
 // decisions become the block's ballot words through LDS.  The first
 // kSideStage buckets of a block have their pivot difference, its norm and
-// their records staged in LDS, so a point reads only its own sketch row; the
-// points of further buckets read the pivot rows themselves.  The undecided
+// their records staged in LDS, so a point reads only its own sketch row (the
+// int8 one when given: half the bytes of the FP16 row); the points of further
+// buckets read the pivot rows themselves.  The undecided
 // ones of a 1024-point block are queued in LDS and take the exact chain
 // together, so a straggler does not hold a whole wave.
 __global__ __launch_bounds__(256) void forest_side_sketch_kernel(
@@ -277,7 +373,8 @@ __global__ __launch_bounds__(256) void forest_side_sketch_kernel(
     const int32_t* __restrict__ act_ids,
     const int32_t* __restrict__ act_bkt, const Bucket* __restrict__ bkt, const ForestCtl* ctl, int parity,
     uint64_t* __restrict__ rmask, int32_t* __restrict__ wpre, int32_t* __restrict__ blk_r,
-    unsigned long long* __restrict__ amb_count) {
+    unsigned long long* __restrict__ amb_count, const int8_t* __restrict__ q8, int ld8,
+    const float2* __restrict__ se8) {
   __shared__ uint64_t bits[kWordsPerBlock];
   __shared__ int wcnt[kWordsPerBlock];
   __shared__ int16_t amb[kSideBlock];
@@ -288,6 +385,7 @@ __global__ __launch_bounds__(256) void forest_side_sketch_kernel(
   __shared__ int32_t bst[kSideStage][3];  // astart, q1, q2
   __shared__ int32_t pst[kSideStage][2];  // pivot ids
   __shared__ int32_t b_first_s, nst_s;
+  __shared__ int32_t ids_s[kSideBlock], bk_s[kSideBlock];  // the block's points and their buckets
   const int nact = ctl->nact[parity];
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -298,6 +396,10 @@ __global__ __launch_bounds__(256) void forest_side_sketch_kernel(
   // cannot hold) falls back to each point's own values.
   const float emax = eps[n], nmax = nh[n];
   const bool per_point = !(emax < __builtin_inff());
+  // every point's ||x|| <= ||xh|| + eps (infinite when a row is out of range:
+  // then the int8 bound decides nothing and the exact chain runs)
+  const double xn = sqrt(static_cast<double>(nmax)) * (1.0 + sb.kap) + static_cast<double>(emax);
+  const int wl = q8 != nullptr && ld8 > ld ? ld8 : ld;  // staged row width
   for (int64_t blk = blockIdx.x; blk * kSideBlock < nact; blk += gridDim.x) {
     if (threadIdx.x == 0) {
       namb = 0;
@@ -307,7 +409,14 @@ __global__ __launch_bounds__(256) void forest_side_sketch_kernel(
       const int64_t m1 = (m0 + kSideBlock < nact ? m0 + kSideBlock : nact) - 1;
       const int32_t bf = act_bkt[m0], bl = act_bkt[m1];
       b_first_s = bf;
-      nst_s = ld <= kSideStageLd ? (bl - bf + 1 < kSideStage ? bl - bf + 1 : kSideStage) : 0;
+      nst_s = wl <= kSideStageLd ? (bl - bf + 1 < kSideStage ? bl - bf + 1 : kSideStage) : 0;
+    }
+    // the block's points and buckets, read coalesced once
+    for (int u = 0; u < kSideBlock / 256; ++u) {
+      const int ml = u * 256 + threadIdx.x;
+      const int64_t m = blk * kSideBlock + ml;
+      ids_s[ml] = m < nact ? act_ids[m] : -1;
+      bk_s[ml] = m < nact ? act_bkt[m] : -1;
     }
     __syncthreads();
     const int32_t b_first = b_first_s;
@@ -321,9 +430,10 @@ __global__ __launch_bounds__(256) void forest_side_sketch_kernel(
       pst[threadIdx.x][1] = act_ids[b.astart + b.q2];
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < nst * ld; e += 256) {
-      const int j = e / ld, dd = e - j * ld;
-      wst[j][dd] = x[static_cast<int64_t>(pst[j][1]) * ld + dd] - x[static_cast<int64_t>(pst[j][0]) * ld + dd];
+    for (int e = threadIdx.x; e < nst * wl; e += 256) {
+      const int j = e / wl, dd = e - j * wl;
+      wst[j][dd] = dd < ld ? x[static_cast<int64_t>(pst[j][1]) * ld + dd] - x[static_cast<int64_t>(pst[j][0]) * ld + dd]
+                           : 0.0f;
     }
     __syncthreads();
     for (int j = wave; j < nst; j += 4) {
@@ -334,40 +444,92 @@ __global__ __launch_bounds__(256) void forest_side_sketch_kernel(
       if (lane == 0) wwst[j] = acc;
     }
     __syncthreads();
-    // one point per quad at a time (64 quads, 16 points each)
-    for (int t = 0; t < kSideBlock / 64; ++t) {
-      const int ml = t * 64 + quad;
+    // one point per quad at a time (64 quads, 16 points each).  A point's
+    // class: 0 LEFT without a test (past the list, or p1), 1 RIGHT (p2), 2
+    // a staged bucket's test, 3 another bucket's test (pivot rows read).
+    auto classify = [&](int ml, int32_t& id, int& j) {
       const int64_t m = blk * kSideBlock + ml;
-      int r = 0;
-      if (m < nact) {
-        const int32_t bi = act_bkt[m];
-        const int j = bi - b_first;
-        if (j >= 0 && j < nst) {
-          const int32_t rel = static_cast<int32_t>(m) - bst[j][0];
-          if (rel == bst[j][2]) {
-            r = 1;  // Leaf2 writes RIGHT for p2 last (.cu:2178-2179)
-          } else if (rel != bst[j][1]) {
-            const int32_t id = act_ids[m];
-            const int sd = sketch_side_quad_w(ld, xh, ldh, per_point ? eps[id] : emax, per_point ? nh[id] : nmax, id,
-                                              wst[j], wwst[j], sb, q);
-            r = sd > 0 ? 1 : (sd == 0 ? 2 : 0);
-          }
-        } else {
-          const Bucket& b = bkt[bi];
-          const int32_t rel = static_cast<int32_t>(m) - b.astart;
-          if (rel == b.q2) {
-            r = 1;
-          } else if (rel != b.q1) {
-            const int32_t id = act_ids[m];
-            const int sd = sketch_side_quad(x, ld, xh, ldh, per_point ? eps[id] : emax, per_point ? nh[id] : nmax, id,
-                                            act_ids[b.astart + b.q1], act_ids[b.astart + b.q2], sb, q);
-            r = sd > 0 ? 1 : (sd == 0 ? 2 : 0);
-          }
-        }
+      id = ids_s[ml];
+      j = -1;
+      if (m >= nact) return 0;
+      const int32_t bi = bk_s[ml];
+      const int jj = bi - b_first;
+      if (jj >= 0 && jj < nst) {
+        const int32_t rel = static_cast<int32_t>(m) - bst[jj][0];
+        if (rel == bst[jj][2]) return 1;  // Leaf2 writes RIGHT for p2 last (.cu:2178-2179)
+        if (rel == bst[jj][1]) return 0;
+        j = jj;
+        return 2;
       }
+      const Bucket& b = bkt[bi];
+      const int32_t rel = static_cast<int32_t>(m) - b.astart;
+      if (rel == b.q2) return 1;
+      if (rel == b.q1) return 0;
+      return 3;
+    };
+    auto record = [&](int ml, int r) {
       if (q == 0) {
         dec[ml] = static_cast<int8_t>(r);
         if (r == 2) amb[atomicAdd(&namb, 1)] = static_cast<int16_t>(ml);
+      }
+    };
+    auto slow = [&](int ml, int32_t id) {
+      const Bucket& b = bkt[bk_s[ml]];
+      return sketch_side_quad(x, ld, xh, ldh, per_point ? eps[id] : emax, per_point ? nh[id] : nmax, id,
+                              act_ids[b.astart + b.q1], act_ids[b.astart + b.q2], sb, q);
+    };
+    if (q8 != nullptr && ld8 <= 128) {
+      // int8 rows of at most 8 pieces: four points per quad in flight, their
+      // pieces requested before any is used (addresses clamped to row 0 for
+      // points without a staged test, whose loads are not used)
+      const int pieces = ld8 / 16;
+      for (int t0 = 0; t0 < kSideBlock / 64; t0 += 4) {
+        int cls[4], jj[4];
+        int32_t ids[4];
+        int4 qa[4], qb[4];
+        float2 sev[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int ml = (t0 + u) * 64 + quad;
+          cls[u] = classify(ml, ids[u], jj[u]);
+          const int64_t row = static_cast<int64_t>(cls[u] == 2 ? ids[u] : 0) * ld8;
+          qa[u] = *reinterpret_cast<const int4*>(q8 + row + 16 * (q < pieces ? q : 0));
+          qb[u] = *reinterpret_cast<const int4*>(q8 + row + 16 * (q + 4 < pieces ? q + 4 : 0));
+          sev[u] = se8[cls[u] == 2 ? ids[u] : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int ml = (t0 + u) * 64 + quad;
+          const bool act = cls[u] == 2;
+          const int j = act ? jj[u] : 0;
+          const int sd = int8_side_pre(d, pieces, act, qa[u], qb[u], sev[u], xn, wst[j], wwst[j], sb, q);
+          int r = cls[u] == 1 ? 1 : 0;
+          if (act) r = sd > 0 ? 1 : (sd == 0 ? 2 : 0);
+          if (cls[u] == 3) {
+            const int s3 = slow(ml, ids[u]);
+            r = s3 > 0 ? 1 : (s3 == 0 ? 2 : 0);
+          }
+          record(ml, r);
+        }
+      }
+    } else {
+      for (int t = 0; t < kSideBlock / 64; ++t) {
+        const int ml = t * 64 + quad;
+        int32_t id;
+        int j;
+        const int c = classify(ml, id, j);
+        int r = c == 1 ? 1 : 0;
+        if (c == 2) {
+          const int sd = q8 != nullptr
+                             ? int8_side_quad_w(d, q8, ld8, id, se8[id], xn, wst[j], wwst[j], sb, q)
+                             : sketch_side_quad_w(ld, xh, ldh, per_point ? eps[id] : emax, per_point ? nh[id] : nmax,
+                                                  id, wst[j], wwst[j], sb, q);
+          r = sd > 0 ? 1 : (sd == 0 ? 2 : 0);
+        } else if (c == 3) {
+          const int sd = slow(ml, id);
+          r = sd > 0 ? 1 : (sd == 0 ? 2 : 0);
+        }
+        record(ml, r);
       }
     }
     __syncthreads();
@@ -379,10 +541,15 @@ __global__ __launch_bounds__(256) void forest_side_sketch_kernel(
     const int na = namb;
     for (int t = threadIdx.x; t < na; t += 256) {
       const int ml = amb[t];
-      const int64_t m = blk * kSideBlock + ml;
-      const Bucket& b = bkt[act_bkt[m]];
-      if (side_is_right<0>(x, d, ld, act_ids[m], act_ids[b.astart + b.q1], act_ids[b.astart + b.q2]))
-        atomicOr(reinterpret_cast<unsigned long long*>(&bits[ml >> 6]), 1ull << (ml & 63));
+      const int j = bk_s[ml] - b_first;
+      bool right;
+      if (j >= 0 && j < nst) {
+        right = side_is_right_w(x + static_cast<int64_t>(ids_s[ml]) * ld, wst[j], d);
+      } else {
+        const Bucket& b = bkt[bk_s[ml]];
+        right = side_is_right<0>(x, d, ld, ids_s[ml], act_ids[b.astart + b.q1], act_ids[b.astart + b.q2]);
+      }
+      if (right) atomicOr(reinterpret_cast<unsigned long long*>(&bits[ml >> 6]), 1ull << (ml & 63));
     }
     __syncthreads();
     if (threadIdx.x < kWordsPerBlock) {
@@ -1104,9 +1271,11 @@ void launch_forest_level(const ForestBuffers& fb, const float* x, int d, int ld,
   if (fb.xh != nullptr) {
     if (fb.eps == nullptr || fb.nh == nullptr || fb.ldh % 8 != 0 || fb.ldh < ld || ld < d || ld % 4 != 0)
       throw HipError("forest: malformed sketch");
+    if (fb.q8 != nullptr && (fb.se8 == nullptr || fb.ld8 % 16 != 0 || fb.ld8 < d))
+      throw HipError("forest: malformed int8 sketch");
     hipLaunchKernelGGL(forest_side_sketch_kernel, sg, dim3(256), 0, s, x, d, ld, fb.xh, fb.ldh, fb.eps, fb.nh,
                        fb.n, sketch_bound(d), fb.act_ids[par], fb.act_bkt[par], fb.bkt[par], fb.ctl, par, fb.rmask,
-                       fb.wpre, fb.blk_r, fb.amb_count);
+                       fb.wpre, fb.blk_r, fb.amb_count, fb.q8, fb.ld8, fb.se8);
   } else switch (row_width_class(d)) {
     case 8:
       hipLaunchKernelGGL(forest_side_kernel<8>, sg, dim3(256), 0, s, x, d, ld, fb.act_ids[par], fb.act_bkt[par],

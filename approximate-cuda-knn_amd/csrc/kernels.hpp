@@ -60,6 +60,11 @@ struct ForestBuffers {
   const float* eps = nullptr;
   const float* nh = nullptr;
   unsigned long long* amb_count = nullptr;
+  // int8 sketch for the side test of buckets staged in LDS (nullable: the
+  // FP16 sketch decides): rows [n][ld8], (scale, radius) pairs
+  const int8_t* q8 = nullptr;
+  int ld8 = 0;
+  const float2* se8 = nullptr;
 };
 void launch_forest_init(const ForestBuffers& fb, int32_t* perm, Leaf* leaves, int64_t n, int32_t q1, int32_t q2,
                         int leaf_eff, int kp1, hipStream_t s);
@@ -220,9 +225,10 @@ void launch_max_nonneg(const float* v, int64_t n, float* out, hipStream_t s);
 // ---- sketches of the normalized points ---------------------------------------
 // (the FP16 sketch: launch_normalize's SketchOut)
 // Int8 sketch: q8 [n][ld8] (ld8 % 16 == 0, ld8 >= d), per-row scale s8,
-// a8 = sum q^2, e8 = upper bound of ||x - s8 q8||_2.
+// a8 = sum q^2, e8 = upper bound of ||x - s8 q8||_2; se (nullable): the
+// pairs (s8, e8) side by side, one 8-byte load per point.
 void launch_sketch8(const float* x, int64_t n, int d, int ld, int8_t* q8, int ld8, float* s8, int32_t* a8, float* e8,
-                    hipStream_t s);
+                    float2* se, hipStream_t s);
 
 // Initial leaf k-NN for one tree, EUCLIDEAN or COSINE with carried distances,
 // with the candidate pairs screened by half-precision MFMA dot products of the

@@ -21,7 +21,8 @@ constexpr int kSketchWaves = 4;
 // integers (v_dot4_i32_i8), so a distance bound needs only the two radii.
 __global__ __launch_bounds__(256) void sketch8_kernel(const float* __restrict__ x, int64_t n, int d, int ld,
                                                       int8_t* __restrict__ q8, int ld8, float* __restrict__ s8,
-                                                      int32_t* __restrict__ a8, float* __restrict__ e8) {
+                                                      int32_t* __restrict__ a8, float* __restrict__ e8,
+                                                      float2* __restrict__ se) {
   const int lane = threadIdx.x & 63;
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kSketchWaves + (threadIdx.x >> 6);
   if (i >= n) return;
@@ -64,17 +65,18 @@ __global__ __launch_bounds__(256) void sketch8_kernel(const float* __restrict__ 
     s8[i] = sc;
     a8[i] = sq;
     e8[i] = bad ? __builtin_inff() : f;
+    if (se != nullptr) se[i] = make_float2(sc, bad ? __builtin_inff() : f);
   }
 }
 
 }  // namespace
 
 void launch_sketch8(const float* x, int64_t n, int d, int ld, int8_t* q8, int ld8, float* s8, int32_t* a8, float* e8,
-                    hipStream_t s) {
+                    float2* se, hipStream_t s) {
   if (ld8 % 16 != 0 || ld8 < d) throw HipError("sketch8: row stride must be a multiple of 16 covering d");
   const int64_t blocks = (n + kSketchWaves - 1) / kSketchWaves;
   hipLaunchKernelGGL(sketch8_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, x, n, d, ld, q8, ld8, s8,
-                     a8, e8);
+                     a8, e8, se);
   AKNN_HIP_TRY(hipGetLastError());
 }
 

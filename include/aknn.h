@@ -111,6 +111,11 @@ int aknn_deinitialize(aknn_handle* h);
  *                   int8 copy (per-point scale, exact integer dot products,
  *                   half the bytes) instead of the FP16 one; 0 (default): FP16
  *                   (its tighter bound re-scores fewer candidates)
+ *   "forest_int8"   1 (default): with "sketch", the forest side test of the
+ *                   buckets a block stages (D <= 512) reads each point's int8
+ *                   row (per-point scale and radius, a proven bound; the exact
+ *                   chain decides the rest) instead of its FP16 row, half the
+ *                   bytes (the trees are the same); 0: the FP16 row
  *   "leaf_mfma"     1: EUCLIDEAN leaf k-NN of every tree after the first
  *                   screens its pairs with FP16 dot products on the matrix
  *                   cores and proven lower bounds, computing exact FP32 keys

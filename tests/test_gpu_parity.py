@@ -372,6 +372,30 @@ def test_forest_sketch_same_trees(aknn, oracle, datasets, d, metric):
     check_forest(oracle, g, xn, 3, 20, 8)
 
 
+@pytest.mark.parametrize("kind,d,leaf", [("gmm", 128, 60), ("uniform", 128, 256), ("gmm", 33, 16),
+                                          ("gmm", 300, 100), ("adversarial", 300, 40)])
+def test_forest_int8_same_trees(aknn, oracle, datasets, kind, d, leaf):
+    """The int8 side screen of staged buckets (option forest_int8, the
+    default) against the FP16 one and the oracle: the same trees.  The int8
+    bound leaves more points to the exact chain."""
+    if kind == "gmm":
+        x = datasets.gmm_c1(4000, d, clusters=6, seed=70 + d)[0]
+    elif kind == "uniform":
+        x = datasets.uniform_c2(4000, d, seed=71)
+    else:
+        x = _adversarial(n=3000, d=d)
+    counts = []
+    for f8 in (1, 0):
+        g = gpu_run(aknn, x, 10, 3, leaf, 0, 0, sketch=1, forest_int8=f8)
+        ex, rs = g.candidate_counts("forest")
+        print(f"[gpu] forest_int8={f8}: side tests {ex:.0f} exact {rs:.0f}", flush=True)
+        counts.append((ex, rs))
+        xn = g.normalized()
+        check_forest(oracle, g, xn, 3, leaf, 10)
+        g.close()
+    assert counts[0][0] == counts[1][0] and counts[0][1] >= counts[1][1] and counts[0][1] < counts[0][0]
+
+
 @pytest.mark.parametrize("int8", [0, 1])
 def test_sketch_precisions_same_graph(aknn, oracle, datasets, int8):
     """EUCLIDEAN propagation screened by the int8 or the FP16 sketch: the plain graph and the oracle's."""
