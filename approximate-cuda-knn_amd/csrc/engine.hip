@@ -289,6 +289,8 @@ void Engine::set_data(const float* x_rowmajor, int64_t n, int d, const Params& p
   fb_.blk_r = alloc<int32_t>(blocks);
   fb_.blk_roff = alloc<int32_t>(blocks);
   fb_.bchild = alloc<int32_t>(fb_.max_buckets);
+  fb_.piv[0] = alloc<int32_t>(2 * fb_.max_buckets);
+  fb_.piv[1] = alloc<int32_t>(2 * fb_.max_buckets);
   fb_.bact = alloc<int32_t>(fb_.max_buckets);
   fb_.ctl = alloc<ForestCtl>(1);
   ncand_ = alloc<unsigned long long>(static_cast<size_t>(kCountWords) * std::max(p.props, 1));

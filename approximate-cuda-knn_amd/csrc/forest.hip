@@ -99,6 +99,10 @@ __global__ __launch_bounds__(256) void forest_init_kernel(ForestBuffers fb, int3
       b.rstart = 0;
       b.rcount = static_cast<int32_t>(n);
       fb.bkt[0][0] = b;
+      if (fb.piv[0] != nullptr) {  // the root's active list is the identity
+        fb.piv[0][0] = q1;
+        fb.piv[0][1] = q2;
+      }
     } else {
       leaves[0] = Leaf{0, static_cast<int32_t>(n), 0, static_cast<int32_t>(n)};
       c.nleaves = 1;
@@ -196,7 +200,7 @@ __device__ __forceinline__ int sketch_side_quad(const float* __restrict__ x, int
   const float* xb = x + static_cast<int64_t>(p2) * ld;
   float sp = 0.0f, ww = 0.0f;
   const int pieces = ld / 8;
-#pragma unroll 4
+#pragma unroll 2
   for (int pc = q; pc < pieces; pc += 4) {
     const int dd = 8 * pc;
     const uint4 h = *reinterpret_cast<const uint4*>(hc + dd);
@@ -367,14 +371,15 @@ constexpr int kSideStageLd = 512;
 // buckets read the pivot rows themselves.  The undecided
 // ones of a 1024-point block are queued in LDS and take the exact chain
 // together, so a straggler does not hold a whole wave.
-__global__ __launch_bounds__(256) void forest_side_sketch_kernel(
+// four waves per SIMD: a level's ~n/1024 blocks fit the chip at once
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void forest_side_sketch_kernel(
     const float* __restrict__ x, int d, int ld, const _Float16* __restrict__ xh, int ldh,
     const float* __restrict__ eps, const float* __restrict__ nh, int64_t n, SketchBound sb,
     const int32_t* __restrict__ act_ids,
     const int32_t* __restrict__ act_bkt, const Bucket* __restrict__ bkt, const ForestCtl* ctl, int parity,
     uint64_t* __restrict__ rmask, int32_t* __restrict__ wpre, int32_t* __restrict__ blk_r,
     unsigned long long* __restrict__ amb_count, const int8_t* __restrict__ q8, int ld8,
-    const float2* __restrict__ se8) {
+    const float2* __restrict__ se8, const int32_t* __restrict__ piv) {
   __shared__ uint64_t bits[kWordsPerBlock];
   __shared__ int wcnt[kWordsPerBlock];
   __shared__ int16_t amb[kSideBlock];
@@ -384,7 +389,6 @@ __global__ __launch_bounds__(256) void forest_side_sketch_kernel(
   __shared__ float wwst[kSideStage];
   __shared__ int32_t bst[kSideStage][3];  // astart, q1, q2
   __shared__ int32_t pst[kSideStage][2];  // pivot ids
-  __shared__ int32_t b_first_s, nst_s;
   __shared__ int32_t ids_s[kSideBlock], bk_s[kSideBlock];  // the block's points and their buckets
   const int nact = ctl->nact[parity];
   const int wave = threadIdx.x >> 6;
@@ -401,16 +405,7 @@ __global__ __launch_bounds__(256) void forest_side_sketch_kernel(
   const double xn = sqrt(static_cast<double>(nmax)) * (1.0 + sb.kap) + static_cast<double>(emax);
   const int wl = q8 != nullptr && ld8 > ld ? ld8 : ld;  // staged row width
   for (int64_t blk = blockIdx.x; blk * kSideBlock < nact; blk += gridDim.x) {
-    if (threadIdx.x == 0) {
-      namb = 0;
-      // the active list runs bucket by bucket: the block's buckets are a
-      // contiguous range of bucket indices
-      const int64_t m0 = blk * kSideBlock;
-      const int64_t m1 = (m0 + kSideBlock < nact ? m0 + kSideBlock : nact) - 1;
-      const int32_t bf = act_bkt[m0], bl = act_bkt[m1];
-      b_first_s = bf;
-      nst_s = wl <= kSideStageLd ? (bl - bf + 1 < kSideStage ? bl - bf + 1 : kSideStage) : 0;
-    }
+    if (threadIdx.x == 0) namb = 0;
     // the block's points and buckets, read coalesced once
     for (int u = 0; u < kSideBlock / 256; ++u) {
       const int ml = u * 256 + threadIdx.x;
@@ -419,15 +414,19 @@ __global__ __launch_bounds__(256) void forest_side_sketch_kernel(
       bk_s[ml] = m < nact ? act_bkt[m] : -1;
     }
     __syncthreads();
-    const int32_t b_first = b_first_s;
-    const int nst = nst_s;
+    // the active list runs bucket by bucket: the block's buckets are a
+    // contiguous range of bucket indices
+    const int cnt_blk = nact - blk * kSideBlock < kSideBlock ? static_cast<int>(nact - blk * kSideBlock) : kSideBlock;
+    const int32_t b_first = bk_s[0], b_last = bk_s[cnt_blk - 1];
+    const int nst = wl <= kSideStageLd ? (b_last - b_first + 1 < kSideStage ? b_last - b_first + 1 : kSideStage) : 0;
     if (threadIdx.x < nst) {
-      const Bucket& b = bkt[b_first + threadIdx.x];
+      const int32_t bi = b_first + threadIdx.x;
+      const Bucket& b = bkt[bi];
       bst[threadIdx.x][0] = b.astart;
       bst[threadIdx.x][1] = b.q1;
       bst[threadIdx.x][2] = b.q2;
-      pst[threadIdx.x][0] = act_ids[b.astart + b.q1];
-      pst[threadIdx.x][1] = act_ids[b.astart + b.q2];
+      pst[threadIdx.x][0] = piv != nullptr ? piv[2 * bi] : act_ids[b.astart + b.q1];
+      pst[threadIdx.x][1] = piv != nullptr ? piv[2 * bi + 1] : act_ids[b.astart + b.q2];
     }
     __syncthreads();
     for (int e = threadIdx.x; e < nst * wl; e += 256) {
@@ -505,11 +504,15 @@ __global__ __launch_bounds__(256) void forest_side_sketch_kernel(
           const int sd = int8_side_pre(d, pieces, act, qa[u], qb[u], sev[u], xn, wst[j], wwst[j], sb, q);
           int r = cls[u] == 1 ? 1 : 0;
           if (act) r = sd > 0 ? 1 : (sd == 0 ? 2 : 0);
-          if (cls[u] == 3) {
-            const int s3 = slow(ml, ids[u]);
-            r = s3 > 0 ? 1 : (s3 == 0 ? 2 : 0);
-          }
-          record(ml, r);
+          if (cls[u] != 3) record(ml, r);
+        }
+        // points of unstaged buckets (rare): the pivot rows from global memory
+#pragma unroll 1
+        for (int u = 0; u < 4; ++u) {
+          if (cls[u] != 3) continue;
+          const int ml = (t0 + u) * 64 + quad;
+          const int s3 = slow(ml, ids[u]);
+          record(ml, s3 > 0 ? 1 : (s3 == 0 ? 2 : 0));
         }
       }
     } else {
@@ -900,7 +903,7 @@ __global__ __launch_bounds__(256) void forest_scatter_kernel(const int32_t* __re
                                                              int leaf_eff, int kp1, int32_t* __restrict__ act_ids_n,
                                                              int32_t* __restrict__ act_bkt_n,
                                                              Bucket* __restrict__ bnext, int32_t* __restrict__ perm,
-                                                             int64_t perm_stride) {
+                                                             int64_t perm_stride, int32_t* __restrict__ piv_n) {
   const int nact = ctl->nact[parity];
   for (int64_t m = blockIdx.x * 256ll + threadIdx.x; m < nact; m += gridDim.x * 256ll) {
     const int32_t id = act_ids[m];
@@ -918,6 +921,15 @@ __global__ __launch_bounds__(256) void forest_scatter_kernel(const int32_t* __re
       const int32_t cast = cast0 + (off ? sz[0] : 0);
       act_ids_n[cast + rank] = id;
       act_bkt_n[cast + rank] = cidx0 + off;
+      if (piv_n != nullptr) {
+        // the child's pivots (the same draws as its record's q1, q2 below)
+        const int32_t size = sz[bit];
+        const int32_t q1 = v1 % size;
+        int32_t q2 = v2 % size;
+        if (q1 == q2) q2 = (q1 + 1) % size;
+        if (rank == q1) piv_n[2 * (cidx0 + off)] = id;
+        if (rank == q2) piv_n[2 * (cidx0 + off) + 1] = id;
+      }
     } else {
       perm[b.tree * perm_stride + b.fstart + (bit ? sz[0] : 0) + rank] = id;
     }
@@ -1250,7 +1262,7 @@ void level_tail(const ForestBuffers& fb, int par, int level, int32_t v1, int32_t
   hipLaunchKernelGGL(forest_scatter_kernel, dim3(clamp_grid((fb.n + 255) / 256, 8192)), dim3(256), 0, s,
                      fb.act_ids[par], fb.act_bkt[par], fb.bkt[par], fb.bchild, fb.bact, fb.rmask, fb.wpre, fb.blk_roff,
                      fb.ctl, par, level, v1, v2, leaf_eff, kp1, fb.act_ids[par ^ 1], fb.act_bkt[par ^ 1],
-                     fb.bkt[par ^ 1], perm, sink.stride);
+                     fb.bkt[par ^ 1], perm, sink.stride, fb.piv[par ^ 1]);
   AKNN_HIP_TRY(hipGetLastError());
 }
 
@@ -1275,7 +1287,7 @@ void launch_forest_level(const ForestBuffers& fb, const float* x, int d, int ld,
       throw HipError("forest: malformed int8 sketch");
     hipLaunchKernelGGL(forest_side_sketch_kernel, sg, dim3(256), 0, s, x, d, ld, fb.xh, fb.ldh, fb.eps, fb.nh,
                        fb.n, sketch_bound(d), fb.act_ids[par], fb.act_bkt[par], fb.bkt[par], fb.ctl, par, fb.rmask,
-                       fb.wpre, fb.blk_r, fb.amb_count, fb.q8, fb.ld8, fb.se8);
+                       fb.wpre, fb.blk_r, fb.amb_count, fb.q8, fb.ld8, fb.se8, fb.piv[par]);
   } else switch (row_width_class(d)) {
     case 8:
       hipLaunchKernelGGL(forest_side_kernel<8>, sg, dim3(256), 0, s, x, d, ld, fb.act_ids[par], fb.act_bkt[par],

@@ -49,6 +49,9 @@ struct ForestBuffers {
   // first child; active elements of the children, then their first offset
   int32_t* bchild;
   int32_t* bact;
+  // per bucket of a level (even/odd): the ids of its two pivots, written by
+  // the elements that land on them (nullable: the side test looks them up)
+  int32_t* piv[2] = {nullptr, nullptr};
   ForestCtl* ctl;
   int64_t n;
   int64_t max_buckets;
