@@ -109,6 +109,8 @@ Engine::Engine(int device) : device_(device) {
   }
   AKNN_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&ctl_host_), sizeof(ForestCtl), hipHostMallocDefault));
   AKNN_HIP_TRY(hipEventCreate(&pca_ev_));
+  AKNN_HIP_TRY(hipEventCreate(&router_ev_));
+  AKNN_HIP_TRY(hipStreamCreateWithFlags(&aux_, hipStreamNonBlocking));
 }
 
 Engine::~Engine() {
@@ -119,6 +121,11 @@ Engine::~Engine() {
   for (auto e : events_) (void)hipEventDestroy(e);
   for (auto e : launch_events_) (void)hipEventDestroy(e);
   if (pca_ev_) (void)hipEventDestroy(pca_ev_);
+  if (router_ev_) (void)hipEventDestroy(router_ev_);
+  if (aux_) {
+    (void)hipStreamSynchronize(aux_);
+    (void)hipStreamDestroy(aux_);
+  }
   if (ctl_host_) (void)hipHostFree(ctl_host_);
   for (Pinned& b : pinned_)
     if (b.p) (void)hipHostFree(b.p);
@@ -378,22 +385,14 @@ void Engine::build_forest_median() {
     dirs_key_ = key;
   }
   int nbins_sets = bins_set_ ? 1 : 0;
-  if (!routers_.empty() && p_.router_trees > 0) {
-    // learned leaf assignments of this data set (inside the timed bracket),
-    // one bin array per router
-    const int nr = static_cast<int>(routers_.size());
-    for (Router* r : routers_) {
-      if (r->device() != device_ || r->d() != d_) throw std::invalid_argument("router does not fit the data");
-      if (r->bin_bits() != routers_[0]->bin_bits()) throw std::invalid_argument("routers must share bin_bits");
-    }
-    if (!bins_ || bins_cap_ < nr) {
-      bins_ = alloc<int32_t>(static_cast<size_t>(n_) * nr);
-      bins_cap_ = nr;
-    }
-    for (int i = 0; i < nr; ++i) routers_[i]->assign(x_raw_, n_, ld_, bins_ + static_cast<int64_t>(i) * n_, stream_);
+  if (routers_launched_) {
+    // the learned leaf assignments of this data set, one bin array per
+    // router, made on the side stream since the build began
+    AKNN_HIP_TRY(hipStreamWaitEvent(stream_, router_ev_, 0));
+    routers_launched_ = false;
     bin_bits_ = routers_[0]->bin_bits();
     bins_set_ = true;
-    nbins_sets = nr;
+    nbins_sets = static_cast<int>(routers_.size());
   }
   const int routed_trees = bins_set_ ? std::min(p_.router_trees, p_.trees) : 0;
   // Trees are independent here (no shared draw stream), so a batch of them is
@@ -1153,6 +1152,25 @@ void Engine::calculate() {
   fb_.ld8 = ld8_;
   fb_.se8 = forest8 ? se8_ : nullptr;
   AKNN_HIP_TRY(hipEventRecord(ev(0), stream_));
+  // Router inference reads only the uploaded points: it runs on the side
+  // stream from the start of the build (inside the timed bracket), beside
+  // normalization, and the median forest waits for its bins.
+  routers_launched_ = false;
+  if (p_.split == 1 && !routers_.empty() && p_.router_trees > 0) {
+    const int nr = static_cast<int>(routers_.size());
+    for (Router* r : routers_) {
+      if (r->device() != device_ || r->d() != d_) throw std::invalid_argument("router does not fit the data");
+      if (r->bin_bits() != routers_[0]->bin_bits()) throw std::invalid_argument("routers must share bin_bits");
+    }
+    if (!bins_ || bins_cap_ < nr) {
+      bins_ = alloc<int32_t>(static_cast<size_t>(n_) * nr);
+      bins_cap_ = nr;
+    }
+    AKNN_HIP_TRY(hipStreamWaitEvent(aux_, ev(0), 0));  // after the previous build's readers of the bins
+    for (int i = 0; i < nr; ++i) routers_[i]->assign(x_raw_, n_, ld_, bins_ + static_cast<int64_t>(i) * n_, aux_);
+    AKNN_HIP_TRY(hipEventRecord(router_ev_, aux_));
+    routers_launched_ = true;
+  }
   if (p_.mean_fp64 && !mean64_) mean64_ = alloc<double>(static_cast<size_t>((n_ + 2047) / 2048) * d_);
   // the uploaded points stay as they are: normalization writes x_ from them
   // (and the FP16 sketch in the same row pass)

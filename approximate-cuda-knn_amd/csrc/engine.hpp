@@ -279,6 +279,9 @@ class Engine {
   float* pca_e_ = nullptr;
   PcaBasis pca_;
   hipEvent_t pca_ev_ = nullptr;  // the sample moments have reached the host
+  hipStream_t aux_ = nullptr;     // side stream (default priority): router inference beside normalization
+  hipEvent_t router_ev_ = nullptr;
+  bool routers_launched_ = false;  // this build's router bins are being made on aux_
   hipGraphExec_t round_exec_ = nullptr;
   int round_graph_mode_ = -1;
   double round_graph_sigma_ = 0.0;
