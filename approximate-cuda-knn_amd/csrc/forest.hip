@@ -357,10 +357,10 @@ __device__ __forceinline__ int int8_side_pre(int d, int pieces, bool active, int
 }
 
 // Buckets of a 1024-point block whose pivot data is staged in LDS (an active
-// bucket holds more than L points, so a block spans few), and the widest row
-// staged.
+// bucket holds more than L points, so a block spans few), within a budget of
+// staged values (rows wider than it stage nothing).
 constexpr int kSideStage = 8;
-constexpr int kSideStageLd = 512;
+constexpr int kSideStageFloats = 4096;  // 16 KB: up to 8 rows of 512 values, fewer of wider rows
 
 // Side kernel with the sketch: one pass decides most points from their half
 // rows, a quad of lanes per point so the rows are read coalesced; the
@@ -385,7 +385,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
   __shared__ int16_t amb[kSideBlock];
   __shared__ int8_t dec[kSideBlock];  // 0 LEFT, 1 RIGHT, 2 undecided
   __shared__ int namb;
-  __shared__ __attribute__((aligned(16))) float wst[kSideStage][kSideStageLd];
+  __shared__ __attribute__((aligned(16))) float wst[kSideStageFloats];  // row j at j * wl
   __shared__ float wwst[kSideStage];
   __shared__ int32_t bst[kSideStage][3];  // astart, q1, q2
   __shared__ int32_t pst[kSideStage][2];  // pivot ids
@@ -418,7 +418,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     // contiguous range of bucket indices
     const int cnt_blk = nact - blk * kSideBlock < kSideBlock ? static_cast<int>(nact - blk * kSideBlock) : kSideBlock;
     const int32_t b_first = bk_s[0], b_last = bk_s[cnt_blk - 1];
-    const int nst = wl <= kSideStageLd ? (b_last - b_first + 1 < kSideStage ? b_last - b_first + 1 : kSideStage) : 0;
+    const int fit = kSideStageFloats / wl < kSideStage ? kSideStageFloats / wl : kSideStage;
+    const int nst = b_last - b_first + 1 < fit ? b_last - b_first + 1 : fit;
     if (threadIdx.x < nst) {
       const int32_t bi = b_first + threadIdx.x;
       const Bucket& b = bkt[bi];
@@ -431,14 +432,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     __syncthreads();
     for (int e = threadIdx.x; e < nst * wl; e += 256) {
       const int j = e / wl, dd = e - j * wl;
-      wst[j][dd] = dd < ld ? x[static_cast<int64_t>(pst[j][1]) * ld + dd] - x[static_cast<int64_t>(pst[j][0]) * ld + dd]
+      wst[j * wl + dd] = dd < ld ? x[static_cast<int64_t>(pst[j][1]) * ld + dd] - x[static_cast<int64_t>(pst[j][0]) * ld + dd]
                            : 0.0f;
     }
     __syncthreads();
     for (int j = wave; j < nst; j += 4) {
       // any order of the FP32 sum stays within the bound's rounding allowance
       float acc = 0.0f;
-      for (int dd = lane; dd < ld; dd += 64) acc = __builtin_fmaf(wst[j][dd], wst[j][dd], acc);
+      for (int dd = lane; dd < ld; dd += 64) acc = __builtin_fmaf(wst[j * wl + dd], wst[j * wl + dd], acc);
       for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
       if (lane == 0) wwst[j] = acc;
     }
@@ -501,7 +502,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
           const int ml = (t0 + u) * 64 + quad;
           const bool act = cls[u] == 2;
           const int j = act ? jj[u] : 0;
-          const int sd = int8_side_pre(d, pieces, act, qa[u], qb[u], sev[u], xn, wst[j], wwst[j], sb, q);
+          const int sd = int8_side_pre(d, pieces, act, qa[u], qb[u], sev[u], xn, wst + j * wl, wwst[j], sb, q);
           int r = cls[u] == 1 ? 1 : 0;
           if (act) r = sd > 0 ? 1 : (sd == 0 ? 2 : 0);
           if (cls[u] != 3) record(ml, r);
@@ -524,9 +525,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         int r = c == 1 ? 1 : 0;
         if (c == 2) {
           const int sd = q8 != nullptr
-                             ? int8_side_quad_w(d, q8, ld8, id, se8[id], xn, wst[j], wwst[j], sb, q)
+                             ? int8_side_quad_w(d, q8, ld8, id, se8[id], xn, wst + j * wl, wwst[j], sb, q)
                              : sketch_side_quad_w(ld, xh, ldh, per_point ? eps[id] : emax, per_point ? nh[id] : nmax,
-                                                  id, wst[j], wwst[j], sb, q);
+                                                  id, wst + j * wl, wwst[j], sb, q);
           r = sd > 0 ? 1 : (sd == 0 ? 2 : 0);
         } else if (c == 3) {
           const int sd = slow(ml, id);
@@ -547,7 +548,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
       const int j = bk_s[ml] - b_first;
       bool right;
       if (j >= 0 && j < nst) {
-        right = side_is_right_w(x + static_cast<int64_t>(ids_s[ml]) * ld, wst[j], d);
+        right = side_is_right_w(x + static_cast<int64_t>(ids_s[ml]) * ld, wst + j * wl, d);
       } else {
         const Bucket& b = bkt[bk_s[ml]];
         right = side_is_right<0>(x, d, ld, ids_s[ml], act_ids[b.astart + b.q1], act_ids[b.astart + b.q2]);
