@@ -218,6 +218,8 @@ int aknn_set_option(aknn_handle* h, const char* name, int64_t value) {
       h->params.sketch_int8 = value != 0;
     } else if (n == "forest_int8") {
       h->params.forest_int8 = value != 0;
+    } else if (n == "flip_int8") {
+      h->params.flip_int8 = value != 0;
     } else if (n == "split") {
       if (value != 0 && value != 1) return fail(h, AKNN_ERR_ARGUMENT, "split is 0 (sign) or 1 (median)");
       h->params.split = static_cast<int>(value);

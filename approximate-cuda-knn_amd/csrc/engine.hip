@@ -944,7 +944,9 @@ static PropagateArgs round_args(const Params& p, int kmetric, bool flip, bool sc
     a.xh = xh;
     a.ldh = ldh;
     a.eps = eps;
-    if (p.sketch_int8 && p.metric == kEuclidean) {
+    // the int8 sketch: EUCLIDEAN rounds with sketch_int8; COSINE parity
+    // rounds (the flip) with flip_int8 whenever the int8 rows were made
+    if ((p.sketch_int8 && p.metric == kEuclidean) || (flip && p.flip_int8 && q8 != nullptr)) {
       a.q8 = q8;
       a.ld8 = ld8;
       a.s8 = s8;
@@ -1051,7 +1053,7 @@ void Engine::propagate() {
     }
     round_screened_[r] = pca ? 2 : (screen ? 1 : 0);
     // buffers alternate each round: round r reads buffer cur_ (= c0 ^ (r & 1))
-    const PropagateArgs a = round_args(p_, kmetric(), flip, screen, pca_, pca_proj_, pca_e_, x_, d_, ld_, xh_, ldh_, eps_, q8_, ld8_, s8_,
+    const PropagateArgs a = round_args(p_, kmetric(), flip, screen, pca_, pca_proj_, pca_e_, x_, d_, ld_, xh_, ldh_, eps_, q8_ready_ ? q8_ : nullptr, ld8_, s8_,
                                        a8_, e8_, norms_, perm_, n_, gids_, gvals_, cur_, r, ncand_, changed_);
     AKNN_HIP_TRY(hipEventRecord(launch_ev(base + 2 * r), stream_));
     launch_propagate(a, stream_);
@@ -1091,7 +1093,7 @@ bool Engine::propagate_graph(int base) {
     AKNN_HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     try {
       for (int r = 0; r < p_.props; ++r)
-        launch_propagate(round_args(p_, kmetric(), flip, screen, pca_, pca_proj_, pca_e_, x_, d_, ld_, xh_, ldh_, eps_, q8_, ld8_, s8_, a8_,
+        launch_propagate(round_args(p_, kmetric(), flip, screen, pca_, pca_proj_, pca_e_, x_, d_, ld_, xh_, ldh_, eps_, q8_ready_ ? q8_ : nullptr, ld8_, s8_, a8_,
                                     e8_, norms_, perm_, n_, gids_, gvals_, (cur_ + r) & 1, r, ncand_, changed_),
                          stream_);
     } catch (...) {
@@ -1188,6 +1190,7 @@ void Engine::calculate() {
     launch_max_nonneg(nh_, n_, nh_ + n_, stream_);
   }
   if (sketch8) launch_sketch8(x_, n_, d_, ld_, q8_, ld8_, s8_, a8_, e8_, forest8 ? se8_ : nullptr, stream_);
+  q8_ready_ = sketch8;  // the int8 rows are this build's
   pca_ = PcaBasis{};
   // This build's principal subspace: the sample moments on the device, then
   // the eigen-solve on a host thread while the device builds the forest and

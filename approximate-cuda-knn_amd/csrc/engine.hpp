@@ -29,6 +29,7 @@ struct Params {
   int leaf_mfma = 0;      // leaf k-NN of trees after the first screens pairs on the matrix cores (same results)
   int sketch_int8 = 0;    // with sketch, EUCLIDEAN propagation screens with the int8 sketch (same results)
   int forest_int8 = 1;    // with sketch, the forest side test of staged buckets reads the int8 sketch (same results)
+  int flip_int8 = 1;      // COSINE parity rounds screen with the int8 sketch where it is made (same results)
   int split = 0;          // forest rule: 0 the reference's sign split, 1 median split on seeded directions
   int router_trees = 0;   // median split: trees 0..router_trees-1 take the router bins as their first levels
   int taxicab_l1 = 0;     // TAXICAB keys and stores the L1 total instead of its square root
@@ -82,6 +83,8 @@ class Engine {
     p_.leaf_mfma = p.leaf_mfma;
     p_.sketch_int8 = p.sketch_int8;
     p_.forest_int8 = p.forest_int8;
+    if (p_.flip_int8 != p.flip_int8) prop_mode_ = -1;
+    p_.flip_int8 = p.flip_int8;
     if (p_.split != p.split) prop_mode_ = -1;
     p_.split = p.split;
     p_.router_trees = p.router_trees;
@@ -198,6 +201,7 @@ class Engine {
   int32_t* a8_ = nullptr;
   float* e8_ = nullptr;
   float2* se8_ = nullptr;  // (scale, radius) pairs of the int8 sketch for the forest side test [n]
+  bool q8_ready_ = false;   // the int8 sketch was made by the current build
   int ld8_ = 0;
   unsigned long long* leaf_rescored_ = nullptr;
   int ldh_ = 0;

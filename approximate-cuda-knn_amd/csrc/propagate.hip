@@ -342,24 +342,58 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
       uint64_t* kv = reinterpret_cast<uint64_t*>(hs);  // [m] (hs has >= 2 m words)
       bool nan = lane < k && key != key;
       bool exact_all = true;
-      if constexpr (SKM == 1) {
+      if constexpr (SKM == 1 || SKM == 2) {
         // The screen of the candidates' own first k: each candidate's sketch
         // bounds lb <= key <= ub; T, the k-th smallest (ub, id), bounds the
         // k-th smallest exact key of the candidates from above, so a
         // candidate whose lb is above T is not among their first k and only
         // the others are keyed exactly.  Degenerate bounds, NaN carried
         // values or at most k candidates: every candidate is keyed.
+        // SKM 2: the sketch is the int8 reconstruction s q (radius e8): its
+        // dot s_i s_c (q_i . q_c) is an exact integer product, rounded once,
+        // within the bounds' allowance for the sketch dot's rounding.
         float* lbv = reinterpret_cast<float*>(cand);  // cand[ci]'s slot once its id is in kv
-        const float ei = eps[i];
-        const float ec_max = eps[n];
+        const float ei = SKM == 2 ? e8[i] : eps[i];
+        const float ec_max = SKM == 2 ? __builtin_inff() : eps[n];
         const bool per_cand = !(ec_max < __builtin_inff());
+        const double si = SKM == 2 ? static_cast<double>(s8[i]) : 0.0;
         bool degen = false;
         for (int base = 0; base < m; base += 64) {
           const int ci = base + lane;
           const int32_t c = ci < m ? cand[ci] : -1;
           if (c >= 0) {
-            const float p = sketch_chain<METRIC>(xs, xh + static_cast<int64_t>(c) * ldh, ldh);
-            const float ec = per_cand ? eps[c] : ec_max;
+            float p;
+            if constexpr (SKM == 2) {
+              const int4* qc = reinterpret_cast<const int4*>(q8 + static_cast<int64_t>(c) * ld8);
+              const int4* qi = reinterpret_cast<const int4*>(xrow);
+              int dot = 0;
+              const int pieces = ld8 / 16;
+              int g = 0;
+              for (; g + 8 <= pieces; g += 8) {
+                int4 v[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) v[j] = qc[g + j];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                  const int4 u = qi[g + j];
+                  dot = __builtin_amdgcn_sdot4(v[j].x, u.x, dot, false);
+                  dot = __builtin_amdgcn_sdot4(v[j].y, u.y, dot, false);
+                  dot = __builtin_amdgcn_sdot4(v[j].z, u.z, dot, false);
+                  dot = __builtin_amdgcn_sdot4(v[j].w, u.w, dot, false);
+                }
+              }
+              for (; g < pieces; ++g) {
+                const int4 v = qc[g], u = qi[g];
+                dot = __builtin_amdgcn_sdot4(v.x, u.x, dot, false);
+                dot = __builtin_amdgcn_sdot4(v.y, u.y, dot, false);
+                dot = __builtin_amdgcn_sdot4(v.z, u.z, dot, false);
+                dot = __builtin_amdgcn_sdot4(v.w, u.w, dot, false);
+              }
+              p = static_cast<float>(si * static_cast<double>(s8[c]) * static_cast<double>(dot));
+            } else {
+              p = sketch_chain<METRIC>(xs, xh + static_cast<int64_t>(c) * ldh, ldh);
+            }
+            const float ec = SKM == 2 ? e8[c] : (per_cand ? eps[c] : ec_max);
             const double lb = sketch_lower_bound<METRIC>(p, ei, ec, ni, norms[c], sb);
             const double ub = sketch_upper_bound_cosine(p, ei, ec, ni, norms[c], sb);
             degen |= !(lb > -__builtin_inf()) || !(ub < __builtin_inf());
@@ -559,7 +593,7 @@ WaveLds plan_lds(int k, int d, int ld, int ldh, int ld8, int pca_r, int skm, boo
   // COSINE parity: the selection's histogram and lists reuse the list; with
   // the sketch screen they follow the candidates' lower bounds
   if (flip && w.cand_cap < 256 + 4 * k) w.cand_cap = (256 + 4 * k + 3) / 4 * 4;
-  if (flip && skm == 1) w.cand_cap = (k * k + 3) / 4 * 4 + 256 + 4 * k;
+  if (flip && (skm == 1 || skm == 2)) w.cand_cap = (k * k + 3) / 4 * 4 + 256 + 4 * k;
   w.xi_stride = skm == 1 ? ldh + 3 * ld : skm == 2 ? ld8 / 4 + 3 * ld : skm >= 3 ? pca_r + ld : (d + 3) / 4 * 4;
   int pow2 = 128;
   while (pow2 < need) pow2 *= 2;
@@ -602,7 +636,9 @@ void launch_t(const PropagateArgs& a, hipStream_t s) {
   // bounds the candidates' own first k instead
   auto fits = [&](int skm) { return plan_lds(a.k, a.d, a.ld, a.ldh, a.ld8, a.pca_r, skm, FLIP).waves > 0; };
   if constexpr (FLIP) {
-    if (a.xh != nullptr && fits(1))
+    if (a.q8 != nullptr && fits(2))
+      launch_w<METRIC, FLIP, 2>(a, s);
+    else if (a.xh != nullptr && fits(1))
       launch_w<METRIC, FLIP, 1>(a, s);
     else
       launch_w<METRIC, FLIP, 0>(a, s);

@@ -111,6 +111,11 @@ int aknn_deinitialize(aknn_handle* h);
  *                   int8 copy (per-point scale, exact integer dot products,
  *                   half the bytes) instead of the FP16 one; 0 (default): FP16
  *                   (its tighter bound re-scores fewer candidates)
+ *   "flip_int8"     1 (default): COSINE parity rounds with "sketch" screen their
+ *                   candidates with the int8 sketch where it is made (the sign
+ *                   forest's, "forest_int8"): an exact integer dot product of
+ *                   half the bytes of the FP16 row (the graph is the same); 0:
+ *                   the FP16 row
  *   "forest_int8"   1 (default): with "sketch", the forest side test of the
  *                   buckets a block stages (D <= 512) reads each point's int8
  *                   row (per-point scale and radius, a proven bound; the exact

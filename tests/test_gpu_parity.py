@@ -235,14 +235,16 @@ def test_sketch_changes_nothing(aknn, oracle, datasets, metric, fixed, d):
     check_graph(oracle, a, x, 8, 2, 24, 4, metric, cosine_fixed=fixed)
 
 
+@pytest.mark.parametrize("flip8", [1, 0])
 @pytest.mark.parametrize("n,d,k,props", [(5000, 96, 30, 3), (3000, 300, 20, 2), (2000, 24, 12, 4)])
-def test_cosine_parity_screen(aknn, oracle, datasets, n, d, k, props):
+def test_cosine_parity_screen(aknn, oracle, datasets, n, d, k, props, flip8):
     """COSINE parity rounds with the sketch screen: the candidates' own first
-    k bounded by the k-th smallest sketch upper bound; the oracle's graph, and
-    that of the unscreened rounds, with fewer candidates keyed exactly."""
+    k bounded by the k-th smallest sketch upper bound (the int8 sketch's with
+    flip_int8, the FP16 one's without); the oracle's graph, and that of the
+    unscreened rounds, with fewer candidates keyed exactly."""
     x = datasets.sphere_c4(n, d, seed=7) if d == 96 else datasets.gmm_c1(n, d, clusters=8, seed=7)[0]
-    print(f"[gpu] COSINE parity screen n={n} d={d} k={k} rounds={props}", flush=True)
-    a = gpu_run(aknn, x, k, 3, 2 * k, props, 2, sketch=1)
+    print(f"[gpu] COSINE parity screen n={n} d={d} k={k} rounds={props} flip_int8={flip8}", flush=True)
+    a = gpu_run(aknn, x, k, 3, 2 * k, props, 2, sketch=1, flip_int8=flip8)
     b = gpu_run(aknn, x, k, 3, 2 * k, props, 2, sketch=0)
     _same_graph(a, b)
     ex, rs = a.candidate_counts()
