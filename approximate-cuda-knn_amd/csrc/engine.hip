@@ -826,6 +826,7 @@ void Engine::initial_knn_big() {
 void Engine::propagate_big() {
   const int base = 2 * p_.trees;
   round_screened_.assign(std::max(p_.props, 0), 0);
+  round_form_.assign(std::max(p_.props, 0), 0);
   rounds_from_graph_ = false;
   AKNN_HIP_TRY(hipMemsetAsync(ncand_, 0, sizeof(unsigned long long) * kCountWords * std::max(p_.props, 1), stream_));
   for (int r = 0; r < p_.props; ++r) {
@@ -1019,6 +1020,7 @@ void Engine::propagate() {
   const int base = 2 * p_.trees;
   AKNN_HIP_TRY(hipMemsetAsync(ncand_, 0, sizeof(unsigned long long) * kCountWords * std::max(p_.props, 1), stream_));
   round_screened_.assign(std::max(p_.props, 0), 0);
+  round_form_.assign(std::max(p_.props, 0), 0);
   // Screened or exact rounds (the graph is the same either way).  The first
   // calculate() on a data set runs round 0 screened and round 1 exact and
   // keeps the mode with the lower time per examined candidate for the other
@@ -1056,7 +1058,7 @@ void Engine::propagate() {
     const PropagateArgs a = round_args(p_, kmetric(), flip, screen, pca_, pca_proj_, pca_e_, x_, d_, ld_, xh_, ldh_, eps_, q8_ready_ ? q8_ : nullptr, ld8_, s8_,
                                        a8_, e8_, norms_, perm_, n_, gids_, gvals_, cur_, r, ncand_, changed_);
     AKNN_HIP_TRY(hipEventRecord(launch_ev(base + 2 * r), stream_));
-    launch_propagate(a, stream_);
+    round_form_[r] = static_cast<uint8_t>(launch_propagate(a, stream_));
     AKNN_HIP_TRY(hipEventRecord(launch_ev(base + 2 * r + 1), stream_));
     cur_ ^= 1;  // buffer swap (.cu:1969-1975)
   }
@@ -1090,12 +1092,13 @@ bool Engine::propagate_graph(int base) {
   if (!round_exec_ || round_graph_mode_ != mode) {
     drop_round_graph();
     hipGraph_t graph = nullptr;
+    graph_form_.assign(p_.props, 0);
     AKNN_HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     try {
       for (int r = 0; r < p_.props; ++r)
-        launch_propagate(round_args(p_, kmetric(), flip, screen, pca_, pca_proj_, pca_e_, x_, d_, ld_, xh_, ldh_, eps_, q8_ready_ ? q8_ : nullptr, ld8_, s8_, a8_,
+        graph_form_[r] = static_cast<uint8_t>(launch_propagate(round_args(p_, kmetric(), flip, screen, pca_, pca_proj_, pca_e_, x_, d_, ld_, xh_, ldh_, eps_, q8_ready_ ? q8_ : nullptr, ld8_, s8_, a8_,
                                     e8_, norms_, perm_, n_, gids_, gvals_, (cur_ + r) & 1, r, ncand_, changed_),
-                         stream_);
+                         stream_));
     } catch (...) {
       (void)hipStreamEndCapture(stream_, &graph);
       if (graph) (void)hipGraphDestroy(graph);
@@ -1116,6 +1119,7 @@ bool Engine::propagate_graph(int base) {
   AKNN_HIP_TRY(hipGraphLaunch(round_exec_, stream_));
   AKNN_HIP_TRY(hipEventRecord(launch_ev(base + 1), stream_));
   round_screened_.assign(p_.props, pca ? 2 : (screen ? 1 : 0));
+  round_form_ = graph_form_;
   cur_ ^= p_.props & 1;
   rounds_from_graph_ = true;
   stats_propagate.launches = p_.props;
@@ -1290,23 +1294,25 @@ void Engine::calculate() {
     const double k = p_.k;
     // per point: own row (8k), k neighbour id rows (4k^2), x_i (4D), output
     // row (8k); per unique new candidate its row (4D), or with a screen its
-    // sketch row (2 ldh; COSINE also its norm) or projection (4r), plus 4D for
-    // each exactly re-scored one.  The screens bound every candidate's radius
-    // by the largest, read once per point; the int8 screen reads three
-    // scalars per candidate.
+    // screen row plus 4D for each exactly re-scored one: the FP16 sketch row
+    // (2 ldh; COSINE also its norm), the int8 row with its three scalars
+    // (scale, radius and sum of squares or, COSINE, the norm), or the FP32
+    // (4r) or FP16 (2r) projection.  The FP16 sketch and projection screens
+    // bound every candidate's radius by the largest, read once per point.
     double per_point = 16.0 * k + 4.0 * k * k + 4.0 * d_;
     double cand = 4.0 * d_ * nc[2 * r];
-    if (round_screened_[r]) {
-      const bool i8 = p_.sketch_int8 && p_.metric == kEuclidean;
-      const double row = round_screened_[r] == 2 ? 4.0 * pca_.r
-                                                 : (i8 ? ld8_ + 12.0
-                                                       : 2.0 * ldh_ + (p_.metric == kCosine ? 4.0 : 0.0));
-      per_point += row + 8.0;  // own sketch row or projection, own radius and the radius bound
+    if (round_screened_[r] && round_form_[r] > 0) {
+      const int f = round_form_[r];
+      const double row = f == 1   ? 2.0 * ldh_ + (p_.metric == kCosine ? 4.0 : 0.0)
+                         : f == 2 ? ld8_ + 12.0
+                         : f == 3 ? 4.0 * pca_.r
+                                  : 2.0 * pca_.r;
+      per_point += row + 8.0;  // own screen row, own radius and the radius bound
       cand = row * nc[2 * r] + 4.0 * d_ * nc[2 * r + 1];
     }
     stats_propagate.units += static_cast<double>(n_) * per_point + cand;
     stats_propagate.examined += static_cast<double>(nc[2 * r]);
-    stats_propagate.rescored += static_cast<double>(round_screened_[r] ? nc[2 * r + 1] : nc[2 * r]);
+    stats_propagate.rescored += static_cast<double>(round_screened_[r] && round_form_[r] > 0 ? nc[2 * r + 1] : nc[2 * r]);
   }
   stats_forest.launches = 0;
   for (int t = 0; t < p_.trees; ++t) stats_forest.launches += iters_[t];

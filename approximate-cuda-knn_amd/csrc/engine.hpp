@@ -253,6 +253,8 @@ class Engine {
   bool cosine_table_used_ = false;
   int64_t active_sum_ = 0;
   std::vector<uint8_t> round_screened_, tree_screened_;  // which rounds / trees used the screens
+  std::vector<uint8_t> round_form_;  // per round the screen form launched (launch_propagate's return)
+  std::vector<uint8_t> graph_form_;  // the same for the rounds captured in round_exec_
   int prop_mode_ = -1;  // propagation rounds learned on this data: -1 unknown, 0 screened, 1 exact
   DrawStream draws_;
   // median-split forest: directions [trees][nd][ld] (cached per seed),

@@ -192,7 +192,9 @@ struct PropagateArgs {
   unsigned long long* ncand = nullptr;
   PropagateHistory hist;
 };
-void launch_propagate(const PropagateArgs& a, hipStream_t s);
+// Returns the screen form launched: 0 exact, 1 FP16 sketch, 2 int8 sketch,
+// 3 FP32 and 4 FP16 principal-subspace projections.
+int launch_propagate(const PropagateArgs& a, hipStream_t s);
 // whether a propagation round with k neighbours over d dimensions fits the
 // LDS of at least one wave (the exact pass; screens that do not fit give way
 // to it)

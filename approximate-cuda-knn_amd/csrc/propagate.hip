@@ -630,40 +630,44 @@ void launch_w(const PropagateArgs& a, hipStream_t s) {
 // register-row instantiations measured slower here (fewer waves in flight at
 // 196 VGPRs outweigh the deeper per-lane prefetch).
 template <int METRIC, bool FLIP>
-void launch_t(const PropagateArgs& a, hipStream_t s) {
+int launch_t(const PropagateArgs& a, hipStream_t s) {
   // COSINE parity: carried entries are keyed by their flipped stored values,
   // so the k-th entry is no distance threshold; the sketch screen there
   // bounds the candidates' own first k instead
   auto fits = [&](int skm) { return plan_lds(a.k, a.d, a.ld, a.ldh, a.ld8, a.pca_r, skm, FLIP).waves > 0; };
+  int skm = 0;
   if constexpr (FLIP) {
-    if (a.q8 != nullptr && fits(2))
-      launch_w<METRIC, FLIP, 2>(a, s);
+    skm = a.q8 != nullptr && fits(2) ? 2 : (a.xh != nullptr && fits(1) ? 1 : 0);
+  } else {
+    // a screen whose wider per-wave layout does not fit even one wave's LDS
+    // gives way to the next one, and finally to the exact pass (the same
+    // graph)
+    if (METRIC == kEuclidean && a.pca16 != nullptr && fits(4))
+      skm = 4;
+    else if (METRIC == kEuclidean && a.pca != nullptr && fits(3))
+      skm = 3;
+    else if (METRIC == kEuclidean && a.q8 != nullptr && fits(2))
+      skm = 2;
     else if (a.xh != nullptr && fits(1))
+      skm = 1;
+  }
+  switch (skm) {
+    case 1:
       launch_w<METRIC, FLIP, 1>(a, s);
-    else
+      break;
+    case 2:
+      if constexpr (FLIP || METRIC == kEuclidean) launch_w<METRIC, FLIP, 2>(a, s);
+      break;
+    case 3:
+      if constexpr (!FLIP && METRIC == kEuclidean) launch_w<METRIC, FLIP, 3>(a, s);
+      break;
+    case 4:
+      if constexpr (!FLIP && METRIC == kEuclidean) launch_w<METRIC, FLIP, 4>(a, s);
+      break;
+    default:
       launch_w<METRIC, FLIP, 0>(a, s);
-    return;
   }
-  // a screen whose wider per-wave layout does not fit even one wave's LDS
-  // gives way to the exact pass (the same graph)
-  if constexpr (METRIC == kEuclidean) {
-    if (a.pca16 != nullptr && fits(4)) {
-      launch_w<METRIC, FLIP, 4>(a, s);
-      return;
-    }
-    if (a.pca != nullptr && fits(3)) {
-      launch_w<METRIC, FLIP, 3>(a, s);
-      return;
-    }
-    if (a.q8 != nullptr && fits(2)) {
-      launch_w<METRIC, FLIP, 2>(a, s);
-      return;
-    }
-  }
-  if (a.xh != nullptr && fits(1))
-    launch_w<METRIC, FLIP, 1>(a, s);
-  else
-    launch_w<METRIC, FLIP, 0>(a, s);
+  return skm;
 }
 
 }  // namespace
@@ -672,7 +676,7 @@ bool propagation_fits(int k, int d) {
   return plan_lds(k, d, (d + 3) / 4 * 4, 0, 0, 0, 0, true).waves > 0;
 }
 
-void launch_propagate(const PropagateArgs& a, hipStream_t s) {
+int launch_propagate(const PropagateArgs& a, hipStream_t s) {
   if (a.k > 64) throw HipError("propagation supports k <= 64");
   if (a.ld % 4 != 0 || a.ld < a.d) throw HipError("propagation: row stride must be a multiple of 4 covering d");
   if (a.q8 != nullptr && (a.s8 == nullptr || a.a8 == nullptr || a.e8 == nullptr || a.ld8 % 16 != 0 || a.ld8 < a.d))
@@ -683,20 +687,13 @@ void launch_propagate(const PropagateArgs& a, hipStream_t s) {
     throw HipError("propagation: malformed principal-subspace screen");
   switch (a.metric) {
     case kEuclidean:
-      launch_t<kEuclidean, false>(a, s);
-      break;
+      return launch_t<kEuclidean, false>(a, s);
     case kTaxicab:
-      launch_t<kTaxicab, false>(a, s);
-      break;
+      return launch_t<kTaxicab, false>(a, s);
     case kTaxicabL1:
-      launch_t<kTaxicabL1, false>(a, s);
-      break;
+      return launch_t<kTaxicabL1, false>(a, s);
     case kCosine:
-      if (a.flip)
-        launch_t<kCosine, true>(a, s);
-      else
-        launch_t<kCosine, false>(a, s);
-      break;
+      return a.flip ? launch_t<kCosine, true>(a, s) : launch_t<kCosine, false>(a, s);
     default:
       throw HipError("unknown distance");
   }
