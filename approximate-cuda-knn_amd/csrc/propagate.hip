@@ -54,8 +54,11 @@ __device__ __forceinline__ bool hash_insert(int32_t* hs, uint32_t size, int32_t 
 
 // SKM: 0 exact pass only, 1 FP16 sketch screen, 2 int8 sketch screen,
 // 3 principal-subspace screen (pca.hip) on FP32 projections, 4 the same on
-// FP16 projections (their rounding is inside each point's radius).
-template <int METRIC, bool FLIP, int SKM>
+// FP16 projections (their rounding is inside each point's radius).  PR: the
+// subspace rank of SKM 3 and 4 fixed at compile time (16), so the
+// projections held in registers take only its width; 0 reads it from pca_r
+// (16 or 32) with registers for 32.
+template <int METRIC, bool FLIP, int SKM, int PR = 0>
 __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict__ x, int d, int ld,
                                                         const _Float16* __restrict__ xh, int ldh,
                                                         const float* __restrict__ eps, SketchBound sb,
@@ -206,22 +209,24 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
         // slots at or below ones already read)
         const double k1 = 1.0 - sb.kap;
         for (int base = 0; base < m; base += 128) {
+          constexpr int RG = PR > 0 ? PR : 32;  // register width
+          const int rk = PR > 0 ? PR : pca_r;
           int32_t cc[2];
-          float4 pv[2][8];
-          uint4 ph[2][4];
+          float4 pv[2][RG / 4];
+          uint4 ph[2][RG / 8];
 #pragma unroll
           for (int u = 0; u < 2; ++u) {
             const int ci = base + 64 * u + lane;
             cc[u] = ci < m ? cand[ci] : -1;
-            const int64_t row = static_cast<int64_t>(cc[u] >= 0 ? cc[u] : 0) * pca_r;
+            const int64_t row = static_cast<int64_t>(cc[u] >= 0 ? cc[u] : 0) * rk;
             if constexpr (SKM == 4) {
               const uint4* pc = reinterpret_cast<const uint4*>(pca16 + row);
 #pragma unroll
-              for (int g = 0; g < 4; ++g) ph[u][g] = g < pca_r / 8 ? pc[g] : make_uint4(0u, 0u, 0u, 0u);
+              for (int g = 0; g < RG / 8; ++g) ph[u][g] = g < rk / 8 ? pc[g] : make_uint4(0u, 0u, 0u, 0u);
             } else {
               const float4* pc = reinterpret_cast<const float4*>(pca + row);
 #pragma unroll
-              for (int g = 0; g < 8; ++g) pv[u][g] = g < pca_r / 4 ? pc[g] : make_float4(0.f, 0.f, 0.f, 0.f);
+              for (int g = 0; g < RG / 4; ++g) pv[u][g] = g < rk / 4 ? pc[g] : make_float4(0.f, 0.f, 0.f, 0.f);
             }
           }
           if constexpr (SKM == 4) {
@@ -229,7 +234,7 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
 #pragma unroll
             for (int u = 0; u < 2; ++u)
 #pragma unroll
-              for (int g = 0; g < 8; ++g) {
+              for (int g = 0; g < RG / 4; ++g) {
                 const uint4 h = ph[u][g >> 1];
                 const uint32_t w0 = (g & 1) ? h.z : h.x, w1 = (g & 1) ? h.w : h.y;
                 pv[u][g] = make_float4(half_lo(w0), half_hi(w0), half_lo(w1), half_hi(w1));
@@ -240,8 +245,8 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
           for (int u = 0; u < 2; ++u) {
             float p = 0.0f;
 #pragma unroll
-            for (int g = 0; g < 8; ++g) {
-              if (g < pca_r / 4) {
+            for (int g = 0; g < RG / 4; ++g) {
+              if (g < rk / 4) {
                 const float4 v = pq[g], q = pv[u][g];
                 float t = v.x - q.x;
                 p = __builtin_fmaf(t, t, p);
@@ -609,7 +614,7 @@ WaveLds plan_lds(int k, int d, int ld, int ldh, int ld8, int pca_r, int skm, boo
   return w;
 }
 
-template <int METRIC, bool FLIP, int SKM>
+template <int METRIC, bool FLIP, int SKM, int PR = 0>
 void launch_w(const PropagateArgs& a, hipStream_t s) {
   const WaveLds w = plan_lds(a.k, a.d, a.ld, a.ldh, a.ld8, a.pca_r, SKM, FLIP);
   if (w.waves == 0) throw HipError("propagation: k or d too large for LDS");
@@ -617,7 +622,7 @@ void launch_w(const PropagateArgs& a, hipStream_t s) {
   const size_t shmem = w.bytes();
   const int64_t blocks = xcd_grid((a.n + w.waves - 1) / w.waves);
   const SketchBound sb = sketch_bound(a.d);
-  hipLaunchKernelGGL((propagate_kernel<METRIC, FLIP, SKM>), dim3(static_cast<unsigned>(blocks)), dim3(64 * w.waves), shmem,
+  hipLaunchKernelGGL((propagate_kernel<METRIC, FLIP, SKM, PR>), dim3(static_cast<unsigned>(blocks)), dim3(64 * w.waves), shmem,
                      s, a.x, a.d, a.ld, a.xh, a.ldh, a.eps, sb, a.q8, a.ld8, a.s8, a.a8, a.e8, a.pca, a.pca16, a.pca_r,
                      a.pca_e,
                      a.pca_sigma, a.norms, a.order, a.n, a.k, hs_size, cand_cap,
@@ -659,10 +664,20 @@ int launch_t(const PropagateArgs& a, hipStream_t s) {
       if constexpr (FLIP || METRIC == kEuclidean) launch_w<METRIC, FLIP, 2>(a, s);
       break;
     case 3:
-      if constexpr (!FLIP && METRIC == kEuclidean) launch_w<METRIC, FLIP, 3>(a, s);
+      if constexpr (!FLIP && METRIC == kEuclidean) {
+        if (a.pca_r == 16)
+          launch_w<METRIC, FLIP, 3, 16>(a, s);
+        else
+          launch_w<METRIC, FLIP, 3, 0>(a, s);
+      }
       break;
     case 4:
-      if constexpr (!FLIP && METRIC == kEuclidean) launch_w<METRIC, FLIP, 4>(a, s);
+      if constexpr (!FLIP && METRIC == kEuclidean) {
+        if (a.pca_r == 16)
+          launch_w<METRIC, FLIP, 4, 16>(a, s);
+        else
+          launch_w<METRIC, FLIP, 4, 0>(a, s);
+      }
       break;
     default:
       launch_w<METRIC, FLIP, 0>(a, s);
