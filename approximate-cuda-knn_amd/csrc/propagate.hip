@@ -208,6 +208,28 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
         // projections are loaded before either is tested (compaction writes
         // slots at or below ones already read)
         const double k1 = 1.0 - sb.kap;
+        // The bound below is a nondecreasing function of p, evaluated in FP64
+        // by monotone steps: it exceeds thk exactly when p > P, where
+        //   P = (S^2 + tiny) / k1,  S = ei + ec + sigma R,
+        //   R = sqrt(((thk / k1)^2 + tiny) / k1)
+        // in exact arithmetic, and its rounding moves that boundary by a few
+        // units of 2^-52 relative.  So p above p_hi (P raised by 2^-13, then
+        // up to a float) is certainly dropped and p below p_lo certainly
+        // kept, with one FP32 comparison; the band between them, and every
+        // point whose quantities leave that derivation (a per-candidate
+        // radius, non-finite or negative values), take the FP64 bound.
+        float p_hi = __builtin_inff(), p_lo = -__builtin_inff();
+        if (!per_cand && thk >= 0.0f && thk < __builtin_inff() && ei >= 0.0f && ei < __builtin_inff() &&
+            ec_max >= 0.0f) {
+          const double t = static_cast<double>(thk) / k1;
+          const double rr = sqrt((t * t + sb.tiny) / k1);
+          const double ss = static_cast<double>(ei) + static_cast<double>(ec_max) + pca_sigma * rr;
+          const double pp = (ss * ss + sb.tiny) / k1;
+          if (pp < 1e300) {
+            p_hi = float_up(pp * (1.0 + 1.0 / 8192.0));
+            p_lo = float_down(pp * (1.0 - 1.0 / 8192.0));
+          }
+        }
         for (int base = 0; base < m; base += 128) {
           constexpr int RG = PR > 0 ? PR : 32;  // register width
           const int rk = PR > 0 ? PR : pca_r;
@@ -258,13 +280,17 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
                 p = __builtin_fmaf(t, t, p);
               }
             }
-            // ||x - y|| >= (||p(x) - p(y)|| - e(x) - e(y)) / sigma, with the
-            // r-term chain's rounding taken off; then the exact chain's error
-            const double sp = fmax(0.0, static_cast<double>(p) * k1 - sb.tiny);
-            const float ec = per_cand ? pca_e[cc[u] >= 0 ? cc[u] : 0] : ec_max;
-            const double r = fmax(0.0, sqrt(sp) - static_cast<double>(ei) - static_cast<double>(ec)) / pca_sigma;
-            const double lb = sqrt(fmax(0.0, r * r * k1 - sb.tiny)) * k1;
-            const bool pass = cc[u] >= 0 && !(lb > static_cast<double>(thk));
+            bool drop = p > p_hi;
+            if (!drop && !(p < p_lo)) {
+              // ||x - y|| >= (||p(x) - p(y)|| - e(x) - e(y)) / sigma, with the
+              // r-term chain's rounding taken off; then the exact chain's error
+              const double sp = fmax(0.0, static_cast<double>(p) * k1 - sb.tiny);
+              const float ec = per_cand ? pca_e[cc[u] >= 0 ? cc[u] : 0] : ec_max;
+              const double r = fmax(0.0, sqrt(sp) - static_cast<double>(ei) - static_cast<double>(ec)) / pca_sigma;
+              const double lb = sqrt(fmax(0.0, r * r * k1 - sb.tiny)) * k1;
+              drop = lb > static_cast<double>(thk);
+            }
+            const bool pass = cc[u] >= 0 && !drop;
             const uint64_t bal = __ballot(pass);
             if (pass) cand[w + __popcll(bal & ((1ull << lane) - 1))] = cc[u];
             w += __popcll(bal);
