@@ -55,15 +55,15 @@ __device__ __forceinline__ float chain_step(float a, float b, float p) {
 
 // Chain over dimensions [0, d) of row a (any address space) and row b
 // (global, 16-byte aligned, padded to a multiple of 4).  Returns the total.
-template <int METRIC>
+template <int METRIC, int UNROLL = 8>
 __device__ __forceinline__ float row_chain(const float* a, const float* __restrict__ b, int d) {
   float tot = 0.0f;
   for (int c0 = 0; c0 < d; c0 += 256) {
     const int c1 = c0 + 256 < d ? c0 + 256 : d;
     float p = 0.0f;
     int dd = c0;
-    // eight 16-byte loads in flight per lane before their FMAs
-#pragma unroll 8
+    // UNROLL 16-byte loads in flight per lane before their FMAs
+#pragma unroll UNROLL
     for (; dd + 4 <= c1; dd += 4) {
       const float4 bv = *reinterpret_cast<const float4*>(b + dd);
       p = chain_step<METRIC>(a[dd + 0], bv.x, p);

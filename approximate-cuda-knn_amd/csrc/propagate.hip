@@ -584,8 +584,11 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
       } else {
         c = ci < m ? cand[ci] : -1;
         if (c >= 0) {
+          // after the subspace screen (its projection registers free by
+          // now) sixteen 16-byte loads in flight per lane, else eight
           const float* xc = x + static_cast<int64_t>(c) * ld;
-          dk = finish_key<METRIC>(row_chain<METRIC>(xi, xc, d), ni, METRIC == kCosine ? norms[c] : 1.0f);
+          const float t = SKM >= 3 ? row_chain<METRIC, 16>(xi, xc, d) : row_chain<METRIC>(xi, xc, d);
+          dk = finish_key<METRIC>(t, ni, METRIC == kCosine ? norms[c] : 1.0f);
         }
       }
       uint64_t surv = __ballot(c >= 0 && key_less(dk, c, thk, thi));
