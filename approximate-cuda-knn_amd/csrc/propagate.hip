@@ -82,7 +82,7 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   // per wave: x_i (FP32, plain form) or, with the sketch, x_i's sketch as
   // floats, x_i (FP32) and two staged candidate rows; then the hash set, the
-  // candidate list and four spare words.  xi_stride is a multiple of 4.
+  // candidate list and its counter.  xi_stride is a multiple of 4.
   const size_t per_wave_words = static_cast<size_t>(xi_stride) + hs_size + cand_cap + 4;
   float* xrow = reinterpret_cast<float*>(smem) + per_wave_words * wave;  // 16-byte aligned
   constexpr bool SK = SKM > 0;
@@ -90,6 +90,7 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
   float* stg = xf + ld;
   int32_t* hs = reinterpret_cast<int32_t*>(xrow + xi_stride);
   int32_t* cand = hs + hs_size;
+  int32_t* cnt = cand + cand_cap;
   // COSINE parity, once the candidates' keys are packed: a histogram and
   // two lists of k packed pairs over the candidate list (its capacity is at
   // least 256 + 4k words then)
@@ -151,6 +152,7 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
   {
     // two-hop ids (.cu:1078-1085), deduplicated in the LDS hash set that
     // holds i and i's row
+    if (lane == 0) *cnt = 0;
     const uint32_t hsz = static_cast<uint32_t>(hs_size);
     if (lane == 0) hash_insert(hs, hsz, static_cast<int32_t>(i));
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -162,7 +164,11 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
     // Slot f is neighbour f / k's entry f % k: the quotient from the float
     // reciprocal, (f + 1/2) * fl(1/k) truncated, which is exact for every
     // k <= 64 and f < 2^13.  A neighbour that is not expanded reads as -1.
-    // New ids take list slots by a ballot prefix in slot order.
+    // New ids take list slots by a ballot prefix in slot order, or, in the
+    // COSINE parity rounds, from a counter in LDS: the lists and graphs are
+    // the same either way, and each form keeps the one measured faster for
+    // it (the ballot on the subspace-screened EUCLIDEAN rounds, 4%; the
+    // counter on the int8-screened parity rounds, 60%).
     constexpr int kHop = 8;
     const int kk_pad = kk + (64 - kk % 64) % 64;
     const float inv_k = 1.0f / static_cast<float>(k);
@@ -181,15 +187,22 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
 #pragma unroll
       for (int u = 0; u < kHop; ++u) {
         const int32_t c = cv[u];
-        const bool ins = c >= 0 && hash_insert(hs, hsz, c);
-        const uint64_t bal = __ballot(ins);
-        if (ins) cand[w + __popcll(bal & ((1ull << lane) - 1))] = c;
-        w += __popcll(bal);
+        if constexpr (FLIP) {
+          if (c >= 0 && hash_insert(hs, hsz, c)) {
+            const int pos = atomicAdd(cnt, 1);
+            cand[pos] = c;
+          }
+        } else {
+          const bool ins = c >= 0 && hash_insert(hs, hsz, c);
+          const uint64_t bal = __ballot(ins);
+          if (ins) cand[w + __popcll(bal & ((1ull << lane) - 1))] = c;
+          w += __popcll(bal);
+        }
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    m = w;
+    m = FLIP ? *cnt : w;
     nnew = m;
     if (ncand != nullptr && lane == 0)
       atomicAdd(ncand + (blockIdx.x % kCountSlots) * kCountStride, static_cast<unsigned long long>(m));
@@ -614,7 +627,7 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
 }
 
 // LDS layout of one wave: x_i (and with a screen its screening row and two
-// staged rows), the hash set, the candidate list and four spare words.
+// staged rows), the hash set, the candidate list and its counter.
 struct WaveLds {
   int hs_size = 0, cand_cap = 0, xi_stride = 0, waves = 0;
   size_t bytes() const { return (static_cast<size_t>(xi_stride) + hs_size + cand_cap + 4) * 4 * waves; }
