@@ -77,6 +77,44 @@ __device__ __forceinline__ float row_chain(const float* a, const float* __restri
   return tot;
 }
 
+// The same chain with the row of b read U 16-byte loads at a time into
+// registers, all issued before the first step that uses them (a scheduling
+// barrier keeps the compiler from interleaving them with the steps, which
+// it otherwise does to save registers): one memory round trip per 4U
+// dimensions.  For kernels whose occupancy is bound by LDS, not registers.
+template <int METRIC, int U>
+__device__ __forceinline__ float row_chain_batched(const float* a, const float* __restrict__ b, int d) {
+  float tot = 0.0f;
+  for (int c0 = 0; c0 < d; c0 += 256) {
+    const int c1 = c0 + 256 < d ? c0 + 256 : d;
+    float p = 0.0f;
+    int dd = c0;
+    for (; dd + 4 * U <= c1; dd += 4 * U) {
+      float4 v[U];
+#pragma unroll
+      for (int j = 0; j < U; ++j) v[j] = *reinterpret_cast<const float4*>(b + dd + 4 * j);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        p = chain_step<METRIC>(a[dd + 4 * j + 0], v[j].x, p);
+        p = chain_step<METRIC>(a[dd + 4 * j + 1], v[j].y, p);
+        p = chain_step<METRIC>(a[dd + 4 * j + 2], v[j].z, p);
+        p = chain_step<METRIC>(a[dd + 4 * j + 3], v[j].w, p);
+      }
+    }
+    for (; dd + 4 <= c1; dd += 4) {
+      const float4 bv = *reinterpret_cast<const float4*>(b + dd);
+      p = chain_step<METRIC>(a[dd + 0], bv.x, p);
+      p = chain_step<METRIC>(a[dd + 1], bv.y, p);
+      p = chain_step<METRIC>(a[dd + 2], bv.z, p);
+      p = chain_step<METRIC>(a[dd + 3], bv.w, p);
+    }
+    for (; dd < c1; ++dd) p = chain_step<METRIC>(a[dd], b[dd], p);
+    tot = tot + p;
+  }
+  return tot;
+}
+
 // Same chain with both rows in global memory (16-byte aligned, padded to a
 // multiple of 4): 16-byte loads from both, 2 * UNROLL in flight per lane.
 template <int METRIC, int UNROLL = 4>

@@ -34,7 +34,12 @@ namespace aknn {
 
 namespace {
 
-constexpr int kWavesPerBlock = 4;  // waves per block when their LDS fits (fewer otherwise)
+constexpr int kWavesPerBlock = 4;
+// 16-byte row loads in flight per lane in the exact COSINE parity rounds:
+// their LDS (a k^2 list beside the hash set) holds them to a few waves per
+// SIMD, so registers are free and a row of up to 96 floats is read in one
+// round trip
+constexpr int kFlipRowLoads = 24;  // waves per block when their LDS fits (fewer otherwise)
 constexpr size_t kLdsBytes = 160 * 1024;
 
 
@@ -513,8 +518,14 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
           const int ci = base + lane;
           const int32_t c = ci < m ? cand[ci] : -1;
           if (c >= 0) {
-            const float dk = finish_key<METRIC>(row_chain<METRIC>(xi, x + static_cast<int64_t>(c) * ld, d), ni,
-                                                METRIC == kCosine ? norms[c] : 1.0f);
+            // the exact form with rows of at most kFlipRowLoads 16-byte
+            // pieces: the whole row in one round trip; longer rows and the
+            // screened forms keep the compiler's pipelined loads (batches
+            // drain between them: slower there)
+            const float* xc = x + static_cast<int64_t>(c) * ld;
+            const float t = SKM == 0 && d <= 4 * kFlipRowLoads ? row_chain_batched<METRIC, kFlipRowLoads>(xi, xc, d)
+                                                               : row_chain<METRIC>(xi, xc, d);
+            const float dk = finish_key<METRIC>(t, ni, METRIC == kCosine ? norms[c] : 1.0f);
             nan |= dk != dk;
             kv[ci] = pack_pair(dk, order_id(c));
           }
