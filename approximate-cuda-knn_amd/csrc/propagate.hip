@@ -609,10 +609,12 @@ __global__ __launch_bounds__(256) void propagate_kernel(const float* __restrict_
         c = ci < m ? cand[ci] : -1;
         if (c >= 0) {
           // after the subspace screen (its projection registers free by
-          // now) eight 16-byte loads issued together per batch (the
-          // compiler's own schedule keeps four in flight), else its schedule
+          // now) and in the plain form for rows of up to 256 floats, eight
+          // 16-byte loads issued together per batch (the compiler's own
+          // schedule keeps four in flight); longer rows keep its schedule
           const float* xc = x + static_cast<int64_t>(c) * ld;
-          const float t = SKM >= 3 ? row_chain_batched<METRIC, 8>(xi, xc, d) : row_chain<METRIC>(xi, xc, d);
+          const float t = SKM >= 3 || (SKM == 0 && d <= 256) ? row_chain_batched<METRIC, 8>(xi, xc, d)
+                                                             : row_chain<METRIC>(xi, xc, d);
           dk = finish_key<METRIC>(t, ni, METRIC == kCosine ? norms[c] : 1.0f);
         }
       }
